@@ -1,0 +1,355 @@
+"""Write the golden input/output vectors under tests/golden/ from the reference solver.
+
+This script is the only place that executes the reference's Python, and it runs only in the
+build container (the reference tree is not present where the GPU tests run).  It pins the
+functions that closed forms cannot pin on their own: the loop ordering of
+``iblbm/runtime/loop.py:44-151``, the boundary closures with their corner rules
+(``iblbm/boundary/dirichlet_neumann.py:39-118``, ``iblbm/boundary/settling_channel.py:42-142``),
+the MDF control flow (``iblbm/ibm/mdf.py:175-229``), the omega estimate
+(``iblbm/ibm/a_norm.py:204-266``), the DFC sequencing (``iblbm/ibm/dfc.py:256-342``,
+``iblbm/ibm/strategies/dfc.py:31-54``) and the sedimentation hooks.
+
+The reference's CPU backend (``IBLBM_GPU=0``) JIT-compiles its lattice kernels with numba,
+which is not installed here.  A stand-in ``numba`` module is installed first: its ``prange``
+is ``range`` and its ``njit`` recompiles the decorated function as plain Python with the one
+lowering rule of numba's that plain Python does not share: a power with a small integer
+literal exponent (``x ** 2``) is lowered by numba to repeated multiplication (``x * x``),
+whereas CPython calls libm ``pow``, which misrounds about 1 in 1300 squares.  The kernels
+then run as ordinary IEEE-double Python loops with the same expression order (numba without
+fastmath emits no contraction, so the values are the ones its compiled kernels produce).
+
+Usage:  python tests/golden/make_golden.py [case ...]
+Outputs: tests/golden/<case>.npz (numeric arrays) and tests/golden/<case>.json (scalars).
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+import time
+import types
+
+import numpy as np
+
+REF = "/root/reference"
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def _numba_like(fn):
+    """Recompile ``fn`` with ``x ** <int literal>`` lowered to repeated multiplication."""
+    import ast
+    import inspect
+    import textwrap
+
+    tree = ast.parse(textwrap.dedent(inspect.getsource(fn)))
+    fdef = tree.body[0]
+    fdef.decorator_list = []
+
+    class _Pow(ast.NodeTransformer):
+        def visit_BinOp(self, node):
+            self.generic_visit(node)
+            if (isinstance(node.op, ast.Pow) and isinstance(node.right, ast.Constant)
+                    and type(node.right.value) is int and node.right.value == 2):
+                return ast.BinOp(left=node.left, op=ast.Mult(), right=node.left)
+            return node
+
+    tree = ast.fix_missing_locations(_Pow().visit(tree))
+    ns = {}
+    exec(compile(tree, inspect.getsourcefile(fn), "exec"), fn.__globals__, ns)
+    return ns[fdef.name]
+
+
+def _install_reference():
+    os.environ["IBLBM_GPU"] = "0"
+    sys.dont_write_bytecode = True
+    nb = types.ModuleType("numba")
+
+    def njit(*args, **kwargs):
+        if args and callable(args[0]) and not kwargs:
+            return _numba_like(args[0])
+        return _numba_like
+
+    nb.njit = njit
+    nb.prange = range
+    sys.modules["numba"] = nb
+    if REF not in sys.path:
+        sys.path.insert(0, REF)
+    import iblbm  # noqa: F401
+
+
+def _save(name, arrays, scalars):
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrays)
+    with open(os.path.join(OUT, name + ".json"), "w") as fh:
+        json.dump(scalars, fh, indent=1, sort_keys=True)
+    print(f"wrote {name}: {sorted(arrays)}", flush=True)
+
+
+# --------------------------------------------------------------------------- unit vectors
+
+def case_units():
+    """Per-function vectors on seeded random inputs (small)."""
+    from iblbm.lbm import make_d2q9
+    from iblbm.lbm.equilibrium import _compute_feq_nb
+    from iblbm.lbm.macroscopic import _macroscopic_nb
+    from iblbm.lbm.streaming import _streaming_step_nb
+    from iblbm.lbm.collision.bgk import _collision_bgk_nb
+    from iblbm.lbm.collision.trt import _collision_trt_nb
+    from iblbm.lbm.collision.cm_mrt import _collision_cm_mrt_nb
+    from iblbm.ibm.common import delta_hat, delta_peskin4pt
+    from iblbm.boundary.base import BoundaryIndices
+    from iblbm.boundary.dirichlet_neumann import apply_bc_dirichlet_neumann
+    from iblbm.boundary.settling_channel import apply_bc_settling_channel
+    from iblbm.ibm.df import ibm_direct_forcing
+    from iblbm.ibm.mdf import ibm_multi_direct_forcing
+    from iblbm.ibm.dfc import apply_dfc_correction, compute_lambda
+    from iblbm.ibm.a_norm import estimate_optimal_omega_from_type
+
+    lat = make_d2q9()
+    rng = np.random.default_rng(20240611)
+    A = {}
+    S = {}
+    N = 64
+    ro = 1.0 + 0.05 * rng.standard_normal(N)
+    U = 0.05 * rng.standard_normal((N, 2))
+    A["feq_ro"], A["feq_U"] = ro, U
+    A["feq_c"] = _compute_feq_nb(ro, U, lat.e, lat.w, False)
+    A["feq_i"] = _compute_feq_nb(ro, U, lat.e, lat.w, True)
+
+    fst = A["feq_c"] + 1e-3 * rng.standard_normal((N, 9))
+    A["macro_f"] = fst
+    A["macro_ro_c"], A["macro_U_c"] = _macroscopic_nb(fst, lat.e, False)
+    A["macro_ro_i"], A["macro_U_i"] = _macroscopic_nb(fst, lat.e, True)
+
+    ny, nx = 6, 8
+    fs_old = rng.standard_normal((ny * nx, 9))
+    f_post = rng.standard_normal((ny * nx, 9))
+    A["stream_old"], A["stream_post"] = fs_old, f_post
+    A["stream_out"] = _streaming_step_nb(fs_old, f_post, nx, ny)
+
+    feq = A["feq_c"]
+    fib = 1e-3 * rng.standard_normal((N, 2))
+    A["col_fstar"], A["col_feq"], A["col_U"], A["col_fib"] = fst, feq, U, fib
+    tau = 0.6125
+    S["col_tau"] = tau
+    A["col_bgk"] = _collision_bgk_nb(fst, feq, U, fib, tau, 1.0, lat.e, lat.w)
+    tau_minus = 0.5 + 0.25 / (tau - 0.5)
+    S["col_tau_minus"] = tau_minus
+    A["col_trt"] = _collision_trt_nb(fst, feq, U, fib, 1.0 / tau, 1.0 / tau_minus, 1.0,
+                                     lat.e, lat.w, lat.opp)
+    S_diag = np.array([0, 0, 0, 1 / tau, 1 / tau, 1 / tau, 1.2, 1.2, 1.4], dtype=np.float64)
+    A["col_cm_sdiag"] = S_diag
+    A["col_cm"] = _collision_cm_mrt_nb(fst, feq, U, fib, S_diag, 1.0, lat.e, lat.w)
+
+    r = np.concatenate([np.linspace(-2.6, 2.6, 201), rng.uniform(-2.5, 2.5, 200),
+                        np.array([0.0, 1.0, -1.0, 2.0, -2.0, 0.5, 1.5])])
+    A["delta_r"] = r
+    A["delta_hat"] = delta_hat(r)
+    A["delta_p4"] = delta_peskin4pt(r)
+
+    # boundary closures on a random post-stream field
+    ny, nx = 7, 9
+    from iblbm.boundary.indices import build_boundary_indices
+
+    class _C:
+        bc_type = "dirichlet_neumann"
+
+    idx = build_boundary_indices(_C, nx, ny)
+    fb = 1.0 / 9.0 + 0.01 * rng.standard_normal((ny * nx, 9))
+    A["bc_in"] = fb.copy()
+    f1 = fb.copy()
+    rb, Ub = np.ones(ny * nx), np.zeros((ny * nx, 2))
+    apply_bc_dirichlet_neumann(f1, rb, Ub, 0.05, 1.0, idx, lat)
+    A["bc_dn"] = f1
+    f2 = fb.copy()
+    apply_bc_settling_channel(f2, rb, Ub, 1.0, idx, lat, incompressible=False)
+    A["bc_sc"] = f2
+    S["bc_ny"], S["bc_nx"] = ny, nx
+
+    # IBM on a random smooth field: 24 x 20 grid, ring of 13 markers (domain units)
+    ny, nx = 20, 24
+    dx = 1.0 / 19.0
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    Eux = 0.04 + 0.01 * np.sin(0.3 * ii + 0.2 * jj) + 1e-3 * rng.standard_normal((ny, nx))
+    Euy = 0.01 * np.cos(0.25 * ii - 0.4 * jj) + 1e-3 * rng.standard_normal((ny, nx))
+    Ro = 1.0 + 0.01 * rng.standard_normal((ny, nx))
+    th = 2 * np.pi * np.arange(13) / 13
+    Lx = (11.3 + 3.1 * np.cos(th)) * dx
+    Ly = (9.6 + 3.1 * np.sin(th)) * dx
+    ud = 0.002 * rng.standard_normal((13, 2))
+    Larea = 2 * np.pi * 3.1 / 13
+    A["ibm_Eux"], A["ibm_Euy"], A["ibm_Ro"] = Eux, Euy, Ro
+    A["ibm_Lx"], A["ibm_Ly"], A["ibm_ud"] = Lx, Ly, ud
+    S["ibm_dx"], S["ibm_Larea"], S["ibm_ny"], S["ibm_nx"] = dx, Larea, ny, nx
+    for dt_name in ("hat", "peskin4pt"):
+        for inc in (False, True):
+            fib, Lux, Luy, Lfx, Lfy, R = ibm_direct_forcing(
+                Lx, Ly, ud, Eux, Euy, Ro, dx, dx, 1.0, Larea, ny, nx,
+                delta_type=dt_name, incompressible=inc)
+            k = f"df_{dt_name}_{'i' if inc else 'c'}"
+            A[k + "_fib"], A[k + "_Lux"], A[k + "_Luy"] = fib, Lux, Luy
+            A[k + "_Lfx"], A[k + "_Lfy"], A[k + "_R"] = Lfx, Lfy, R
+        om = estimate_optimal_omega_from_type(Lx, Ly, dt_name, Larea, dx)
+        S[f"omega_{dt_name}"] = om
+        Uf = np.stack([Eux.ravel(), Euy.ravel()], axis=1)
+        for n_iter, min_iter, tol in ((20, 5, 1e-4), (20, 3, 1e-2), (1, 1, 1e-4), (6, 2, 1e-12)):
+            fibt, st = ibm_multi_direct_forcing(
+                Lx, Ly, ud, Uf, Ro.ravel(), dx, dx, 1.0, Larea, ny, nx,
+                n_iter=n_iter, min_iter=min_iter, delta_type=dt_name, omega=om, tol=tol)
+            k = f"mdf_{dt_name}_{n_iter}_{min_iter}"
+            A[k + "_fib"] = fibt
+            S[k + "_stats"] = st
+        # divergence guard: oversize omega
+        fibt, st = ibm_multi_direct_forcing(
+            Lx, Ly, ud, Uf, Ro.ravel(), dx, dx, 1.0, Larea, ny, nx,
+            n_iter=20, min_iter=5, delta_type=dt_name, omega=6.0 * om, tol=1e-4)
+        A[f"mdf_{dt_name}_div_fib"] = fibt
+        S[f"mdf_{dt_name}_div_stats"] = st
+        lam = compute_lambda(Lx, Ly, 1.0, Larea, dx, dx, ny, nx, dt_name)
+        A[f"dfc_{dt_name}_lambda"] = lam
+        fstar = _compute_feq_nb(Ro.ravel(), Uf, lat.e, lat.w, False) \
+            + 1e-4 * rng.standard_normal((ny * nx, 9))
+        A[f"dfc_{dt_name}_fstar"] = fstar
+        fc, force, _ = apply_dfc_correction(Lx, Ly, ud, fstar, dx, dx, Larea, ny, nx,
+                                            dt_name, lat, lambda_cache=None)
+        A[f"dfc_{dt_name}_fcorr"] = fc
+        A[f"dfc_{dt_name}_force"] = force
+    _save("units", A, S)
+
+
+# --------------------------------------------------------------------------- whole runs
+
+def _perturbing_initialize(seed, amp):
+    """Wrap ``initialize`` so the initial velocity carries a seeded relative perturbation.
+
+    u <- u * (1 + amp * xi), xi ~ U(-1, 1) i.i.d. per node and component (numpy PCG64), then
+    feq = fstar = f_eq(rho, u).  The reference has no such option; the same arrays are
+    produced by the build's host code from the same seed.
+    """
+    import iblbm.runtime.loop as loop_mod
+    from iblbm.lbm import compute_feq
+    base = loop_mod.initialize
+
+    def init(cfg):
+        st = base(cfg)
+        rng = np.random.default_rng(seed)
+        xi = rng.uniform(-1.0, 1.0, size=st.U.shape)
+        st.U = st.U * (1.0 + amp * xi)
+        st.feq = compute_feq(st.ro, st.U, st.lattice, incompressible=cfg.incompressible_lbgk)
+        st.fstar = st.feq.copy()
+        return st
+
+    loop_mod.initialize = init
+    return lambda: setattr(loop_mod, "initialize", base)
+
+
+def _run_case(name, cfg_kwargs, perturb=None, record_mdf=False):
+    import iblbm
+    from iblbm.runtime import step as step_mod
+
+    cfg = iblbm.SimConfig(**cfg_kwargs)
+    restore = _perturbing_initialize(*perturb) if perturb else (lambda: None)
+    mdf_hist = []
+    sed = []
+
+    def cb(step, Cd, Cl, error, state, elapsed, converged):
+        st = getattr(state, "_mdf_iter_stats", None)
+        if record_mdf and st is not None:
+            mdf_hist.append([step, st["iter_count"], st["last_residual"],
+                             int(st["diverged"]), int(st["converged"])])
+
+    t0 = time.time()
+    try:
+        res = iblbm.run(cfg, verbose=False, callback=cb)
+    finally:
+        restore()
+    st = res["state"]
+    f_post = step_mod.collide(st, cfg)
+    A = {
+        "fstar": st.fstar, "f_post": f_post, "ro": st.ro, "U": st.U, "fib": st.fib,
+        "feq": st.feq, "Lx": np.asarray(st.Lx), "Ly": np.asarray(st.Ly),
+        "Cd": res["Cd_history"], "Cl": res["Cl_history"],
+    }
+    if mdf_hist:
+        A["mdf_stats"] = np.array(mdf_hist, dtype=np.float64)
+    if getattr(st, "dfc_force_lagr", None) is not None:
+        A["dfc_force"] = st.dfc_force_lagr
+    if getattr(st, "lambda_cache", None) is not None:
+        A["lambda"] = st.lambda_cache
+    S = {"cfg": {k: v for k, v in cfg_kwargs.items()}, "nx": st.nx, "ny": st.ny,
+         "tau": st.tau, "Larea": float(st.Larea), "Lb": int(st.Lb),
+         "lattice_r": float(st.lattice_r), "final_step": res["final_step"],
+         "omega": getattr(st, "_mdf_omega", None),
+         "tau_minus": getattr(st, "tau_minus", None),
+         "gen_seconds": time.time() - t0}
+    if "sedimentation_history" in res:
+        hist = res["sedimentation_history"]
+        keys = sorted(k for k in hist[0] if isinstance(hist[0][k], (int, float)))
+        A["sed_hist"] = np.array([[float(h[k]) for k in keys] for h in hist])
+        S["sed_keys"] = keys
+        S["particle_pos"] = [float(v) for v in np.asarray(st.particle_pos)]
+        S["particle_vel"] = [float(v) for v in np.asarray(st.particle_vel)]
+        S["termination_reason"] = res.get("termination_reason")
+    if perturb:
+        S["perturb_seed"], S["perturb_amp"] = perturb
+    _save(name, A, S)
+
+
+def case_run_df_bgk_hat():
+    _run_case("run_df_bgk_hat", dict(
+        Re=5.0, NN=21, xmax=2.0, ymax=1.0, cylinder_center=(0.5, 0.5), cylinder_D_ratio=0.25,
+        lattice_u=0.05, inflow_u=0.05, bc_type="dirichlet_neumann", delta_type="hat",
+        ibm_method="DF", collision_model="BGK", marker_spacing_factor=1.0, max_steps=200,
+        check_interval=10, use_convergence=False, mdf_iterations=1))
+
+
+def case_run_mdf_trt_p4():
+    _run_case("run_mdf_trt_p4", dict(
+        Re=10.0, NN=25, xmax=2.0, ymax=1.0, cylinder_center=(0.5, 0.5), cylinder_D_ratio=0.25,
+        lattice_u=0.04, inflow_u=0.04, bc_type="dirichlet_neumann", delta_type="peskin4pt",
+        ibm_method="MDF", collision_model="TRT", trt_magic_param=0.25,
+        mdf_iterations=20, mdf_min_iterations=5, mdf_tolerance=1e-4,
+        marker_spacing_factor=1.0, max_steps=120, check_interval=1, use_convergence=False),
+        perturb=(1234, 1e-3), record_mdf=True)
+
+
+def case_run_dfc_cm_hat():
+    _run_case("run_dfc_cm_hat", dict(
+        Re=8.0, NN=21, xmax=2.0, ymax=1.0, cylinder_center=(0.45, 0.5), cylinder_D_ratio=0.25,
+        lattice_u=0.05, inflow_u=0.05, bc_type="dirichlet_neumann", delta_type="hat",
+        ibm_method="DFC", collision_model="CM_MRT", marker_spacing_factor=1.0,
+        max_steps=150, check_interval=10, use_convergence=False))
+
+
+def case_run_df_cm_p4():
+    _run_case("run_df_cm_p4", dict(
+        Re=12.0, NN=21, xmax=2.0, ymax=1.0, cylinder_center=(0.55, 0.48), cylinder_D_ratio=0.3,
+        lattice_u=0.05, inflow_u=0.05, bc_type="dirichlet_neumann", delta_type="peskin4pt",
+        ibm_method="DF", collision_model="CM_MRT", marker_spacing_factor=1.0,
+        max_steps=120, check_interval=10, use_convergence=False),
+        perturb=(99, 1e-3))
+
+
+def case_run_dfc_trt_p4():
+    _run_case("run_dfc_trt_p4", dict(
+        Re=10.0, NN=21, xmax=2.0, ymax=1.0, cylinder_center=(0.5, 0.52), cylinder_D_ratio=0.25,
+        lattice_u=0.04, inflow_u=0.04, bc_type="dirichlet_neumann", delta_type="peskin4pt",
+        ibm_method="DFC", collision_model="TRT", marker_spacing_factor=1.0,
+        max_steps=120, check_interval=10, use_convergence=False))
+
+
+def case_run_mdf_bgk_hat():
+    _run_case("run_mdf_bgk_hat", dict(
+        Re=6.0, NN=21, xmax=2.0, ymax=1.0, cylinder_center=(0.5, 0.5), cylinder_D_ratio=0.25,
+        lattice_u=0.05, inflow_u=0.05, bc_type="dirichlet_neumann", delta_type="hat",
+        ibm_method="MDF", collision_model="BGK", mdf_iterations=20, mdf_min_iterations=5,
+        mdf_tolerance=1e-4, marker_spacing_factor=1.0, max_steps=100, check_interval=1,
+        use_convergence=False), record_mdf=True)
+
+
+CASES = {k[5:]: v for k, v in dict(globals()).items() if k.startswith("case_")}
+
+if __name__ == "__main__":
+    _install_reference()
+    names = sys.argv[1:] or sorted(CASES)
+    for n in names:
+        CASES[n]()
