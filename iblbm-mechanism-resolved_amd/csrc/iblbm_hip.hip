@@ -1,0 +1,713 @@
+// iblbm_hip.hip -- gfx950 kernels and the C ABI of libiblbm_hip.so (see include/iblbm_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <algorithm>
+#include <climits>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/iblbm_hip.h"
+#include "ibm_block.h"
+
+using namespace iblbm;
+
+// ------------------------------------------------------------------ kernels
+
+__global__ __launch_bounds__(NT) void k_step(LaunchTable T) {
+    __shared__ double red[NT];
+    __shared__ int redi[NT];
+    const int b = blockIdx.x;
+    if (b < T.n_ibm) {
+        Inst &I = T.insts[T.ibm_inst[b]];
+        ibm_role(I, T.parity, T.diag, red, redi);
+        return;
+    }
+    const int tb = b - T.n_ibm;
+    int inst = 0;
+    while (inst + 1 < T.n_inst && T.tile_prefix[inst + 1] <= tb) ++inst;
+    const Inst &I = T.insts[inst];
+    const int lt = tb - T.tile_prefix[inst];
+    const int tx = lt % I.tiles_x, ty = lt / I.tiles_x;
+    const int i = tx * TILE_X + (int)(threadIdx.x % TILE_X);
+    const int j = ty * TILE_Y + (int)(threadIdx.x / TILE_X);
+    double rho = 0.0;
+    const bool in_grid = i < I.nx && j < I.ny;
+    const bool in_box = I.ibm != 0 && i >= I.bx0 && i < I.bx0 + I.bw && j >= I.by0 &&
+                        j < I.by0 + I.bh;
+    if (in_grid && !in_box) {
+        const double *__restrict__ A = I.f[T.parity];
+        double *__restrict__ B = I.f[T.parity ^ 1];
+        double fs[9], fe[9], out[9];
+        fstar_at(A, I, i, j, fs);
+        double r, u0, u1;
+        macro9(fs, I.inc != 0, r, u0, u1);
+        if (I.cp.model != 2) feq9(r, u0, u1, I.inc != 0, fe);
+        collide<false>(I.cp, fs, fe, u0, u1, 0.0, 0.0, out);
+        store9(B, I, i, j, out);
+        rho = r;
+    }
+    if (T.diag) {
+        const double s = block_sum_fixed(rho, red);
+        if (threadIdx.x == 0) T.rho_partial[b] = s;
+    }
+}
+
+// state.fstar / ro / U of the last step (source buffer A = that step's input); box nodes come
+// from the IBM scratch (DFC-corrected fstar, velocity-corrected U).
+__global__ __launch_bounds__(256) void k_state(const Inst *Ip, int parity, double *fstar,
+                                               double *ro, double *U) {
+    const Inst &I = *Ip;
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (int64_t)I.nx * I.ny) return;
+    const int i = (int)(n % I.nx), j = (int)(n / I.nx);
+    double fs[9], r, u0, u1;
+    const bool in_box = I.ibm != 0 && i >= I.bx0 && i < I.bx0 + I.bw && j >= I.by0 &&
+                        j < I.by0 + I.bh;
+    if (in_box) {
+        const int sl = (j - I.by0) * I.bw + (i - I.bx0);
+        for (int q = 0; q < 9; ++q) fs[q] = I.s.fs[q * I.box_cap + sl];
+        r = I.s.ro[sl];
+        u0 = I.s.ux0[sl];
+        u1 = I.s.uy0[sl];
+    } else {
+        fstar_at(I.f[parity], I, i, j, fs);
+        macro9(fs, I.inc != 0, r, u0, u1);
+    }
+    for (int q = 0; q < 9; ++q) fstar[9 * n + q] = fs[q];
+    ro[n] = r;
+    U[2 * n] = u0;
+    U[2 * n + 1] = u1;
+}
+
+// AoS (N,9) <-> SoA f[q][j][pitch]
+__global__ void k_aos_to_soa(const double *aos, double *soa, int nx, int ny, int pitch,
+                             int64_t qs) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (int64_t)nx * ny) return;
+    const int i = (int)(n % nx), j = (int)(n / nx);
+    for (int q = 0; q < 9; ++q) soa[q * qs + (int64_t)j * pitch + i] = aos[9 * n + q];
+}
+__global__ void k_soa_to_aos(const double *soa, double *aos, int nx, int ny, int pitch,
+                             int64_t qs) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (int64_t)nx * ny) return;
+    const int i = (int)(n % nx), j = (int)(n / nx);
+    for (int q = 0; q < 9; ++q) aos[9 * n + q] = soa[q * qs + (int64_t)j * pitch + i];
+}
+
+// ------------------------------------------------------------------ host side
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(IBLBM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));  \
+    } while (0)
+
+struct DevAlloc {
+    std::vector<void *> ptrs;
+    hipError_t alloc(void **p, size_t bytes) {
+        hipError_t e = hipMalloc(p, bytes < 8 ? 8 : bytes);
+        if (e == hipSuccess) ptrs.push_back(*p);
+        return e;
+    }
+    void release() {
+        for (void *p : ptrs) (void)hipFree(p);
+        ptrs.clear();
+    }
+};
+
+struct HostInst {
+    iblbm_case c;
+    Inst d;  // shadow of the device descriptor
+    int Lb = 0;
+    double Larea = 0.0;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevAlloc mem;
+    std::vector<HostInst> inst;
+    Inst *d_insts = nullptr;
+    int *d_ibm_inst = nullptr, *d_tile_prefix = nullptr;
+    double *d_rho_partial = nullptr;
+    int cap_inst = 0, cap_blocks = 0;
+    bool tables_dirty = true;
+    int n_ibm = 0, n_tiles = 0;
+    std::vector<int> h_tile_prefix, h_ibm_inst;
+    int64_t step = 0;
+    int parity = 0;
+    float last_ms = 0.0f;
+    int last_steps = 0;
+    int64_t diag_step = -1;
+    std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
+};
+
+int ensure_tables(Ctx &C) {
+    if (!C.tables_dirty) return 0;
+    const int n = (int)C.inst.size();
+    C.h_tile_prefix.assign(n + 1, 0);
+    C.h_ibm_inst.clear();
+    for (int k = 0; k < n; ++k) {
+        const Inst &d = C.inst[k].d;
+        C.h_tile_prefix[k + 1] = C.h_tile_prefix[k] + d.tiles_x * d.tiles_y;
+        if (d.ibm != 0) C.h_ibm_inst.push_back(k);
+    }
+    C.n_tiles = C.h_tile_prefix[n];
+    C.n_ibm = (int)C.h_ibm_inst.size();
+    const int blocks = C.n_tiles + C.n_ibm;
+    if (n > C.cap_inst) {
+        int cap = n < 16 ? 16 : 2 * n;
+        void *p;
+        HIPCHK(C.mem.alloc(&p, sizeof(Inst) * cap));
+        C.d_insts = (Inst *)p;
+        HIPCHK(C.mem.alloc(&p, sizeof(int) * (cap + 1)));
+        C.d_tile_prefix = (int *)p;
+        HIPCHK(C.mem.alloc(&p, sizeof(int) * (cap + 1)));
+        C.d_ibm_inst = (int *)p;
+        C.cap_inst = cap;
+    }
+    if (blocks > C.cap_blocks) {
+        void *p;
+        HIPCHK(C.mem.alloc(&p, sizeof(double) * blocks));
+        C.d_rho_partial = (double *)p;
+        C.cap_blocks = blocks;
+    }
+    std::vector<Inst> hd(n);
+    for (int k = 0; k < n; ++k) hd[k] = C.inst[k].d;
+    HIPCHK(hipMemcpyAsync(C.d_insts, hd.data(), sizeof(Inst) * n, hipMemcpyHostToDevice,
+                          C.stream));
+    HIPCHK(hipMemcpyAsync(C.d_tile_prefix, C.h_tile_prefix.data(), sizeof(int) * (n + 1),
+                          hipMemcpyHostToDevice, C.stream));
+    if (C.n_ibm)
+        HIPCHK(hipMemcpyAsync(C.d_ibm_inst, C.h_ibm_inst.data(), sizeof(int) * C.n_ibm,
+                              hipMemcpyHostToDevice, C.stream));
+    HIPCHK(hipStreamSynchronize(C.stream));
+    for (auto &g : C.graphs) (void)hipGraphExecDestroy(g.second);
+    C.graphs.clear();
+    C.tables_dirty = false;
+    return 0;
+}
+
+int upload_inst(Ctx &C, int id) {
+    // replaces the device descriptor (the IBM workgroup's cached-table flags are reset by the
+    // shadow, which forces a rebuild on the next step)
+    HIPCHK(hipMemcpyAsync(C.d_insts + id, &C.inst[id].d, sizeof(Inst), hipMemcpyHostToDevice,
+                          C.stream));
+    return 0;
+}
+
+LaunchTable make_table(Ctx &C, int parity, int diag) {
+    LaunchTable T;
+    T.insts = C.d_insts;
+    T.ibm_inst = C.d_ibm_inst;
+    T.tile_prefix = C.d_tile_prefix;
+    T.rho_partial = C.d_rho_partial;
+    T.n_inst = (int)C.inst.size();
+    T.n_ibm = C.n_ibm;
+    T.parity = parity;
+    T.diag = diag;
+    return T;
+}
+
+int launch_steps(Ctx &C, int nsteps, int diag_every) {
+    const int blocks = C.n_tiles + C.n_ibm;
+    for (int s = 0; s < nsteps; ++s) {
+        const int64_t stepno = C.step + s + 1;
+        const int diag = (diag_every > 0 && stepno % diag_every == 0) ? 1 : 0;
+        LaunchTable T = make_table(C, (C.parity + s) & 1, diag);
+        hipLaunchKernelGGL(k_step, dim3(blocks), dim3(NT), 0, C.stream, T);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+Ctx *as_ctx(void *p) { return static_cast<Ctx *>(p); }
+
+bool valid_id(Ctx *C, int id) { return C && id >= 0 && id < (int)C->inst.size(); }
+
+}  // namespace
+
+extern "C" {
+
+const char *iblbm_last_error(void) { return g_err.c_str(); }
+
+__attribute__((visibility("hidden"))) int iblbm_set_op_error(int code, const char *m) {
+    return fail(code, m);
+}
+
+int iblbm_device_name(int device, char *buf, int buflen) {
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, device));
+    snprintf(buf, buflen, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    return 0;
+}
+
+int iblbm_ctx_create(int device, void **out) {
+    if (!out) return fail(IBLBM_EINVAL, "ctx_create: null output pointer");
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n)
+        return fail(IBLBM_EDEVICE, "ctx_create: no HIP device " + std::to_string(device));
+    HIPCHK(hipSetDevice(device));
+    Ctx *C = new Ctx();
+    C->device = device;
+    if (hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&C->ev0) != hipSuccess || hipEventCreate(&C->ev1) != hipSuccess) {
+        delete C;
+        return fail(IBLBM_EDEVICE, "ctx_create: stream/event creation failed");
+    }
+    *out = C;
+    return 0;
+}
+
+int iblbm_ctx_destroy(void *p) {
+    Ctx *C = as_ctx(p);
+    if (!C) return 0;
+    (void)hipSetDevice(C->device);
+    (void)hipStreamSynchronize(C->stream);
+    for (auto &g : C->graphs) (void)hipGraphExecDestroy(g.second);
+    C->mem.release();
+    (void)hipEventDestroy(C->ev0);
+    (void)hipEventDestroy(C->ev1);
+    (void)hipStreamDestroy(C->stream);
+    delete C;
+    return 0;
+}
+
+int iblbm_instance_add(void *p, const iblbm_case *c, int *id) {
+    Ctx *C = as_ctx(p);
+    if (!C || !c || !id) return fail(IBLBM_EINVAL, "instance_add: null argument");
+    if (c->nx < 3 || c->ny < 3) return fail(IBLBM_EINVAL, "instance_add: grid must be >= 3x3");
+    if (c->collision < 0 || c->collision > 2)
+        return fail(IBLBM_EKEY, "instance_add: unknown collision model");
+    if (c->bc < 0 || c->bc > 1) return fail(IBLBM_EKEY, "instance_add: unknown bc_type");
+    if (c->ibm < 0 || c->ibm > 3) return fail(IBLBM_EKEY, "instance_add: unknown ibm_method");
+    if (c->delta < 0 || c->delta > 1) return fail(IBLBM_EKEY, "instance_add: unknown delta_type");
+    if (c->precision != IBLBM_FP64)
+        return fail(IBLBM_EINVAL, "instance_add: only FP64 storage is built in this library");
+    if (c->ibm != 0 && (c->lb_capacity < 1 || c->box_capacity < 1))
+        return fail(IBLBM_EINVAL, "instance_add: IBM needs lb_capacity and box_capacity");
+    if (c->ibm == IBLBM_IBM_MDF && (c->mdf_iterations < 1 || c->mdf_min_iterations < 1))
+        return fail(IBLBM_EINVAL, "instance_add: mdf iterations must be >= 1");
+    HIPCHK(hipSetDevice(C->device));
+    HostInst h;
+    h.c = *c;
+    Inst &d = h.d;
+    memset(&d, 0, sizeof d);
+    d.nx = c->nx;
+    d.ny = c->ny;
+    d.pitch = (c->nx + 31) / 32 * 32;
+    d.qs = (int64_t)d.pitch * d.ny;
+    d.bc = c->bc;
+    d.inc = c->incompressible;
+    d.tiles_x = (c->nx + TILE_X - 1) / TILE_X;
+    d.tiles_y = (c->ny + TILE_Y - 1) / TILE_Y;
+    CollParams &cp = d.cp;
+    cp.model = c->collision;
+    cp.dt = c->dt;
+    cp.inv_tau = 1.0 / c->tau;             // bgk.py:28-29
+    cp.guo_pref = 1.0 - 0.5 / c->tau;
+    cp.s_plus = 1.0 / c->tau;              // trt.py:81-82
+    cp.s_minus = 1.0 / c->tau_minus;
+    cp.gpp = 1.0 - 0.5 * cp.s_plus;        // trt.py:42-43
+    cp.gpm = 1.0 - 0.5 * cp.s_minus;
+    for (int q = 0; q < 9; ++q) cp.S[q] = c->s_diag[q];
+    d.in_u = c->in_u;
+    d.dens = c->dens;
+    if (c->bc == IBLBM_BC_DIRICHLET_NEUMANN)
+        feq9(c->dens, c->in_u, 0.0, false, d.feq_bc);  // dirichlet_neumann.py:186-199
+    else
+        feq9(c->dens, 0.0, 0.0, c->incompressible != 0, d.feq_bc);  // settling_channel.py:137-142
+    d.ibm = c->ibm;
+    d.delta = c->delta;
+    d.dx = c->dx;
+    d.dy = c->dy;
+    d.omega = c->mdf_omega;
+    d.tol = c->mdf_tolerance;
+    d.n_iter = c->mdf_iterations;
+    d.min_iter = c->mdf_min_iterations;
+    d.fixed_body = c->fixed_body;
+    void *ptr;
+    const size_t fbytes = sizeof(double) * 9 * (size_t)d.qs;
+    for (int k = 0; k < 2; ++k) {
+        if (C->mem.alloc(&ptr, fbytes) != hipSuccess)
+            return fail(IBLBM_ENOMEM, "instance_add: population buffers do not fit");
+        d.f[k] = (double *)ptr;
+        HIPCHK(hipMemsetAsync(ptr, 0, fbytes, C->stream));
+    }
+    const int L = c->ibm ? c->lb_capacity : 1, B = c->ibm ? c->box_capacity : 1;
+    const int K = 25;
+    d.lb_cap = L;
+    d.box_cap = B;
+    auto dalloc = [&](double **pp, size_t n) {
+        void *q;
+        hipError_t e = C->mem.alloc(&q, sizeof(double) * n);
+        *pp = (double *)q;
+        return e;
+    };
+    auto ialloc = [&](int **pp, size_t n) {
+        void *q;
+        hipError_t e = C->mem.alloc(&q, sizeof(int) * n);
+        *pp = (int *)q;
+        return e;
+    };
+    IbmScratch &s = d.s;
+    double *lx, *ly, *ud;
+    bool ok = dalloc(&lx, L) == hipSuccess && dalloc(&ly, L) == hipSuccess &&
+              dalloc(&ud, 2 * (size_t)L) == hipSuccess && dalloc(&s.fs, 9 * (size_t)B) == hipSuccess &&
+              dalloc(&s.ro, B) == hipSuccess && dalloc(&s.ux0, B) == hipSuccess &&
+              dalloc(&s.uy0, B) == hipSuccess && dalloc(&s.uwx, B) == hipSuccess &&
+              dalloc(&s.uwy, B) == hipSuccess && dalloc(&s.vs, B) == hipSuccess &&
+              dalloc(&s.fibx, B) == hipSuccess && dalloc(&s.fiby, B) == hipSuccess &&
+              dalloc(&s.wt, B) == hipSuccess && ialloc(&s.cnt, B) == hipSuccess &&
+              ialloc(&s.off, B + 1) == hipSuccess && ialloc(&s.cur, B) == hipSuccess &&
+              dalloc(&s.cw, (size_t)K * L) == hipSuccess && ialloc(&s.cslot, (size_t)K * L) == hipSuccess &&
+              ialloc(&s.list, (size_t)K * L) == hipSuccess && dalloc(&s.Lux, L) == hipSuccess &&
+              dalloc(&s.Luy, L) == hipSuccess && dalloc(&s.R, L) == hipSuccess &&
+              dalloc(&s.Lfx, L) == hipSuccess && dalloc(&s.Lfy, L) == hipSuccess &&
+              dalloc(&s.lam, L) == hipSuccess && dalloc(&s.q, 2 * (size_t)L) == hipSuccess &&
+              dalloc(&s.fi, 9 * (size_t)L) == hipSuccess && dalloc(&s.dfl, 9 * (size_t)L) == hipSuccess &&
+              dalloc(&d.stats, 4) == hipSuccess && dalloc(&d.diag, 8) == hipSuccess;
+    if (!ok) return fail(IBLBM_ENOMEM, "instance_add: IBM scratch allocation failed");
+    HIPCHK(hipMemsetAsync(s.fibx, 0, sizeof(double) * B, C->stream));
+    HIPCHK(hipMemsetAsync(s.fiby, 0, sizeof(double) * B, C->stream));
+    HIPCHK(hipMemsetAsync(d.stats, 0, sizeof(double) * 4, C->stream));
+    HIPCHK(hipMemsetAsync(d.diag, 0, sizeof(double) * 8, C->stream));
+    d.Lx = lx;
+    d.Ly = ly;
+    d.ud = ud;
+    d.Lb = 0;
+    if (c->ibm != 0) d.ibm = 0;  // inactive until markers arrive
+    h.c.ibm = c->ibm;
+    C->inst.push_back(h);
+    C->tables_dirty = true;
+    *id = (int)C->inst.size() - 1;
+    return 0;
+}
+
+int iblbm_marker_box(const double *Lx, const double *Ly, int Lb, double dx, double dy, int nx,
+                     int ny, int delta, int box[4]) {
+    if (!Lx || !Ly || !box || Lb < 1 || nx < 1 || ny < 1 || !(dx > 0.0) || !(dy > 0.0))
+        return fail(IBLBM_EINVAL, "marker_box: bad argument");
+    const int a = delta == IBLBM_DELTA_HAT ? 1 : 2;
+    int64_t x0 = INT64_MAX, x1 = INT64_MIN, y0 = INT64_MAX, y1 = INT64_MIN;
+    for (int k = 0; k < Lb; ++k) {
+        if (!std::isfinite(Lx[k]) || !std::isfinite(Ly[k]))
+            return fail(IBLBM_EINVAL, "marker_box: non-finite marker coordinate");
+        const int64_t ix0 = (int64_t)std::floor(Lx[k] / dx + 0.5);  // df.py:229-230
+        const int64_t iy0 = (int64_t)std::floor(Ly[k] / dy + 0.5);
+        x0 = std::min(x0, ix0 - a);
+        x1 = std::max(x1, ix0 + a);
+        y0 = std::min(y0, iy0 - a);
+        y1 = std::max(y1, iy0 + a);
+    }
+    auto cl = [](int64_t v, int64_t hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
+    x0 = cl(x0, nx - 1);
+    x1 = cl(x1, nx - 1);
+    y0 = cl(y0, ny - 1);
+    y1 = cl(y1, ny - 1);
+    box[0] = (int)x0;
+    box[1] = (int)y0;
+    box[2] = (int)(x1 - x0 + 1);
+    box[3] = (int)(y1 - y0 + 1);
+    return 0;
+}
+
+int iblbm_set_markers(void *p, int id, const double *Lx, const double *Ly, const double *ud,
+                      int Lb, double Larea) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id)) return fail(IBLBM_EINVAL, "set_markers: bad instance id");
+    HostInst &h = C->inst[id];
+    Inst &d = h.d;
+    if (h.c.ibm == 0) return fail(IBLBM_EINVAL, "set_markers: instance has no IBM method");
+    if (!Lx || !Ly || !ud) return fail(IBLBM_EINVAL, "set_markers: null marker array");
+    if (Lb < 1 || Lb > d.lb_cap)
+        return fail(IBLBM_EINVAL, "set_markers: Lb outside [1, lb_capacity]");
+    int box[4];
+    int brc = iblbm_marker_box(Lx, Ly, Lb, d.dx, d.dy, d.nx, d.ny, d.delta, box);
+    if (brc) return brc;
+    const int bx0 = box[0], by0 = box[1], bw = box[2], bh = box[3];
+    if (bw < 1 || bh < 1 || (int64_t)bw * bh > d.box_cap)
+        return fail(IBLBM_EINVAL, "set_markers: window box exceeds box_capacity");
+    if (bx0 < 0 || by0 < 0 || bx0 + bw > d.nx || by0 + bh > d.ny)
+        return fail(IBLBM_EINVAL, "set_markers: window box outside the grid");
+    if (!(Larea > 0.0)) return fail(IBLBM_EINVAL, "set_markers: Larea must be positive");
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipMemcpyAsync((void *)d.Lx, Lx, sizeof(double) * Lb, hipMemcpyHostToDevice, C->stream));
+    HIPCHK(hipMemcpyAsync((void *)d.Ly, Ly, sizeof(double) * Lb, hipMemcpyHostToDevice, C->stream));
+    HIPCHK(hipMemcpyAsync((void *)d.ud, ud, sizeof(double) * 2 * Lb, hipMemcpyHostToDevice,
+                          C->stream));
+    const bool was_active = d.ibm != 0;
+    d.ibm = h.c.ibm;
+    d.Lb = Lb;
+    d.Larea = Larea;
+    d.bx0 = bx0;
+    d.by0 = by0;
+    d.bw = bw;
+    d.bh = bh;
+    d.tables_valid = 0;
+    d.lambda_valid = 0;
+    h.Lb = Lb;
+    h.Larea = Larea;
+    // the host copies are pageable: make sure they have been consumed before returning
+    if (!was_active || C->tables_dirty) {
+        C->tables_dirty = true;
+        int rc = ensure_tables(*C);
+        if (rc) return rc;
+    } else {
+        int rc = upload_inst(*C, id);
+        if (rc) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return 0;
+}
+
+int iblbm_set_torque_center(void *p, int id, double cx, double cy) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id)) return fail(IBLBM_EINVAL, "set_torque_center: bad instance id");
+    Inst &d = C->inst[id].d;
+    d.cxd = cx;
+    d.cyd = cy;
+    d.cxc = cx / d.dx;
+    d.cyc = cy / d.dy;
+    if (C->tables_dirty) return ensure_tables(*C);
+    int rc = upload_inst(*C, id);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return 0;
+}
+
+int iblbm_upload_f(void *p, int id, const double *f_aos) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || !f_aos) return fail(IBLBM_EINVAL, "upload_f: bad argument");
+    HIPCHK(hipSetDevice(C->device));
+    int rc = ensure_tables(*C);
+    if (rc) return rc;
+    const Inst &d = C->inst[id].d;
+    const int64_t N = (int64_t)d.nx * d.ny;
+    void *tmp;
+    HIPCHK(hipMalloc(&tmp, sizeof(double) * 9 * N));
+    hipError_t e = hipMemcpy(tmp, f_aos, sizeof(double) * 9 * N, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
+                           C->stream, (const double *)tmp, d.f[C->parity], d.nx, d.ny, d.pitch,
+                           d.qs);
+        e = hipStreamSynchronize(C->stream);
+    }
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(IBLBM_EDEVICE, std::string("upload_f: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int iblbm_download_f(void *p, int id, double *f_aos) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || !f_aos) return fail(IBLBM_EINVAL, "download_f: bad argument");
+    HIPCHK(hipSetDevice(C->device));
+    const Inst &d = C->inst[id].d;
+    const int64_t N = (int64_t)d.nx * d.ny;
+    void *tmp;
+    HIPCHK(hipMalloc(&tmp, sizeof(double) * 9 * N));
+    hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, C->stream,
+                       (const double *)d.f[C->parity], (double *)tmp, d.nx, d.ny, d.pitch, d.qs);
+    hipError_t e = hipStreamSynchronize(C->stream);
+    if (e == hipSuccess) e = hipMemcpy(f_aos, tmp, sizeof(double) * 9 * N, hipMemcpyDeviceToHost);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(IBLBM_EDEVICE, std::string("download_f: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int iblbm_download_state(void *p, int id, double *fstar, double *ro, double *U) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || !fstar || !ro || !U)
+        return fail(IBLBM_EINVAL, "download_state: bad argument");
+    if (C->step == 0) return fail(IBLBM_EINVAL, "download_state: no step has run yet");
+    HIPCHK(hipSetDevice(C->device));
+    const Inst &d = C->inst[id].d;
+    const int64_t N = (int64_t)d.nx * d.ny;
+    void *tmp;
+    HIPCHK(hipMalloc(&tmp, sizeof(double) * 12 * N));
+    double *tf = (double *)tmp, *tr = tf + 9 * N, *tu = tr + N;
+    hipLaunchKernelGGL(k_state, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, C->stream,
+                       (const Inst *)(C->d_insts + id), C->parity ^ 1, tf, tr, tu);
+    hipError_t e = hipStreamSynchronize(C->stream);
+    if (e == hipSuccess) e = hipMemcpy(fstar, tf, sizeof(double) * 9 * N, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(ro, tr, sizeof(double) * N, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(U, tu, sizeof(double) * 2 * N, hipMemcpyDeviceToHost);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(IBLBM_EDEVICE, std::string("download_state: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int iblbm_download_fib(void *p, int id, double *fib) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || !fib) return fail(IBLBM_EINVAL, "download_fib: bad argument");
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    const HostInst &h = C->inst[id];
+    const Inst &d = h.d;
+    const int64_t N = (int64_t)d.nx * d.ny;
+    memset(fib, 0, sizeof(double) * 2 * N);
+    if (d.ibm == 0 || d.ibm == IBLBM_IBM_DFC) return 0;
+    const int nb = d.bw * d.bh;
+    std::vector<double> fx(nb), fy(nb);
+    HIPCHK(hipMemcpy(fx.data(), d.s.fibx, sizeof(double) * nb, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(fy.data(), d.s.fiby, sizeof(double) * nb, hipMemcpyDeviceToHost));
+    for (int sl = 0; sl < nb; ++sl) {
+        const int64_t n = (int64_t)(d.by0 + sl / d.bw) * d.nx + d.bx0 + sl % d.bw;
+        fib[2 * n] = fx[sl];
+        fib[2 * n + 1] = fy[sl];
+    }
+    return 0;
+}
+
+int iblbm_download_dfc_force(void *p, int id, double *q) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || !q) return fail(IBLBM_EINVAL, "download_dfc_force: bad argument");
+    const HostInst &h = C->inst[id];
+    if (h.d.ibm != IBLBM_IBM_DFC) return fail(IBLBM_EINVAL, "download_dfc_force: not a DFC instance");
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    HIPCHK(hipMemcpy(q, h.d.s.q, sizeof(double) * 2 * h.Lb, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int iblbm_step(void *p, int nsteps, int diag_every) {
+    Ctx *C = as_ctx(p);
+    if (!C || nsteps < 0) return fail(IBLBM_EINVAL, "step: bad argument");
+    if (C->inst.empty()) return fail(IBLBM_EINVAL, "step: no instance");
+    HIPCHK(hipSetDevice(C->device));
+    int rc = ensure_tables(*C);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(C->ev0, C->stream));
+    rc = launch_steps(*C, nsteps, diag_every);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(C->ev1, C->stream));
+    if (diag_every > 0) {
+        const int64_t last = ((C->step + nsteps) / diag_every) * diag_every;
+        if (last > C->step) C->diag_step = last;
+    }
+    C->step += nsteps;
+    C->parity = (C->parity + nsteps) & 1;
+    C->last_steps = nsteps;
+    return 0;
+}
+
+int iblbm_graph_run(void *p, int nsteps, int diag_every) {
+    Ctx *C = as_ctx(p);
+    if (!C || nsteps < 1) return fail(IBLBM_EINVAL, "graph_run: bad argument");
+    if (C->inst.empty()) return fail(IBLBM_EINVAL, "graph_run: no instance");
+    HIPCHK(hipSetDevice(C->device));
+    int rc = ensure_tables(*C);
+    if (rc) return rc;
+    const int phase = diag_every > 0 ? (int)(C->step % diag_every) : 0;
+    auto key = std::make_tuple(nsteps, diag_every, C->parity, phase);
+    auto it = C->graphs.find(key);
+    hipGraphExec_t exec;
+    if (it == C->graphs.end()) {
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(C->stream, hipStreamCaptureModeThreadLocal));
+        rc = launch_steps(*C, nsteps, diag_every);
+        hipError_t e = hipStreamEndCapture(C->stream, &g);
+        if (rc) return rc;
+        if (e != hipSuccess) return fail(IBLBM_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(IBLBM_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+        C->graphs[key] = exec;
+    } else {
+        exec = it->second;
+    }
+    HIPCHK(hipEventRecord(C->ev0, C->stream));
+    HIPCHK(hipGraphLaunch(exec, C->stream));
+    HIPCHK(hipEventRecord(C->ev1, C->stream));
+    if (diag_every > 0) {
+        const int64_t last = ((C->step + nsteps) / diag_every) * diag_every;
+        if (last > C->step) C->diag_step = last;
+    }
+    C->step += nsteps;
+    C->parity = (C->parity + nsteps) & 1;
+    C->last_steps = nsteps;
+    return 0;
+}
+
+int iblbm_sync(void *p) {
+    Ctx *C = as_ctx(p);
+    if (!C) return fail(IBLBM_EINVAL, "sync: null context");
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return 0;
+}
+
+int iblbm_last_timing(void *p, double *ms, int *steps) {
+    Ctx *C = as_ctx(p);
+    if (!C || !ms || !steps) return fail(IBLBM_EINVAL, "last_timing: bad argument");
+    HIPCHK(hipEventSynchronize(C->ev1));
+    float t = 0.0f;
+    HIPCHK(hipEventElapsedTime(&t, C->ev0, C->ev1));
+    *ms = t;
+    *steps = C->last_steps;
+    return 0;
+}
+
+int64_t iblbm_step_counter(void *p) {
+    Ctx *C = as_ctx(p);
+    return C ? C->step : -1;
+}
+
+int iblbm_get_mdf_stats(void *p, int id, iblbm_mdf_stats *out) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || !out) return fail(IBLBM_EINVAL, "mdf_stats: bad argument");
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    double s[4];
+    HIPCHK(hipMemcpy(s, C->inst[id].d.stats, sizeof s, hipMemcpyDeviceToHost));
+    out->iter_count = (int32_t)s[0];
+    out->last_residual = s[1];
+    out->diverged = (int32_t)s[2];
+    out->converged = (int32_t)s[3];
+    out->reserved = 0;
+    return 0;
+}
+
+int iblbm_get_diag(void *p, int id, iblbm_diag *out) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || !out) return fail(IBLBM_EINVAL, "get_diag: bad argument");
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    const Inst &d = C->inst[id].d;
+    std::vector<double> part(C->n_tiles + C->n_ibm);
+    HIPCHK(hipMemcpy(part.data(), C->d_rho_partial, sizeof(double) * part.size(),
+                     hipMemcpyDeviceToHost));
+    double box[5] = {0, 0, 0, 0, 0};
+    if (d.ibm != 0) HIPCHK(hipMemcpy(box, d.diag, sizeof box, hipMemcpyDeviceToHost));
+    double rs = 0.0;
+    const int b0 = C->n_ibm + C->h_tile_prefix[id], b1 = C->n_ibm + C->h_tile_prefix[id + 1];
+    for (int b = b0; b < b1; ++b) rs += part[b];
+    rs += box[0];
+    out->rho_sum = rs;
+    out->force[0] = box[1];
+    out->force[1] = box[2];
+    out->torque_fib = box[3];
+    out->torque_q = box[4];
+    out->step = C->diag_step;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,606 @@
+// ops.hip -- operator-level entry points with the argument meaning of the reference's
+// gpu_kernels.py wrappers (host arrays in and out), and the host-side omega estimate.
+//
+// They share the per-node arithmetic (lbm_math.h) and the window machinery (ibm_block.h) with
+// the fused step, so a parity test of an operator also pins that piece of the fused kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/iblbm_hip.h"
+#include "ibm_block.h"
+
+using namespace iblbm;
+
+namespace {
+
+thread_local std::string g_op_err;
+
+// shares the error slot with the context API through iblbm_last_error's fallback
+int op_fail(int code, const std::string &m);
+
+struct Dev {
+    std::vector<void *> p;
+    ~Dev() {
+        for (void *q : p) (void)hipFree(q);
+    }
+    template <typename T>
+    T *get(size_t n) {
+        void *q = nullptr;
+        if (hipMalloc(&q, sizeof(T) * (n ? n : 1)) != hipSuccess) return nullptr;
+        p.push_back(q);
+        return (T *)q;
+    }
+    template <typename T>
+    T *up(const T *h, size_t n) {
+        T *d = get<T>(n);
+        if (d && h && n && hipMemcpy(d, h, sizeof(T) * n, hipMemcpyHostToDevice) != hipSuccess)
+            return nullptr;
+        return d;
+    }
+};
+
+#define OPCHK(expr)                                                                          \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return op_fail(IBLBM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+#define NEED(ptr)                                                                  \
+    do {                                                                           \
+        if (!(ptr)) return op_fail(IBLBM_ENOMEM, "device allocation failed");     \
+    } while (0)
+
+unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+extern "C" int iblbm_set_op_error(int code, const char *m);
+
+namespace {
+int op_fail(int code, const std::string &m) { return iblbm_set_op_error(code, m.c_str()); }
+}  // namespace
+
+// ------------------------------------------------------------------ lattice operators
+
+__global__ void k_op_feq(const double *ro, const double *U, int64_t N, int inc, double *out) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    feq9(ro[n], U[2 * n], U[2 * n + 1], inc != 0, out + 9 * n);
+}
+
+__global__ void k_op_macro(const double *f, int64_t N, int inc, double *ro, double *U) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double r, u0, u1;
+    macro9(f + 9 * n, inc != 0, r, u0, u1);
+    ro[n] = r;
+    U[2 * n] = u0;
+    U[2 * n + 1] = u1;
+}
+
+// push streaming onto a copy of the previous fstar (streaming.py:38-77), written as a pull
+__global__ void k_op_stream(const double *old, const double *f, int nx, int ny, double *out) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (int64_t)nx * ny) return;
+    const int i = (int)(n % nx), j = (int)(n / nx);
+    for (int q = 0; q < 9; ++q) {
+        const int si = i - EXi[q], sj = j - EYi[q];
+        out[9 * n + q] = (si >= 0 && si < nx && sj >= 0 && sj < ny)
+                             ? f[9 * ((int64_t)sj * nx + si) + q]
+                             : old[9 * n + q];
+    }
+}
+
+__global__ void k_op_collide(CollParams cp, const double *fs, const double *fe, const double *U,
+                             const double *fib, int64_t N, double *out) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    collide<true>(cp, fs + 9 * n, fe + 9 * n, U[2 * n], U[2 * n + 1], fib[2 * n], fib[2 * n + 1],
+                  out + 9 * n);
+}
+
+// boundary closures in place on a post-stream field (one thread per boundary node)
+__global__ void k_op_boundary(int bc, double *fs, int nx, int ny, double in_u, double dens,
+                              int inc) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nb = 2LL * nx + 2LL * (ny - 2);
+    if (k >= nb) return;
+    int i, j;
+    if (k < nx) {
+        i = (int)k;
+        j = 0;
+    } else if (k < 2LL * nx) {
+        i = (int)(k - nx);
+        j = ny - 1;
+    } else {
+        const int64_t r = k - 2LL * nx;
+        i = (r < ny - 2) ? 0 : nx - 1;
+        j = 1 + (int)(r % (ny - 2));
+    }
+    double *f = fs + 9 * ((int64_t)j * nx + i);
+    if (bc == IBLBM_BC_DIRICHLET_NEUMANN) {
+        if (i == nx - 1) {
+            const int jj = j == 0 ? 1 : (j == ny - 1 ? ny - 2 : j);
+            neumann_copy(fs + 9 * ((int64_t)jj * nx + nx - 2), f);
+        } else {
+            feq9(dens, in_u, 0.0, false, f);
+        }
+        return;
+    }
+    const bool corner = (i == 0 || i == nx - 1) && (j == 0 || j == ny - 1);
+    if (corner) {
+        feq9(dens, 0.0, 0.0, inc != 0, f);
+        return;
+    }
+    settling_face(i == 0 ? 0 : (i == nx - 1 ? 1 : (j == ny - 1 ? 2 : 3)), f);
+}
+
+// ------------------------------------------------------------------ IBM operators
+
+// mode 0: DF fields, 1: MDF, 2: DFC lambda, 3: DFC correction
+__global__ __launch_bounds__(NT) void k_op_ibm(Inst *Ip, int mode, const double *Eux,
+                                               const double *Euy, const double *Ro,
+                                               double *fstar) {
+    __shared__ double red[NT];
+    __shared__ int redi[NT];
+    Inst &I = *Ip;
+    IbmScratch &S = I.s;
+    const int t = threadIdx.x;
+    const int nbox = I.bw * I.bh;
+    ibm_build_tables(I, redi);
+    if (mode >= 2) {
+        if (mode != 4) ibm_dfc_lambda(I);  // mode 4 keeps the caller's lambda
+        if (mode == 2) return;
+        // DFC correction on the given fstar
+        for (int sl = t; sl < nbox; sl += NT) {
+            const int64_t n = (int64_t)(I.by0 + sl / I.bw) * I.nx + I.bx0 + sl % I.bw;
+            for (int q = 0; q < 9; ++q) S.fs[q * I.box_cap + sl] = fstar[9 * n + q];
+        }
+        __syncthreads();
+        const int a = I.delta == 1 ? 1 : 2, K = (2 * a + 1) * (2 * a + 1), Lb = I.Lb;
+        const double cs2 = 1.0 / 3.0, csc = I.tol;  // correction_scale passed in tol
+        for (int k = t; k < Lb; k += NT) {
+            double fi[9];
+            for (int q = 0; q < 9; ++q) fi[q] = 0.0;
+            for (int s = 0; s < K; ++s) {
+                const int c = s * Lb + k;
+                const int sl = S.cslot[c];
+                const double w = S.cw[c];
+                for (int q = 0; q < 9; ++q) fi[q] += S.fs[q * I.box_cap + sl] * w;
+            }
+            const double udx = I.ud[2 * k], udy = I.ud[2 * k + 1], lam = S.lam[k];
+            double qx = 0.0, qy = 0.0;
+            for (int q = 0; q < 9; ++q) {
+                const double edu = udx * EX(q) + udy * EY(q);
+                const double fbb = fi[OPP[q]] + 2.0 * W(q) * 1.0 * edu / cs2;
+                const double dev = fbb - fi[q];
+                S.dfl[9 * k + q] = csc * lam * dev;
+                const double sd = -(lam * dev * I.Larea);
+                qx += sd * EX(q);
+                qy += sd * EY(q);
+            }
+            S.q[2 * k] = csc * qx;
+            S.q[2 * k + 1] = csc * qy;
+        }
+        __syncthreads();
+        for (int sl = t; sl < nbox; sl += NT) {
+            if (S.cnt[sl] == 0) continue;
+            const int64_t n = (int64_t)(I.by0 + sl / I.bw) * I.nx + I.bx0 + sl % I.bw;
+            double e[9];
+            for (int q = 0; q < 9; ++q) e[q] = 0.0;
+            const int *L = S.list + S.off[sl];
+            for (int x = 0, m = S.cnt[sl]; x < m; ++x) {
+                const int c = L[x];
+                const double w = S.cw[c] * I.Larea;
+                for (int q = 0; q < 9; ++q) e[q] += S.dfl[9 * (c % Lb) + q] * w;
+            }
+            for (int q = 0; q < 9; ++q) fstar[9 * n + q] = fstar[9 * n + q] + e[q];
+        }
+        return;
+    }
+    // load the Eulerian fields of the box
+    for (int sl = t; sl < nbox; sl += NT) {
+        const int64_t n = (int64_t)(I.by0 + sl / I.bw) * I.nx + I.bx0 + sl % I.bw;
+        S.ro[sl] = Ro[n];
+        S.ux0[sl] = Eux[n];
+        S.uy0[sl] = Euy[n];
+        S.fibx[sl] = 0.0;
+        S.fiby[sl] = 0.0;
+    }
+    __syncthreads();
+    if (mode == 0) {
+        ibm_interp_force(I, S.ux0, S.uy0, I.inc != 0, red);
+        for (int sl = t; sl < nbox; sl += NT) {
+            if (S.cnt[sl] == 0) continue;
+            ibm_gather_force(I, sl, S.fibx[sl], S.fiby[sl]);
+        }
+        return;
+    }
+    for (int sl = t; sl < nbox; sl += NT) {
+        S.uwx[sl] = S.ux0[sl];
+        S.uwy[sl] = S.uy0[sl];
+        S.vs[sl] = I.cp.dt / (2.0 * S.ro[sl]);
+    }
+    __syncthreads();
+    const int min_iter = max(1, min(I.min_iter, I.n_iter));
+    double prev = INFINITY, last = INFINITY;
+    int iter_count = 0, diverged = 0, converged = 0;
+    for (int it = 0; it < I.n_iter; ++it) {
+        const double res = ibm_interp_force(I, S.uwx, S.uwy, false, red);
+        last = res;
+        if (it > 0 && res > prev * 1.5) {
+            diverged = 1;
+            iter_count = it + 1;
+            break;
+        }
+        for (int sl = t; sl < nbox; sl += NT) {
+            if (S.cnt[sl] == 0) continue;
+            double ex, ey;
+            ibm_gather_force(I, sl, ex, ey);
+            const double fx = ex * I.omega, fy = ey * I.omega;
+            S.fibx[sl] = S.fibx[sl] + fx;
+            S.fiby[sl] = S.fiby[sl] + fy;
+            S.uwx[sl] = S.uwx[sl] + fx * S.vs[sl];
+            S.uwy[sl] = S.uwy[sl] + fy * S.vs[sl];
+        }
+        __syncthreads();
+        prev = res;
+        iter_count = it + 1;
+        if (it + 1 >= min_iter && res < I.tol) {
+            converged = 1;
+            break;
+        }
+    }
+    if (t == 0) {
+        I.stats[0] = iter_count;
+        I.stats[1] = last;
+        I.stats[2] = diverged;
+        I.stats[3] = converged;
+    }
+}
+
+namespace {
+
+// Scratch for one operator call on a (ny, nx) grid with Lb markers.
+struct OpIbm {
+    Dev mem;
+    Inst h;
+    Inst *d = nullptr;
+    int setup(const double *Lx, const double *Ly, const double *ud, int Lb, double dx, double dy,
+              double dt, double Larea, int ny, int nx, int delta, int inc) {
+        if (Lb < 1 || nx < 1 || ny < 1) return op_fail(IBLBM_EINVAL, "ibm op: empty input");
+        if (delta != 0 && delta != 1) return op_fail(IBLBM_EKEY, "ibm op: unknown delta type");
+        memset(&h, 0, sizeof h);
+        int box[4];
+        int rc = iblbm_marker_box(Lx, Ly, Lb, dx, dy, nx, ny, delta, box);
+        if (rc) return rc;
+        h.nx = nx;
+        h.ny = ny;
+        h.inc = inc;
+        h.cp.dt = dt;
+        h.delta = delta;
+        h.Lb = Lb;
+        h.lb_cap = Lb;
+        h.bx0 = box[0];
+        h.by0 = box[1];
+        h.bw = box[2];
+        h.bh = box[3];
+        const int B = box[2] * box[3], K = 25;
+        h.box_cap = B;
+        h.dx = dx;
+        h.dy = dy;
+        h.Larea = Larea;
+        IbmScratch &s = h.s;
+        h.Lx = mem.up(Lx, Lb);
+        h.Ly = mem.up(Ly, Lb);
+        h.ud = ud ? mem.up(ud, 2 * (size_t)Lb) : mem.get<double>(2 * (size_t)Lb);
+        s.fs = mem.get<double>(9 * (size_t)B);
+        s.ro = mem.get<double>(B);
+        s.ux0 = mem.get<double>(B);
+        s.uy0 = mem.get<double>(B);
+        s.uwx = mem.get<double>(B);
+        s.uwy = mem.get<double>(B);
+        s.vs = mem.get<double>(B);
+        s.fibx = mem.get<double>(B);
+        s.fiby = mem.get<double>(B);
+        s.wt = mem.get<double>(B);
+        s.cnt = mem.get<int>(B);
+        s.off = mem.get<int>(B + 1);
+        s.cur = mem.get<int>(B);
+        s.cw = mem.get<double>((size_t)K * Lb);
+        s.cslot = mem.get<int>((size_t)K * Lb);
+        s.list = mem.get<int>((size_t)K * Lb);
+        s.Lux = mem.get<double>(Lb);
+        s.Luy = mem.get<double>(Lb);
+        s.R = mem.get<double>(Lb);
+        s.Lfx = mem.get<double>(Lb);
+        s.Lfy = mem.get<double>(Lb);
+        s.lam = mem.get<double>(Lb);
+        s.q = mem.get<double>(2 * (size_t)Lb);
+        s.fi = mem.get<double>(9 * (size_t)Lb);
+        s.dfl = mem.get<double>(9 * (size_t)Lb);
+        h.stats = mem.get<double>(4);
+        h.diag = mem.get<double>(8);
+        if (!h.Lx || !h.Ly || !h.ud || !s.fs || !s.ro || !s.ux0 || !s.uy0 || !s.uwx || !s.uwy ||
+            !s.vs || !s.fibx || !s.fiby || !s.wt || !s.cnt || !s.off || !s.cur || !s.cw ||
+            !s.cslot || !s.list || !s.Lux || !s.Luy || !s.R || !s.Lfx || !s.Lfy || !s.lam ||
+            !s.q || !s.fi || !s.dfl || !h.stats || !h.diag)
+            return op_fail(IBLBM_ENOMEM, "ibm op: scratch allocation failed");
+        return 0;
+    }
+    int push() {
+        d = mem.get<Inst>(1);
+        if (!d) return op_fail(IBLBM_ENOMEM, "ibm op: allocation failed");
+        OPCHK(hipMemcpy(d, &h, sizeof h, hipMemcpyHostToDevice));
+        return 0;
+    }
+    // scatter box fib into a zeroed (N, 2) host array
+    int fib_out(double *fib) {
+        const int64_t N = (int64_t)h.nx * h.ny;
+        const int B = h.bw * h.bh;
+        std::vector<double> fx(B), fy(B);
+        OPCHK(hipMemcpy(fx.data(), h.s.fibx, sizeof(double) * B, hipMemcpyDeviceToHost));
+        OPCHK(hipMemcpy(fy.data(), h.s.fiby, sizeof(double) * B, hipMemcpyDeviceToHost));
+        memset(fib, 0, sizeof(double) * 2 * N);
+        for (int sl = 0; sl < B; ++sl) {
+            const int64_t n = (int64_t)(h.by0 + sl / h.bw) * h.nx + h.bx0 + sl % h.bw;
+            fib[2 * n] = fx[sl];
+            fib[2 * n + 1] = fy[sl];
+        }
+        return 0;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int iblbm_op_feq(const double *ro, const double *U, int64_t N, int inc, double *feq) {
+    if (!ro || !U || !feq || N < 0) return op_fail(IBLBM_EINVAL, "op_feq: bad argument");
+    if (N == 0) return 0;
+    Dev m;
+    const double *dr = m.up(ro, N), *du = m.up(U, 2 * N);
+    double *dout = m.get<double>(9 * N);
+    NEED(dr && du && dout);
+    hipLaunchKernelGGL(k_op_feq, dim3(nblk(N, 256)), dim3(256), 0, 0, dr, du, N, inc, dout);
+    OPCHK(hipGetLastError());
+    OPCHK(hipMemcpy(feq, dout, sizeof(double) * 9 * N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int iblbm_op_macroscopic(const double *f, int64_t N, int inc, double *ro, double *U) {
+    if (!f || !ro || !U || N < 0) return op_fail(IBLBM_EINVAL, "op_macroscopic: bad argument");
+    if (N == 0) return 0;
+    Dev m;
+    const double *df = m.up(f, 9 * N);
+    double *dr = m.get<double>(N), *du = m.get<double>(2 * N);
+    NEED(df && dr && du);
+    hipLaunchKernelGGL(k_op_macro, dim3(nblk(N, 256)), dim3(256), 0, 0, df, N, inc, dr, du);
+    OPCHK(hipGetLastError());
+    OPCHK(hipMemcpy(ro, dr, sizeof(double) * N, hipMemcpyDeviceToHost));
+    OPCHK(hipMemcpy(U, du, sizeof(double) * 2 * N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int iblbm_op_streaming(const double *old, const double *f, int nx, int ny, double *out) {
+    if (!old || !f || !out || nx < 1 || ny < 1)
+        return op_fail(IBLBM_EINVAL, "op_streaming: bad argument");
+    const int64_t N = (int64_t)nx * ny;
+    Dev m;
+    const double *dold = m.up(old, 9 * N), *df = m.up(f, 9 * N);
+    double *dout = m.get<double>(9 * N);
+    NEED(dold && df && dout);
+    hipLaunchKernelGGL(k_op_stream, dim3(nblk(N, 256)), dim3(256), 0, 0, dold, df, nx, ny, dout);
+    OPCHK(hipGetLastError());
+    OPCHK(hipMemcpy(out, dout, sizeof(double) * 9 * N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int iblbm_op_collision(int model, const double *fstar, const double *feq, const double *U,
+                       const double *fib, int64_t N, double tau, double tau_minus,
+                       const double *s_diag, double dt, double *out) {
+    if (model < 0 || model > 2) return op_fail(IBLBM_EKEY, "op_collision: unknown model");
+    if (!fstar || !U || !fib || !out || N < 0 || (model != 2 && !feq) || (model == 2 && !s_diag))
+        return op_fail(IBLBM_EINVAL, "op_collision: bad argument");
+    if (N == 0) return 0;
+    CollParams cp;
+    memset(&cp, 0, sizeof cp);
+    cp.model = model;
+    cp.dt = dt;
+    cp.inv_tau = 1.0 / tau;
+    cp.guo_pref = 1.0 - 0.5 / tau;
+    cp.s_plus = 1.0 / tau;
+    cp.s_minus = 1.0 / tau_minus;
+    cp.gpp = 1.0 - 0.5 * cp.s_plus;
+    cp.gpm = 1.0 - 0.5 * cp.s_minus;
+    if (model == 2)
+        for (int q = 0; q < 9; ++q) cp.S[q] = s_diag[q];
+    Dev m;
+    const double *dfs = m.up(fstar, 9 * N), *dfe = model != 2 ? m.up(feq, 9 * N) : dfs;
+    const double *du = m.up(U, 2 * N), *dfib = m.up(fib, 2 * N);
+    double *dout = m.get<double>(9 * N);
+    NEED(dfs && dfe && du && dfib && dout);
+    hipLaunchKernelGGL(k_op_collide, dim3(nblk(N, 256)), dim3(256), 0, 0, cp, dfs, dfe, du, dfib,
+                       N, dout);
+    OPCHK(hipGetLastError());
+    OPCHK(hipMemcpy(out, dout, sizeof(double) * 9 * N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int iblbm_op_boundary(int bc, double *fstar, int nx, int ny, double in_u, double dens, int inc) {
+    if (bc != 0 && bc != 1) return op_fail(IBLBM_EKEY, "op_boundary: unknown bc_type");
+    if (!fstar || nx < 3 || ny < 3) return op_fail(IBLBM_EINVAL, "op_boundary: bad argument");
+    const int64_t N = (int64_t)nx * ny, nb = 2LL * nx + 2LL * (ny - 2);
+    Dev m;
+    double *d = m.up(fstar, 9 * N);
+    NEED(d);
+    hipLaunchKernelGGL(k_op_boundary, dim3(nblk(nb, 256)), dim3(256), 0, 0, bc, d, nx, ny, in_u,
+                       dens, inc);
+    OPCHK(hipGetLastError());
+    OPCHK(hipMemcpy(fstar, d, sizeof(double) * 9 * N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int iblbm_op_direct_forcing(const double *Lx, const double *Ly, const double *ud, int Lb,
+                            const double *Eux, const double *Euy, const double *Ro, double dx,
+                            double dy, double dt, double Larea, int ny, int nx, int delta,
+                            int inc, double *fib, double *Lux, double *Luy) {
+    if (!Lx || !Ly || !ud || !Eux || !Euy || !Ro || !fib)
+        return op_fail(IBLBM_EINVAL, "op_direct_forcing: null argument");
+    OpIbm op;
+    int rc = op.setup(Lx, Ly, ud, Lb, dx, dy, dt, Larea, ny, nx, delta, inc);
+    if (rc) return rc;
+    rc = op.push();
+    if (rc) return rc;
+    const int64_t N = (int64_t)nx * ny;
+    const double *dux = op.mem.up(Eux, N), *duy = op.mem.up(Euy, N), *dro = op.mem.up(Ro, N);
+    NEED(dux && duy && dro);
+    hipLaunchKernelGGL(k_op_ibm, dim3(1), dim3(NT), 0, 0, op.d, 0, dux, duy, dro, (double *)nullptr);
+    OPCHK(hipGetLastError());
+    OPCHK(hipDeviceSynchronize());
+    rc = op.fib_out(fib);
+    if (rc) return rc;
+    if (Lux) OPCHK(hipMemcpy(Lux, op.h.s.Lux, sizeof(double) * Lb, hipMemcpyDeviceToHost));
+    if (Luy) OPCHK(hipMemcpy(Luy, op.h.s.Luy, sizeof(double) * Lb, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int iblbm_op_multi_direct_forcing(const double *Lx, const double *Ly, const double *ud, int Lb,
+                                  const double *U, const double *ro, double dx, double dy,
+                                  double dt, double Larea, int ny, int nx, int n_iter,
+                                  int min_iter, int delta, double omega, double tol,
+                                  double *fib_total, iblbm_mdf_stats *stats) {
+    if (!Lx || !Ly || !ud || !U || !ro || !fib_total)
+        return op_fail(IBLBM_EINVAL, "op_multi_direct_forcing: null argument");
+    if (n_iter < 1) return op_fail(IBLBM_EINVAL, "op_multi_direct_forcing: n_iter must be >= 1");
+    OpIbm op;
+    int rc = op.setup(Lx, Ly, ud, Lb, dx, dy, dt, Larea, ny, nx, delta, 0);
+    if (rc) return rc;
+    op.h.n_iter = n_iter;
+    op.h.min_iter = min_iter;
+    op.h.omega = omega;
+    op.h.tol = tol;
+    rc = op.push();
+    if (rc) return rc;
+    const int64_t N = (int64_t)nx * ny;
+    std::vector<double> ux(N), uy(N);
+    for (int64_t n = 0; n < N; ++n) {
+        ux[n] = U[2 * n];
+        uy[n] = U[2 * n + 1];
+    }
+    const double *dux = op.mem.up(ux.data(), N), *duy = op.mem.up(uy.data(), N);
+    const double *dro = op.mem.up(ro, N);
+    NEED(dux && duy && dro);
+    hipLaunchKernelGGL(k_op_ibm, dim3(1), dim3(NT), 0, 0, op.d, 1, dux, duy, dro, (double *)nullptr);
+    OPCHK(hipGetLastError());
+    OPCHK(hipDeviceSynchronize());
+    rc = op.fib_out(fib_total);
+    if (rc) return rc;
+    if (stats) {
+        double s[4];
+        OPCHK(hipMemcpy(s, op.h.stats, sizeof s, hipMemcpyDeviceToHost));
+        stats->iter_count = (int32_t)s[0];
+        stats->last_residual = s[1];
+        stats->diverged = (int32_t)s[2];
+        stats->converged = (int32_t)s[3];
+        stats->reserved = 0;
+    }
+    return 0;
+}
+
+int iblbm_op_dfc_lambda(const double *Lx, const double *Ly, int Lb, double rho_f, double Larea,
+                        double dx, double dy, int ny, int nx, int delta, double *lambda_k) {
+    if (!Lx || !Ly || !lambda_k) return op_fail(IBLBM_EINVAL, "op_dfc_lambda: null argument");
+    if (rho_f != 1.0) return op_fail(IBLBM_EINVAL, "op_dfc_lambda: only rho_f = 1 (dfc.py:297)");
+    OpIbm op;
+    int rc = op.setup(Lx, Ly, nullptr, Lb, dx, dy, 1.0, Larea, ny, nx, delta, 0);
+    if (rc) return rc;
+    rc = op.push();
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_op_ibm, dim3(1), dim3(NT), 0, 0, op.d, 2, (const double *)nullptr,
+                       (const double *)nullptr, (const double *)nullptr, (double *)nullptr);
+    OPCHK(hipGetLastError());
+    OPCHK(hipMemcpy(lambda_k, op.h.s.lam, sizeof(double) * Lb, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int iblbm_op_dfc_correction(const double *Lx, const double *Ly, const double *ud, int Lb,
+                            double *fstar, const double *lambda_k, double dx, double dy,
+                            double Larea, int ny, int nx, int delta, double scale,
+                            double *force) {
+    if (!Lx || !Ly || !ud || !fstar || !force)
+        return op_fail(IBLBM_EINVAL, "op_dfc_correction: null argument");
+    OpIbm op;
+    int rc = op.setup(Lx, Ly, ud, Lb, dx, dy, 1.0, Larea, ny, nx, delta, 0);
+    if (rc) return rc;
+    op.h.tol = scale;
+    rc = op.push();
+    if (rc) return rc;
+    const int64_t N = (int64_t)nx * ny;
+    double *dfs = op.mem.up(fstar, 9 * N);
+    NEED(dfs);
+    // lambda: the caller's (the reference's lambda_cache) or recomputed (compute_lambda)
+    if (lambda_k)
+        OPCHK(hipMemcpy(op.h.s.lam, lambda_k, sizeof(double) * Lb, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_op_ibm, dim3(1), dim3(NT), 0, 0, op.d, lambda_k ? 4 : 3,
+                       (const double *)nullptr, (const double *)nullptr, (const double *)nullptr,
+                       dfs);
+    OPCHK(hipGetLastError());
+    OPCHK(hipDeviceSynchronize());
+    OPCHK(hipMemcpy(fstar, dfs, sizeof(double) * 9 * N, hipMemcpyDeviceToHost));
+    OPCHK(hipMemcpy(force, op.h.s.q, sizeof(double) * 2 * Lb, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// iblbm/ibm/a_norm.py:75-172 in lattice units, as called by :204-266
+double iblbm_omega_estimate(const double *Lx, const double *Ly, int Lb, int delta, double Larea,
+                            double dx) {
+    if (!Lx || !Ly || Lb < 1 || !(dx > 0.0) || (delta != 0 && delta != 1)) return NAN;
+    std::vector<double> x(Lb), y(Lb);
+    for (int k = 0; k < Lb; ++k) {
+        x[k] = Lx[k] / dx;
+        y[k] = Ly[k] / dx;
+    }
+    const int a = delta == 1 ? 1 : 2;
+    const double cutoff = 2.0 * a * 1.0, factor = Larea / (1.0 * 1.0);
+    double best = -INFINITY;
+    for (int l = 0; l < Lb; ++l) {
+        double acc = 0.0;
+        for (int m = 0; m < Lb; ++m) {
+            if (std::fabs(x[l] - x[m]) >= cutoff || std::fabs(y[l] - y[m]) >= cutoff) continue;
+            double sums[2];
+            bool skip = false;
+            for (int ax = 0; ax < 2 && !skip; ++ax) {
+                const double pl = ax ? y[l] : x[l], pm = ax ? y[m] : x[m];
+                const double lo = std::max(pl, pm) - a * 1.0, hi = std::min(pl, pm) + a * 1.0;
+                const int64_t ilo = (int64_t)std::ceil(lo / 1.0);
+                const int64_t ihi = (int64_t)std::floor(hi / 1.0) + 1;
+                if (ilo >= ihi) {
+                    skip = true;
+                    break;
+                }
+                double s = 0.0;
+                for (int64_t i = ilo; i < ihi; ++i) {
+                    const double xi = (double)i * 1.0;
+                    s += delta_phi(delta, (xi - pl) / 1.0) * delta_phi(delta, (xi - pm) / 1.0);
+                }
+                if (s == 0.0) {
+                    skip = true;
+                    break;
+                }
+                sums[ax] = s;
+            }
+            if (skip) continue;
+            acc += factor * sums[0] * sums[1];
+        }
+        best = std::max(best, acc);
+    }
+    return 1.0 / best;
+}
+
+}  // extern "C"

@@ -1,0 +1,260 @@
+"""Time loop of the reference (``iblbm/runtime/loop.py:44-151``) on the fused gfx950 step.
+
+``run(cfg)`` keeps the reference's call surface and result dictionary.  The device owns the
+populations; one launch per step runs streaming, the boundary closure, the moments, the
+immersed-boundary stage and the collision; the host only reads FP64 reductions at the
+diagnostic interval.
+
+Step order (``iblbm/runtime/loop.py:59-87``): the reference collides once before the loop and
+then runs stream -> boundary -> macroscopic -> IBM -> (velocity correction) -> feq -> collide
+for DF/MDF, and collide -> stream -> boundary -> macroscopic -> DFC -> macroscopic -> feq for
+DFC.  Both are the same sequence once the first collision is hoisted out of the loop, which
+is what the device runs: the buffer between launches always holds post-collision f.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+
+import numpy as np
+
+from . import _abi, gpu_kernels
+from ._abi import check, dptr, f64
+from .init import initialize
+
+
+class Context:
+    """One HIP stream and the instances stepped together on it (``iblbm_ctx_*``)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _abi.load()
+        h = ctypes.c_void_p()
+        check(self.lib.iblbm_ctx_create(int(device), ctypes.byref(h)), "ctx_create")
+        self.h = h
+        self.n = []  # node counts per instance
+
+    def close(self):
+        if self.h:
+            self.lib.iblbm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, case: _abi.Case) -> int:
+        i = ctypes.c_int()
+        check(self.lib.iblbm_instance_add(self.h, ctypes.byref(case), ctypes.byref(i)),
+              "instance_add")
+        self.n.append((case.nx, case.ny))
+        return i.value
+
+    def set_markers(self, k, Lx, Ly, ud, Larea):
+        Lx, Ly, ud = f64(Lx), f64(Ly), f64(ud).reshape(-1, 2)
+        check(self.lib.iblbm_set_markers(self.h, k, dptr(Lx), dptr(Ly), dptr(ud), Lx.shape[0],
+                                         float(Larea)), "set_markers")
+
+    def set_torque_center(self, k, cx, cy):
+        check(self.lib.iblbm_set_torque_center(self.h, k, float(cx), float(cy)), "torque_center")
+
+    def upload_f(self, k, f):
+        f = f64(f).reshape(-1, 9)
+        check(self.lib.iblbm_upload_f(self.h, k, dptr(f)), "upload_f")
+
+    def download_f(self, k):
+        nx, ny = self.n[k]
+        out = np.empty((nx * ny, 9))
+        check(self.lib.iblbm_download_f(self.h, k, dptr(out)), "download_f")
+        return out
+
+    def download_state(self, k):
+        nx, ny = self.n[k]
+        N = nx * ny
+        fs, ro, U = np.empty((N, 9)), np.empty(N), np.empty((N, 2))
+        check(self.lib.iblbm_download_state(self.h, k, dptr(fs), dptr(ro), dptr(U)),
+              "download_state")
+        return fs, ro, U
+
+    def download_fib(self, k):
+        nx, ny = self.n[k]
+        out = np.empty((nx * ny, 2))
+        check(self.lib.iblbm_download_fib(self.h, k, dptr(out)), "download_fib")
+        return out
+
+    def download_dfc_force(self, k, Lb):
+        out = np.empty((Lb, 2))
+        check(self.lib.iblbm_download_dfc_force(self.h, k, dptr(out)), "download_dfc_force")
+        return out
+
+    def step(self, n, diag_every=0, graph=False):
+        fn = self.lib.iblbm_graph_run if graph else self.lib.iblbm_step
+        check(fn(self.h, int(n), int(diag_every)), "step")
+
+    def sync(self):
+        check(self.lib.iblbm_sync(self.h), "sync")
+
+    def mdf_stats(self, k):
+        st = _abi.MdfStats()
+        check(self.lib.iblbm_get_mdf_stats(self.h, k, ctypes.byref(st)), "mdf_stats")
+        return {"iter_count": st.iter_count, "last_residual": st.last_residual,
+                "diverged": bool(st.diverged), "converged": bool(st.converged)}
+
+    def diag(self, k):
+        d = _abi.Diag()
+        check(self.lib.iblbm_get_diag(self.h, k, ctypes.byref(d)), "get_diag")
+        return d
+
+    def last_timing(self):
+        ms, n = ctypes.c_double(), ctypes.c_int()
+        check(self.lib.iblbm_last_timing(self.h, ctypes.byref(ms), ctypes.byref(n)), "timing")
+        return ms.value, n.value
+
+
+def make_case(cfg, st, precision=_abi.FP64) -> _abi.Case:
+    """Translate a ``SimConfig`` + initial state into the library's instance description."""
+    if cfg.bc_type not in _abi.BC:
+        raise KeyError(f"bc_type={cfg.bc_type!r} is outside the GPU hot path "
+                       f"(supported: {sorted(_abi.BC)})")
+    if cfg.ibm_method not in ("DF", "MDF", "DFC"):
+        raise KeyError(f"unknown ibm_method={cfg.ibm_method!r} (registered: ['DF', 'DFC', 'MDF'])")
+    if cfg.delta_type not in _abi.DELTA:
+        raise KeyError(cfg.delta_type)
+    c = _abi.Case()
+    c.nx, c.ny = st.nx, st.ny
+    c.collision = _abi.COLL[cfg.collision_model]
+    c.bc = _abi.BC[cfg.bc_type]
+    c.ibm = _abi.IBM[cfg.ibm_method]
+    c.delta = _abi.DELTA[cfg.delta_type]
+    c.incompressible = int(bool(cfg.incompressible_lbgk))
+    c.precision = precision
+    c.tau = st.tau
+    c.tau_minus = st.tau_minus if st.tau_minus is not None else 1.0
+    S = st.S_diag if st.S_diag is not None else np.zeros(9)
+    for q in range(9):
+        c.s_diag[q] = float(S[q])
+    c.dt, c.dx, c.dy = st.dt, st.dx, st.dy
+    c.in_u, c.dens = st.in_u, st.dens
+    c.mdf_iterations, c.mdf_min_iterations = cfg.mdf_iterations, cfg.mdf_min_iterations
+    c.mdf_tolerance = cfg.mdf_tolerance
+    c.mdf_omega = 1.0
+    a = _abi.DELTA_RADIUS[cfg.delta_type]
+    span = int(math.ceil(2.0 * st.lattice_r)) + 2 * a + 4
+    c.lb_capacity = max(st.Lb, 1)
+    c.box_capacity = span * span
+    c.fixed_body = 1 if cfg.motion_type is None else 0
+    return c
+
+
+def pre_collide(cfg, st):
+    """The collision before the loop (``iblbm/runtime/loop.py:59-61``): f from fstar = feq."""
+    fib = np.zeros((st.nodenums, 2))
+    if cfg.collision_model == "BGK":
+        return gpu_kernels.collision_gpu(st.feq0, st.feq0, st.U0, fib, st.tau, st.dt)
+    if cfg.collision_model == "TRT":
+        return gpu_kernels.collision_trt_gpu(st.feq0, st.feq0, st.U0, fib, st.tau, st.tau_minus,
+                                             st.dt)
+    return gpu_kernels.collision_cm_mrt_gpu(st.feq0, st.feq0, st.U0, fib, st.S_diag, st.dt)
+
+
+class FixedCylinder:
+    """A fixed (or prescribed-marker) cylinder on the device, stepped in chunks."""
+
+    def __init__(self, cfg, perturb=None, device=0, ctx=None):
+        if cfg.motion_type is not None:
+            raise ValueError("FixedCylinder runs motion_type=None; use the sedimentation driver")
+        self.cfg = cfg
+        self.st = initialize(cfg, perturb)
+        self.ctx = ctx or Context(device)
+        case = make_case(cfg, self.st)
+        self.omega = None
+        if cfg.ibm_method == "MDF":
+            self.omega = gpu_kernels.estimate_optimal_omega_from_type(
+                self.st.Lx, self.st.Ly, cfg.delta_type, self.st.Larea, self.st.dx)
+            case.mdf_omega = self.omega
+        self.k = self.ctx.add(case)
+        self.ctx.set_markers(self.k, self.st.Lx, self.st.Ly, self.st.desired_velocity,
+                             self.st.Larea)
+        cx, cy = cfg.cylinder_center
+        self.ctx.set_torque_center(self.k, cx, cy)
+        self.ctx.upload_f(self.k, pre_collide(cfg, self.st))
+        self.ttt = 0
+
+    def advance(self, n, diag_every=0, graph=False):
+        if n > 0:
+            self.ctx.step(n, diag_every, graph)
+            self.ttt += n
+
+    def cd_cl(self):
+        """Cd, Cl of the last diagnostic step (``iblbm/diagnostics/core.py:53-113``)."""
+        d = self.ctx.diag(self.k)
+        ro_mean = d.rho_sum / self.st.nodenums
+        denom = ro_mean * self.st.in_u ** 2 * self.st.lattice_r
+        return float(abs(d.force[0]) / denom), float(d.force[1] / denom)
+
+
+def run(cfg, verbose: bool = True, callback=None, save_dir=None, perturb=None,
+        device: int = 0, graph: bool = True) -> dict:
+    """``iblbm.run`` for the fixed-cylinder scenario on the fused GPU step.
+
+    Returns the reference's result keys: Cd_history, Cl_history, converged, final_step,
+    termination_reason, plus ``mdf_stats`` (last step of each diagnostic interval) and
+    ``sim`` (the device-resident simulation, whose state can be downloaded).
+    """
+    if cfg.motion_type is not None:
+        raise ValueError(f"motion_type={cfg.motion_type!r}: sedimentation runs through "
+                         "iblbm_amd.sedimentation.run")
+    if cfg.stream_first:
+        raise ValueError("stream_first=True is outside the GPU hot path")
+    sim = FixedCylinder(cfg, perturb=perturb, device=device)
+    ci = int(cfg.check_interval)
+    Cd_hist, Cl_hist, mdf_hist = [], [], []
+    converged = False
+    termination = "max_steps"
+    pre_U = None
+    error = 1.0
+    t0 = time.time()
+    while sim.ttt < cfg.max_steps:
+        if converged and (cfg.use_convergence or cfg.cd_convergence_tol is not None):
+            if termination == "max_steps":
+                termination = "converged"
+            break
+        n = min(ci - sim.ttt % ci, cfg.max_steps - sim.ttt)
+        sim.advance(n, diag_every=ci, graph=graph)
+        ttt = sim.ttt
+        if ttt % ci != 0:
+            continue
+        Cd, Cl = sim.cd_cl()
+        Cd_hist.append(Cd)
+        Cl_hist.append(Cl)
+        if cfg.ibm_method == "MDF":
+            mdf_hist.append(sim.ctx.mdf_stats(sim.k))
+        if verbose:
+            print(f"step {ttt:>7d} | Cd={Cd:.6f} Cl={Cl:.6f} | err={error:.2e} | "
+                  f"{time.time() - t0:.1f}s")
+        if callback is not None:
+            callback(step=ttt, Cd=Cd, Cl=Cl, error=error, state=sim, elapsed=time.time() - t0,
+                     converged=converged)
+        if cfg.use_convergence and ttt >= cfg.convergence_start:
+            _, _, U = sim.ctx.download_state(sim.k)
+            if ttt == cfg.convergence_start or pre_U is None:
+                pre_U = U.copy()
+            else:  # iblbm/diagnostics/core.py:116-137
+                numer = np.sum((U[:, 0] - pre_U[:, 0]) ** 2 + (U[:, 1] - pre_U[:, 1]) ** 2)
+                denom = np.sum(pre_U[:, 0] ** 2 + pre_U[:, 1] ** 2)
+                error = 1.0 if denom == 0.0 else float(np.sqrt(numer / denom))
+                pre_U = U.copy()
+                if error < cfg.convergence_threshold:
+                    converged = True
+        if (cfg.cd_convergence_tol is not None and ttt > cfg.cd_convergence_start_step
+                and len(Cd_hist) >= cfg.cd_convergence_window):
+            recent = Cd_hist[-cfg.cd_convergence_window:]
+            if max(recent) - min(recent) < cfg.cd_convergence_tol:
+                converged = True
+                termination = "cd_converged"
+    sim.ctx.sync()
+    return {"Cd_history": np.array(Cd_hist), "Cl_history": np.array(Cl_hist),
+            "converged": converged, "final_step": sim.ttt, "termination_reason": termination,
+            "mdf_stats": mdf_hist, "sim": sim, "state": sim.st}

@@ -1,0 +1,194 @@
+/*
+ * iblbm_hip.h -- C ABI of libiblbm_hip.so, the gfx950 IB-LBM per-step hot path.
+ *
+ * The reference binds its hot path as Python-level operators over CuPy arrays
+ * (iblbm/gpu_kernels.py:760-1223, selected by iblbm/backend.py:14-36).  This library
+ * replaces that surface with two layers behind plain C types (no torch or CuPy types):
+ *
+ *  1. A state-owning, fused step interface (iblbm_ctx_*, iblbm_instance_*, iblbm_step):
+ *     one launch per time step runs pull-streaming, the boundary closure, the macroscopic
+ *     moments, the equilibrium and the collision for every node, and one workgroup per
+ *     instance runs the immersed-boundary stage (DF, MDF or DFC) over the marker window in
+ *     the same launch.  It replaces the step sequence of iblbm/runtime/step.py:21-78 and
+ *     iblbm/runtime/loop.py:59-87.
+ *  2. Operator-level entry points with the argument meaning of the reference's
+ *     gpu_kernels.py wrappers (iblbm_op_*): host arrays in, host arrays out.
+ *
+ * Conventions (identical to the reference): populations are (N, 9) row-major AoS on the
+ * host with node n = j*nx + i; U is (N, 2); Eulerian fields are (ny, nx); marker
+ * coordinates Lx, Ly are in domain units (x = i*dx); the lattice is D2Q9 with
+ * e = [(0,0),(1,0),(0,1),(-1,0),(0,-1),(1,1),(-1,1),(-1,-1),(1,-1)].
+ *
+ * Every function returns 0 on success or a negative IBLBM_E* code; iblbm_last_error()
+ * returns the message of the most recent failure on the calling thread.  A context is not
+ * thread-safe; iblbm_step is asynchronous on the context's stream and the download and
+ * reduction calls synchronise it.
+ */
+#ifndef IBLBM_HIP_H
+#define IBLBM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IBLBM_OK 0
+#define IBLBM_EINVAL (-1)   /* invalid argument: maps to ValueError */
+#define IBLBM_EKEY (-2)     /* unknown model / method / kernel name: maps to KeyError */
+#define IBLBM_EDEVICE (-3)  /* HIP runtime or device failure: maps to RuntimeError */
+#define IBLBM_ENOMEM (-4)   /* device allocation failed: maps to MemoryError */
+
+/* Collision models: iblbm/lbm/collision/{bgk,trt,cm_mrt}.py */
+#define IBLBM_COLL_BGK 0
+#define IBLBM_COLL_TRT 1
+#define IBLBM_COLL_CM_MRT 2
+/* Boundary closures: iblbm/boundary/{dirichlet_neumann,settling_channel}.py */
+#define IBLBM_BC_DIRICHLET_NEUMANN 0
+#define IBLBM_BC_SETTLING_CHANNEL 1
+/* IBM methods: iblbm/ibm/strategies/{df,mdf,dfc}.py */
+#define IBLBM_IBM_NONE 0
+#define IBLBM_IBM_DF 1
+#define IBLBM_IBM_MDF 2
+#define IBLBM_IBM_DFC 3
+/* Delta kernels: iblbm/gpu_kernels.py:952-953 ids (peskin4pt = 0, hat = 1) */
+#define IBLBM_DELTA_PESKIN4PT 0
+#define IBLBM_DELTA_HAT 1
+/* Storage precision of the populations */
+#define IBLBM_FP64 0
+#define IBLBM_FP32_DEV 1 /* FP32 deviations f_i - w_i * rho0, FP64 IBM reductions */
+
+/* One lattice instance (a SimConfig after iblbm/init.py:116-265 has run). */
+typedef struct iblbm_case {
+    int32_t nx, ny;
+    int32_t collision;       /* IBLBM_COLL_* */
+    int32_t bc;              /* IBLBM_BC_* */
+    int32_t ibm;             /* IBLBM_IBM_* */
+    int32_t delta;           /* IBLBM_DELTA_* */
+    int32_t incompressible;  /* SimConfig.incompressible_lbgk */
+    int32_t precision;       /* IBLBM_FP64 | IBLBM_FP32_DEV */
+    double tau;              /* init.py:165-168 */
+    double tau_minus;        /* TRT: 0.5 + Lambda/(tau - 0.5), trt.py:75 */
+    double s_diag[9];        /* CM_MRT relaxation rates, cm_mrt.py:171-188 */
+    double dt, dx, dy;       /* dt = 1 (lattice), dx = dy = 1/(NN-1) (domain units) */
+    double in_u, dens;       /* Dirichlet inflow velocity and reference density */
+    int32_t mdf_iterations, mdf_min_iterations;
+    double mdf_tolerance, mdf_omega; /* omega from iblbm_omega_estimate */
+    int32_t lb_capacity;     /* maximum markers this instance will hold */
+    int32_t box_capacity;    /* maximum window nodes (bw*bh) the marker set will cover */
+    int32_t fixed_body;      /* 1: markers never move (window tables and DFC lambda cached) */
+    int32_t reserved;
+} iblbm_case;
+
+/* Per-step IBM statistics (iblbm/ibm/mdf.py:224-229). */
+typedef struct iblbm_mdf_stats {
+    int32_t iter_count;
+    int32_t diverged;
+    int32_t converged;
+    int32_t reserved;
+    double last_residual;
+} iblbm_mdf_stats;
+
+/* Diagnostic reductions of the most recent step that requested them (FP64). */
+typedef struct iblbm_diag {
+    double rho_sum;     /* sum of rho over all N nodes (mean = rho_sum / N) */
+    double force[2];    /* DF/MDF: sum of fib over the grid; DFC: sum of marker forces q_k */
+    double torque_fib;  /* DF/MDF: sum over the grid of (i - cx)*fib_y - (j - cy)*fib_x */
+    double torque_q;    /* DFC: sum_k (Lx_k - cx)/dx * q_y - (Ly_k - cy)/dy * q_x */
+    int64_t step;       /* step counter at which these were taken */
+} iblbm_diag;
+
+/* --- context and instances (replaces iblbm/runtime/step.py + loop.py:59-87) --- */
+int iblbm_ctx_create(int device, void **ctx);
+int iblbm_ctx_destroy(void *ctx);
+const char *iblbm_last_error(void);
+int iblbm_device_name(int device, char *buf, int buflen);
+
+/* Registers an instance; the device buffers are allocated here.  *id receives its index. */
+int iblbm_instance_add(void *ctx, const iblbm_case *c, int *id);
+
+/* Post-collision populations f (the array the next step streams), AoS (N,9) FP64. */
+int iblbm_upload_f(void *ctx, int id, const double *f_aos);
+int iblbm_download_f(void *ctx, int id, double *f_aos);
+
+/* Markers in domain units, desired velocity (Lb,2) in lattice units and arc length Larea.
+ * The window box (every clamped stencil node, see iblbm_marker_box) must fit box_capacity.
+ * Replaces the per-step H2D of iblbm/runtime/scenarios/sedimentation_single.py:156-160. */
+int iblbm_set_markers(void *ctx, int id, const double *Lx, const double *Ly, const double *ud,
+                      int Lb, double Larea);
+/* Centre (domain units) about which the diagnostic torque sums are taken. */
+int iblbm_set_torque_center(void *ctx, int id, double cx, double cy);
+/* Window box [box[0], box[0]+box[2]) x [box[1], box[1]+box[3]) covering every clamped
+ * stencil node of the markers (stencil radius 2 for peskin4pt, 1 for hat). */
+int iblbm_marker_box(const double *Lx, const double *Ly, int Lb, double dx, double dy, int nx,
+                     int ny, int delta, int box[4]);
+
+/* Runs nsteps steps of every instance (asynchronous).  diag_every > 0 takes the diagnostic
+ * reductions on steps whose counter is a multiple of it. */
+int iblbm_step(void *ctx, int nsteps, int diag_every);
+int iblbm_sync(void *ctx);
+
+/* Captures nsteps steps into a hipGraph and replays it (the step counter advances). */
+int iblbm_graph_run(void *ctx, int nsteps, int diag_every);
+
+int iblbm_get_mdf_stats(void *ctx, int id, iblbm_mdf_stats *out);
+int iblbm_get_diag(void *ctx, int id, iblbm_diag *out);
+/* Window fib of the last step scattered into a zero (N,2) array. */
+int iblbm_download_fib(void *ctx, int id, double *fib_aos);
+/* Post-stream/boundary/IBM populations fstar, rho and U (velocity-corrected for DF/MDF) of
+ * the last step, i.e. state.fstar, state.ro, state.U of the reference after that step. */
+int iblbm_download_state(void *ctx, int id, double *fstar_aos, double *ro, double *U);
+/* DFC marker forces q_k (Lb,2) of the last step (iblbm/ibm/dfc.py:232-253). */
+int iblbm_download_dfc_force(void *ctx, int id, double *q);
+int64_t iblbm_step_counter(void *ctx);
+
+/* Time of the last iblbm_step/iblbm_graph_run measured with HIP events on the context
+ * stream, in milliseconds, and the number of steps it covered. */
+int iblbm_last_timing(void *ctx, double *ms, int *steps);
+
+/* --- host-side helper (iblbm/ibm/a_norm.py:204-266) --- */
+/* omega = 1/||A||_inf for markers in domain units; returns NaN on bad input. */
+double iblbm_omega_estimate(const double *Lx, const double *Ly, int Lb, int delta,
+                            double Larea, double dx);
+
+/* --- operator-level entry points (iblbm/gpu_kernels.py wrappers; host arrays) --- */
+/* feq_gpu (gpu_kernels.py:906): ro (N), U (N,2) -> feq (N,9) */
+int iblbm_op_feq(const double *ro, const double *U, int64_t N, int incompressible, double *feq);
+/* macroscopic_gpu (:878): fstar (N,9) -> ro (N), U (N,2) */
+int iblbm_op_macroscopic(const double *fstar, int64_t N, int incompressible, double *ro,
+                         double *U);
+/* streaming_gpu (:846) with the CPU push semantics of streaming.py:38-77 */
+int iblbm_op_streaming(const double *fstar_old, const double *f, int nx, int ny, double *out);
+/* collision_gpu / collision_trt_gpu / collision_cm_mrt_gpu (:760, :787, :815).
+ * p0 = tau (BGK, TRT) or unused (CM_MRT); p1 = tau_minus (TRT); s_diag for CM_MRT. */
+int iblbm_op_collision(int model, const double *fstar, const double *feq, const double *U,
+                       const double *fib, int64_t N, double tau, double tau_minus,
+                       const double *s_diag, double dt, double *out);
+/* apply_bc_dirichlet_neumann / apply_bc_settling_channel on a post-stream fstar, in place */
+int iblbm_op_boundary(int bc, double *fstar, int nx, int ny, double in_u, double dens,
+                      int incompressible);
+/* ibm_direct_forcing (df.py:37-154): fib (N,2), Lux, Luy (Lb) */
+int iblbm_op_direct_forcing(const double *Lx, const double *Ly, const double *ud, int Lb,
+                            const double *Eux, const double *Euy, const double *Ro, double dx,
+                            double dy, double dt, double Larea, int ny, int nx, int delta,
+                            int incompressible, double *fib, double *Lux, double *Luy);
+/* ibm_multi_direct_forcing (mdf.py:61-229) without rotation coupling */
+int iblbm_op_multi_direct_forcing(const double *Lx, const double *Ly, const double *ud, int Lb,
+                                  const double *U, const double *ro, double dx, double dy,
+                                  double dt, double Larea, int ny, int nx, int n_iter,
+                                  int min_iter, int delta, double omega, double tol,
+                                  double *fib_total, iblbm_mdf_stats *stats);
+/* compute_lambda / compute_lambda_gpu (dfc.py:126-186, gpu_kernels.py:1091) */
+int iblbm_op_dfc_lambda(const double *Lx, const double *Ly, int Lb, double rho_f, double Larea,
+                        double dx, double dy, int ny, int nx, int delta, double *lambda_k);
+/* apply_dfc_correction with a given lambda (dfc.py:256-342): fstar corrected in place */
+int iblbm_op_dfc_correction(const double *Lx, const double *Ly, const double *ud, int Lb,
+                            double *fstar, const double *lambda_k, double dx, double dy,
+                            double Larea, int ny, int nx, int delta, double correction_scale,
+                            double *force);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IBLBM_HIP_H */

@@ -1,0 +1,44 @@
+"""The fused per-step path (one launch per step) against the reference's whole runs and the
+oracle: populations bit-exact (the 1e-12 bar is met with zero error), Cd/Cl within 1e-10
+relative (the device reduces rho and fib in a fixed tree order, not NumPy's)."""
+import numpy as np
+import pytest
+
+from conftest import RUN_CASES, golden, golden_cfg, perturb_of, rel_linf
+from iblbm_amd.solver import FixedCylinder
+
+pytestmark = pytest.mark.gpu
+CD_TOL = 1e-10
+
+
+@pytest.mark.parametrize("graph", [False, True])
+@pytest.mark.parametrize("case", RUN_CASES)
+def test_fused_step_matches_reference_run(case, graph):
+    g, sc = golden(case)
+    cfg = golden_cfg(sc)
+    sim = FixedCylinder(cfg, perturb=perturb_of(sc))
+    ci = cfg.check_interval
+    cd, cl, mdf = [], [], []
+    while sim.ttt < cfg.max_steps:
+        sim.advance(ci, diag_every=ci, graph=graph)
+        c, l = sim.cd_cl()
+        cd.append(c)
+        cl.append(l)
+        if cfg.ibm_method == "MDF":
+            s = sim.ctx.mdf_stats(sim.k)
+            mdf.append([sim.ttt, s["iter_count"], s["last_residual"], s["diverged"],
+                        s["converged"]])
+    f = sim.ctx.download_f(sim.k)
+    np.testing.assert_array_equal(f, g["f_post"])
+    fs, ro, U = sim.ctx.download_state(sim.k)
+    np.testing.assert_array_equal(fs, g["fstar"])
+    np.testing.assert_array_equal(ro, g["ro"])
+    np.testing.assert_array_equal(U, g["U"])
+    np.testing.assert_array_equal(sim.ctx.download_fib(sim.k), g["fib"])
+    assert rel_linf(cd, g["Cd"]) <= CD_TOL
+    assert np.max(np.abs(np.array(cl) - g["Cl"]) / np.abs(g["Cd"])) <= CD_TOL
+    if "mdf_stats" in g:
+        np.testing.assert_array_equal(np.array(mdf, dtype=np.float64), g["mdf_stats"])
+    if "dfc_force" in g:
+        np.testing.assert_array_equal(sim.ctx.download_dfc_force(sim.k, sc["Lb"]),
+                                      g["dfc_force"])
