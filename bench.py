@@ -80,6 +80,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--precision", choices=["fp64", "fp32dev"], default="fp64")
+    ap.add_argument("--serial", action="store_true",
+                    help="run the window and bulk kernels in order on one stream")
+    ap.add_argument("--no-graph", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the lattice fits one MI355X and this build has no "
@@ -91,17 +95,20 @@ def main():
     ci = cfg.check_interval
     steps = max(ci, (args.steps // ci) * ci)
     warm = max(ci, (args.warmup // ci) * ci)
-    sim = FixedCylinder(cfg, perturb=PERTURB)
+    sim = FixedCylinder(cfg, perturb=PERTURB, precision=args.precision)
+    if args.serial:
+        sim.ctx.set_concurrent(False)
+    graph = not args.no_graph
     N = sim.st.nodenums
     for _ in range(warm // ci):
-        sim.advance(ci, diag_every=ci, graph=True)
+        sim.advance(ci, diag_every=ci, graph=graph)
         sim.cd_cl()
     sim.ctx.sync()
     ev_ms = 0.0
     iters = []
     t0 = time.perf_counter()
     for _ in range(steps // ci):
-        sim.advance(ci, diag_every=ci, graph=True)
+        sim.advance(ci, diag_every=ci, graph=graph)
         cd, cl = sim.cd_cl()  # synchronises, as the reference's diagnostics do
         ms, _n = sim.ctx.last_timing()
         ev_ms += ms
@@ -110,9 +117,10 @@ def main():
     wall = time.perf_counter() - t0
     mlups = N * steps / wall / 1e6
     kernel_s = ev_ms / 1e3 / steps
-    achieved = BYTES_PER_NODE["fp64"] * N / kernel_s / 1e9
+    bpn = BYTES_PER_NODE[args.precision]
+    achieved = bpn * N / kernel_s / 1e9
     out = {
-        "metric": "MLUPS (FP64 fused TRT+MDF step, Re=100 cylinder 1601x641)",
+        "metric": f"MLUPS ({args.precision} TRT+MDF step, Re=100 cylinder 1601x641)",
         "value": round(mlups, 2),
         "unit": "MLUPS",
         "n_gpus": 1,
@@ -122,16 +130,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32 (deviation storage; FP64 IBM)",
         "data": "synthetic (seeded 1e-3 inflow perturbation, seed 1234)",
         "config": {"workload": "fixed cylinder Re=100, 1601x641, D=40, 126 markers, TRT "
-                               "Lambda=1/4, MDF [5,20] tol 1e-4, Peskin-4, U=0.04, FP64",
+                               "Lambda=1/4, MDF [5,20] tol 1e-4, Peskin-4, U=0.04",
+                   "precision": args.precision, "streams": 1 if args.serial else 2,
+                   "graph": graph,
                    "nodes": N, "check_interval": ci, "Cd_last": cd, "Cl_last": cl,
                    "mdf_iter_last": iters[-1] if iters else None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None,
-                     "kernel": "k_step", "bytes_per_node": BYTES_PER_NODE["fp64"],
+                     "kernel": "k_bulk + k_ibm (one step)", "bytes_per_node": bpn,
                      "avg_launch_us": round(kernel_s * 1e6, 3)},
     }
     if not args.no_cpu:

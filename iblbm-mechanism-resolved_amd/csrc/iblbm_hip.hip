@@ -13,21 +13,68 @@
 
 #include "../../include/iblbm_hip.h"
 #include "ibm_block.h"
+#include "scratch_alloc.h"
 
 using namespace iblbm;
 
 // ------------------------------------------------------------------ kernels
 
-__global__ __launch_bounds__(NT) void k_step(LaunchTable T) {
-    __shared__ double red[NT];
-    __shared__ int redi[NT];
-    const int b = blockIdx.x;
-    if (b < T.n_ibm) {
-        Inst &I = T.insts[T.ibm_inst[b]];
-        ibm_role(I, T.parity, T.diag, red, redi);
-        return;
+// Bulk lattice node in FP64 (reference arithmetic, bit-exact).
+template <int MODEL>
+__device__ inline double bulk_node_fp64(const Inst &I, int parity, int i, int j) {
+    const double *__restrict__ A = buf<double>(I, parity);
+    double *__restrict__ B = buf<double>(I, parity ^ 1);
+    double fs[9], fe[9], out[9];
+    fstar_at(A, I, i, j, fs);
+    double r, u0, u1;
+    const bool inc = I.inc != 0;
+    macro9(fs, inc, r, u0, u1);
+    if constexpr (MODEL == 0) {
+        feq9(r, u0, u1, inc, fe);
+        collide_bgk<false>(I.cp, fs, fe, u0, u1, 0.0, 0.0, out);
+    } else if constexpr (MODEL == 1) {
+        feq9(r, u0, u1, inc, fe);
+        collide_trt<false>(I.cp, fs, fe, u0, u1, 0.0, 0.0, out);
+    } else if constexpr (MODEL == 2) {
+        collide_cm<false>(I.cp, fs, u0, u1, 0.0, 0.0, out);
+    } else {
+        if (I.cp.model != 2) feq9(r, u0, u1, inc, fe);
+        collide<false>(I.cp, fs, fe, u0, u1, 0.0, 0.0, out);
     }
-    const int tb = b - T.n_ibm;
+    store9(B, I, i, j, out);
+    return r;
+}
+
+// Bulk lattice node in FP32 deviation form.
+template <int MODEL>
+__device__ inline double bulk_node_fp32(const Inst &I, int parity, int i, int j) {
+    const float *__restrict__ A = buf<float>(I, parity);
+    float *__restrict__ B = buf<float>(I, parity ^ 1);
+    float g[9], out[9];
+    fstar_dev32(A, I, i, j, g);
+    float drho, ux, uy;
+    const bool inc = I.inc != 0;
+    moments32(g, inc, drho, ux, uy);
+    if constexpr (MODEL >= 0) {
+        collide32<MODEL>(I.cp32, g, drho, ux, uy, inc, out);
+    } else {
+        if (I.cp.model == 0)
+            collide32<0>(I.cp32, g, drho, ux, uy, inc, out);
+        else if (I.cp.model == 1)
+            collide32<1>(I.cp32, g, drho, ux, uy, inc, out);
+        else
+            collide32<2>(I.cp32, g, drho, ux, uy, inc, out);
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) B[q * I.qs + (int64_t)j * I.pitch + i] = out[q];
+    return 1.0 + (double)drho;
+}
+
+// Bulk tiles of one time step: every node outside the instance's marker window.
+template <typename S, int MODEL>
+__global__ __launch_bounds__(NT_BULK) void k_bulk(LaunchTable T) {
+    __shared__ double red[NT_BULK / 64];
+    const int tb = blockIdx.x;
     int inst = 0;
     while (inst + 1 < T.n_inst && T.tile_prefix[inst + 1] <= tb) ++inst;
     const Inst &I = T.insts[inst];
@@ -40,25 +87,29 @@ __global__ __launch_bounds__(NT) void k_step(LaunchTable T) {
     const bool in_box = I.ibm != 0 && i >= I.bx0 && i < I.bx0 + I.bw && j >= I.by0 &&
                         j < I.by0 + I.bh;
     if (in_grid && !in_box) {
-        const double *__restrict__ A = I.f[T.parity];
-        double *__restrict__ B = I.f[T.parity ^ 1];
-        double fs[9], fe[9], out[9];
-        fstar_at(A, I, i, j, fs);
-        double r, u0, u1;
-        macro9(fs, I.inc != 0, r, u0, u1);
-        if (I.cp.model != 2) feq9(r, u0, u1, I.inc != 0, fe);
-        collide<false>(I.cp, fs, fe, u0, u1, 0.0, 0.0, out);
-        store9(B, I, i, j, out);
-        rho = r;
+        if constexpr (sizeof(S) == 8)
+            rho = bulk_node_fp64<MODEL>(I, T.parity, i, j);
+        else
+            rho = bulk_node_fp32<MODEL>(I, T.parity, i, j);
     }
     if (T.diag) {
-        const double s = block_sum_fixed(rho, red);
-        if (threadIdx.x == 0) T.rho_partial[b] = s;
+        const double s = bulk_block_sum(rho, red);
+        if (threadIdx.x == 0) T.rho_partial[tb] = s;
     }
+}
+
+// Immersed-boundary workgroups of one time step (one per instance with markers).
+template <typename S, int MODEL>
+__global__ __launch_bounds__(NTI) void k_ibm(LaunchTable T) {
+    __shared__ double red[NTI];
+    __shared__ int redi[NTI];
+    Inst &I = T.insts[T.ibm_inst[blockIdx.x]];
+    ibm_role<S, MODEL>(I, T.parity, T.diag, red, redi);
 }
 
 // state.fstar / ro / U of the last step (source buffer A = that step's input); box nodes come
 // from the IBM scratch (DFC-corrected fstar, velocity-corrected U).
+template <typename S>
 __global__ __launch_bounds__(256) void k_state(const Inst *Ip, int parity, double *fstar,
                                                double *ro, double *U) {
     const Inst &I = *Ip;
@@ -75,7 +126,7 @@ __global__ __launch_bounds__(256) void k_state(const Inst *Ip, int parity, doubl
         u0 = I.s.ux0[sl];
         u1 = I.s.uy0[sl];
     } else {
-        fstar_at(I.f[parity], I, i, j, fs);
+        fstar_at(buf<S>(I, parity), I, i, j, fs);
         macro9(fs, I.inc != 0, r, u0, u1);
     }
     for (int q = 0; q < 9; ++q) fstar[9 * n + q] = fs[q];
@@ -84,20 +135,20 @@ __global__ __launch_bounds__(256) void k_state(const Inst *Ip, int parity, doubl
     U[2 * n + 1] = u1;
 }
 
-// AoS (N,9) <-> SoA f[q][j][pitch]
-__global__ void k_aos_to_soa(const double *aos, double *soa, int nx, int ny, int pitch,
-                             int64_t qs) {
+// AoS (N,9) FP64 <-> SoA f[q][j][pitch] (FP64 f, or FP32 g = f - w)
+template <typename S>
+__global__ void k_aos_to_soa(const double *aos, S *soa, int nx, int ny, int pitch, int64_t qs) {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= (int64_t)nx * ny) return;
     const int i = (int)(n % nx), j = (int)(n / nx);
-    for (int q = 0; q < 9; ++q) soa[q * qs + (int64_t)j * pitch + i] = aos[9 * n + q];
+    for (int q = 0; q < 9; ++q) soa[q * qs + (int64_t)j * pitch + i] = Store<S>::st(aos[9 * n + q], q);
 }
-__global__ void k_soa_to_aos(const double *soa, double *aos, int nx, int ny, int pitch,
-                             int64_t qs) {
+template <typename S>
+__global__ void k_soa_to_aos(const S *soa, double *aos, int nx, int ny, int pitch, int64_t qs) {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= (int64_t)nx * ny) return;
     const int i = (int)(n % nx), j = (int)(n / nx);
-    for (int q = 0; q < 9; ++q) aos[9 * n + q] = soa[q * qs + (int64_t)j * pitch + i];
+    for (int q = 0; q < 9; ++q) aos[9 * n + q] = Store<S>::ld(soa, q * qs + (int64_t)j * pitch + i, q);
 }
 
 // ------------------------------------------------------------------ host side
@@ -140,8 +191,9 @@ struct HostInst {
 
 struct Ctx {
     int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipStream_t stream = nullptr, side = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_main = nullptr, ev_side = nullptr;
+    bool concurrent = true;
     DevAlloc mem;
     std::vector<HostInst> inst;
     Inst *d_insts = nullptr;
@@ -226,14 +278,65 @@ LaunchTable make_table(Ctx &C, int parity, int diag) {
     return T;
 }
 
-int launch_steps(Ctx &C, int nsteps, int diag_every) {
-    const int blocks = C.n_tiles + C.n_ibm;
+typedef void (*StepKernel)(LaunchTable);
+
+template <typename S, int MODEL>
+void pick_pair(StepKernel &bulk, StepKernel &ibm) {
+    bulk = k_bulk<S, MODEL>;
+    ibm = k_ibm<S, MODEL>;
+}
+
+void pick_kernels(const Ctx &C, StepKernel &bulk, StepKernel &ibm) {
+    int model = C.inst[0].d.cp.model;
+    for (const HostInst &h : C.inst)
+        if (h.d.cp.model != model) model = -1;
+    const bool f32 = C.inst[0].d.prec == 1;
+    switch (model * 2 + (f32 ? 1 : 0)) {
+        case 0: return pick_pair<double, 0>(bulk, ibm);
+        case 1: return pick_pair<float, 0>(bulk, ibm);
+        case 2: return pick_pair<double, 1>(bulk, ibm);
+        case 3: return pick_pair<float, 1>(bulk, ibm);
+        case 4: return pick_pair<double, 2>(bulk, ibm);
+        case 5: return pick_pair<float, 2>(bulk, ibm);
+        default: break;
+    }
+    if (f32)
+        pick_pair<float, -1>(bulk, ibm);
+    else
+        pick_pair<double, -1>(bulk, ibm);
+}
+
+// Enqueues nsteps steps.  The bulk kernel runs on the context stream and the IBM kernel on a
+// second stream; both depend on both kernels of the previous step through events, so the
+// window workgroups overlap the bulk tiles of the same step.  With concurrent = false both
+// run in order on the context stream.
+int launch_steps(Ctx &C, int nsteps, int diag_every, bool concurrent) {
+    StepKernel kb, ki;
+    pick_kernels(C, kb, ki);
+    const bool two = concurrent && C.n_ibm > 0;
+    if (two) {  // join the side stream to everything already queued on the main stream
+        HIPCHK(hipEventRecord(C.ev_main, C.stream));
+        HIPCHK(hipStreamWaitEvent(C.side, C.ev_main, 0));
+    }
     for (int s = 0; s < nsteps; ++s) {
         const int64_t stepno = C.step + s + 1;
         const int diag = (diag_every > 0 && stepno % diag_every == 0) ? 1 : 0;
         LaunchTable T = make_table(C, (C.parity + s) & 1, diag);
-        hipLaunchKernelGGL(k_step, dim3(blocks), dim3(NT), 0, C.stream, T);
+        if (two) {
+            if (s > 0) {
+                HIPCHK(hipStreamWaitEvent(C.stream, C.ev_side, 0));
+                HIPCHK(hipStreamWaitEvent(C.side, C.ev_main, 0));
+            }
+            hipLaunchKernelGGL(ki, dim3(C.n_ibm), dim3(NTI), 0, C.side, T);
+            hipLaunchKernelGGL(kb, dim3(C.n_tiles), dim3(NT_BULK), 0, C.stream, T);
+            HIPCHK(hipEventRecord(C.ev_main, C.stream));
+            HIPCHK(hipEventRecord(C.ev_side, C.side));
+        } else {
+            if (C.n_ibm) hipLaunchKernelGGL(ki, dim3(C.n_ibm), dim3(NTI), 0, C.stream, T);
+            if (C.n_tiles) hipLaunchKernelGGL(kb, dim3(C.n_tiles), dim3(NT_BULK), 0, C.stream, T);
+        }
     }
+    if (two) HIPCHK(hipStreamWaitEvent(C.stream, C.ev_side, 0));  // join back
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -269,7 +372,10 @@ int iblbm_ctx_create(int device, void **out) {
     Ctx *C = new Ctx();
     C->device = device;
     if (hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&C->ev0) != hipSuccess || hipEventCreate(&C->ev1) != hipSuccess) {
+        hipStreamCreateWithFlags(&C->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&C->ev0) != hipSuccess || hipEventCreate(&C->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&C->ev_main, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&C->ev_side, hipEventDisableTiming) != hipSuccess) {
         delete C;
         return fail(IBLBM_EDEVICE, "ctx_create: stream/event creation failed");
     }
@@ -286,6 +392,9 @@ int iblbm_ctx_destroy(void *p) {
     C->mem.release();
     (void)hipEventDestroy(C->ev0);
     (void)hipEventDestroy(C->ev1);
+    (void)hipEventDestroy(C->ev_main);
+    (void)hipEventDestroy(C->ev_side);
+    (void)hipStreamDestroy(C->side);
     (void)hipStreamDestroy(C->stream);
     delete C;
     return 0;
@@ -300,8 +409,10 @@ int iblbm_instance_add(void *p, const iblbm_case *c, int *id) {
     if (c->bc < 0 || c->bc > 1) return fail(IBLBM_EKEY, "instance_add: unknown bc_type");
     if (c->ibm < 0 || c->ibm > 3) return fail(IBLBM_EKEY, "instance_add: unknown ibm_method");
     if (c->delta < 0 || c->delta > 1) return fail(IBLBM_EKEY, "instance_add: unknown delta_type");
-    if (c->precision != IBLBM_FP64)
-        return fail(IBLBM_EINVAL, "instance_add: only FP64 storage is built in this library");
+    if (c->precision != IBLBM_FP64 && c->precision != IBLBM_FP32_DEV)
+        return fail(IBLBM_EINVAL, "instance_add: unknown precision");
+    if (!C->inst.empty() && C->inst[0].d.prec != c->precision)
+        return fail(IBLBM_EINVAL, "instance_add: all instances of a context share one precision");
     if (c->ibm != 0 && (c->lb_capacity < 1 || c->box_capacity < 1))
         return fail(IBLBM_EINVAL, "instance_add: IBM needs lb_capacity and box_capacity");
     if (c->ibm == IBLBM_IBM_MDF && (c->mdf_iterations < 1 || c->mdf_min_iterations < 1))
@@ -329,12 +440,18 @@ int iblbm_instance_add(void *p, const iblbm_case *c, int *id) {
     cp.gpp = 1.0 - 0.5 * cp.s_plus;        // trt.py:42-43
     cp.gpm = 1.0 - 0.5 * cp.s_minus;
     for (int q = 0; q < 9; ++q) cp.S[q] = c->s_diag[q];
+    d.prec = c->precision;
+    d.cp32.inv_tau = (float)cp.inv_tau;
+    d.cp32.s_plus = (float)cp.s_plus;
+    d.cp32.s_minus = (float)cp.s_minus;
+    for (int q = 0; q < 9; ++q) d.cp32.S[q] = (float)cp.S[q];
     d.in_u = c->in_u;
     d.dens = c->dens;
     if (c->bc == IBLBM_BC_DIRICHLET_NEUMANN)
         feq9(c->dens, c->in_u, 0.0, false, d.feq_bc);  // dirichlet_neumann.py:186-199
     else
         feq9(c->dens, 0.0, 0.0, c->incompressible != 0, d.feq_bc);  // settling_channel.py:137-142
+    for (int q = 0; q < 9; ++q) d.gbc[q] = (float)(d.feq_bc[q] - W(q));
     d.ibm = c->ibm;
     d.delta = c->delta;
     d.dx = c->dx;
@@ -345,7 +462,7 @@ int iblbm_instance_add(void *p, const iblbm_case *c, int *id) {
     d.min_iter = c->mdf_min_iterations;
     d.fixed_body = c->fixed_body;
     void *ptr;
-    const size_t fbytes = sizeof(double) * 9 * (size_t)d.qs;
+    const size_t fbytes = (d.prec == 1 ? sizeof(float) : sizeof(double)) * 9 * (size_t)d.qs;
     for (int k = 0; k < 2; ++k) {
         if (C->mem.alloc(&ptr, fbytes) != hipSuccess)
             return fail(IBLBM_ENOMEM, "instance_add: population buffers do not fit");
@@ -353,46 +470,16 @@ int iblbm_instance_add(void *p, const iblbm_case *c, int *id) {
         HIPCHK(hipMemsetAsync(ptr, 0, fbytes, C->stream));
     }
     const int L = c->ibm ? c->lb_capacity : 1, B = c->ibm ? c->box_capacity : 1;
-    const int K = 25;
-    d.lb_cap = L;
-    d.box_cap = B;
-    auto dalloc = [&](double **pp, size_t n) {
-        void *q;
-        hipError_t e = C->mem.alloc(&q, sizeof(double) * n);
-        *pp = (double *)q;
-        return e;
-    };
-    auto ialloc = [&](int **pp, size_t n) {
-        void *q;
-        hipError_t e = C->mem.alloc(&q, sizeof(int) * n);
-        *pp = (int *)q;
-        return e;
-    };
-    IbmScratch &s = d.s;
-    double *lx, *ly, *ud;
-    bool ok = dalloc(&lx, L) == hipSuccess && dalloc(&ly, L) == hipSuccess &&
-              dalloc(&ud, 2 * (size_t)L) == hipSuccess && dalloc(&s.fs, 9 * (size_t)B) == hipSuccess &&
-              dalloc(&s.ro, B) == hipSuccess && dalloc(&s.ux0, B) == hipSuccess &&
-              dalloc(&s.uy0, B) == hipSuccess && dalloc(&s.uwx, B) == hipSuccess &&
-              dalloc(&s.uwy, B) == hipSuccess && dalloc(&s.vs, B) == hipSuccess &&
-              dalloc(&s.fibx, B) == hipSuccess && dalloc(&s.fiby, B) == hipSuccess &&
-              dalloc(&s.wt, B) == hipSuccess && ialloc(&s.cnt, B) == hipSuccess &&
-              ialloc(&s.off, B + 1) == hipSuccess && ialloc(&s.cur, B) == hipSuccess &&
-              dalloc(&s.cw, (size_t)K * L) == hipSuccess && ialloc(&s.cslot, (size_t)K * L) == hipSuccess &&
-              ialloc(&s.list, (size_t)K * L) == hipSuccess && dalloc(&s.Lux, L) == hipSuccess &&
-              dalloc(&s.Luy, L) == hipSuccess && dalloc(&s.R, L) == hipSuccess &&
-              dalloc(&s.Lfx, L) == hipSuccess && dalloc(&s.Lfy, L) == hipSuccess &&
-              dalloc(&s.lam, L) == hipSuccess && dalloc(&s.q, 2 * (size_t)L) == hipSuccess &&
-              dalloc(&s.fi, 9 * (size_t)L) == hipSuccess && dalloc(&s.dfl, 9 * (size_t)L) == hipSuccess &&
-              dalloc(&d.stats, 4) == hipSuccess && dalloc(&d.diag, 8) == hipSuccess;
+    d.dfc_scale = 1.0;
+    const bool ok = alloc_ibm_scratch(d, L, B, [&](size_t bytes) -> void * {
+        void *q = nullptr;
+        return C->mem.alloc(&q, bytes) == hipSuccess ? q : nullptr;
+    });
     if (!ok) return fail(IBLBM_ENOMEM, "instance_add: IBM scratch allocation failed");
-    HIPCHK(hipMemsetAsync(s.fibx, 0, sizeof(double) * B, C->stream));
-    HIPCHK(hipMemsetAsync(s.fiby, 0, sizeof(double) * B, C->stream));
+    HIPCHK(hipMemsetAsync(d.s.fibx, 0, sizeof(double) * B, C->stream));
+    HIPCHK(hipMemsetAsync(d.s.fiby, 0, sizeof(double) * B, C->stream));
     HIPCHK(hipMemsetAsync(d.stats, 0, sizeof(double) * 4, C->stream));
     HIPCHK(hipMemsetAsync(d.diag, 0, sizeof(double) * 8, C->stream));
-    d.Lx = lx;
-    d.Ly = ly;
-    d.ud = ud;
     d.Lb = 0;
     if (c->ibm != 0) d.ibm = 0;  // inactive until markers arrive
     h.c.ibm = c->ibm;
@@ -506,9 +593,14 @@ int iblbm_upload_f(void *p, int id, const double *f_aos) {
     HIPCHK(hipMalloc(&tmp, sizeof(double) * 9 * N));
     hipError_t e = hipMemcpy(tmp, f_aos, sizeof(double) * 9 * N, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
-                           C->stream, (const double *)tmp, d.f[C->parity], d.nx, d.ny, d.pitch,
-                           d.qs);
+        if (d.prec == 1)
+            hipLaunchKernelGGL(k_aos_to_soa<float>, dim3((unsigned)((N + 255) / 256)), dim3(256),
+                               0, C->stream, (const double *)tmp, (float *)d.f[C->parity], d.nx,
+                               d.ny, d.pitch, d.qs);
+        else
+            hipLaunchKernelGGL(k_aos_to_soa<double>, dim3((unsigned)((N + 255) / 256)), dim3(256),
+                               0, C->stream, (const double *)tmp, d.f[C->parity], d.nx, d.ny,
+                               d.pitch, d.qs);
         e = hipStreamSynchronize(C->stream);
     }
     (void)hipFree(tmp);
@@ -524,8 +616,14 @@ int iblbm_download_f(void *p, int id, double *f_aos) {
     const int64_t N = (int64_t)d.nx * d.ny;
     void *tmp;
     HIPCHK(hipMalloc(&tmp, sizeof(double) * 9 * N));
-    hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, C->stream,
-                       (const double *)d.f[C->parity], (double *)tmp, d.nx, d.ny, d.pitch, d.qs);
+    if (d.prec == 1)
+        hipLaunchKernelGGL(k_soa_to_aos<float>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
+                           C->stream, (const float *)d.f[C->parity], (double *)tmp, d.nx, d.ny,
+                           d.pitch, d.qs);
+    else
+        hipLaunchKernelGGL(k_soa_to_aos<double>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
+                           C->stream, (const double *)d.f[C->parity], (double *)tmp, d.nx, d.ny,
+                           d.pitch, d.qs);
     hipError_t e = hipStreamSynchronize(C->stream);
     if (e == hipSuccess) e = hipMemcpy(f_aos, tmp, sizeof(double) * 9 * N, hipMemcpyDeviceToHost);
     (void)hipFree(tmp);
@@ -544,8 +642,12 @@ int iblbm_download_state(void *p, int id, double *fstar, double *ro, double *U) 
     void *tmp;
     HIPCHK(hipMalloc(&tmp, sizeof(double) * 12 * N));
     double *tf = (double *)tmp, *tr = tf + 9 * N, *tu = tr + N;
-    hipLaunchKernelGGL(k_state, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, C->stream,
-                       (const Inst *)(C->d_insts + id), C->parity ^ 1, tf, tr, tu);
+    if (d.prec == 1)
+        hipLaunchKernelGGL(k_state<float>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
+                           C->stream, (const Inst *)(C->d_insts + id), C->parity ^ 1, tf, tr, tu);
+    else
+        hipLaunchKernelGGL(k_state<double>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
+                           C->stream, (const Inst *)(C->d_insts + id), C->parity ^ 1, tf, tr, tu);
     hipError_t e = hipStreamSynchronize(C->stream);
     if (e == hipSuccess) e = hipMemcpy(fstar, tf, sizeof(double) * 9 * N, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(ro, tr, sizeof(double) * N, hipMemcpyDeviceToHost);
@@ -596,7 +698,7 @@ int iblbm_step(void *p, int nsteps, int diag_every) {
     int rc = ensure_tables(*C);
     if (rc) return rc;
     HIPCHK(hipEventRecord(C->ev0, C->stream));
-    rc = launch_steps(*C, nsteps, diag_every);
+    rc = launch_steps(*C, nsteps, diag_every, C->concurrent);
     if (rc) return rc;
     HIPCHK(hipEventRecord(C->ev1, C->stream));
     if (diag_every > 0) {
@@ -617,13 +719,13 @@ int iblbm_graph_run(void *p, int nsteps, int diag_every) {
     int rc = ensure_tables(*C);
     if (rc) return rc;
     const int phase = diag_every > 0 ? (int)(C->step % diag_every) : 0;
-    auto key = std::make_tuple(nsteps, diag_every, C->parity, phase);
+    auto key = std::make_tuple(nsteps, diag_every * 2 + (C->concurrent ? 1 : 0), C->parity, phase);
     auto it = C->graphs.find(key);
     hipGraphExec_t exec;
     if (it == C->graphs.end()) {
         hipGraph_t g;
         HIPCHK(hipStreamBeginCapture(C->stream, hipStreamCaptureModeThreadLocal));
-        rc = launch_steps(*C, nsteps, diag_every);
+        rc = launch_steps(*C, nsteps, diag_every, C->concurrent);
         hipError_t e = hipStreamEndCapture(C->stream, &g);
         if (rc) return rc;
         if (e != hipSuccess) return fail(IBLBM_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e));
@@ -644,6 +746,13 @@ int iblbm_graph_run(void *p, int nsteps, int diag_every) {
     C->step += nsteps;
     C->parity = (C->parity + nsteps) & 1;
     C->last_steps = nsteps;
+    return 0;
+}
+
+int iblbm_set_concurrent(void *p, int on) {
+    Ctx *C = as_ctx(p);
+    if (!C) return fail(IBLBM_EINVAL, "set_concurrent: null context");
+    C->concurrent = on != 0;
     return 0;
 }
 

@@ -5,64 +5,170 @@
 // (iblbm/ibm/dfc.py:256-342) on them, applies the velocity correction
 // (iblbm/ibm/dispatch.py:18-33), and finishes the collision of the box nodes.
 //
-// Spreading is a gather: every box slot owns the list of (stencil offset s, marker k)
-// contributions that land on it, sorted by c = s*Lb + k, which is exactly the order in
-// which np.add.at applies them (offset loop outer, markers inner), so the FP64 sums carry
-// the reference's bits and the result does not depend on thread timing (no float atomics).
+// Only "touched" box nodes (those some clamped stencil point lands on) carry a force; their
+// working arrays (MDF velocity, dt/(2 rho), accumulated fib) and the marker forces live in
+// LDS, so an MDF sub-step is two short latency chains (marker interpolation, node gather)
+// and one wavefront-shuffle reduction.
+//
+// Spreading is a gather: every touched node owns the list of (stencil offset s, marker k)
+// contributions that land on it, sorted by c = s*Lb + k, which is exactly the order in which
+// np.add.at applies them (offset loop outer, markers inner), so the FP64 sums carry the
+// reference's bits and do not depend on thread timing (no floating-point atomics).
 #pragma once
 #include "step_kernel.h"
 
 namespace iblbm {
 
+constexpr int MAX_T = 2048;   // touched nodes held in LDS (beyond: global scratch)
+constexpr int MAX_LB = 1024;  // markers held in LDS (beyond: global scratch)
+
+struct IbmShared {
+    double uwx[MAX_T], uwy[MAX_T], vs[MAX_T], fibx[MAX_T], fiby[MAX_T];
+    double Lfx[MAX_LB], Lfy[MAX_LB];
+};
+
 __device__ inline int64_t floor_idx(double v) { return (int64_t)floor(v); }
 
-// Contribution tables: weights, target slots and per-slot ordered lists.
+// ------------------------------------------------------------------ block-wide primitives
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = nanmax2(v, __shfl_xor(v, o, 64));
+    return v;
+}
+// Deterministic block sum/max (fixed shuffle tree per wavefront, then over wavefronts).
+__device__ inline double block_reduce(double v, double *red, bool is_max) {
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    v = is_max ? wave_max(v) : wave_sum(v);
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    if (wv == 0) {
+        double x = lane < NTI / 64 ? red[lane] : (is_max ? -INFINITY : 0.0);
+        x = is_max ? wave_max(x) : wave_sum(x);
+        if (lane == 0) red[NTI / 64] = x;
+    }
+    __syncthreads();
+    const double r = red[NTI / 64];
+    __syncthreads();
+    return r;
+}
+// Exclusive scan of in[0..n) into out[0..n], out[n] = total (in and out may alias).
+__device__ inline void block_scan_int(const int *in, int *out, int n, int *redi) {
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int chunk = (n + NTI - 1) / NTI;
+    const int b = min(n, t * chunk), e = min(n, b + chunk);
+    int s = 0;
+    for (int k = b; k < e; ++k) s += in[k];
+    int incl = s;  // inclusive scan of s across the wavefront
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) redi[wv] = incl;
+    __syncthreads();
+    if (wv == 0) {
+        const int x = lane < NTI / 64 ? redi[lane] : 0;
+        int xi = x;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(xi, o, 64);
+            if (lane >= o) xi += y;
+        }
+        if (lane < NTI / 64) redi[NTI / 64 + lane] = xi - x;  // exclusive wave offsets
+        if (lane == NTI / 64 - 1) redi[2 * (NTI / 64)] = xi;
+    }
+    __syncthreads();
+    int acc = redi[NTI / 64 + wv] + incl - s;
+    const int total = redi[2 * (NTI / 64)];
+    __syncthreads();
+    for (int k = b; k < e; ++k) {
+        const int v = in[k];
+        out[k] = acc;
+        acc += v;
+    }
+    if (t == 0) out[n] = total;
+    __threadfence_block();
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------ window tables
+
+// Contribution tables (marker-major c_m = k*K + s), touched-node compaction and per-node
+// gather lists in np.add.at order.
+template <int A>
 __device__ void ibm_build_tables(Inst &I, int *redi) {
-    const int t = threadIdx.x;
-    const int a = I.delta == 1 ? 1 : 2, side = 2 * a + 1, K = side * side, Lb = I.Lb;
-    const int nc = K * Lb, nbox = I.bw * I.bh;
+    constexpr int side = 2 * A + 1, K = side * side;
+    const int t = threadIdx.x, Lb = I.Lb, nc = K * Lb, nbox = I.bw * I.bh;
     IbmScratch &S = I.s;
-    for (int c = t; c < nc; c += NT) {
-        const int s = c / Lb, k = c - s * Lb;
-        const int di = s / side - a, dj = s % side - a;
+    for (int k = t; k < nbox; k += NTI) {
+        S.cnt[k] = 0;
+        S.cur[k] = 0;
+    }
+    __syncthreads();
+    for (int cm = t; cm < nc; cm += NTI) {
+        const int k = cm / K, s = cm - k * K;
+        const int di = s / side - A, dj = s % side - A;
         const double Lx = I.Lx[k], Ly = I.Ly[k];
         const int64_t ix0 = floor_idx(Lx / I.dx + 0.5), iy0 = floor_idx(Ly / I.dy + 0.5);
         const int64_t ei = ix0 + di, ej = iy0 + dj;
         const double wx = delta_phi(I.delta, (Lx - (double)ei * I.dx) / I.dx);
         const double wy = delta_phi(I.delta, (Ly - (double)ej * I.dy) / I.dy);
-        S.cw[c] = wx * wy;
+        double w = wx * wy;
         const int64_t eic = ei < 0 ? 0 : (ei > I.nx - 1 ? I.nx - 1 : ei);
         const int64_t ejc = ej < 0 ? 0 : (ej > I.ny - 1 ? I.ny - 1 : ej);
+        int slot;
         // The host sizes the box to hold every clamped stencil node.  A node outside it
         // would be a host bug: it is routed to slot 0 with a NaN weight so the step output
         // turns NaN (loudly) instead of touching memory outside the scratch arrays.
         if (eic < I.bx0 || eic >= I.bx0 + I.bw || ejc < I.by0 || ejc >= I.by0 + I.bh) {
-            S.cslot[c] = 0;
-            S.cw[c] = __builtin_nan("");
+            slot = 0;
+            w = __builtin_nan("");
         } else {
-            S.cslot[c] = (int)((ejc - I.by0) * I.bw + (eic - I.bx0));
+            slot = (int)((ejc - I.by0) * I.bw + (eic - I.bx0));
         }
-    }
-    for (int k = t; k < nbox; k += NT) {
-        S.cnt[k] = 0;
-        S.cur[k] = 0;
-    }
-    __syncthreads();
-    for (int c = t; c < nc; c += NT) {
-        atomicAdd(&S.cnt[S.cslot[c]], 1);
-    }
-    __syncthreads();
-    block_scan(S.cnt, S.off, nbox, redi);
-    for (int c = t; c < nc; c += NT) {
-        const int sl = S.cslot[c];
-        const int pos = atomicAdd(&S.cur[sl], 1);
-        S.list[S.off[sl] + pos] = c;
+        S.cw[cm] = w;
+        S.cslot[cm] = slot;
+        atomicAdd(&S.cnt[slot], 1);
     }
     __threadfence_block();
     __syncthreads();
-    for (int sl = t; sl < nbox; sl += NT) {  // insertion sort each slot's list by c
-        int *L = S.list + S.off[sl];
-        const int n = S.cnt[sl];
+    // touched flags -> compact index (tindex has nbox + 1 entries)
+    for (int k = t; k < nbox; k += NTI) S.tindex[k] = S.cnt[k] > 0 ? 1 : 0;
+    __threadfence_block();
+    __syncthreads();
+    block_scan_int(S.tindex, S.tindex, nbox, redi);
+    const int nT = S.tindex[nbox];
+    for (int k = t; k < nbox; k += NTI)
+        if (S.cnt[k] > 0) S.tslot[S.tindex[k]] = k;
+    __threadfence_block();
+    __syncthreads();
+    for (int k = t; k < nbox; k += NTI)
+        if (S.cnt[k] == 0) S.tindex[k] = -1;
+    if (t == 0) I.nT = nT;
+    // gather offsets over touched nodes (toff has nT + 1 entries)
+    for (int ti = t; ti < nT; ti += NTI) S.toff[ti] = S.cnt[S.tslot[ti]];
+    __threadfence_block();
+    __syncthreads();
+    block_scan_int(S.toff, S.toff, nT, redi);
+    for (int cm = t; cm < nc; cm += NTI) {
+        const int sl = S.cslot[cm];
+        const int ti = S.tindex[sl];
+        S.cti[cm] = ti;
+        const int k = cm / K, s = cm - k * K;
+        const int pos = atomicAdd(&S.cur[sl], 1);
+        S.gc[S.toff[ti] + pos] = s * Lb + k;  // np.add.at order key
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (int ti = t; ti < nT; ti += NTI) {  // sort each list by the np order key
+        int *L = S.gc + S.toff[ti];
+        const int n = S.toff[ti + 1] - S.toff[ti];
         for (int x = 1; x < n; ++x) {
             const int v = L[x];
             int y = x - 1;
@@ -72,192 +178,188 @@ __device__ void ibm_build_tables(Inst &I, int *redi) {
             }
             L[y + 1] = v;
         }
+        for (int x = 0; x < n; ++x) {  // (marker, w*Larea) per entry
+            const int c = L[x], s = c / Lb, k = c - s * Lb;
+            S.gk[S.toff[ti] + x] = k;
+            S.gw[S.toff[ti] + x] = S.cw[k * K + s] * I.Larea;
+        }
     }
     __threadfence_block();
     __syncthreads();
 }
 
-// DFC lambda (iblbm/ibm/dfc.py:157-185), rho_f = 1.
-__device__ void ibm_dfc_lambda(Inst &I) {
+// DFC lambda (iblbm/ibm/dfc.py:157-185), rho_f = 1: W_total per touched node in np order,
+// then W_sum per marker in stencil order.
+template <int A>
+__device__ void ibm_dfc_lambda(Inst &I, int nT, int Lb) {
+    constexpr int K = (2 * A + 1) * (2 * A + 1);
     const int t = threadIdx.x;
-    const int a = I.delta == 1 ? 1 : 2, K = (2 * a + 1) * (2 * a + 1), Lb = I.Lb;
-    const int nbox = I.bw * I.bh;
     IbmScratch &S = I.s;
-    for (int sl = t; sl < nbox; sl += NT) {
+    for (int ti = t; ti < nT; ti += NTI) {
         double w = 0.0;
-        const int *L = S.list + S.off[sl];
-        for (int x = 0, n = S.cnt[sl]; x < n; ++x) w += S.cw[L[x]];
-        S.wt[sl] = w;
+        for (int x = S.toff[ti], e = S.toff[ti + 1]; x < e; ++x) {
+            const int k = S.gk[x];
+            const int s = S.gc[x] / Lb;
+            w += S.cw[k * K + s];
+        }
+        S.wt[ti] = w;
     }
     __threadfence_block();
     __syncthreads();
-    for (int k = t; k < Lb; k += NT) {
+    for (int k = t; k < Lb; k += NTI) {
         double ws = 0.0;
-        for (int s = 0; s < K; ++s) {
-            const int c = s * Lb + k;
-            const int sl = S.cslot[c];
-            ws += S.wt[sl] * S.cw[c];
-        }
+#pragma unroll
+        for (int s = 0; s < K; ++s) ws += S.wt[S.cti[k * K + s]] * S.cw[k * K + s];
         S.lam[k] = 1.0 / (2.0 * 1.0 * I.Larea * ws);
     }
     __threadfence_block();
     __syncthreads();
 }
 
-// Interpolate rho and U (from ux/uy planes) to the markers and form the DF force.
-// compressible: F = 2 R (U_d - U~)/dt; incompressible (df.py:256-259): F = 2 (U_d - U~)/dt.
-// Returns the block-wide L-inf slip residual max_k |U_d - U~_k|.
-__device__ double ibm_interp_force(Inst &I, const double *ux, const double *uy, bool inc,
-                                   double *red) {
-    const int t = threadIdx.x;
-    const int a = I.delta == 1 ? 1 : 2, K = (2 * a + 1) * (2 * a + 1), Lb = I.Lb;
+// ------------------------------------------------------------------ the IBM core
+
+// Working-array pointers: LDS when the window fits, global scratch otherwise.
+struct IbmPtrs {
+    double *uwx, *uwy, *vs, *fbx, *fby, *Lfx, *Lfy;
+};
+__device__ inline IbmPtrs ibm_ptrs(Inst &I, int nT, IbmShared &sh) {
     IbmScratch &S = I.s;
-    double rmax = -INFINITY;
-    for (int k = t; k < Lb; k += NT) {
-        double R = 0.0, lx = 0.0, ly = 0.0;
-        for (int s = 0; s < K; ++s) {
-            const int c = s * Lb + k;
-            const int sl = S.cslot[c];
-            const double w = S.cw[c];
-            R += S.ro[sl] * w;
-            lx += ux[sl] * w;
-            ly += uy[sl] * w;
-        }
-        const double udx = I.ud[2 * k], udy = I.ud[2 * k + 1];
-        S.R[k] = R;
-        S.Lux[k] = lx;
-        S.Luy[k] = ly;
-        if (inc) {
-            S.Lfx[k] = 2.0 * (udx - lx) / I.cp.dt;
-            S.Lfy[k] = 2.0 * (udy - ly) / I.cp.dt;
-        } else {
-            S.Lfx[k] = 2.0 * R * (udx - lx) / I.cp.dt;
-            S.Lfy[k] = 2.0 * R * (udy - ly) / I.cp.dt;
-        }
-        const double vx = udx - lx, vy = udy - ly;
-        rmax = nanmax2(sqrt(vx * vx + vy * vy), rmax);
-    }
-    __threadfence_block();
-    return block_max(rmax, red);  // includes the barrier that publishes Lf
+    const bool lt = nT <= MAX_T, lm = I.Lb <= MAX_LB;
+    IbmPtrs p;
+    p.uwx = lt ? sh.uwx : S.uwx;
+    p.uwy = lt ? sh.uwy : S.uwy;
+    p.vs = lt ? sh.vs : S.vs;
+    p.fbx = lt ? sh.fibx : S.tfibx;
+    p.fby = lt ? sh.fiby : S.tfiby;
+    p.Lfx = lm ? sh.Lfx : S.Lfx;
+    p.Lfy = lm ? sh.Lfy : S.Lfy;
+    return p;
 }
 
-// Gather the spread force of one slot in np.add.at order: sum_c Lf[k] * (w_c * Larea).
-__device__ inline void ibm_gather_force(const Inst &I, int sl, double &ex, double &ey) {
-    const IbmScratch &S = I.s;
-    const int *L = S.list + S.off[sl];
-    const int Lb = I.Lb;
-    double sx = 0.0, sy = 0.0;
-    for (int x = 0, n = S.cnt[sl]; x < n; ++x) {
-        const int c = L[x];
-        const int k = c % Lb;
-        const double w = S.cw[c] * I.Larea;
-        sx += S.Lfx[k] * w;
-        sy += S.Lfy[k] * w;
-    }
-    ex = sx;
-    ey = sy;
-}
-
-__device__ void ibm_role(Inst &I, int parity, int diag, double *red, int *redi) {
-    const int t = threadIdx.x;
-    const double *__restrict__ A = I.f[parity];
-    double *__restrict__ B = I.f[parity ^ 1];
+// DF / MDF / DFC on the window given S.ro, S.ux0, S.uy0 (and S.fs for DFC) of the box.
+// DF/MDF leave fib of the touched nodes in P.fbx/P.fby; DFC corrects S.fs and recomputes
+// S.ro, S.ux0, S.uy0 of the touched nodes and leaves the marker forces in S.q.
+template <int A>
+__device__ void ibm_core(Inst &I, int nT, const IbmPtrs &P, double *red, bool store_lu) {
+    constexpr int K = (2 * A + 1) * (2 * A + 1);
+    const int t = threadIdx.x, Lb = I.Lb;
     IbmScratch &S = I.s;
-    const int nbox = I.bw * I.bh;
     const bool inc = I.inc != 0;
-
-    if (!I.tables_valid) {
-        ibm_build_tables(I, redi);
-        if (t == 0) I.tables_valid = I.fixed_body;
-    }
-    if (I.ibm == 3 && !I.lambda_valid) {
-        ibm_dfc_lambda(I);
-        if (t == 0) I.lambda_valid = I.fixed_body;
-    }
-
-    // P1: stream + close the box nodes, macroscopic moments
-    for (int sl = t; sl < nbox; sl += NT) {
-        const int i = I.bx0 + sl % I.bw, j = I.by0 + sl / I.bw;
-        double fs[9];
-        fstar_at(A, I, i, j, fs);
-#pragma unroll
-        for (int q = 0; q < 9; ++q) S.fs[q * I.box_cap + sl] = fs[q];
-        double r, u0, u1;
-        macro9(fs, inc, r, u0, u1);
-        S.ro[sl] = r;
-        S.ux0[sl] = u0;
-        S.uy0[sl] = u1;
-        S.fibx[sl] = 0.0;
-        S.fiby[sl] = 0.0;
+    const double dt = I.cp.dt;
+    for (int ti = t; ti < nT; ti += NTI) {
+        const int sl = S.tslot[ti];
+        P.uwx[ti] = S.ux0[sl];
+        P.uwy[ti] = S.uy0[sl];
+        P.vs[ti] = dt / (2.0 * S.ro[sl]);  // mdf.py:117
+        P.fbx[ti] = 0.0;
+        P.fby[ti] = 0.0;
     }
     __threadfence_block();
     __syncthreads();
-
-    if (I.ibm == 1) {  // DF: one interpolation + spread, iblbm/ibm/strategies/df.py:27-49
-        ibm_interp_force(I, S.ux0, S.uy0, inc, red);
-        for (int sl = t; sl < nbox; sl += NT) {
-            if (S.cnt[sl] == 0) continue;
-            double ex, ey;
-            ibm_gather_force(I, sl, ex, ey);
-            S.fibx[sl] = ex;
-            S.fiby[sl] = ey;
+    if (I.ibm == 1 || I.ibm == 2) {
+        // R = interpolated rho, constant over the MDF sub-steps
+        for (int k = t; k < Lb; k += NTI) {
+            double R = 0.0;
+#pragma unroll
+            for (int s = 0; s < K; ++s) R += S.ro[S.tslot[S.cti[k * K + s]]] * S.cw[k * K + s];
+            S.R[k] = R;
         }
-    } else if (I.ibm == 2) {  // MDF, iblbm/ibm/mdf.py:113-229 (compressible sub-steps)
-        for (int sl = t; sl < nbox; sl += NT) {
-            S.uwx[sl] = S.ux0[sl];
-            S.uwy[sl] = S.uy0[sl];
-            S.vs[sl] = I.cp.dt / (2.0 * S.ro[sl]);
-        }
-        __threadfence_block();
-        __syncthreads();
-        const int n_iter = I.n_iter;
+        const bool mdf = I.ibm == 2;
+        const bool finc = mdf ? false : inc;  // MDF sub-steps are compressible (mdf.py:177-181)
+        const int n_iter = mdf ? I.n_iter : 1;
         const int min_iter = max(1, min(I.min_iter, n_iter));
+        const double omega = I.omega;
         double prev = INFINITY, last = INFINITY;
         int iter_count = 0, diverged = 0, converged = 0;
         for (int it = 0; it < n_iter; ++it) {
-            const double res = ibm_interp_force(I, S.uwx, S.uwy, false, red);
-            last = res;
-            if (it > 0 && res > prev * 1.5) {
-                diverged = 1;
-                iter_count = it + 1;
-                break;
+            // interpolation + force per marker (df.py:239-263)
+            double rmax = -INFINITY;
+            for (int k = t; k < Lb; k += NTI) {
+                int ti[K];
+                double w[K];
+#pragma unroll
+                for (int s = 0; s < K; ++s) {
+                    ti[s] = S.cti[k * K + s];
+                    w[s] = S.cw[k * K + s];
+                }
+                double lx = 0.0, ly = 0.0;
+#pragma unroll
+                for (int s = 0; s < K; ++s) {
+                    lx += P.uwx[ti[s]] * w[s];
+                    ly += P.uwy[ti[s]] * w[s];
+                }
+                const double R = S.R[k];
+                const double udx = I.ud[2 * k], udy = I.ud[2 * k + 1];
+                if (finc) {
+                    P.Lfx[k] = 2.0 * (udx - lx) / dt;
+                    P.Lfy[k] = 2.0 * (udy - ly) / dt;
+                } else {
+                    P.Lfx[k] = 2.0 * R * (udx - lx) / dt;
+                    P.Lfy[k] = 2.0 * R * (udy - ly) / dt;
+                }
+                if (store_lu) {
+                    S.Lux[k] = lx;
+                    S.Luy[k] = ly;
+                }
+                const double vx = udx - lx, vy = udy - ly;
+                rmax = nanmax2(sqrt(vx * vx + vy * vy), rmax);
             }
-            for (int sl = t; sl < nbox; sl += NT) {
-                if (S.cnt[sl] == 0) continue;
-                double ex, ey;
-                ibm_gather_force(I, sl, ex, ey);
-                const double fx = ex * I.omega, fy = ey * I.omega;
-                S.fibx[sl] = S.fibx[sl] + fx;
-                S.fiby[sl] = S.fiby[sl] + fy;
-                S.uwx[sl] = S.uwx[sl] + fx * S.vs[sl];
-                S.uwy[sl] = S.uwy[sl] + fy * S.vs[sl];
+            __threadfence_block();
+            const double res = block_reduce(rmax, red, true);  // its barrier publishes Lf
+            if (mdf) {
+                last = res;
+                if (it > 0 && res > prev * 1.5) {  // divergence guard, mdf.py:192-195
+                    diverged = 1;
+                    iter_count = it + 1;
+                    break;
+                }
+            }
+            // spread as a gather in np.add.at order (df.py:269-283)
+            for (int ti = t; ti < nT; ti += NTI) {
+                double ex = 0.0, ey = 0.0;
+                for (int x = S.toff[ti], e = S.toff[ti + 1]; x < e; ++x) {
+                    const int k = S.gk[x];
+                    const double wl = S.gw[x];
+                    ex += P.Lfx[k] * wl;
+                    ey += P.Lfy[k] * wl;
+                }
+                if (mdf) {  // mdf.py:197-200
+                    const double fx = ex * omega, fy = ey * omega;
+                    P.fbx[ti] = P.fbx[ti] + fx;
+                    P.fby[ti] = P.fby[ti] + fy;
+                    P.uwx[ti] = P.uwx[ti] + fx * P.vs[ti];
+                    P.uwy[ti] = P.uwy[ti] + fy * P.vs[ti];
+                } else {
+                    P.fbx[ti] = ex;
+                    P.fby[ti] = ey;
+                }
             }
             __threadfence_block();
             __syncthreads();
-            prev = res;
-            iter_count = it + 1;
-            if (it + 1 >= min_iter && res < I.tol) {
-                converged = 1;
-                break;
+            if (mdf) {
+                prev = res;
+                iter_count = it + 1;
+                if (it + 1 >= min_iter && res < I.tol) {
+                    converged = 1;
+                    break;
+                }
             }
         }
-        if (t == 0) {
+        if (mdf && t == 0) {
             I.stats[0] = iter_count;
             I.stats[1] = last;
             I.stats[2] = diverged;
             I.stats[3] = converged;
         }
-    } else if (I.ibm == 3) {  // DFC, iblbm/ibm/dfc.py:319-341 (correction_scale = 1)
-        const int a = I.delta == 1 ? 1 : 2, K = (2 * a + 1) * (2 * a + 1), Lb = I.Lb;
-        const double cs2 = 1.0 / 3.0;
-        for (int k = t; k < Lb; k += NT) {
+    } else if (I.ibm == 3) {  // DFC, iblbm/ibm/dfc.py:319-341
+        const double cs2 = 1.0 / 3.0, csc = I.dfc_scale;
+        for (int k = t; k < Lb; k += NTI) {
             double fi[9];
 #pragma unroll
             for (int q = 0; q < 9; ++q) fi[q] = 0.0;
             for (int s = 0; s < K; ++s) {
-                const int c = s * Lb + k;
-                const int sl = S.cslot[c];
-                const double w = S.cw[c];
+                const int sl = S.tslot[S.cti[k * K + s]];
+                const double w = S.cw[k * K + s];
 #pragma unroll
                 for (int q = 0; q < 9; ++q) fi[q] += S.fs[q * I.box_cap + sl] * w;
             }
@@ -268,39 +370,34 @@ __device__ void ibm_role(Inst &I, int parity, int diag, double *red, int *redi) 
                 const double edu = udx * EX(q) + udy * EY(q);
                 const double fbb = fi[OPP[q]] + 2.0 * W(q) * 1.0 * edu / cs2;
                 const double dev = fbb - fi[q];
-                S.dfl[9 * k + q] = 1.0 * lam * dev;
+                S.dfl[9 * k + q] = csc * lam * dev;
                 const double sd = -(lam * dev * I.Larea);
                 qx += sd * EX(q);
                 qy += sd * EY(q);
             }
-            S.q[2 * k] = 1.0 * qx;
-            S.q[2 * k + 1] = 1.0 * qy;
+            S.q[2 * k] = csc * qx;
+            S.q[2 * k + 1] = csc * qy;
         }
         __threadfence_block();
         __syncthreads();
-        for (int sl = t; sl < nbox; sl += NT) {
+        for (int ti = t; ti < nT; ti += NTI) {
+            const int sl = S.tslot[ti];
+            double e[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) e[q] = 0.0;
+            for (int x = S.toff[ti], end = S.toff[ti + 1]; x < end; ++x) {
+                const int k = S.gk[x];
+                const double wl = S.gw[x];
+#pragma unroll
+                for (int q = 0; q < 9; ++q) e[q] += S.dfl[9 * k + q] * wl;
+            }
             double fs[9];
 #pragma unroll
-            for (int q = 0; q < 9; ++q) fs[q] = S.fs[q * I.box_cap + sl];
-            if (S.cnt[sl] != 0) {
-                double e[9];
-#pragma unroll
-                for (int q = 0; q < 9; ++q) e[q] = 0.0;
-                const int *L = S.list + S.off[sl];
-                for (int x = 0, n = S.cnt[sl]; x < n; ++x) {
-                    const int c = L[x];
-                    const int k = c % Lb;
-                    const double w = S.cw[c] * I.Larea;
-#pragma unroll
-                    for (int q = 0; q < 9; ++q) e[q] += S.dfl[9 * k + q] * w;
-                }
-#pragma unroll
-                for (int q = 0; q < 9; ++q) {
-                    fs[q] = fs[q] + e[q];
-                    S.fs[q * I.box_cap + sl] = fs[q];
-                }
+            for (int q = 0; q < 9; ++q) {
+                fs[q] = S.fs[q * I.box_cap + sl] + e[q];
+                S.fs[q * I.box_cap + sl] = fs[q];
             }
-            double r, u0, u1;
+            double r, u0, u1;  // strategies/dfc.py:50-54 macroscopic recompute
             macro9(fs, inc, r, u0, u1);
             S.ro[sl] = r;
             S.ux0[sl] = u0;
@@ -309,31 +406,86 @@ __device__ void ibm_role(Inst &I, int parity, int diag, double *red, int *redi) 
     }
     __threadfence_block();
     __syncthreads();
+}
+
+// ------------------------------------------------------------------ the workgroup
+
+template <typename ST, int MODEL, int A>
+__device__ void ibm_role_impl(Inst &I, int parity, int diag, double *red, int *redi,
+                              IbmShared &sh) {
+    const int t = threadIdx.x;
+    const ST *__restrict__ Asrc = buf<ST>(I, parity);
+    ST *__restrict__ B = buf<ST>(I, parity ^ 1);
+    IbmScratch &S = I.s;
+    const int nbox = I.bw * I.bh;
+    const bool inc = I.inc != 0;
+    const double dt = I.cp.dt;
+    const int Lb = I.Lb;
+
+    const bool build = !I.tables_valid;
+    if (build) ibm_build_tables<A>(I, redi);
+    const int nT = build ? S.tindex[nbox] : I.nT;
+    if (I.ibm == 3 && !I.lambda_valid) ibm_dfc_lambda<A>(I, nT, Lb);
+    __syncthreads();
+    if (t == 0) {
+        if (build) I.tables_valid = I.fixed_body;
+        if (I.ibm == 3) I.lambda_valid = I.fixed_body;
+    }
+    const IbmPtrs P = ibm_ptrs(I, nT, sh);
+
+    // P1: stream + close the box nodes, macroscopic moments
+    for (int sl = t; sl < nbox; sl += NTI) {
+        const int i = I.bx0 + sl % I.bw, j = I.by0 + sl / I.bw;
+        double fs[9];
+        fstar_at(Asrc, I, i, j, fs);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) S.fs[q * I.box_cap + sl] = fs[q];
+        double r, u0, u1;
+        macro9(fs, inc, r, u0, u1);
+        S.ro[sl] = r;
+        S.ux0[sl] = u0;
+        S.uy0[sl] = u1;
+    }
+    __threadfence_block();
+    __syncthreads();
+    ibm_core<A>(I, nT, P, red, false);
 
     // P3: velocity correction (DF/MDF), equilibrium, collision, store
     const bool correct = I.ibm == 1 || I.ibm == 2;
     double rho_acc = 0.0, fxs = 0.0, fys = 0.0, trq = 0.0;
-    for (int sl = t; sl < nbox; sl += NT) {
+    for (int sl = t; sl < nbox; sl += NTI) {
         const int i = I.bx0 + sl % I.bw, j = I.by0 + sl / I.bw;
+        const int ti = S.tindex[sl];
         double fs[9];
 #pragma unroll
         for (int q = 0; q < 9; ++q) fs[q] = S.fs[q * I.box_cap + sl];
-        const double r = S.ro[sl], fx = S.fibx[sl], fy = S.fiby[sl];
+        const double r = S.ro[sl];
+        const double fx = (correct && ti >= 0) ? P.fbx[ti] : 0.0;
+        const double fy = (correct && ti >= 0) ? P.fby[ti] : 0.0;
         double u0 = S.ux0[sl], u1 = S.uy0[sl];
         if (correct) {
             if (inc) {
-                u0 = u0 + 0.5 * fx * I.cp.dt;
-                u1 = u1 + 0.5 * fy * I.cp.dt;
+                u0 = u0 + 0.5 * fx * dt;
+                u1 = u1 + 0.5 * fy * dt;
             } else {
-                u0 = u0 + fx * I.cp.dt / (2.0 * r);
-                u1 = u1 + fy * I.cp.dt / (2.0 * r);
+                u0 = u0 + fx * dt / (2.0 * r);
+                u1 = u1 + fy * dt / (2.0 * r);
             }
             S.ux0[sl] = u0;
             S.uy0[sl] = u1;
         }
+        S.fibx[sl] = fx;
+        S.fiby[sl] = fy;
         double fe[9], out[9];
-        feq9(r, u0, u1, inc, fe);
-        collide<true>(I.cp, fs, fe, u0, u1, fx, fy, out);
+        if (MODEL != 2) feq9(r, u0, u1, inc, fe);
+        if constexpr (MODEL == 0)
+            collide_bgk<true>(I.cp, fs, fe, u0, u1, fx, fy, out);
+        else if constexpr (MODEL == 1)
+            collide_trt<true>(I.cp, fs, fe, u0, u1, fx, fy, out);
+        else if constexpr (MODEL == 2)
+            collide_cm<true>(I.cp, fs, u0, u1, fx, fy, out);
+        else
+            collide<true>(I.cp, fs, fe, u0, u1, fx, fy, out);
         store9(B, I, i, j, out);
         if (diag) {
             rho_acc += r;
@@ -343,21 +495,21 @@ __device__ void ibm_role(Inst &I, int parity, int diag, double *red, int *redi) 
         }
     }
     if (diag) {
-        const double rs = block_sum_fixed(rho_acc, red);
-        double f0 = block_sum_fixed(fxs, red), f1 = block_sum_fixed(fys, red);
-        const double tq = block_sum_fixed(trq, red);
+        const double rs = block_reduce(rho_acc, red, false);
+        double f0 = block_reduce(fxs, red, false), f1 = block_reduce(fys, red, false);
+        const double tq = block_reduce(trq, red, false);
         double tqq = 0.0;
         if (I.ibm == 3) {
             double qx = 0.0, qy = 0.0, tk = 0.0;
-            for (int k = t; k < I.Lb; k += NT) {
+            for (int k = t; k < Lb; k += NTI) {
                 qx += S.q[2 * k];
                 qy += S.q[2 * k + 1];
                 tk += (I.Lx[k] - I.cxd) / I.dx * S.q[2 * k + 1] -
                       (I.Ly[k] - I.cyd) / I.dy * S.q[2 * k];
             }
-            f0 = block_sum_fixed(qx, red);
-            f1 = block_sum_fixed(qy, red);
-            tqq = block_sum_fixed(tk, red);
+            f0 = block_reduce(qx, red, false);
+            f1 = block_reduce(qy, red, false);
+            tqq = block_reduce(tk, red, false);
         }
         if (t == 0) {
             I.diag[0] = rs;
@@ -367,6 +519,15 @@ __device__ void ibm_role(Inst &I, int parity, int diag, double *red, int *redi) 
             I.diag[4] = tqq;
         }
     }
+}
+
+template <typename ST, int MODEL>
+__device__ void ibm_role(Inst &I, int parity, int diag, double *red, int *redi) {
+    __shared__ IbmShared sh;
+    if (I.delta == 1)
+        ibm_role_impl<ST, MODEL, 1>(I, parity, diag, red, redi, sh);
+    else
+        ibm_role_impl<ST, MODEL, 2>(I, parity, diag, red, redi, sh);
 }
 
 }  // namespace iblbm

@@ -13,6 +13,7 @@
 
 #include "../../include/iblbm_hip.h"
 #include "ibm_block.h"
+#include "scratch_alloc.h"
 
 using namespace iblbm;
 
@@ -142,126 +143,66 @@ __global__ void k_op_boundary(int bc, double *fs, int nx, int ny, double in_u, d
 
 // ------------------------------------------------------------------ IBM operators
 
-// mode 0: DF fields, 1: MDF, 2: DFC lambda, 3: DFC correction
-__global__ __launch_bounds__(NT) void k_op_ibm(Inst *Ip, int mode, const double *Eux,
-                                               const double *Euy, const double *Ro,
-                                               double *fstar) {
-    __shared__ double red[NT];
-    __shared__ int redi[NT];
-    Inst &I = *Ip;
+// mode 0: DF fields, 1: MDF, 2: DFC lambda only, 3: DFC correction (lambda recomputed),
+// 4: DFC correction with the caller's lambda.  One workgroup; the window machinery is the
+// fused step's (ibm_build_tables / ibm_core).
+template <int A>
+__device__ void op_ibm_impl(Inst &I, int mode, const double *Eux, const double *Euy,
+                            const double *Ro, double *fstar, double *red, int *redi,
+                            IbmShared &sh) {
     IbmScratch &S = I.s;
     const int t = threadIdx.x;
     const int nbox = I.bw * I.bh;
-    ibm_build_tables(I, redi);
+    ibm_build_tables<A>(I, redi);
+    const int nT = S.tindex[nbox];
+    const IbmPtrs P = ibm_ptrs(I, nT, sh);
     if (mode >= 2) {
-        if (mode != 4) ibm_dfc_lambda(I);  // mode 4 keeps the caller's lambda
+        if (mode != 4) ibm_dfc_lambda<A>(I, nT, I.Lb);
         if (mode == 2) return;
-        // DFC correction on the given fstar
-        for (int sl = t; sl < nbox; sl += NT) {
+        for (int sl = t; sl < nbox; sl += NTI) {
             const int64_t n = (int64_t)(I.by0 + sl / I.bw) * I.nx + I.bx0 + sl % I.bw;
             for (int q = 0; q < 9; ++q) S.fs[q * I.box_cap + sl] = fstar[9 * n + q];
+            S.ro[sl] = 1.0;
+            S.ux0[sl] = 0.0;
+            S.uy0[sl] = 0.0;
         }
+        __threadfence_block();
         __syncthreads();
-        const int a = I.delta == 1 ? 1 : 2, K = (2 * a + 1) * (2 * a + 1), Lb = I.Lb;
-        const double cs2 = 1.0 / 3.0, csc = I.tol;  // correction_scale passed in tol
-        for (int k = t; k < Lb; k += NT) {
-            double fi[9];
-            for (int q = 0; q < 9; ++q) fi[q] = 0.0;
-            for (int s = 0; s < K; ++s) {
-                const int c = s * Lb + k;
-                const int sl = S.cslot[c];
-                const double w = S.cw[c];
-                for (int q = 0; q < 9; ++q) fi[q] += S.fs[q * I.box_cap + sl] * w;
-            }
-            const double udx = I.ud[2 * k], udy = I.ud[2 * k + 1], lam = S.lam[k];
-            double qx = 0.0, qy = 0.0;
-            for (int q = 0; q < 9; ++q) {
-                const double edu = udx * EX(q) + udy * EY(q);
-                const double fbb = fi[OPP[q]] + 2.0 * W(q) * 1.0 * edu / cs2;
-                const double dev = fbb - fi[q];
-                S.dfl[9 * k + q] = csc * lam * dev;
-                const double sd = -(lam * dev * I.Larea);
-                qx += sd * EX(q);
-                qy += sd * EY(q);
-            }
-            S.q[2 * k] = csc * qx;
-            S.q[2 * k + 1] = csc * qy;
-        }
-        __syncthreads();
-        for (int sl = t; sl < nbox; sl += NT) {
-            if (S.cnt[sl] == 0) continue;
+        ibm_core<A>(I, nT, P, red, false);
+        for (int ti = t; ti < nT; ti += NTI) {
+            const int sl = S.tslot[ti];
             const int64_t n = (int64_t)(I.by0 + sl / I.bw) * I.nx + I.bx0 + sl % I.bw;
-            double e[9];
-            for (int q = 0; q < 9; ++q) e[q] = 0.0;
-            const int *L = S.list + S.off[sl];
-            for (int x = 0, m = S.cnt[sl]; x < m; ++x) {
-                const int c = L[x];
-                const double w = S.cw[c] * I.Larea;
-                for (int q = 0; q < 9; ++q) e[q] += S.dfl[9 * (c % Lb) + q] * w;
-            }
-            for (int q = 0; q < 9; ++q) fstar[9 * n + q] = fstar[9 * n + q] + e[q];
+            for (int q = 0; q < 9; ++q) fstar[9 * n + q] = S.fs[q * I.box_cap + sl];
         }
         return;
     }
-    // load the Eulerian fields of the box
-    for (int sl = t; sl < nbox; sl += NT) {
+    for (int sl = t; sl < nbox; sl += NTI) {
         const int64_t n = (int64_t)(I.by0 + sl / I.bw) * I.nx + I.bx0 + sl % I.bw;
         S.ro[sl] = Ro[n];
         S.ux0[sl] = Eux[n];
         S.uy0[sl] = Euy[n];
-        S.fibx[sl] = 0.0;
-        S.fiby[sl] = 0.0;
     }
+    __threadfence_block();
     __syncthreads();
-    if (mode == 0) {
-        ibm_interp_force(I, S.ux0, S.uy0, I.inc != 0, red);
-        for (int sl = t; sl < nbox; sl += NT) {
-            if (S.cnt[sl] == 0) continue;
-            ibm_gather_force(I, sl, S.fibx[sl], S.fiby[sl]);
-        }
-        return;
+    ibm_core<A>(I, nT, P, red, mode == 0);
+    for (int sl = t; sl < nbox; sl += NTI) {
+        const int ti = S.tindex[sl];
+        S.fibx[sl] = ti >= 0 ? P.fbx[ti] : 0.0;
+        S.fiby[sl] = ti >= 0 ? P.fby[ti] : 0.0;
     }
-    for (int sl = t; sl < nbox; sl += NT) {
-        S.uwx[sl] = S.ux0[sl];
-        S.uwy[sl] = S.uy0[sl];
-        S.vs[sl] = I.cp.dt / (2.0 * S.ro[sl]);
-    }
-    __syncthreads();
-    const int min_iter = max(1, min(I.min_iter, I.n_iter));
-    double prev = INFINITY, last = INFINITY;
-    int iter_count = 0, diverged = 0, converged = 0;
-    for (int it = 0; it < I.n_iter; ++it) {
-        const double res = ibm_interp_force(I, S.uwx, S.uwy, false, red);
-        last = res;
-        if (it > 0 && res > prev * 1.5) {
-            diverged = 1;
-            iter_count = it + 1;
-            break;
-        }
-        for (int sl = t; sl < nbox; sl += NT) {
-            if (S.cnt[sl] == 0) continue;
-            double ex, ey;
-            ibm_gather_force(I, sl, ex, ey);
-            const double fx = ex * I.omega, fy = ey * I.omega;
-            S.fibx[sl] = S.fibx[sl] + fx;
-            S.fiby[sl] = S.fiby[sl] + fy;
-            S.uwx[sl] = S.uwx[sl] + fx * S.vs[sl];
-            S.uwy[sl] = S.uwy[sl] + fy * S.vs[sl];
-        }
-        __syncthreads();
-        prev = res;
-        iter_count = it + 1;
-        if (it + 1 >= min_iter && res < I.tol) {
-            converged = 1;
-            break;
-        }
-    }
-    if (t == 0) {
-        I.stats[0] = iter_count;
-        I.stats[1] = last;
-        I.stats[2] = diverged;
-        I.stats[3] = converged;
-    }
+}
+
+__global__ __launch_bounds__(NTI) void k_op_ibm(Inst *Ip, int mode, const double *Eux,
+                                               const double *Euy, const double *Ro,
+                                               double *fstar) {
+    __shared__ double red[NTI];
+    __shared__ int redi[NTI];
+    __shared__ IbmShared sh;
+    Inst &I = *Ip;
+    if (I.delta == 1)
+        op_ibm_impl<1>(I, mode, Eux, Euy, Ro, fstar, red, redi, sh);
+    else
+        op_ibm_impl<2>(I, mode, Eux, Euy, Ro, fstar, red, redi, sh);
 }
 
 namespace {
@@ -290,47 +231,20 @@ struct OpIbm {
         h.by0 = box[1];
         h.bw = box[2];
         h.bh = box[3];
-        const int B = box[2] * box[3], K = 25;
-        h.box_cap = B;
+        const int B = box[2] * box[3];
         h.dx = dx;
         h.dy = dy;
         h.Larea = Larea;
-        IbmScratch &s = h.s;
-        h.Lx = mem.up(Lx, Lb);
-        h.Ly = mem.up(Ly, Lb);
-        h.ud = ud ? mem.up(ud, 2 * (size_t)Lb) : mem.get<double>(2 * (size_t)Lb);
-        s.fs = mem.get<double>(9 * (size_t)B);
-        s.ro = mem.get<double>(B);
-        s.ux0 = mem.get<double>(B);
-        s.uy0 = mem.get<double>(B);
-        s.uwx = mem.get<double>(B);
-        s.uwy = mem.get<double>(B);
-        s.vs = mem.get<double>(B);
-        s.fibx = mem.get<double>(B);
-        s.fiby = mem.get<double>(B);
-        s.wt = mem.get<double>(B);
-        s.cnt = mem.get<int>(B);
-        s.off = mem.get<int>(B + 1);
-        s.cur = mem.get<int>(B);
-        s.cw = mem.get<double>((size_t)K * Lb);
-        s.cslot = mem.get<int>((size_t)K * Lb);
-        s.list = mem.get<int>((size_t)K * Lb);
-        s.Lux = mem.get<double>(Lb);
-        s.Luy = mem.get<double>(Lb);
-        s.R = mem.get<double>(Lb);
-        s.Lfx = mem.get<double>(Lb);
-        s.Lfy = mem.get<double>(Lb);
-        s.lam = mem.get<double>(Lb);
-        s.q = mem.get<double>(2 * (size_t)Lb);
-        s.fi = mem.get<double>(9 * (size_t)Lb);
-        s.dfl = mem.get<double>(9 * (size_t)Lb);
-        h.stats = mem.get<double>(4);
-        h.diag = mem.get<double>(8);
-        if (!h.Lx || !h.Ly || !h.ud || !s.fs || !s.ro || !s.ux0 || !s.uy0 || !s.uwx || !s.uwy ||
-            !s.vs || !s.fibx || !s.fiby || !s.wt || !s.cnt || !s.off || !s.cur || !s.cw ||
-            !s.cslot || !s.list || !s.Lux || !s.Luy || !s.R || !s.Lfx || !s.Lfy || !s.lam ||
-            !s.q || !s.fi || !s.dfl || !h.stats || !h.diag)
+        h.dfc_scale = 1.0;
+        if (!alloc_ibm_scratch(h, Lb, B, [&](size_t bytes) -> void * { return mem.get<char>(bytes); }))
             return op_fail(IBLBM_ENOMEM, "ibm op: scratch allocation failed");
+        if (hipMemcpy((void *)h.Lx, Lx, sizeof(double) * Lb, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy((void *)h.Ly, Ly, sizeof(double) * Lb, hipMemcpyHostToDevice) != hipSuccess)
+            return op_fail(IBLBM_EDEVICE, "ibm op: marker upload failed");
+        if (ud && hipMemcpy((void *)h.ud, ud, sizeof(double) * 2 * Lb, hipMemcpyHostToDevice) !=
+                      hipSuccess)
+            return op_fail(IBLBM_EDEVICE, "ibm op: desired velocity upload failed");
+        h.Lb = Lb;
         return 0;
     }
     int push() {
@@ -455,12 +369,13 @@ int iblbm_op_direct_forcing(const double *Lx, const double *Ly, const double *ud
     OpIbm op;
     int rc = op.setup(Lx, Ly, ud, Lb, dx, dy, dt, Larea, ny, nx, delta, inc);
     if (rc) return rc;
+    op.h.ibm = 1;
     rc = op.push();
     if (rc) return rc;
     const int64_t N = (int64_t)nx * ny;
     const double *dux = op.mem.up(Eux, N), *duy = op.mem.up(Euy, N), *dro = op.mem.up(Ro, N);
     NEED(dux && duy && dro);
-    hipLaunchKernelGGL(k_op_ibm, dim3(1), dim3(NT), 0, 0, op.d, 0, dux, duy, dro, (double *)nullptr);
+    hipLaunchKernelGGL(k_op_ibm, dim3(1), dim3(NTI), 0, 0, op.d, 0, dux, duy, dro, (double *)nullptr);
     OPCHK(hipGetLastError());
     OPCHK(hipDeviceSynchronize());
     rc = op.fib_out(fib);
@@ -481,6 +396,7 @@ int iblbm_op_multi_direct_forcing(const double *Lx, const double *Ly, const doub
     OpIbm op;
     int rc = op.setup(Lx, Ly, ud, Lb, dx, dy, dt, Larea, ny, nx, delta, 0);
     if (rc) return rc;
+    op.h.ibm = 2;
     op.h.n_iter = n_iter;
     op.h.min_iter = min_iter;
     op.h.omega = omega;
@@ -496,7 +412,7 @@ int iblbm_op_multi_direct_forcing(const double *Lx, const double *Ly, const doub
     const double *dux = op.mem.up(ux.data(), N), *duy = op.mem.up(uy.data(), N);
     const double *dro = op.mem.up(ro, N);
     NEED(dux && duy && dro);
-    hipLaunchKernelGGL(k_op_ibm, dim3(1), dim3(NT), 0, 0, op.d, 1, dux, duy, dro, (double *)nullptr);
+    hipLaunchKernelGGL(k_op_ibm, dim3(1), dim3(NTI), 0, 0, op.d, 1, dux, duy, dro, (double *)nullptr);
     OPCHK(hipGetLastError());
     OPCHK(hipDeviceSynchronize());
     rc = op.fib_out(fib_total);
@@ -520,9 +436,10 @@ int iblbm_op_dfc_lambda(const double *Lx, const double *Ly, int Lb, double rho_f
     OpIbm op;
     int rc = op.setup(Lx, Ly, nullptr, Lb, dx, dy, 1.0, Larea, ny, nx, delta, 0);
     if (rc) return rc;
+    op.h.ibm = 3;
     rc = op.push();
     if (rc) return rc;
-    hipLaunchKernelGGL(k_op_ibm, dim3(1), dim3(NT), 0, 0, op.d, 2, (const double *)nullptr,
+    hipLaunchKernelGGL(k_op_ibm, dim3(1), dim3(NTI), 0, 0, op.d, 2, (const double *)nullptr,
                        (const double *)nullptr, (const double *)nullptr, (double *)nullptr);
     OPCHK(hipGetLastError());
     OPCHK(hipMemcpy(lambda_k, op.h.s.lam, sizeof(double) * Lb, hipMemcpyDeviceToHost));
@@ -538,7 +455,8 @@ int iblbm_op_dfc_correction(const double *Lx, const double *Ly, const double *ud
     OpIbm op;
     int rc = op.setup(Lx, Ly, ud, Lb, dx, dy, 1.0, Larea, ny, nx, delta, 0);
     if (rc) return rc;
-    op.h.tol = scale;
+    op.h.ibm = 3;
+    op.h.dfc_scale = scale;
     rc = op.push();
     if (rc) return rc;
     const int64_t N = (int64_t)nx * ny;
@@ -547,7 +465,7 @@ int iblbm_op_dfc_correction(const double *Lx, const double *Ly, const double *ud
     // lambda: the caller's (the reference's lambda_cache) or recomputed (compute_lambda)
     if (lambda_k)
         OPCHK(hipMemcpy(op.h.s.lam, lambda_k, sizeof(double) * Lb, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_op_ibm, dim3(1), dim3(NT), 0, 0, op.d, lambda_k ? 4 : 3,
+    hipLaunchKernelGGL(k_op_ibm, dim3(1), dim3(NTI), 0, 0, op.d, lambda_k ? 4 : 3,
                        (const double *)nullptr, (const double *)nullptr, (const double *)nullptr,
                        dfs);
     OPCHK(hipGetLastError());

@@ -8,46 +8,56 @@
 // and skipped by the bulk tiles, so the two roles run concurrently and each node is read
 // once and written once per step.
 #pragma once
+#include "dev32.h"
 #include "lbm_math.h"
 
 namespace iblbm {
 
-constexpr int NT = 1024;  // threads per workgroup (16 wavefronts of 64)
+constexpr int NT_BULK = 256;  // bulk tile workgroup: 4 wavefronts, a 64 x 4 node tile
 constexpr int TILE_X = 64;
-constexpr int TILE_Y = NT / TILE_X;
+constexpr int TILE_Y = NT_BULK / TILE_X;
+constexpr int NTI = 512;      // immersed-boundary workgroup: 8 wavefronts
 
 struct IbmScratch {
-    // per box slot
-    double *fs;     // 9 planes of box_cap: post-stream/boundary (and DFC-corrected) fstar
-    double *ro, *ux0, *uy0;   // macroscopic rho, U from fstar
-    double *uwx, *uwy, *vs;   // MDF working velocity and dt/(2 rho)
-    double *fibx, *fiby;      // fib (DF) or fib_total (MDF); corrected U is written to ux0/uy0
-    double *wt;               // DFC W_total
-    int *cnt, *off, *cur;     // per-slot contribution count, offset (box_cap + 1), cursor
-    // per contribution c = s*Lb + k (s = stencil offset index, k = marker)
-    double *cw;    // wx*wy
-    int *cslot;    // clamped target slot
-    int *list;     // per-slot contribution ids sorted ascending
-    // per marker
+    // per box slot (box_cap)
+    double *fs;               // 9 planes: post-stream/boundary (and DFC-corrected) fstar
+    double *ro, *ux0, *uy0;   // macroscopic rho, U (velocity-corrected U after the step)
+    double *fibx, *fiby;      // fib of the step (0 on untouched nodes)
+    int *cnt, *cur;           // contributions per slot, fill cursor
+    int *tindex;              // slot -> touched index or -1 (box_cap + 1 entries)
+    // per touched node (<= box_cap)
+    int *tslot;               // touched index -> box slot
+    int *toff;                // gather-list offsets (box_cap + 1)
+    double *uwx, *uwy, *vs, *tfibx, *tfiby, *wt;  // global fallback of the LDS arrays, W_total
+    // per contribution, marker-major c_m = k*K + s (K = stencil points, lb_cap*25)
+    double *cw;               // wx*wy
+    int *cslot, *cti;         // box slot, touched index
+    // gather lists (same length), np.add.at order per touched node
+    int *gc, *gk;             // order key s*Lb + k, marker k
+    double *gw;               // wx*wy*Larea
+    // per marker (lb_cap)
     double *Lux, *Luy, *R, *Lfx, *Lfy, *lam, *q;  // q: DFC force (Lb, 2)
-    double *fi, *dfl;  // DFC interpolated f and delta f (Lb, 9)
+    double *dfl;              // DFC delta f (Lb, 9)
 };
 
 struct Inst {
     // lattice
     int nx, ny, pitch, pad0;
     int64_t qs;  // elements per direction plane
-    double *f[2];
+    double *f[2];  // float* when prec == 1 (FP32 deviations g = f - w)
     int bc, inc, tiles_x, tiles_y;
+    int prec, pad_p;
     CollParams cp;
+    CollParams32 cp32;
+    float gbc[9];      // feq_bc - w in FP32 (deviation-form boundary constants)
     double in_u, dens;
     double feq_bc[9];  // dirichlet: f_eq(dens, (in_u, 0)); settling corners: f_eq(dens, 0)
     // immersed boundary
     int ibm, delta, Lb, lb_cap;
     int bx0, by0, bw, bh;
     int box_cap, n_iter, min_iter, fixed_body;
-    int tables_valid, lambda_valid, pad1, pad2;
-    double dx, dy, Larea, omega, tol;
+    int tables_valid, lambda_valid, nT, pad2;
+    double dx, dy, Larea, omega, tol, dfc_scale;
     const double *Lx, *Ly, *ud;
     IbmScratch s;
     // outputs of the IBM workgroup
@@ -67,21 +77,44 @@ struct LaunchTable {
     int diag;                // take diagnostic reductions this step
 };
 
+// ---------------------------------------------------------------- storage traits
+// FP64 storage holds f itself; FP32 storage holds g = f - w.  ld() returns f in FP64 and st()
+// stores an FP64 f, so the immersed-boundary workgroup computes in FP64 for both.
+template <typename S>
+struct Store;
+template <>
+struct Store<double> {
+    __device__ static double ld(const double *__restrict__ A, int64_t idx, int) { return A[idx]; }
+    __device__ static double st(double f, int) { return f; }
+};
+template <>
+struct Store<float> {
+    __device__ static double ld(const float *__restrict__ A, int64_t idx, int q) {
+        return W(q) + (double)A[idx];
+    }
+    __device__ static float st(double f, int q) { return (float)(f - W(q)); }
+};
+
+template <typename S>
+__device__ inline S *buf(const Inst &I, int k) {
+    return reinterpret_cast<S *>(I.f[k]);
+}
+
 // ---------------------------------------------------------------- node-level helpers
 
-__device__ inline const double *src_buf(const Inst &I, int parity) { return I.f[parity]; }
-
 // Pull the 9 post-stream values of an interior node (all sources inside the grid).
-__device__ inline void pull_interior(const double *__restrict__ A, const Inst &I, int i, int j,
+template <typename S>
+__device__ inline void pull_interior(const S *__restrict__ A, const Inst &I, int i, int j,
                                      double *fs) {
 #pragma unroll
     for (int q = 0; q < 9; ++q)
-        fs[q] = A[q * I.qs + (int64_t)(j - EYi[q]) * I.pitch + (i - EXi[q])];
+        fs[q] = Store<S>::ld(A, q * I.qs + (int64_t)(j - EYi[q]) * I.pitch + (i - EXi[q]), q);
 }
 
 // Post-stream + boundary-closure populations of node (i, j): the value of state.fstar after
 // iblbm/runtime/step.py:49-56 (streaming.py push + the registered closure).
-__device__ inline void fstar_at(const double *__restrict__ A, const Inst &I, int i, int j,
+template <typename S>
+__device__ inline void fstar_at(const S *__restrict__ A, const Inst &I, int i, int j,
                                 double *fs) {
     const int nx = I.nx, ny = I.ny;
     const bool edge = (i == 0) | (j == 0) | (i == nx - 1) | (j == ny - 1);
@@ -112,77 +145,78 @@ __device__ inline void fstar_at(const double *__restrict__ A, const Inst &I, int
     for (int q = 0; q < 9; ++q) {
         const int si = i - EXi[q], sj = j - EYi[q];
         fs[q] = (si >= 0 && si < nx && sj >= 0 && sj < ny)
-                    ? A[q * I.qs + (int64_t)sj * I.pitch + si]
+                    ? Store<S>::ld(A, q * I.qs + (int64_t)sj * I.pitch + si, q)
                     : 0.0;
     }
     const int face = i == 0 ? 0 : (i == nx - 1 ? 1 : (j == ny - 1 ? 2 : 3));
     settling_face(face, fs);
 }
 
-__device__ inline void store9(double *__restrict__ B, const Inst &I, int i, int j,
-                              const double *f) {
+template <typename S>
+__device__ inline void store9(S *__restrict__ B, const Inst &I, int i, int j, const double *f) {
 #pragma unroll
-    for (int q = 0; q < 9; ++q) B[q * I.qs + (int64_t)j * I.pitch + i] = f[q];
+    for (int q = 0; q < 9; ++q) B[q * I.qs + (int64_t)j * I.pitch + i] = Store<S>::st(f[q], q);
+}
+
+// FP32-deviation post-stream + closure (same closures as fstar_at, on g = f - w).
+__device__ inline void fstar_dev32(const float *__restrict__ A, const Inst &I, int i, int j,
+                                   float *g) {
+    const int nx = I.nx, ny = I.ny;
+    const bool edge = (i == 0) | (j == 0) | (i == nx - 1) | (j == ny - 1);
+    if (!edge) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q)
+            g[q] = A[q * I.qs + (int64_t)(j - EYi[q]) * I.pitch + (i - EXi[q])];
+        return;
+    }
+    if (I.bc == 0) {
+        if (i == nx - 1) {
+            const int jj = j == 0 ? 1 : (j == ny - 1 ? ny - 2 : j);
+            float gin[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q)
+                gin[q] = A[q * I.qs + (int64_t)(jj - EYi[q]) * I.pitch + (nx - 2 - EXi[q])];
+            neumann_copy32(gin, g);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) g[q] = I.gbc[q];
+        }
+        return;
+    }
+    const bool corner = (i == 0 || i == nx - 1) && (j == 0 || j == ny - 1);
+    if (corner) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) g[q] = I.gbc[q];
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const int si = i - EXi[q], sj = j - EYi[q];
+        g[q] = (si >= 0 && si < nx && sj >= 0 && sj < ny) ? A[q * I.qs + (int64_t)sj * I.pitch + si]
+                                                          : 0.f;
+    }
+    settling_face32(i == 0 ? 0 : (i == nx - 1 ? 1 : (j == ny - 1 ? 2 : 3)), g);
 }
 
 // ---------------------------------------------------------------- block-wide helpers
-
-__device__ inline double block_sum_fixed(double v, double *red) {
-    // deterministic tree over the NT lanes (fixed order, independent of timing)
-    const int t = threadIdx.x;
-    red[t] = v;
-    __syncthreads();
-    for (int s = NT / 2; s > 0; s >>= 1) {
-        if (t < s) red[t] = red[t] + red[t + s];
-        __syncthreads();
-    }
-    const double r = red[0];
-    __syncthreads();
-    return r;
-}
 
 __device__ inline double nanmax2(double a, double b) {
     return (a != a || a > b) ? a : b;  // np.max semantics: NaN propagates
 }
 
-__device__ inline double block_max(double v, double *red) {
-    const int t = threadIdx.x;
-    red[t] = v;
+// Deterministic sum over the NT_BULK threads of a bulk workgroup (fixed shuffle tree).
+__device__ inline double bulk_block_sum(double v, double *red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) red[wv] = v;
     __syncthreads();
-    for (int s = NT / 2; s > 0; s >>= 1) {
-        if (t < s) red[t] = nanmax2(red[t], red[t + s]);
-        __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+        r = red[0];
+        for (int k = 1; k < NT_BULK / 64; ++k) r = r + red[k];
     }
-    const double r = red[0];
-    __syncthreads();
     return r;
-}
-
-// exclusive scan of cnt[0..n) into off[0..n], off[n] = total (block-wide, chunked)
-__device__ inline void block_scan(const int *cnt, int *off, int n, int *red) {
-    const int t = threadIdx.x;
-    const int chunk = (n + NT - 1) / NT;
-    const int b = t * chunk, e = min(n, b + chunk);
-    int s = 0;
-    for (int k = b; k < e; ++k) s += cnt[k];
-    red[t] = s;
-    __syncthreads();
-    if (t == 0) {
-        int acc = 0;
-        for (int k = 0; k < NT; ++k) {
-            const int v = red[k];
-            red[k] = acc;
-            acc += v;
-        }
-        off[n] = acc;
-    }
-    __syncthreads();
-    int acc = red[t];
-    for (int k = b; k < e; ++k) {
-        off[k] = acc;
-        acc += cnt[k];
-    }
-    __syncthreads();
 }
 
 }  // namespace iblbm

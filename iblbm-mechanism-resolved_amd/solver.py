@@ -96,6 +96,9 @@ class Context:
     def sync(self):
         check(self.lib.iblbm_sync(self.h), "sync")
 
+    def set_concurrent(self, on: bool):
+        check(self.lib.iblbm_set_concurrent(self.h, int(bool(on))), "set_concurrent")
+
     def mdf_stats(self, k):
         st = _abi.MdfStats()
         check(self.lib.iblbm_get_mdf_stats(self.h, k, ctypes.byref(st)), "mdf_stats")
@@ -159,16 +162,27 @@ def pre_collide(cfg, st):
     return gpu_kernels.collision_cm_mrt_gpu(st.feq0, st.feq0, st.U0, fib, st.S_diag, st.dt)
 
 
-class FixedCylinder:
-    """A fixed (or prescribed-marker) cylinder on the device, stepped in chunks."""
+PRECISIONS = {"fp64": _abi.FP64, "fp32dev": _abi.FP32_DEV}
 
-    def __init__(self, cfg, perturb=None, device=0, ctx=None):
+
+class FixedCylinder:
+    """A fixed cylinder on the device, stepped in chunks.
+
+    precision "fp64" reproduces the reference's FP64 arithmetic bit for bit; "fp32dev" stores
+    the populations as FP32 deviations from the weights and keeps the immersed-boundary stage
+    (interpolation, forces, MDF residual, reductions) in FP64.
+    """
+
+    def __init__(self, cfg, perturb=None, device=0, ctx=None, precision="fp64"):
         if cfg.motion_type is not None:
             raise ValueError("FixedCylinder runs motion_type=None; use the sedimentation driver")
+        if precision not in PRECISIONS:
+            raise KeyError(f"precision={precision!r} (expected one of {sorted(PRECISIONS)})")
         self.cfg = cfg
+        self.precision = precision
         self.st = initialize(cfg, perturb)
         self.ctx = ctx or Context(device)
-        case = make_case(cfg, self.st)
+        case = make_case(cfg, self.st, PRECISIONS[precision])
         self.omega = None
         if cfg.ibm_method == "MDF":
             self.omega = gpu_kernels.estimate_optimal_omega_from_type(
@@ -196,7 +210,7 @@ class FixedCylinder:
 
 
 def run(cfg, verbose: bool = True, callback=None, save_dir=None, perturb=None,
-        device: int = 0, graph: bool = True) -> dict:
+        device: int = 0, graph: bool = True, precision: str = "fp64") -> dict:
     """``iblbm.run`` for the fixed-cylinder scenario on the fused GPU step.
 
     Returns the reference's result keys: Cd_history, Cl_history, converged, final_step,
@@ -208,7 +222,7 @@ def run(cfg, verbose: bool = True, callback=None, save_dir=None, perturb=None,
                          "iblbm_amd.sedimentation.run")
     if cfg.stream_first:
         raise ValueError("stream_first=True is outside the GPU hot path")
-    sim = FixedCylinder(cfg, perturb=perturb, device=device)
+    sim = FixedCylinder(cfg, perturb=perturb, device=device, precision=precision)
     ci = int(cfg.check_interval)
     Cd_hist, Cl_hist, mdf_hist = [], [], []
     converged = False

@@ -131,6 +131,9 @@ int iblbm_sync(void *ctx);
 
 /* Captures nsteps steps into a hipGraph and replays it (the step counter advances). */
 int iblbm_graph_run(void *ctx, int nsteps, int diag_every);
+/* on = 1 (default): the marker-window workgroups run on a second stream concurrently with
+ * the bulk tiles of the same step; on = 0: both kernels run in order on one stream. */
+int iblbm_set_concurrent(void *ctx, int on);
 
 int iblbm_get_mdf_stats(void *ctx, int id, iblbm_mdf_stats *out);
 int iblbm_get_diag(void *ctx, int id, iblbm_diag *out);

@@ -346,6 +346,82 @@ def case_run_mdf_bgk_hat():
         use_convergence=False), record_mdf=True)
 
 
+def _run_sed(name, cfg_kwargs):
+    """Sedimentation run: final fields, per-record particle states, per-step forces."""
+    import iblbm
+    from iblbm.runtime import step as step_mod
+
+    cfg = iblbm.SimConfig(**cfg_kwargs)
+    steps = []
+
+    def cb(step, Cd, Cl, error, state, elapsed, converged):
+        if state.particles is not None:
+            row = [step]
+            for p in state.particles:
+                row += [float(p.pos[0]), float(p.pos[1]), float(p.vel[0]), float(p.vel[1]),
+                        float(p.omega), float(p.angle), float(p.force[0]), float(p.force[1]),
+                        float(p.torque)]
+        else:
+            row = [step, float(state.particle_pos[0]), float(state.particle_pos[1]),
+                   float(state.particle_vel[0]), float(state.particle_vel[1]),
+                   float(state.particle_omega), float(state.particle_angle),
+                   float(state.particle_force[0]), float(state.particle_force[1]),
+                   float(state.particle_torque)]
+            st = getattr(state, "_mdf_iter_stats", None)
+            if st is not None:
+                row += [st["iter_count"], st["last_residual"]]
+        steps.append(row)
+
+    t0 = time.time()
+    res = iblbm.run(cfg, verbose=False, callback=cb)
+    st = res["state"]
+    f_post = step_mod.collide(st, cfg)
+    A = {"fstar": st.fstar, "f_post": f_post, "ro": st.ro, "U": st.U, "fib": st.fib,
+         "particles": np.array(steps, dtype=np.float64)}
+    if st.particles is not None:
+        A["Lx"] = np.concatenate([np.asarray(p.Lx) for p in st.particles])
+        A["Ly"] = np.concatenate([np.asarray(p.Ly) for p in st.particles])
+    else:
+        A["Lx"], A["Ly"] = np.asarray(st.Lx), np.asarray(st.Ly)
+    hist = res["sedimentation_history"]
+    S = {"cfg": cfg_kwargs, "nx": st.nx, "ny": st.ny, "tau": st.tau, "Larea": float(st.Larea),
+         "Lb": int(st.Lb), "lattice_r": float(st.lattice_r), "final_step": res["final_step"],
+         "omega": getattr(st, "_mdf_omega", None), "termination_reason":
+         res.get("termination_reason"), "history": hist, "gen_seconds": time.time() - t0}
+    _save(name, A, S)
+
+
+def case_sed_single_mdf_cm_p4():
+    _run_sed("sed_single_mdf_cm_p4", dict(
+        Re=38.4, NN=129, xmax=1.0, ymax=1.5, cylinder_center=(0.5, 1.0), cylinder_D_ratio=0.25,
+        lattice_u=0.02, inflow_u=0.0, bc_type="settling_channel", delta_type="peskin4pt",
+        ibm_method="MDF", collision_model="CM_MRT", mdf_iterations=20, mdf_min_iterations=5,
+        mdf_tolerance=1e-4, marker_spacing_factor=1.0, max_steps=40, check_interval=1,
+        use_convergence=False, motion_type="sedimentation", rho_ratio=1.5, gravity=2.7e-5,
+        enable_rotation=True, rotation_coupling="indirect",
+        settling_inertia_model="explicit_history", time_integrator="verlet"))
+
+
+def case_sed_single_df_bgk_hat():
+    _run_sed("sed_single_df_bgk_hat", dict(
+        Re=2.0, NN=25, xmax=1.0, ymax=3.0, cylinder_center=(0.48, 2.2), cylinder_D_ratio=0.25,
+        lattice_u=0.02, inflow_u=0.0, bc_type="settling_channel", delta_type="hat",
+        ibm_method="DF", collision_model="BGK", marker_spacing_factor=1.0, max_steps=150,
+        check_interval=1, use_convergence=False, motion_type="sedimentation", rho_ratio=1.5,
+        gravity=2e-4, enable_rotation=True, settling_inertia_model="explicit_history"))
+
+
+def case_sed_multi_dfc_cm_hat():
+    _run_sed("sed_multi_dfc_cm_hat", dict(
+        Re=2.0, NN=25, xmax=1.0, ymax=4.0, cylinder_center=(0.5, 3.0), cylinder_D_ratio=0.25,
+        lattice_u=0.02, inflow_u=0.0, bc_type="settling_channel", delta_type="hat",
+        ibm_method="DFC", collision_model="CM_MRT", marker_spacing_factor=1.0, max_steps=150,
+        check_interval=1, use_convergence=False, motion_type="sedimentation", rho_ratio=1.5,
+        gravity=2e-4, enable_rotation=True, settling_inertia_model="explicit_history",
+        particles_config=[{"center": (0.5, 3.0), "rho_ratio": 1.5},
+                          {"center": (0.53, 2.55), "rho_ratio": 1.5}]))
+
+
 CASES = {k[5:]: v for k, v in dict(globals()).items() if k.startswith("case_")}
 
 if __name__ == "__main__":

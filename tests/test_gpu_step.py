@@ -42,3 +42,17 @@ def test_fused_step_matches_reference_run(case, graph):
     if "dfc_force" in g:
         np.testing.assert_array_equal(sim.ctx.download_dfc_force(sim.k, sc["Lb"]),
                                       g["dfc_force"])
+
+
+@pytest.mark.parametrize("case", ["run_mdf_trt_p4", "run_dfc_cm_hat", "run_df_bgk_hat"])
+def test_one_stream_and_two_streams_agree(case):
+    g, sc = golden(case)
+    cfg = golden_cfg(sc)
+    outs = []
+    for concurrent in (True, False):
+        sim = FixedCylinder(cfg, perturb=perturb_of(sc))
+        sim.ctx.set_concurrent(concurrent)
+        sim.advance(cfg.max_steps, diag_every=cfg.check_interval, graph=concurrent)
+        outs.append(sim.ctx.download_f(sim.k))
+    np.testing.assert_array_equal(outs[0], outs[1])
+    np.testing.assert_array_equal(outs[0], g["f_post"])
