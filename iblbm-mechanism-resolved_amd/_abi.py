@@ -78,6 +78,7 @@ SIGNATURES = {
     "iblbm_download_fib": (_i, [_VP, _i, _D]),
     "iblbm_download_state": (_i, [_VP, _i, _D, _D, _D]),
     "iblbm_download_dfc_force": (_i, [_VP, _i, _D]),
+    "iblbm_download_window": (_i, [_VP, _i, _D, _D, _I]),
     "iblbm_step_counter": (_l, [_VP]),
     "iblbm_last_timing": (_i, [_VP, _D, _I]),
     "iblbm_omega_estimate": (_d, [_D, _D, _i, _i, _d, _d]),

@@ -141,23 +141,7 @@ __device__ inline void neumann_copy32(const float *gin, float *gout) {
 
 __device__ inline void settling_face32(int face, float *f) {
     // identical to the FP64 closure: w is symmetric under every swap it performs
-    if (face == 0) {
-        f[1] = f[3];
-        f[5] = f[7] - 0.5f * (f[2] - f[4]);
-        f[8] = f[6] + 0.5f * (f[2] - f[4]);
-    } else if (face == 1) {
-        f[3] = f[1];
-        f[7] = f[5] + 0.5f * (f[2] - f[4]);
-        f[6] = f[8] - 0.5f * (f[2] - f[4]);
-    } else if (face == 2) {
-        f[4] = f[2];
-        f[7] = f[5] + 0.5f * (f[1] - f[3]);
-        f[8] = f[6] - 0.5f * (f[1] - f[3]);
-    } else {
-        f[2] = f[4];
-        f[5] = f[7] - 0.5f * (f[1] - f[3]);
-        f[6] = f[8] + 0.5f * (f[1] - f[3]);
-    }
+    settling_face_t(face, f);
 }
 
 }  // namespace iblbm

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
@@ -70,7 +71,7 @@ __device__ inline double bulk_node_fp32(const Inst &I, int parity, int i, int j)
     return 1.0 + (double)drho;
 }
 
-// Bulk tiles of one time step: every node outside the instance's marker window.
+// Bulk tiles of one time step: every node that no marker stencil touches.
 template <typename S, int MODEL>
 __global__ __launch_bounds__(NT_BULK) void k_bulk(LaunchTable T) {
     __shared__ double red[NT_BULK / 64];
@@ -84,9 +85,11 @@ __global__ __launch_bounds__(NT_BULK) void k_bulk(LaunchTable T) {
     const int j = ty * TILE_Y + (int)(threadIdx.x / TILE_X);
     double rho = 0.0;
     const bool in_grid = i < I.nx && j < I.ny;
-    const bool in_box = I.ibm != 0 && i >= I.bx0 && i < I.bx0 + I.bw && j >= I.by0 &&
-                        j < I.by0 + I.bh;
-    if (in_grid && !in_box) {
+    // nodes a marker stencil touches belong to the window workgroup
+    const bool touched = I.ibm != 0 && i >= I.bx0 && i < I.bx0 + I.bw && j >= I.by0 &&
+                         j < I.by0 + I.bh &&
+                         I.s.tindex[(j - I.by0) * I.bw + (i - I.bx0)] >= 0;
+    if (in_grid && !touched) {
         if constexpr (sizeof(S) == 8)
             rho = bulk_node_fp64<MODEL>(I, T.parity, i, j);
         else
@@ -102,13 +105,18 @@ __global__ __launch_bounds__(NT_BULK) void k_bulk(LaunchTable T) {
 template <typename S, int MODEL>
 __global__ __launch_bounds__(NTI) void k_ibm(LaunchTable T) {
     __shared__ double red[NTI];
-    __shared__ int redi[NTI];
     Inst &I = T.insts[T.ibm_inst[blockIdx.x]];
-    ibm_role<S, MODEL>(I, T.parity, T.diag, red, redi);
+    ibm_role<S, MODEL>(I, T.parity, T.diag, red);
 }
 
-// state.fstar / ro / U of the last step (source buffer A = that step's input); box nodes come
-// from the IBM scratch (DFC-corrected fstar, velocity-corrected U).
+// Window tables of one instance for newly set markers (one workgroup).
+__global__ __launch_bounds__(NTI) void k_tables(Inst *Ip) {
+    __shared__ int redi[NTI];
+    ibm_tables_role(*Ip, redi);
+}
+
+// state.fstar / ro / U of the last step (source buffer A = that step's input); touched nodes
+// come from the IBM scratch (DFC-corrected fstar, velocity-corrected U).
 template <typename S>
 __global__ __launch_bounds__(256) void k_state(const Inst *Ip, int parity, double *fstar,
                                                double *ro, double *U) {
@@ -119,8 +127,8 @@ __global__ __launch_bounds__(256) void k_state(const Inst *Ip, int parity, doubl
     double fs[9], r, u0, u1;
     const bool in_box = I.ibm != 0 && i >= I.bx0 && i < I.bx0 + I.bw && j >= I.by0 &&
                         j < I.by0 + I.bh;
-    if (in_box) {
-        const int sl = (j - I.by0) * I.bw + (i - I.bx0);
+    const int sl = in_box ? (j - I.by0) * I.bw + (i - I.bx0) : 0;
+    if (in_box && I.s.tindex[sl] >= 0) {
         for (int q = 0; q < 9; ++q) fs[q] = I.s.fs[q * I.box_cap + sl];
         r = I.s.ro[sl];
         u0 = I.s.ux0[sl];
@@ -258,8 +266,7 @@ int ensure_tables(Ctx &C) {
 }
 
 int upload_inst(Ctx &C, int id) {
-    // replaces the device descriptor (the IBM workgroup's cached-table flags are reset by the
-    // shadow, which forces a rebuild on the next step)
+    // replaces the device descriptor with the host shadow
     HIPCHK(hipMemcpyAsync(C.d_insts + id, &C.inst[id].d, sizeof(Inst), hipMemcpyHostToDevice,
                           C.stream));
     return 0;
@@ -549,11 +556,8 @@ int iblbm_set_markers(void *p, int id, const double *Lx, const double *Ly, const
     d.by0 = by0;
     d.bw = bw;
     d.bh = bh;
-    d.tables_valid = 0;
-    d.lambda_valid = 0;
     h.Lb = Lb;
     h.Larea = Larea;
-    // the host copies are pageable: make sure they have been consumed before returning
     if (!was_active || C->tables_dirty) {
         C->tables_dirty = true;
         int rc = ensure_tables(*C);
@@ -562,6 +566,13 @@ int iblbm_set_markers(void *p, int id, const double *Lx, const double *Ly, const
         int rc = upload_inst(*C, id);
         if (rc) return rc;
     }
+    // window tables, DFC lambda and a zeroed box force, ordered after every queued step
+    hipLaunchKernelGGL(k_tables, dim3(1), dim3(NTI), 0, C->stream, C->d_insts + id);
+    HIPCHK(hipGetLastError());
+    // the touched-node count goes back into the host descriptor, so later uploads keep it;
+    // the synchronisation also retires the pageable marker copies before returning
+    HIPCHK(hipMemcpyAsync(&d.nT, (const char *)(C->d_insts + id) + offsetof(Inst, nT),
+                          sizeof(int), hipMemcpyDeviceToHost, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
     return 0;
 }
@@ -690,6 +701,24 @@ int iblbm_download_dfc_force(void *p, int id, double *q) {
     return 0;
 }
 
+int iblbm_download_window(void *p, int id, double *fibx, double *fiby, int box[4]) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || !fibx || !fiby || !box)
+        return fail(IBLBM_EINVAL, "download_window: bad argument");
+    const Inst &d = C->inst[id].d;
+    if (d.ibm == 0) return fail(IBLBM_EINVAL, "download_window: instance has no markers");
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    const int nb = d.bw * d.bh;
+    HIPCHK(hipMemcpy(fibx, d.s.fibx, sizeof(double) * nb, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(fiby, d.s.fiby, sizeof(double) * nb, hipMemcpyDeviceToHost));
+    box[0] = d.bx0;
+    box[1] = d.by0;
+    box[2] = d.bw;
+    box[3] = d.bh;
+    return 0;
+}
+
 int iblbm_step(void *p, int nsteps, int diag_every) {
     Ctx *C = as_ctx(p);
     if (!C || nsteps < 0) return fail(IBLBM_EINVAL, "step: bad argument");
@@ -807,7 +836,7 @@ int iblbm_get_diag(void *p, int id, iblbm_diag *out) {
     double box[5] = {0, 0, 0, 0, 0};
     if (d.ibm != 0) HIPCHK(hipMemcpy(box, d.diag, sizeof box, hipMemcpyDeviceToHost));
     double rs = 0.0;
-    const int b0 = C->n_ibm + C->h_tile_prefix[id], b1 = C->n_ibm + C->h_tile_prefix[id + 1];
+    const int b0 = C->h_tile_prefix[id], b1 = C->h_tile_prefix[id + 1];  // bulk tile partials
     for (int b = b0; b < b1; ++b) rs += part[b];
     rs += box[0];
     out->rho_sum = rs;

@@ -1,14 +1,17 @@
 // ibm_block.h -- the immersed-boundary workgroup of the fused step (one per instance).
 //
-// It owns the window box: it streams and closes the box nodes itself, computes rho and U,
-// runs DF (iblbm/ibm/df.py:94-154), MDF (iblbm/ibm/mdf.py:175-229) or DFC
-// (iblbm/ibm/dfc.py:256-342) on them, applies the velocity correction
-// (iblbm/ibm/dispatch.py:18-33), and finishes the collision of the box nodes.
+// It owns the nodes some clamped marker stencil lands on ("touched" nodes; the bulk tiles
+// skip exactly those): it streams and closes them, computes rho and U, runs DF
+// (iblbm/ibm/df.py:94-154), MDF (iblbm/ibm/mdf.py:175-229) or DFC (iblbm/ibm/dfc.py:256-342)
+// on them, applies the velocity correction (iblbm/ibm/dispatch.py:18-33) and finishes their
+// collision.  Every other node of the window box carries fib = 0 exactly and is collided by
+// the bulk tiles with the same bits.
 //
-// Only "touched" box nodes (those some clamped stencil point lands on) carry a force; their
-// working arrays (MDF velocity, dt/(2 rho), accumulated fib) and the marker forces live in
-// LDS, so an MDF sub-step is two short latency chains (marker interpolation, node gather)
-// and one wavefront-shuffle reduction.
+// The tables (contributions, touched compaction, gather lists, DFC lambda) are built once per
+// marker set by k_tables.  During a step the touched-node working arrays (MDF velocity,
+// dt/(2 rho), accumulated fib), the marker forces and the gather lists live in LDS, and each
+// marker's stencil (touched index, weight) lives in registers, so an MDF sub-step is two
+// short latency chains (marker interpolation, node gather) and one wavefront-shuffle reduction.
 //
 // Spreading is a gather: every touched node owns the list of (stencil offset s, marker k)
 // contributions that land on it, sorted by c = s*Lb + k, which is exactly the order in which
@@ -19,12 +22,16 @@
 
 namespace iblbm {
 
-constexpr int MAX_T = 2048;   // touched nodes held in LDS (beyond: global scratch)
-constexpr int MAX_LB = 1024;  // markers held in LDS (beyond: global scratch)
+constexpr int MAX_T = 1536;   // touched nodes held in LDS (beyond: global scratch)
+constexpr int MAX_LB = 512;   // markers held in LDS (beyond: global scratch)
+constexpr int MAX_G = 4096;   // gather-list entries held in LDS (beyond: global scratch)
 
 struct IbmShared {
     double uwx[MAX_T], uwy[MAX_T], vs[MAX_T], fibx[MAX_T], fiby[MAX_T];
     double Lfx[MAX_LB], Lfy[MAX_LB];
+    double gw[MAX_G];
+    int gk[MAX_G];
+    int toff[MAX_T + 1];
 };
 
 __device__ inline int64_t floor_idx(double v) { return (int64_t)floor(v); }
@@ -220,7 +227,8 @@ __device__ void ibm_dfc_lambda(Inst &I, int nT, int Lb) {
 
 // Working-array pointers: LDS when the window fits, global scratch otherwise.
 struct IbmPtrs {
-    double *uwx, *uwy, *vs, *fbx, *fby, *Lfx, *Lfy;
+    double *uwx, *uwy, *vs, *fbx, *fby, *Lfx, *Lfy, *gw;
+    int *gk, *toff;
 };
 __device__ inline IbmPtrs ibm_ptrs(Inst &I, int nT, IbmShared &sh) {
     IbmScratch &S = I.s;
@@ -233,12 +241,61 @@ __device__ inline IbmPtrs ibm_ptrs(Inst &I, int nT, IbmShared &sh) {
     p.fby = lt ? sh.fiby : S.tfiby;
     p.Lfx = lm ? sh.Lfx : S.Lfx;
     p.Lfy = lm ? sh.Lfy : S.Lfy;
+    p.gk = S.gk;
+    p.gw = S.gw;
+    p.toff = S.toff;
     return p;
 }
 
-// DF / MDF / DFC on the window given S.ro, S.ux0, S.uy0 (and S.fs for DFC) of the box.
-// DF/MDF leave fib of the touched nodes in P.fbx/P.fby; DFC corrects S.fs and recomputes
-// S.ro, S.ux0, S.uy0 of the touched nodes and leaves the marker forces in S.q.
+// Copies the gather lists of the window into LDS when they fit (one pass per step).
+__device__ inline void ibm_stage_lists(Inst &I, int nT, IbmShared &sh, IbmPtrs &P) {
+    IbmScratch &S = I.s;
+    const int t = threadIdx.x;
+    const int ng = S.toff[nT];
+    if (nT <= MAX_T && ng <= MAX_G) {
+        for (int x = t; x <= nT; x += NTI) sh.toff[x] = S.toff[x];
+        for (int x = t; x < ng; x += NTI) {
+            sh.gk[x] = S.gk[x];
+            sh.gw[x] = S.gw[x];
+        }
+        P.gk = sh.gk;
+        P.gw = sh.gw;
+        P.toff = sh.toff;
+    }
+}
+
+// Spread of one touched node as a gather in np.add.at order: sum_x Lf[k_x] * w_x.
+__device__ inline void gather2(const IbmPtrs &P, int ti, double &ex, double &ey) {
+    int x = P.toff[ti];
+    const int e = P.toff[ti + 1];
+    double sx = 0.0, sy = 0.0;
+    for (; x + 4 <= e; x += 4) {
+        const int k0 = P.gk[x], k1 = P.gk[x + 1], k2 = P.gk[x + 2], k3 = P.gk[x + 3];
+        const double w0 = P.gw[x], w1 = P.gw[x + 1], w2 = P.gw[x + 2], w3 = P.gw[x + 3];
+        const double a0 = P.Lfx[k0], a1 = P.Lfx[k1], a2 = P.Lfx[k2], a3 = P.Lfx[k3];
+        const double b0 = P.Lfy[k0], b1 = P.Lfy[k1], b2 = P.Lfy[k2], b3 = P.Lfy[k3];
+        sx += a0 * w0;
+        sy += b0 * w0;
+        sx += a1 * w1;
+        sy += b1 * w1;
+        sx += a2 * w2;
+        sy += b2 * w2;
+        sx += a3 * w3;
+        sy += b3 * w3;
+    }
+    for (; x < e; ++x) {
+        const int k = P.gk[x];
+        const double w = P.gw[x];
+        sx += P.Lfx[k] * w;
+        sy += P.Lfy[k] * w;
+    }
+    ex = sx;
+    ey = sy;
+}
+
+// DF / MDF / DFC on the window, given S.ro, S.ux0, S.uy0 (and S.fs for DFC) of the touched
+// nodes.  DF/MDF leave fib of the touched nodes in P.fbx/P.fby; DFC corrects S.fs and
+// recomputes S.ro, S.ux0, S.uy0 of the touched nodes and leaves the marker forces in S.q.
 template <int A>
 __device__ void ibm_core(Inst &I, int nT, const IbmPtrs &P, double *red, bool store_lu) {
     constexpr int K = (2 * A + 1) * (2 * A + 1);
@@ -257,12 +314,31 @@ __device__ void ibm_core(Inst &I, int nT, const IbmPtrs &P, double *red, bool st
     __threadfence_block();
     __syncthreads();
     if (I.ibm == 1 || I.ibm == 2) {
-        // R = interpolated rho, constant over the MDF sub-steps
-        for (int k = t; k < Lb; k += NTI) {
-            double R = 0.0;
+        // the marker's stencil (touched index, weight) stays in registers over the sub-steps
+        // when every marker has its own thread
+        const bool regs = Lb <= NTI;
+        int rti[K];
+        double rw[K];
+        double R = 0.0;
+        if (regs && t < Lb) {
 #pragma unroll
-            for (int s = 0; s < K; ++s) R += S.ro[S.tslot[S.cti[k * K + s]]] * S.cw[k * K + s];
-            S.R[k] = R;
+            for (int s = 0; s < K; ++s) {
+                rti[s] = S.cti[t * K + s];
+                rw[s] = S.cw[t * K + s];
+            }
+#pragma unroll
+            for (int s = 0; s < K; ++s) R += S.ro[S.tslot[rti[s]]] * rw[s];  // rho at the marker
+        }
+        if (!regs) {
+            for (int k = t; k < Lb; k += NTI) {
+                double r = 0.0;
+#pragma unroll
+                for (int s = 0; s < K; ++s)
+                    r += S.ro[S.tslot[S.cti[k * K + s]]] * S.cw[k * K + s];
+                S.R[k] = r;
+            }
+            __threadfence_block();
+            __syncthreads();
         }
         const bool mdf = I.ibm == 2;
         const bool finc = mdf ? false : inc;  // MDF sub-steps are compressible (mdf.py:177-181)
@@ -275,27 +351,31 @@ __device__ void ibm_core(Inst &I, int nT, const IbmPtrs &P, double *red, bool st
             // interpolation + force per marker (df.py:239-263)
             double rmax = -INFINITY;
             for (int k = t; k < Lb; k += NTI) {
-                int ti[K];
-                double w[K];
+                double lx = 0.0, ly = 0.0, Rk;
+                if (regs) {
 #pragma unroll
-                for (int s = 0; s < K; ++s) {
-                    ti[s] = S.cti[k * K + s];
-                    w[s] = S.cw[k * K + s];
-                }
-                double lx = 0.0, ly = 0.0;
+                    for (int s = 0; s < K; ++s) {
+                        lx += P.uwx[rti[s]] * rw[s];
+                        ly += P.uwy[rti[s]] * rw[s];
+                    }
+                    Rk = R;
+                } else {
 #pragma unroll
-                for (int s = 0; s < K; ++s) {
-                    lx += P.uwx[ti[s]] * w[s];
-                    ly += P.uwy[ti[s]] * w[s];
+                    for (int s = 0; s < K; ++s) {
+                        const int ti = S.cti[k * K + s];
+                        const double w = S.cw[k * K + s];
+                        lx += P.uwx[ti] * w;
+                        ly += P.uwy[ti] * w;
+                    }
+                    Rk = S.R[k];
                 }
-                const double R = S.R[k];
                 const double udx = I.ud[2 * k], udy = I.ud[2 * k + 1];
                 if (finc) {
                     P.Lfx[k] = 2.0 * (udx - lx) / dt;
                     P.Lfy[k] = 2.0 * (udy - ly) / dt;
                 } else {
-                    P.Lfx[k] = 2.0 * R * (udx - lx) / dt;
-                    P.Lfy[k] = 2.0 * R * (udy - ly) / dt;
+                    P.Lfx[k] = 2.0 * Rk * (udx - lx) / dt;
+                    P.Lfy[k] = 2.0 * Rk * (udy - ly) / dt;
                 }
                 if (store_lu) {
                     S.Lux[k] = lx;
@@ -316,13 +396,8 @@ __device__ void ibm_core(Inst &I, int nT, const IbmPtrs &P, double *red, bool st
             }
             // spread as a gather in np.add.at order (df.py:269-283)
             for (int ti = t; ti < nT; ti += NTI) {
-                double ex = 0.0, ey = 0.0;
-                for (int x = S.toff[ti], e = S.toff[ti + 1]; x < e; ++x) {
-                    const int k = S.gk[x];
-                    const double wl = S.gw[x];
-                    ex += P.Lfx[k] * wl;
-                    ey += P.Lfy[k] * wl;
-                }
+                double ex, ey;
+                gather2(P, ti, ex, ey);
                 if (mdf) {  // mdf.py:197-200
                     const double fx = ex * omega, fy = ey * omega;
                     P.fbx[ti] = P.fbx[ti] + fx;
@@ -385,9 +460,9 @@ __device__ void ibm_core(Inst &I, int nT, const IbmPtrs &P, double *red, bool st
             double e[9];
 #pragma unroll
             for (int q = 0; q < 9; ++q) e[q] = 0.0;
-            for (int x = S.toff[ti], end = S.toff[ti + 1]; x < end; ++x) {
-                const int k = S.gk[x];
-                const double wl = S.gw[x];
+            for (int x = P.toff[ti], end = P.toff[ti + 1]; x < end; ++x) {
+                const int k = P.gk[x];
+                const double wl = P.gw[x];
 #pragma unroll
                 for (int q = 0; q < 9; ++q) e[q] += S.dfl[9 * k + q] * wl;
             }
@@ -410,31 +485,24 @@ __device__ void ibm_core(Inst &I, int nT, const IbmPtrs &P, double *red, bool st
 
 // ------------------------------------------------------------------ the workgroup
 
+// Window workgroup of one step.  It owns the touched nodes only (the bulk tiles skip exactly
+// those): stream/close them, moments, the IBM stage, velocity correction, forced collision.
+// The tables it reads were built by k_tables when the markers were set.
 template <typename ST, int MODEL, int A>
-__device__ void ibm_role_impl(Inst &I, int parity, int diag, double *red, int *redi,
-                              IbmShared &sh) {
+__device__ void ibm_role_impl(Inst &I, int parity, int diag, double *red, IbmShared &sh) {
     const int t = threadIdx.x;
     const ST *__restrict__ Asrc = buf<ST>(I, parity);
     ST *__restrict__ B = buf<ST>(I, parity ^ 1);
     IbmScratch &S = I.s;
-    const int nbox = I.bw * I.bh;
     const bool inc = I.inc != 0;
     const double dt = I.cp.dt;
-    const int Lb = I.Lb;
+    const int Lb = I.Lb, nT = I.nT;
+    IbmPtrs P = ibm_ptrs(I, nT, sh);
+    ibm_stage_lists(I, nT, sh, P);
 
-    const bool build = !I.tables_valid;
-    if (build) ibm_build_tables<A>(I, redi);
-    const int nT = build ? S.tindex[nbox] : I.nT;
-    if (I.ibm == 3 && !I.lambda_valid) ibm_dfc_lambda<A>(I, nT, Lb);
-    __syncthreads();
-    if (t == 0) {
-        if (build) I.tables_valid = I.fixed_body;
-        if (I.ibm == 3) I.lambda_valid = I.fixed_body;
-    }
-    const IbmPtrs P = ibm_ptrs(I, nT, sh);
-
-    // P1: stream + close the box nodes, macroscopic moments
-    for (int sl = t; sl < nbox; sl += NTI) {
+    // P1: stream + close the touched nodes, macroscopic moments
+    for (int ti = t; ti < nT; ti += NTI) {
+        const int sl = S.tslot[ti];
         const int i = I.bx0 + sl % I.bw, j = I.by0 + sl / I.bw;
         double fs[9];
         fstar_at(Asrc, I, i, j, fs);
@@ -448,20 +516,23 @@ __device__ void ibm_role_impl(Inst &I, int parity, int diag, double *red, int *r
     }
     __threadfence_block();
     __syncthreads();
-    ibm_core<A>(I, nT, P, red, false);
+    if (A == 1)
+        ibm_core<1>(I, nT, P, red, false);
+    else
+        ibm_core<2>(I, nT, P, red, false);
 
     // P3: velocity correction (DF/MDF), equilibrium, collision, store
     const bool correct = I.ibm == 1 || I.ibm == 2;
     double rho_acc = 0.0, fxs = 0.0, fys = 0.0, trq = 0.0;
-    for (int sl = t; sl < nbox; sl += NTI) {
+    for (int ti = t; ti < nT; ti += NTI) {
+        const int sl = S.tslot[ti];
         const int i = I.bx0 + sl % I.bw, j = I.by0 + sl / I.bw;
-        const int ti = S.tindex[sl];
         double fs[9];
 #pragma unroll
         for (int q = 0; q < 9; ++q) fs[q] = S.fs[q * I.box_cap + sl];
         const double r = S.ro[sl];
-        const double fx = (correct && ti >= 0) ? P.fbx[ti] : 0.0;
-        const double fy = (correct && ti >= 0) ? P.fby[ti] : 0.0;
+        const double fx = correct ? P.fbx[ti] : 0.0;
+        const double fy = correct ? P.fby[ti] : 0.0;
         double u0 = S.ux0[sl], u1 = S.uy0[sl];
         if (correct) {
             if (inc) {
@@ -522,12 +593,27 @@ __device__ void ibm_role_impl(Inst &I, int parity, int diag, double *red, int *r
 }
 
 template <typename ST, int MODEL>
-__device__ void ibm_role(Inst &I, int parity, int diag, double *red, int *redi) {
+__device__ void ibm_role(Inst &I, int parity, int diag, double *red) {
     __shared__ IbmShared sh;
     if (I.delta == 1)
-        ibm_role_impl<ST, MODEL, 1>(I, parity, diag, red, redi, sh);
+        ibm_role_impl<ST, MODEL, 1>(I, parity, diag, red, sh);
     else
-        ibm_role_impl<ST, MODEL, 2>(I, parity, diag, red, redi, sh);
+        ibm_role_impl<ST, MODEL, 2>(I, parity, diag, red, sh);
+}
+
+// Window tables (and the DFC lambda) for new markers; zeroes the box fib.
+__device__ inline void ibm_tables_role(Inst &I, int *redi) {
+    if (I.delta == 1) {
+        ibm_build_tables<1>(I, redi);
+        if (I.ibm == 3) ibm_dfc_lambda<1>(I, I.s.tindex[I.bw * I.bh], I.Lb);
+    } else {
+        ibm_build_tables<2>(I, redi);
+        if (I.ibm == 3) ibm_dfc_lambda<2>(I, I.s.tindex[I.bw * I.bh], I.Lb);
+    }
+    for (int sl = threadIdx.x; sl < I.bw * I.bh; sl += NTI) {
+        I.s.fibx[sl] = 0.0;
+        I.s.fiby[sl] = 0.0;
+    }
 }
 
 }  // namespace iblbm

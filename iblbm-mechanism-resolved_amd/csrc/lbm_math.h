@@ -78,26 +78,24 @@ __host__ __device__ inline void neumann_copy(const double *fin, double *fout) {
 }
 
 // iblbm/boundary/settling_channel.py:65-127: Zou-He no-slip closure of one face node.
-// face: 0 left, 1 right, 2 top, 3 bottom.
-__host__ __device__ inline void settling_face(int face, double *f) {
-    if (face == 0) {
-        f[1] = f[3];
-        f[5] = f[7] - 0.5 * (f[2] - f[4]);
-        f[8] = f[6] + 0.5 * (f[2] - f[4]);
-    } else if (face == 1) {
-        f[3] = f[1];
-        f[7] = f[5] + 0.5 * (f[2] - f[4]);
-        f[6] = f[8] - 0.5 * (f[2] - f[4]);
-    } else if (face == 2) {
-        f[4] = f[2];
-        f[7] = f[5] + 0.5 * (f[1] - f[3]);
-        f[8] = f[6] - 0.5 * (f[1] - f[3]);
-    } else {
-        f[2] = f[4];
-        f[5] = f[7] - 0.5 * (f[1] - f[3]);
-        f[6] = f[8] + 0.5 * (f[1] - f[3]);
-    }
+// face: 0 left, 1 right, 2 top, 3 bottom.  Written as selects with fixed indices (no face
+// writes a value another face's formula reads), so f stays in registers.
+template <typename T>
+__host__ __device__ inline void settling_face_t(int face, T *f) {
+    const T h = (T)0.5;
+    const T f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4];
+    const T f5 = f[5], f6 = f[6], f7 = f[7], f8 = f[8];
+    const T dv = h * (f2 - f4), dh = h * (f1 - f3);
+    f[1] = face == 0 ? f3 : f1;
+    f[3] = face == 1 ? f1 : f3;
+    f[2] = face == 3 ? f4 : f2;
+    f[4] = face == 2 ? f2 : f4;
+    f[5] = face == 0 ? f7 - dv : (face == 3 ? f7 - dh : f5);
+    f[6] = face == 1 ? f8 - dv : (face == 3 ? f8 + dh : f6);
+    f[7] = face == 1 ? f5 + dv : (face == 2 ? f5 + dh : f7);
+    f[8] = face == 0 ? f6 + dv : (face == 2 ? f6 - dh : f8);
 }
+__host__ __device__ inline void settling_face(int face, double *f) { settling_face_t(face, f); }
 
 // Guo term without prefactor, w_i omitted: iblbm/lbm/collision/bgk.py:32-37
 __host__ __device__ inline double guo_term(int i, double u0, double u1, double fx, double fy) {

@@ -56,7 +56,7 @@ struct Inst {
     int ibm, delta, Lb, lb_cap;
     int bx0, by0, bw, bh;
     int box_cap, n_iter, min_iter, fixed_body;
-    int tables_valid, lambda_valid, nT, pad2;
+    int nT, pad2;
     double dx, dy, Larea, omega, tol, dfc_scale;
     const double *Lx, *Ly, *ud;
     IbmScratch s;

@@ -1,0 +1,359 @@
+"""Settling-particle runs on the GPU hot path (single disc and multi-particle DFC).
+
+The lattice, the marker window (MDF/DF/DFC) and the collision run on the device; the rigid-
+body update (Velocity Verlet with the ``none`` or ``explicit_history`` inertia model,
+``iblbm/physics/inertia``) stays on the host in FP64 with the reference's expressions, as in
+``iblbm/runtime/scenarios/sedimentation_single.py:91-293`` and ``sedimentation_multi.py:49-124``.
+Per step the host uploads the moved markers and reads back the window fib (DF/MDF) or the
+marker forces (DFC).  The force and torque sums are formed in NumPy's own summation order
+(pairwise inside 8192-element blocks, blocks in sequence), so the trajectories carry the
+reference's bits.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _abi, gpu_kernels
+from ._abi import check
+from .init import circle_markers, initialize
+from .solver import PRECISIONS, Context, make_case, pre_collide
+
+_NP_BLOCK = 8192  # NumPy's reduction buffer: pairwise inside blocks, blocks in sequence
+
+
+def np_sum_sparse(vals: np.ndarray, start: int, n_total: int) -> float:
+    """``float(np.sum(v))`` of a length-``n_total`` vector that is ``vals`` on
+    [start, start+len(vals)) and +0.0 elsewhere, without materialising it: only the NumPy
+    reduction blocks that overlap ``vals`` are summed (with NumPy itself), in order."""
+    end = start + len(vals)
+    res = 0.0
+    b0 = (start // _NP_BLOCK) * _NP_BLOCK
+    for b in range(b0, end, _NP_BLOCK):
+        blk = np.zeros(min(_NP_BLOCK, n_total - b))
+        lo, hi = max(start, b), min(end, b + len(blk))
+        blk[lo - b:hi - b] = vals[lo - start:hi - start]
+        res += float(np.sum(blk))
+    return res
+
+
+def compute_net_gravity(rho_ratio, r_lattice, g_lattice, gravity_direction="down"):
+    """``iblbm/physics/gravity.py:16-34``."""
+    mag = (rho_ratio - 1.0) * (np.pi * r_lattice**2) * g_lattice
+    if gravity_direction == "right":
+        return np.array([mag, 0.0])
+    return np.array([0.0, -mag])
+
+
+def _history_dv(model, vel, vel_prev, mass, r_lattice):
+    if model == "explicit_history":  # inertia/explicit_history.py:41-54
+        m_fluid = np.pi * r_lattice**2
+        return (m_fluid / mass) * (vel - vel_prev)
+    return None
+
+
+def verlet_half(model, pos, vel, vel_prev, force, mass, rho_ratio, r_lat, g, dt, dx, gdir):
+    """``velocity_verlet_half`` of the ``none`` / ``explicit_history`` inertia models."""
+    F_total = force + compute_net_gravity(rho_ratio, r_lat, g, gdir)
+    hdv = _history_dv(model, vel, vel_prev, mass, r_lat)
+    if hdv is None:
+        vel_half = vel + 0.5 * dt * F_total / mass
+    else:
+        vel_half = vel + 0.5 * dt * F_total / mass + 0.5 * hdv
+    return pos + dt * vel_half * dx, vel_half
+
+
+def verlet_full(model, vel_half, vel, vel_prev, force, mass, rho_ratio, r_lat, g, dt, gdir):
+    F_total = force + compute_net_gravity(rho_ratio, r_lat, g, gdir)
+    hdv = _history_dv(model, vel, vel_prev, mass, r_lat)
+    if hdv is None:
+        return vel_half + 0.5 * dt * F_total / mass
+    return vel_half + 0.5 * dt * F_total / mass + 0.5 * hdv
+
+
+def update_markers(Lx_c, Ly_c, cx_init, cy_init, pos_new, angle=0.0):
+    """``iblbm/physics/markers.py:23-38``."""
+    cos_a = np.cos(angle)
+    sin_a = np.sin(angle)
+    rx = Lx_c - cx_init
+    ry = Ly_c - cy_init
+    return cos_a * rx - sin_a * ry + pos_new[0], sin_a * rx + cos_a * ry + pos_new[1]
+
+
+def desired_velocity(vel, Lb, omega=0.0, Lx=None, Ly=None, cx=0.0, cy=0.0, dx=1.0):
+    """``iblbm/physics/markers.py:41-57``."""
+    d = np.empty((Lb, 2))
+    d[:, 0] = float(vel[0])
+    d[:, 1] = float(vel[1])
+    if omega != 0.0 and Lx is not None:
+        d[:, 0] += -omega * (Ly - cy) / dx
+        d[:, 1] += omega * (Lx - cx) / dx
+    return d
+
+
+def record_state(pos, vel, t, d_lattice, g, rho_ratio, y0, dx):
+    """``iblbm/diagnostics/internal_flow.py:156-218``."""
+    u_g = np.sqrt(abs(rho_ratio - 1.0) * g * d_lattice)
+    D_domain = d_lattice * dx
+    if u_g < 1e-15:
+        return {"step": int(t), "t_star": 0.0, "y_star": 0.0, "vy_star": 0.0, "vx_star": 0.0,
+                "x": float(pos[0]), "y": float(pos[1]), "vx": float(vel[0]),
+                "vy": float(vel[1])}
+    return {"step": int(t), "t_star": float(t * u_g / d_lattice),
+            "y_star": float((y0 - pos[1]) / D_domain), "vy_star": float(-vel[1] / u_g),
+            "vx_star": float(vel[0] / u_g), "x": float(pos[0]), "y": float(pos[1]),
+            "vx": float(vel[0]), "vy": float(vel[1])}
+
+
+class Particle:
+    def __init__(self, cx, cy, rho_ratio, lattice_r, r_domain, dx, spacing, retraction, Lx=None,
+                 Ly=None, Lb=None, Larea=None):
+        if Lx is None:
+            Lx, Ly, Lb, Larea = circle_markers(cx, cy, r_domain, lattice_r, dx, spacing, retraction)
+        self.pos = np.array([cx, cy], dtype=float)
+        self.vel = np.array([0.0, 0.0], dtype=float)
+        self.vel_prev = np.array([0.0, 0.0], dtype=float)
+        self.force = np.array([0.0, 0.0], dtype=float)
+        self.rho_ratio = rho_ratio
+        self.lattice_r = lattice_r
+        self.r_domain = r_domain
+        self.mass = rho_ratio * np.pi * lattice_r**2  # physics/sedimentation.py:126-127
+        self.I = 0.5 * self.mass * lattice_r**2
+        self.omega = 0.0
+        self.torque = 0.0
+        self.angle = 0.0
+        self.Lx_c, self.Ly_c = Lx.copy(), Ly.copy()
+        self.Lx, self.Ly = Lx, Ly
+        self.Lb, self.Larea = Lb, Larea
+        self.desired = np.zeros((Lb, 2))
+        self.cx_init, self.cy_init = cx, cy
+        self.vel_half = None
+        self.omega_half = 0.0
+
+
+class Sedimentation:
+    """One settling-particle instance on the device, stepped one time step at a time."""
+
+    def __init__(self, cfg, device=0, precision="fp64", ctx=None):
+        if cfg.motion_type != "sedimentation":
+            raise ValueError("Sedimentation needs motion_type='sedimentation'")
+        if cfg.settling_inertia_model not in ("none", "explicit_history"):
+            raise ValueError(f"settling_inertia_model={cfg.settling_inertia_model!r} is outside "
+                             "the GPU hot path (supported: 'none', 'explicit_history')")
+        if cfg.time_integrator != "verlet":
+            raise ValueError("only the Velocity Verlet integrator is on the GPU hot path")
+        if cfg.rotation_coupling == "iterative" and cfg.enable_rotation:
+            raise ValueError("rotation_coupling='iterative' is outside the GPU hot path")
+        if cfg.prescribed_velocity is not None:
+            raise ValueError("prescribed_velocity is outside the GPU hot path")
+        if cfg.gravity <= 0.0:
+            raise ValueError(f"sedimentation needs gravity > 0 (got {cfg.gravity})")
+        if cfg.rho_ratio <= 1.0:
+            raise ValueError(f"sedimentation needs rho_ratio > 1.0 (got {cfg.rho_ratio})")
+        self.cfg = cfg
+        st = initialize(cfg)
+        self.st = st
+        self.g = cfg.gravity
+        self.multi = cfg.particles_config is not None
+        if self.multi:
+            if cfg.ibm_method != "DFC":
+                raise ValueError("multi-particle sedimentation on the GPU path runs DFC")
+            self.particles = [Particle(pc["center"][0], pc["center"][1], pc["rho_ratio"],
+                                       st.lattice_r, st.r, st.dx, cfg.marker_spacing_factor,
+                                       cfg.retraction_dx) for pc in cfg.particles_config]
+            larea = self.particles[0].Larea
+            for p in self.particles:
+                if abs(float(p.Larea) - larea) > 1e-12:
+                    raise ValueError("multi-particle DFC needs one Larea for every particle")
+        else:
+            cx, cy = cfg.cylinder_center
+            self.particles = [Particle(cx, cy, cfg.rho_ratio, st.lattice_r, st.r, st.dx,
+                                       cfg.marker_spacing_factor, cfg.retraction_dx,
+                                       Lx=st.Lx, Ly=st.Ly, Lb=st.Lb, Larea=st.Larea)]
+        self.ctx = ctx or Context(device)
+        case = make_case(cfg, st, PRECISIONS[precision])
+        case.fixed_body = 0
+        case.lb_capacity = sum(p.Lb for p in self.particles)
+        case.box_capacity = st.nx * st.ny
+        self.omega = None
+        if cfg.ibm_method == "MDF":  # strategies/mdf.py:27-41
+            om = gpu_kernels.estimate_optimal_omega_from_type(st.Lx, st.Ly, cfg.delta_type,
+                                                              st.Larea, st.dx)
+            if cfg.enable_rotation:
+                om = min(float(om), 1.0)
+            self.omega = om
+            case.mdf_omega = om
+        self.k = self.ctx.add(case)
+        self._push_markers()
+        self.ctx.upload_f(self.k, pre_collide(cfg, st))
+        self.ttt = 0
+        self.history = []
+        self.mdf_stats = None
+        self.d_lattice = 2.0 * st.lattice_r
+        self.termination_reason = "max_steps"
+
+    def _push_markers(self):
+        Lx = np.concatenate([p.Lx for p in self.particles])
+        Ly = np.concatenate([p.Ly for p in self.particles])
+        ud = np.concatenate([p.desired for p in self.particles])
+        self.ctx.set_markers(self.k, Lx, Ly, ud, self.particles[0].Larea)
+        p0 = self.particles[0]
+        self.ctx.set_torque_center(self.k, float(p0.pos[0]), float(p0.pos[1]))
+
+    # -- the reference's pre_step / post_ibm hooks ------------------------------------------
+    def _half_step(self):
+        cfg, st = self.cfg, self.st
+        for p in self.particles:
+            pos_new, vel_half = verlet_half(cfg.settling_inertia_model, p.pos, p.vel, p.vel_prev,
+                                            p.force, p.mass, p.rho_ratio, p.lattice_r, self.g,
+                                            st.dt, st.dx, cfg.gravity_direction)
+            p.pos = pos_new
+            if cfg.enable_rotation:  # rigid_body.py:65-74
+                p.omega_half = p.omega + 0.5 * st.dt * p.torque / p.I
+                p.angle += st.dt * p.omega_half
+            else:
+                p.omega_half = 0.0
+            p.Lx, p.Ly = update_markers(p.Lx_c, p.Ly_c, p.cx_init, p.cy_init, pos_new,
+                                        angle=p.angle if cfg.enable_rotation else 0.0)
+            if cfg.rotation_coupling == "semi_implicit" and cfg.enable_rotation:
+                omega = p.omega_half if not self.multi else p.omega
+                p.desired = desired_velocity(vel_half, p.Lb, omega=omega, Lx=p.Lx, Ly=p.Ly,
+                                             cx=float(p.pos[0]), cy=float(p.pos[1]), dx=st.dx)
+            else:
+                p.desired = desired_velocity(vel_half, p.Lb)
+            p.vel_half = vel_half
+
+    def _forces(self):
+        """Hydrodynamic force and torque per particle (strategies/*.py extract_force/torque)."""
+        st, cfg = self.st, self.cfg
+        out = []
+        if cfg.ibm_method == "DFC":
+            Lbt = sum(p.Lb for p in self.particles)
+            q = self.ctx.download_dfc_force(self.k, Lbt)
+            c = 0
+            for p in self.particles:
+                fq = q[c:c + p.Lb]
+                c += p.Lb
+                F = np.array([float(fq[:, 0].sum()), float(fq[:, 1].sum())])
+                T = 0.0
+                if cfg.enable_rotation:
+                    rx = (p.Lx - float(p.pos[0])) / st.dx
+                    ry = (p.Ly - float(p.pos[1])) / st.dy
+                    T = float(np.sum(rx * fq[:, 1] - ry * fq[:, 0]))
+                out.append((F, T))
+            return out
+        fx, fy, box = self.ctx.download_window(self.k)
+        x0, y0, bw, bh = box
+        nx, N = st.nx, st.nodenums
+        start = y0 * nx
+        n_rows = bh * nx
+        seg_x = np.zeros(n_rows)
+        seg_y = np.zeros(n_rows)
+        for r in range(bh):
+            seg_x[r * nx + x0:r * nx + x0 + bw] = fx[r * bw:(r + 1) * bw]
+            seg_y[r * nx + x0:r * nx + x0 + bw] = fy[r * bw:(r + 1) * bw]
+        F = -np.array([np_sum_sparse(seg_x, start, N), np_sum_sparse(seg_y, start, N)])
+        T = 0.0
+        if cfg.enable_rotation:  # strategies/mdf.py:94-101
+            p = self.particles[0]
+            cx_lat = float(p.pos[0]) / st.dx
+            cy_lat = float(p.pos[1]) / st.dy
+            II = np.tile(np.arange(nx, dtype=float), bh)
+            JJ = np.repeat(np.arange(y0, y0 + bh, dtype=float), nx)
+            term = (II - cx_lat) * seg_y - (JJ - cy_lat) * seg_x
+            T = -np_sum_sparse(term, start, N)
+        out.append((F, T))
+        return out
+
+    def _full_step(self, forces):
+        cfg, st = self.cfg, self.st
+        for p, (F, T) in zip(self.particles, forces):
+            vel_new = verlet_full(cfg.settling_inertia_model, p.vel_half, p.vel, p.vel_prev, F,
+                                  p.mass, p.rho_ratio, p.lattice_r, self.g, st.dt,
+                                  cfg.gravity_direction)
+            if cfg.enable_rotation:  # rigid_body.py:77-82
+                p.omega = p.omega_half + 0.5 * st.dt * T / p.I
+                p.torque = T
+            p.vel_prev = p.vel.copy()
+            p.vel = vel_new
+            p.force = F
+
+    def step(self):
+        self.ttt += 1
+        self._half_step()
+        self._push_markers()
+        ci = int(self.cfg.check_interval)
+        self.ctx.step(1, diag_every=1 if self.ttt % ci == 0 else 0)
+        self._full_step(self._forces())
+        if self.cfg.ibm_method == "MDF":
+            self.mdf_stats = self.ctx.mdf_stats(self.k)
+
+    def record(self):
+        cfg, st = self.cfg, self.st
+        recs = [record_state(p.pos, p.vel, self.ttt, self.d_lattice, self.g, p.rho_ratio,
+                             p.cy_init, st.dx) for p in self.particles]
+        return recs
+
+    def should_stop(self):
+        """``sedimentation_common.py:14-66`` (contact / offset / nan / domain bounds)."""
+        cfg, st = self.cfg, self.st
+        p0 = self.particles[0]
+        v = float(p0.vel[0]) if cfg.gravity_direction == "right" else float(p0.vel[1])
+        if not math.isfinite(v):
+            return "nan"
+        r = p0.r_domain
+        pos = p0.pos
+        if cfg.sedimentation_stop_at_contact:
+            if cfg.gravity_direction == "right":
+                if float(pos[0]) >= cfg.xmax - r:
+                    return "contact"
+            elif float(pos[1]) <= r:
+                return "contact"
+        if cfg.sedimentation_stop_offset_d > 0.0:
+            d_dom = cfg.cylinder_D_ratio
+            if cfg.gravity_direction == "right":
+                if float(pos[0]) >= cfg.xmax - cfg.sedimentation_stop_offset_d * d_dom:
+                    return "offset"
+            elif float(pos[1]) <= cfg.sedimentation_stop_offset_d * d_dom:
+                return "offset"
+        margin = 2.0 * st.dx
+        terminal = cfg.sedimentation_stop_at_contact or cfg.sedimentation_stop_offset_d > 0.0
+        for p in self.particles:
+            x, y, rr = float(p.pos[0]), float(p.pos[1]), p.r_domain
+            if cfg.gravity_direction == "right":
+                near = (y - rr < margin or y + rr > cfg.ymax - margin or x - rr < margin or
+                        (not terminal and x + rr > cfg.xmax - margin))
+            else:
+                near = ((not terminal and y - rr < margin) or y + rr > cfg.ymax - margin or
+                        x - rr < margin or x + rr > cfg.xmax - margin)
+            if near:
+                return "domain_bounds"
+        return None
+
+
+def run(cfg, verbose: bool = True, callback=None, save_dir=None, device: int = 0,
+        precision: str = "fp64") -> dict:
+    """``iblbm.run`` for sedimentation: result keys of the reference plus the history."""
+    sim = Sedimentation(cfg, device=device, precision=precision)
+    ci = int(cfg.check_interval)
+    while sim.ttt < cfg.max_steps:
+        sim.step()
+        if sim.ttt % ci == 0:
+            recs = sim.record()
+            sim.history.append(recs if sim.multi else recs[0])
+            if verbose:
+                r0 = recs[0]
+                print(f"step {sim.ttt:>7d} | y*={r0['y_star']:.4f} vy*={r0['vy_star']:.6f}")
+            if callback is not None:
+                callback(step=sim.ttt, Cd=0.0, Cl=0.0, error=1.0, state=sim, elapsed=0.0,
+                         converged=False)
+            reason = sim.should_stop()
+            if reason is not None:
+                sim.termination_reason = reason
+                break
+    sim.ctx.sync()
+    return {"Cd_history": np.array([]), "Cl_history": np.array([]), "converged": False,
+            "final_step": sim.ttt, "termination_reason": sim.termination_reason,
+            "sedimentation_history": sim.history, "sim": sim}

@@ -89,6 +89,16 @@ class Context:
         check(self.lib.iblbm_download_dfc_force(self.h, k, dptr(out)), "download_dfc_force")
         return out
 
+    def download_window(self, k):
+        """(fib_x, fib_y) on the window box (box-major) and the box [x0, y0, bw, bh]."""
+        nx, ny = self.n[k]
+        fx, fy = np.empty(nx * ny), np.empty(nx * ny)
+        box = (ctypes.c_int * 4)()
+        check(self.lib.iblbm_download_window(self.h, k, dptr(fx), dptr(fy), box),
+              "download_window")
+        nb = box[2] * box[3]
+        return fx[:nb], fy[:nb], (box[0], box[1], box[2], box[3])
+
     def step(self, n, diag_every=0, graph=False):
         fn = self.lib.iblbm_graph_run if graph else self.lib.iblbm_step
         check(fn(self.h, int(n), int(diag_every)), "step")

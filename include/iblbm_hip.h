@@ -144,6 +144,10 @@ int iblbm_download_fib(void *ctx, int id, double *fib_aos);
 int iblbm_download_state(void *ctx, int id, double *fstar_aos, double *ro, double *U);
 /* DFC marker forces q_k (Lb,2) of the last step (iblbm/ibm/dfc.py:232-253). */
 int iblbm_download_dfc_force(void *ctx, int id, double *q);
+/* fib of the last step on the window box only (box-major, bw*bh each) and the box
+ * [x0, y0, bw, bh]; fib is zero outside it.  Replaces the per-step full-grid to_cpu(fib) of
+ * iblbm/ibm/strategies/mdf.py:172-185 (force and torque extraction). */
+int iblbm_download_window(void *ctx, int id, double *fibx, double *fiby, int box[4]);
 int64_t iblbm_step_counter(void *ctx);
 
 /* Time of the last iblbm_step/iblbm_graph_run measured with HIP events on the context
