@@ -27,7 +27,7 @@ constexpr int MAX_LB = 512;   // markers held in LDS (beyond: global scratch)
 constexpr int MAX_G = 4096;   // gather-list entries held in LDS (beyond: global scratch)
 
 struct IbmShared {
-    double uwx[MAX_T], uwy[MAX_T], vs[MAX_T], fibx[MAX_T], fiby[MAX_T];
+    double uwx[MAX_T], uwy[MAX_T], vs[MAX_T], fibx[MAX_T], fiby[MAX_T], rot[MAX_T];
     double Lfx[MAX_LB], Lfy[MAX_LB];
     double gw[MAX_G];
     int gk[MAX_G];
@@ -225,42 +225,75 @@ __device__ void ibm_dfc_lambda(Inst &I, int nT, int Lb) {
 
 // ------------------------------------------------------------------ the IBM core
 
-// Working-array pointers: LDS when the window fits, global scratch otherwise.
+// Touched-node working arrays (MDF velocity, dt/(2 rho), fib, rho), marker forces and gather
+// lists.  LDS or global scratch is a compile-time choice, so that LDS accesses compile to
+// ds_* instructions rather than flat ones.
 struct IbmPtrs {
-    double *uwx, *uwy, *vs, *fbx, *fby, *Lfx, *Lfy, *gw;
+    double *uwx, *uwy, *vs, *fbx, *fby, *rot, *Lfx, *Lfy, *gw;
     int *gk, *toff;
 };
-__device__ inline IbmPtrs ibm_ptrs(Inst &I, int nT, IbmShared &sh) {
+template <bool LDS>
+__device__ inline IbmPtrs ibm_ptrs(Inst &I, IbmShared &sh) {
     IbmScratch &S = I.s;
-    const bool lt = nT <= MAX_T, lm = I.Lb <= MAX_LB;
-    IbmPtrs p;
-    p.uwx = lt ? sh.uwx : S.uwx;
-    p.uwy = lt ? sh.uwy : S.uwy;
-    p.vs = lt ? sh.vs : S.vs;
-    p.fbx = lt ? sh.fibx : S.tfibx;
-    p.fby = lt ? sh.fiby : S.tfiby;
-    p.Lfx = lm ? sh.Lfx : S.Lfx;
-    p.Lfy = lm ? sh.Lfy : S.Lfy;
-    p.gk = S.gk;
-    p.gw = S.gw;
-    p.toff = S.toff;
-    return p;
+    if constexpr (LDS)
+        return {sh.uwx, sh.uwy, sh.vs, sh.fibx, sh.fiby, sh.rot, sh.Lfx, sh.Lfy, sh.gw, sh.gk,
+                sh.toff};
+    else
+        return {S.uwx, S.uwy, S.vs, S.tfibx, S.tfiby, S.rot, S.Lfx, S.Lfy, S.gw, S.gk, S.toff};
+}
+__device__ inline bool ibm_fits_lds(const Inst &I, int nT) {
+    return nT <= MAX_T && I.Lb <= MAX_LB && I.s.toff[nT] <= MAX_G;
 }
 
-// Copies the gather lists of the window into LDS when they fit (one pass per step).
-__device__ inline void ibm_stage_lists(Inst &I, int nT, IbmShared &sh, IbmPtrs &P) {
-    IbmScratch &S = I.s;
+// Copies the gather lists of the window into LDS (several loads in flight per thread).
+__device__ inline void ibm_stage_lists(const Inst &I, int nT, IbmShared &sh) {
+    const IbmScratch &S = I.s;
     const int t = threadIdx.x;
     const int ng = S.toff[nT];
-    if (nT <= MAX_T && ng <= MAX_G) {
-        for (int x = t; x <= nT; x += NTI) sh.toff[x] = S.toff[x];
-        for (int x = t; x < ng; x += NTI) {
-            sh.gk[x] = S.gk[x];
-            sh.gw[x] = S.gw[x];
+    for (int x = t; x <= nT; x += NTI) sh.toff[x] = S.toff[x];
+    for (int x0 = t; x0 < ng; x0 += 4 * NTI) {
+        int k[4];
+        double w[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int x = x0 + u * NTI;
+            if (x < ng) {
+                k[u] = S.gk[x];
+                w[u] = S.gw[x];
+            }
         }
-        P.gk = sh.gk;
-        P.gw = sh.gw;
-        P.toff = sh.toff;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int x = x0 + u * NTI;
+            if (x < ng) {
+                sh.gk[x] = k[u];
+                sh.gw[x] = w[u];
+            }
+        }
+    }
+}
+
+// One marker's stencil (touched index, weight) and desired velocity, held in registers by the
+// thread that owns the marker (Lb <= NTI).
+template <int A>
+struct MarkerRegs {
+    static constexpr int K = (2 * A + 1) * (2 * A + 1);
+    int ti[K];
+    double w[K];
+    double udx, udy;
+};
+template <int A>
+__device__ inline void load_marker_regs(const Inst &I, MarkerRegs<A> &M) {
+    constexpr int K = MarkerRegs<A>::K;
+    const int t = threadIdx.x;
+    if (I.Lb <= NTI && t < I.Lb) {
+#pragma unroll
+        for (int s = 0; s < K; ++s) {
+            M.ti[s] = I.s.cti[t * K + s];
+            M.w[s] = I.s.cw[t * K + s];
+        }
+        M.udx = I.ud[2 * t];
+        M.udy = I.ud[2 * t + 1];
     }
 }
 
@@ -293,72 +326,75 @@ __device__ inline void gather2(const IbmPtrs &P, int ti, double &ex, double &ey)
     ey = sy;
 }
 
-// DF / MDF / DFC on the window, given S.ro, S.ux0, S.uy0 (and S.fs for DFC) of the touched
-// nodes.  DF/MDF leave fib of the touched nodes in P.fbx/P.fby; DFC corrects S.fs and
-// recomputes S.ro, S.ux0, S.uy0 of the touched nodes and leaves the marker forces in S.q.
+// Touched-node inputs of the core from box-slot arrays S.ro, S.ux0, S.uy0 (operator path).
+__device__ inline void ibm_touched_init(const Inst &I, int nT, const IbmPtrs &P) {
+    const IbmScratch &S = I.s;
+    for (int ti = threadIdx.x; ti < nT; ti += NTI) {
+        const int sl = S.tslot[ti];
+        const double r = S.ro[sl];
+        P.uwx[ti] = S.ux0[sl];
+        P.uwy[ti] = S.uy0[sl];
+        P.rot[ti] = r;
+        P.vs[ti] = I.cp.dt / (2.0 * r);  // mdf.py:117
+    }
+    __threadfence_block();
+    __syncthreads();
+}
+
+// DF / MDF / DFC on the window.  Inputs: P.uwx, P.uwy, P.rot, P.vs of the touched nodes
+// (DF/MDF), or S.fs of the touched slots (DFC).  DF/MDF leave fib of the touched nodes in
+// P.fbx/P.fby; DFC corrects S.fs and recomputes S.ro, S.ux0, S.uy0 of the touched slots and
+// leaves the marker forces in S.q.
 template <int A>
-__device__ void ibm_core(Inst &I, int nT, const IbmPtrs &P, double *red, bool store_lu) {
+__device__ void ibm_core(Inst &I, int nT, const IbmPtrs &P, double *red, bool store_lu,
+                         const MarkerRegs<A> &M) {
     constexpr int K = (2 * A + 1) * (2 * A + 1);
     const int t = threadIdx.x, Lb = I.Lb;
     IbmScratch &S = I.s;
     const bool inc = I.inc != 0;
     const double dt = I.cp.dt;
-    for (int ti = t; ti < nT; ti += NTI) {
-        const int sl = S.tslot[ti];
-        P.uwx[ti] = S.ux0[sl];
-        P.uwy[ti] = S.uy0[sl];
-        P.vs[ti] = dt / (2.0 * S.ro[sl]);  // mdf.py:117
-        P.fbx[ti] = 0.0;
-        P.fby[ti] = 0.0;
-    }
-    __threadfence_block();
-    __syncthreads();
     if (I.ibm == 1 || I.ibm == 2) {
-        // the marker's stencil (touched index, weight) stays in registers over the sub-steps
-        // when every marker has its own thread
+        for (int ti = t; ti < nT; ti += NTI) {
+            P.fbx[ti] = 0.0;
+            P.fby[ti] = 0.0;
+        }
         const bool regs = Lb <= NTI;
-        int rti[K];
-        double rw[K];
         double R = 0.0;
         if (regs && t < Lb) {
 #pragma unroll
-            for (int s = 0; s < K; ++s) {
-                rti[s] = S.cti[t * K + s];
-                rw[s] = S.cw[t * K + s];
-            }
-#pragma unroll
-            for (int s = 0; s < K; ++s) R += S.ro[S.tslot[rti[s]]] * rw[s];  // rho at the marker
+            for (int s = 0; s < K; ++s) R += P.rot[M.ti[s]] * M.w[s];  // rho at the marker
         }
         if (!regs) {
             for (int k = t; k < Lb; k += NTI) {
                 double r = 0.0;
 #pragma unroll
-                for (int s = 0; s < K; ++s)
-                    r += S.ro[S.tslot[S.cti[k * K + s]]] * S.cw[k * K + s];
+                for (int s = 0; s < K; ++s) r += P.rot[S.cti[k * K + s]] * S.cw[k * K + s];
                 S.R[k] = r;
             }
-            __threadfence_block();
-            __syncthreads();
         }
+        __threadfence_block();
+        __syncthreads();
         const bool mdf = I.ibm == 2;
         const bool finc = mdf ? false : inc;  // MDF sub-steps are compressible (mdf.py:177-181)
         const int n_iter = mdf ? I.n_iter : 1;
         const int min_iter = max(1, min(I.min_iter, n_iter));
-        const double omega = I.omega;
+        const double omega = I.omega, tol = I.tol;
         double prev = INFINITY, last = INFINITY;
         int iter_count = 0, diverged = 0, converged = 0;
         for (int it = 0; it < n_iter; ++it) {
             // interpolation + force per marker (df.py:239-263)
             double rmax = -INFINITY;
             for (int k = t; k < Lb; k += NTI) {
-                double lx = 0.0, ly = 0.0, Rk;
+                double lx = 0.0, ly = 0.0, Rk, udx, udy;
                 if (regs) {
 #pragma unroll
                     for (int s = 0; s < K; ++s) {
-                        lx += P.uwx[rti[s]] * rw[s];
-                        ly += P.uwy[rti[s]] * rw[s];
+                        lx += P.uwx[M.ti[s]] * M.w[s];
+                        ly += P.uwy[M.ti[s]] * M.w[s];
                     }
                     Rk = R;
+                    udx = M.udx;
+                    udy = M.udy;
                 } else {
 #pragma unroll
                     for (int s = 0; s < K; ++s) {
@@ -368,8 +404,9 @@ __device__ void ibm_core(Inst &I, int nT, const IbmPtrs &P, double *red, bool st
                         ly += P.uwy[ti] * w;
                     }
                     Rk = S.R[k];
+                    udx = I.ud[2 * k];
+                    udy = I.ud[2 * k + 1];
                 }
-                const double udx = I.ud[2 * k], udy = I.ud[2 * k + 1];
                 if (finc) {
                     P.Lfx[k] = 2.0 * (udx - lx) / dt;
                     P.Lfy[k] = 2.0 * (udy - ly) / dt;
@@ -400,10 +437,11 @@ __device__ void ibm_core(Inst &I, int nT, const IbmPtrs &P, double *red, bool st
                 gather2(P, ti, ex, ey);
                 if (mdf) {  // mdf.py:197-200
                     const double fx = ex * omega, fy = ey * omega;
+                    const double vs = P.vs[ti];
                     P.fbx[ti] = P.fbx[ti] + fx;
                     P.fby[ti] = P.fby[ti] + fy;
-                    P.uwx[ti] = P.uwx[ti] + fx * P.vs[ti];
-                    P.uwy[ti] = P.uwy[ti] + fy * P.vs[ti];
+                    P.uwx[ti] = P.uwx[ti] + fx * vs;
+                    P.uwy[ti] = P.uwy[ti] + fy * vs;
                 } else {
                     P.fbx[ti] = ex;
                     P.fby[ti] = ey;
@@ -414,7 +452,7 @@ __device__ void ibm_core(Inst &I, int nT, const IbmPtrs &P, double *red, bool st
             if (mdf) {
                 prev = res;
                 iter_count = it + 1;
-                if (it + 1 >= min_iter && res < I.tol) {
+                if (it + 1 >= min_iter && res < tol) {
                     converged = 1;
                     break;
                 }
@@ -488,7 +526,7 @@ __device__ void ibm_core(Inst &I, int nT, const IbmPtrs &P, double *red, bool st
 // Window workgroup of one step.  It owns the touched nodes only (the bulk tiles skip exactly
 // those): stream/close them, moments, the IBM stage, velocity correction, forced collision.
 // The tables it reads were built by k_tables when the markers were set.
-template <typename ST, int MODEL, int A>
+template <typename ST, int MODEL, int A, bool LDS>
 __device__ void ibm_role_impl(Inst &I, int parity, int diag, double *red, IbmShared &sh) {
     const int t = threadIdx.x;
     const ST *__restrict__ Asrc = buf<ST>(I, parity);
@@ -497,10 +535,12 @@ __device__ void ibm_role_impl(Inst &I, int parity, int diag, double *red, IbmSha
     const bool inc = I.inc != 0;
     const double dt = I.cp.dt;
     const int Lb = I.Lb, nT = I.nT;
-    IbmPtrs P = ibm_ptrs(I, nT, sh);
-    ibm_stage_lists(I, nT, sh, P);
+    const IbmPtrs P = ibm_ptrs<LDS>(I, sh);
+    MarkerRegs<A> M;
+    load_marker_regs<A>(I, M);  // in flight during P1
+    if constexpr (LDS) ibm_stage_lists(I, nT, sh);
 
-    // P1: stream + close the touched nodes, macroscopic moments
+    // P1: stream + close the touched nodes, macroscopic moments, core inputs
     for (int ti = t; ti < nT; ti += NTI) {
         const int sl = S.tslot[ti];
         const int i = I.bx0 + sl % I.bw, j = I.by0 + sl / I.bw;
@@ -513,13 +553,14 @@ __device__ void ibm_role_impl(Inst &I, int parity, int diag, double *red, IbmSha
         S.ro[sl] = r;
         S.ux0[sl] = u0;
         S.uy0[sl] = u1;
+        P.uwx[ti] = u0;
+        P.uwy[ti] = u1;
+        P.rot[ti] = r;
+        P.vs[ti] = dt / (2.0 * r);  // mdf.py:117
     }
     __threadfence_block();
     __syncthreads();
-    if (A == 1)
-        ibm_core<1>(I, nT, P, red, false);
-    else
-        ibm_core<2>(I, nT, P, red, false);
+    ibm_core<A>(I, nT, P, red, false, M);
 
     // P3: velocity correction (DF/MDF), equilibrium, collision, store
     const bool correct = I.ibm == 1 || I.ibm == 2;
@@ -595,10 +636,18 @@ __device__ void ibm_role_impl(Inst &I, int parity, int diag, double *red, IbmSha
 template <typename ST, int MODEL>
 __device__ void ibm_role(Inst &I, int parity, int diag, double *red) {
     __shared__ IbmShared sh;
-    if (I.delta == 1)
-        ibm_role_impl<ST, MODEL, 1>(I, parity, diag, red, sh);
-    else
-        ibm_role_impl<ST, MODEL, 2>(I, parity, diag, red, sh);
+    const bool lds = ibm_fits_lds(I, I.nT);
+    if (I.delta == 1) {
+        if (lds)
+            ibm_role_impl<ST, MODEL, 1, true>(I, parity, diag, red, sh);
+        else
+            ibm_role_impl<ST, MODEL, 1, false>(I, parity, diag, red, sh);
+    } else {
+        if (lds)
+            ibm_role_impl<ST, MODEL, 2, true>(I, parity, diag, red, sh);
+        else
+            ibm_role_impl<ST, MODEL, 2, false>(I, parity, diag, red, sh);
+    }
 }
 
 // Window tables (and the DFC lambda) for new markers; zeroes the box fib.

@@ -155,7 +155,9 @@ __device__ void op_ibm_impl(Inst &I, int mode, const double *Eux, const double *
     const int nbox = I.bw * I.bh;
     ibm_build_tables<A>(I, redi);
     const int nT = S.tindex[nbox];
-    const IbmPtrs P = ibm_ptrs(I, nT, sh);
+    const IbmPtrs P = ibm_ptrs<false>(I, sh);  // global working arrays on this path
+    MarkerRegs<A> M;
+    load_marker_regs<A>(I, M);
     if (mode >= 2) {
         if (mode != 4) ibm_dfc_lambda<A>(I, nT, I.Lb);
         if (mode == 2) return;
@@ -168,7 +170,7 @@ __device__ void op_ibm_impl(Inst &I, int mode, const double *Eux, const double *
         }
         __threadfence_block();
         __syncthreads();
-        ibm_core<A>(I, nT, P, red, false);
+        ibm_core<A>(I, nT, P, red, false, M);
         for (int ti = t; ti < nT; ti += NTI) {
             const int sl = S.tslot[ti];
             const int64_t n = (int64_t)(I.by0 + sl / I.bw) * I.nx + I.bx0 + sl % I.bw;
@@ -184,7 +186,8 @@ __device__ void op_ibm_impl(Inst &I, int mode, const double *Eux, const double *
     }
     __threadfence_block();
     __syncthreads();
-    ibm_core<A>(I, nT, P, red, mode == 0);
+    ibm_touched_init(I, nT, P);
+    ibm_core<A>(I, nT, P, red, mode == 0, M);
     for (int sl = t; sl < nbox; sl += NTI) {
         const int ti = S.tindex[sl];
         S.fibx[sl] = ti >= 0 ? P.fbx[ti] : 0.0;

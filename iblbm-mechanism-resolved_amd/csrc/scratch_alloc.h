@@ -45,6 +45,7 @@ inline bool alloc_ibm_scratch(Inst &d, int lb_cap, int box_cap, Get get) {
     D(&s.vs, B);
     D(&s.tfibx, B);
     D(&s.tfiby, B);
+    D(&s.rot, B);
     D(&s.wt, B);
     D(&s.cw, KL);
     I(&s.cslot, KL);

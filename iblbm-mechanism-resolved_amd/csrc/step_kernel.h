@@ -28,7 +28,8 @@ struct IbmScratch {
     // per touched node (<= box_cap)
     int *tslot;               // touched index -> box slot
     int *toff;                // gather-list offsets (box_cap + 1)
-    double *uwx, *uwy, *vs, *tfibx, *tfiby, *wt;  // global fallback of the LDS arrays, W_total
+    double *uwx, *uwy, *vs, *tfibx, *tfiby, *rot;  // global fallback of the LDS arrays
+    double *wt;               // DFC W_total
     // per contribution, marker-major c_m = k*K + s (K = stencil points, lb_cap*25)
     double *cw;               // wx*wy
     int *cslot, *cti;         // box slot, touched index
