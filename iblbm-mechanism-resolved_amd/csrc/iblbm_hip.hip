@@ -105,8 +105,18 @@ __global__ __launch_bounds__(NT_BULK) void k_bulk(LaunchTable T) {
 template <typename S, int MODEL>
 __global__ __launch_bounds__(NTI) void k_ibm(LaunchTable T) {
     __shared__ double red[NTI];
-    Inst &I = T.insts[T.ibm_inst[blockIdx.x]];
+    Inst &I = T.insts[T.ibm_inst[T.n_fast + blockIdx.x]];
     ibm_role<S, MODEL>(I, T.parity, T.diag, red);
+}
+
+// The same for instances whose window takes the register-resident path.
+template <typename S, int MODEL>
+__global__ __launch_bounds__(NTI) void k_ibm_fast(LaunchTable T) {
+    __shared__ double red[NTI];
+    __builtin_amdgcn_s_setprio(3);  // issue ahead of bulk wavefronts sharing the CU
+    const int b = blockIdx.x;
+    Inst &I = T.insts[b < 8 ? T.fast_id[b] : T.ibm_inst[b]];
+    ibm_fast_dispatch<S, MODEL>(I, T.parity, T.diag, red);
 }
 
 // Window tables of one instance for newly set markers (one workgroup).
@@ -209,7 +219,7 @@ struct Ctx {
     double *d_rho_partial = nullptr;
     int cap_inst = 0, cap_blocks = 0;
     bool tables_dirty = true;
-    int n_ibm = 0, n_tiles = 0;
+    int n_ibm = 0, n_fast = 0, n_tiles = 0;
     std::vector<int> h_tile_prefix, h_ibm_inst;
     int64_t step = 0;
     int parity = 0;
@@ -227,7 +237,12 @@ int ensure_tables(Ctx &C) {
     for (int k = 0; k < n; ++k) {
         const Inst &d = C.inst[k].d;
         C.h_tile_prefix[k + 1] = C.h_tile_prefix[k] + d.tiles_x * d.tiles_y;
-        if (d.ibm != 0) C.h_ibm_inst.push_back(k);
+        if (d.ibm != 0 && d.fast) C.h_ibm_inst.push_back(k);
+    }
+    C.n_fast = (int)C.h_ibm_inst.size();
+    for (int k = 0; k < n; ++k) {
+        const Inst &d = C.inst[k].d;
+        if (d.ibm != 0 && !d.fast) C.h_ibm_inst.push_back(k);
     }
     C.n_tiles = C.h_tile_prefix[n];
     C.n_ibm = (int)C.h_ibm_inst.size();
@@ -280,6 +295,8 @@ LaunchTable make_table(Ctx &C, int parity, int diag) {
     T.rho_partial = C.d_rho_partial;
     T.n_inst = (int)C.inst.size();
     T.n_ibm = C.n_ibm;
+    T.n_fast = C.n_fast;
+    for (int b = 0; b < 8; ++b) T.fast_id[b] = b < C.n_fast ? C.h_ibm_inst[b] : 0;
     T.parity = parity;
     T.diag = diag;
     return T;
@@ -288,29 +305,30 @@ LaunchTable make_table(Ctx &C, int parity, int diag) {
 typedef void (*StepKernel)(LaunchTable);
 
 template <typename S, int MODEL>
-void pick_pair(StepKernel &bulk, StepKernel &ibm) {
+void pick_pair(StepKernel &bulk, StepKernel &ibm, StepKernel &fast) {
     bulk = k_bulk<S, MODEL>;
     ibm = k_ibm<S, MODEL>;
+    fast = k_ibm_fast<S, MODEL>;
 }
 
-void pick_kernels(const Ctx &C, StepKernel &bulk, StepKernel &ibm) {
+void pick_kernels(const Ctx &C, StepKernel &bulk, StepKernel &ibm, StepKernel &fast) {
     int model = C.inst[0].d.cp.model;
     for (const HostInst &h : C.inst)
         if (h.d.cp.model != model) model = -1;
     const bool f32 = C.inst[0].d.prec == 1;
     switch (model * 2 + (f32 ? 1 : 0)) {
-        case 0: return pick_pair<double, 0>(bulk, ibm);
-        case 1: return pick_pair<float, 0>(bulk, ibm);
-        case 2: return pick_pair<double, 1>(bulk, ibm);
-        case 3: return pick_pair<float, 1>(bulk, ibm);
-        case 4: return pick_pair<double, 2>(bulk, ibm);
-        case 5: return pick_pair<float, 2>(bulk, ibm);
+        case 0: return pick_pair<double, 0>(bulk, ibm, fast);
+        case 1: return pick_pair<float, 0>(bulk, ibm, fast);
+        case 2: return pick_pair<double, 1>(bulk, ibm, fast);
+        case 3: return pick_pair<float, 1>(bulk, ibm, fast);
+        case 4: return pick_pair<double, 2>(bulk, ibm, fast);
+        case 5: return pick_pair<float, 2>(bulk, ibm, fast);
         default: break;
     }
     if (f32)
-        pick_pair<float, -1>(bulk, ibm);
+        pick_pair<float, -1>(bulk, ibm, fast);
     else
-        pick_pair<double, -1>(bulk, ibm);
+        pick_pair<double, -1>(bulk, ibm, fast);
 }
 
 // Enqueues nsteps steps.  The bulk kernel runs on the context stream and the IBM kernel on a
@@ -318,8 +336,9 @@ void pick_kernels(const Ctx &C, StepKernel &bulk, StepKernel &ibm) {
 // window workgroups overlap the bulk tiles of the same step.  With concurrent = false both
 // run in order on the context stream.
 int launch_steps(Ctx &C, int nsteps, int diag_every, bool concurrent) {
-    StepKernel kb, ki;
-    pick_kernels(C, kb, ki);
+    StepKernel kb, ki, kf;
+    pick_kernels(C, kb, ki, kf);
+    const int n_gen = C.n_ibm - C.n_fast;
     const bool two = concurrent && C.n_ibm > 0;
     if (two) {  // join the side stream to everything already queued on the main stream
         HIPCHK(hipEventRecord(C.ev_main, C.stream));
@@ -334,12 +353,14 @@ int launch_steps(Ctx &C, int nsteps, int diag_every, bool concurrent) {
                 HIPCHK(hipStreamWaitEvent(C.stream, C.ev_side, 0));
                 HIPCHK(hipStreamWaitEvent(C.side, C.ev_main, 0));
             }
-            hipLaunchKernelGGL(ki, dim3(C.n_ibm), dim3(NTI), 0, C.side, T);
+            if (C.n_fast) hipLaunchKernelGGL(kf, dim3(C.n_fast), dim3(NTI), 0, C.side, T);
+            if (n_gen) hipLaunchKernelGGL(ki, dim3(n_gen), dim3(NTI), 0, C.side, T);
             hipLaunchKernelGGL(kb, dim3(C.n_tiles), dim3(NT_BULK), 0, C.stream, T);
             HIPCHK(hipEventRecord(C.ev_main, C.stream));
             HIPCHK(hipEventRecord(C.ev_side, C.side));
         } else {
-            if (C.n_ibm) hipLaunchKernelGGL(ki, dim3(C.n_ibm), dim3(NTI), 0, C.stream, T);
+            if (C.n_fast) hipLaunchKernelGGL(kf, dim3(C.n_fast), dim3(NTI), 0, C.stream, T);
+            if (n_gen) hipLaunchKernelGGL(ki, dim3(n_gen), dim3(NTI), 0, C.stream, T);
             if (C.n_tiles) hipLaunchKernelGGL(kb, dim3(C.n_tiles), dim3(NT_BULK), 0, C.stream, T);
         }
     }
@@ -569,11 +590,18 @@ int iblbm_set_markers(void *p, int id, const double *Lx, const double *Ly, const
     // window tables, DFC lambda and a zeroed box force, ordered after every queued step
     hipLaunchKernelGGL(k_tables, dim3(1), dim3(NTI), 0, C->stream, C->d_insts + id);
     HIPCHK(hipGetLastError());
-    // the touched-node count goes back into the host descriptor, so later uploads keep it;
-    // the synchronisation also retires the pageable marker copies before returning
+    // the touched-node count and path flag go back into the host descriptor, so later
+    // uploads keep them; the synchronisation also retires the pageable marker copies
+    static_assert(offsetof(Inst, fast) == offsetof(Inst, nT) + sizeof(int), "nT, fast adjacent");
+    const int fast_before = d.fast;
     HIPCHK(hipMemcpyAsync(&d.nT, (const char *)(C->d_insts + id) + offsetof(Inst, nT),
-                          sizeof(int), hipMemcpyDeviceToHost, C->stream));
+                          2 * sizeof(int), hipMemcpyDeviceToHost, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
+    if (d.fast != fast_before) {  // the window workgroup changes kernel: rebuild the lists
+        C->tables_dirty = true;
+        int rc = ensure_tables(*C);
+        if (rc) return rc;
+    }
     return 0;
 }
 

@@ -39,7 +39,7 @@ inline bool alloc_ibm_scratch(Inst &d, int lb_cap, int box_cap, Get get) {
     I(&s.cur, B);
     I(&s.tindex, B + 1);
     I(&s.tslot, B);
-    I(&s.toff, B + 1);
+    I(&s.toff, B + 4);  // padded to 16-byte copy granules
     D(&s.uwx, B);
     D(&s.uwy, B);
     D(&s.vs, B);
@@ -51,8 +51,8 @@ inline bool alloc_ibm_scratch(Inst &d, int lb_cap, int box_cap, Get get) {
     I(&s.cslot, KL);
     I(&s.cti, KL);
     I(&s.gc, KL);
-    I(&s.gk, KL);
-    D(&s.gw, KL);
+    I(&s.gk, KL + 4);
+    D(&s.gw, KL + 2);
     D(&s.Lux, L);
     D(&s.Luy, L);
     D(&s.R, L);
@@ -61,6 +61,9 @@ inline bool alloc_ibm_scratch(Inst &d, int lb_cap, int box_cap, Get get) {
     D(&s.lam, L);
     D(&s.q, 2 * L);
     D(&s.dfl, 9 * L);
+    I(&s.pk, (size_t)FAST_G * FAST_T);
+    I(&s.pn, FAST_T);
+    D(&s.pw, (size_t)FAST_G * FAST_T);
     D(&d.stats, 4);
     D(&d.diag, 8);
     d.lb_cap = (int)L;

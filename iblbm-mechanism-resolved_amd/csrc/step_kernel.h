@@ -16,7 +16,10 @@ namespace iblbm {
 constexpr int NT_BULK = 256;  // bulk tile workgroup: 4 wavefronts, a 64 x 4 node tile
 constexpr int TILE_X = 64;
 constexpr int TILE_Y = NT_BULK / TILE_X;
-constexpr int NTI = 512;      // immersed-boundary workgroup: 8 wavefronts
+constexpr int NTI = 512;
+constexpr int FAST_NPT = 2;                     // touched nodes per node thread (fast path)
+constexpr int FAST_T = FAST_NPT * (NTI - 64);  // touched-node capacity of the fast path
+constexpr int FAST_G = 8;                       // gather entries per touched node (fast path)      // immersed-boundary workgroup: 8 wavefronts
 
 struct IbmScratch {
     // per box slot (box_cap)
@@ -39,6 +42,9 @@ struct IbmScratch {
     // per marker (lb_cap)
     double *Lux, *Luy, *R, *Lfx, *Lfy, *lam, *q;  // q: DFC force (Lb, 2)
     double *dfl;              // DFC delta f (Lb, 9)
+    // gather lists padded to FAST_G entries, entry-major [e * FAST_T + ti] (fast path)
+    int *pk, *pn;             // marker of entry e, entries per touched node
+    double *pw;               // wx*wy*Larea of entry e (0 beyond pn)
 };
 
 struct Inst {
@@ -57,7 +63,7 @@ struct Inst {
     int ibm, delta, Lb, lb_cap;
     int bx0, by0, bw, bh;
     int box_cap, n_iter, min_iter, fixed_body;
-    int nT, pad2;
+    int nT, fast;  // touched nodes; 1 when the register-resident window path applies
     double dx, dy, Larea, omega, tol, dfc_scale;
     const double *Lx, *Ly, *ud;
     IbmScratch s;
@@ -70,10 +76,11 @@ struct Inst {
 
 struct LaunchTable {
     Inst *insts;
-    const int *ibm_inst;     // instance id of each IBM workgroup
+    const int *ibm_inst;     // instance ids of the window workgroups: fast-path ones first
     const int *tile_prefix;  // tile_prefix[i]: first bulk tile of instance i (n_inst + 1 entries)
-    double *rho_partial;     // one FP64 partial rho sum per workgroup (diagnostic steps)
-    int n_inst, n_ibm;
+    double *rho_partial;     // one FP64 partial rho sum per bulk workgroup (diagnostic steps)
+    int n_inst, n_ibm, n_fast;
+    int fast_id[8];          // the first fast-path instance ids, passed by value
     int parity;              // source buffer index of this step
     int diag;                // take diagnostic reductions this step
 };
