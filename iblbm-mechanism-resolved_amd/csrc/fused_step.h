@@ -1,0 +1,393 @@
+// fused_step.h -- the window role of the one-launch time step (k_step).
+//
+// A context whose windows all pass ibm_fast_lists runs each time step as ONE launch of
+// NTF-thread workgroups: the first workgroups run one instance's window each, the others run
+// bulk tiles.  One launch per step leaves no cross-stream wait between steps, and the
+// register budget of 128 VGPRs (4 wavefronts per SIMD) keeps the bulk tiles streaming.
+//
+// Window workgroup: the first ceil(Lb/64) wavefronts are marker wavefronts (thread t < Lb owns
+// marker t: interpolation, force, residual); every other thread owns at most one touched node
+// (its gather list, MDF working velocity and accumulated force in registers).  Marker stencils,
+// the working velocity, the marker forces and the post-stream populations live in LDS.  Both
+// roles run the same barrier sequence and take the same (uniform) loop exits.  Arithmetic and
+// summation orders are those of ibm_role_impl / ibm_core (DF: iblbm/ibm/df.py:94-154,
+// MDF: iblbm/ibm/mdf.py:175-229, velocity correction: iblbm/ibm/dispatch.py:18-33).
+#pragma once
+#include "ibm_block.h"
+
+namespace iblbm {
+
+struct FusedShared {
+    alignas(16) double uw[2 * FAST_T];         // MDF working velocity of the touched nodes
+    alignas(16) double lf[2 * (FAST_LB + 1)];  // marker forces; entry Lb is the zero force
+    double rot[FAST_T];                         // rho of the touched nodes
+    double mw[FAST_K * FAST_LB];                // marker stencil weights, [s * Lb + k]
+    int mti[FAST_K * FAST_LB];                  // marker stencil touched indices, [s * Lb + k]
+    double fs[9 * FAST_T];                      // post-stream populations of the touched nodes
+};
+
+// Copies the descriptor fields the window reads into a local descriptor, in one batch at the
+// start (reads of the global one after a store would be reissued one at a time).
+__device__ inline void light_copy(const Inst &G, Inst &L) {
+    L.nx = G.nx;
+    L.ny = G.ny;
+    L.pitch = G.pitch;
+    L.qs = G.qs;
+    L.f[0] = G.f[0];
+    L.f[1] = G.f[1];
+    L.bc = G.bc;
+    L.inc = G.inc;
+    L.cp = G.cp;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) L.feq_bc[q] = G.feq_bc[q];
+    L.ibm = G.ibm;
+    L.Lb = G.Lb;
+    L.bx0 = G.bx0;
+    L.by0 = G.by0;
+    L.bw = G.bw;
+    L.bh = G.bh;
+    L.box_cap = G.box_cap;
+    L.n_iter = G.n_iter;
+    L.min_iter = G.min_iter;
+    L.nT = G.nT;
+    L.omega = G.omega;
+    L.tol = G.tol;
+    L.ud = G.ud;
+    L.s = G.s;
+    L.stats = G.stats;
+    L.diag = G.diag;
+    L.cxc = G.cxc;
+    L.cyc = G.cyc;
+}
+
+// Sum over the NTF threads: fixed shuffle tree per wavefront, then the wavefronts in order
+// (the result is valid in thread 0).
+__device__ inline double fused_block_sum(double v, double *red) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+        r = red[0];
+        for (int k = 1; k < NTF / 64; ++k) r = r + red[k];
+    }
+    __syncthreads();
+    return r;
+}
+
+template <typename ST, int MODEL, int A>
+__device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int parity, int diag,
+                                                             double *red, FusedShared &sh) {
+    constexpr int K = (2 * A + 1) * (2 * A + 1);
+    constexpr int GRP = 8;  // stencil points per batch of LDS loads
+    Inst I;
+    light_copy(Ig, I);
+    const int t = threadIdx.x;
+    IbmScratch &S = I.s;
+    const bool inc = I.inc != 0;
+    const double dt = I.cp.dt;
+    const int Lb = I.Lb, nT = I.nT;
+    const int mwv = (Lb + 63) >> 6;
+    const bool marker_wave = (t >> 6) < mwv;
+    const int n = t - 64 * mwv;  // touched node of a node thread
+    const bool hn = !marker_wave && n < nT;
+    double2 *uw = reinterpret_cast<double2 *>(sh.uw);
+    double2 *lf = reinterpret_cast<double2 *>(sh.lf);
+    unsigned long long *rbits = reinterpret_cast<unsigned long long *>(red) + NTF / 64;
+    const bool mdf = I.ibm == 2;
+    const bool finc = mdf ? false : inc;  // MDF sub-steps are compressible (mdf.py:177-181)
+    const int n_iter = mdf ? I.n_iter : 1;
+    const int min_iter = max(1, min(I.min_iter, n_iter));
+    const double tol = I.tol;
+    double prev = INFINITY, last = INFINITY;
+    int iter_count = 0, diverged = 0, converged = 0;
+
+    // Every global load of the prologue is issued before the first wait: the marker stencils
+    // (for LDS), a marker's desired velocity, a node's gather list, the box slots of P1.
+    constexpr int SU = (FAST_K * FAST_LB + NTF - 1) / NTF;  // stencil entries per thread
+    int st_i[SU];
+    double st_w[SU];
+#pragma unroll
+    for (int u = 0; u < SU; ++u) {
+        const int e = t + u * NTF;
+        if (e < K * Lb) {
+            st_i[u] = S.mti[e];
+            st_w[u] = S.mw[e];
+        }
+    }
+    double udx = 0.0, udy = 0.0;
+    if (t < Lb) {
+        udx = I.ud[2 * t];
+        udy = I.ud[2 * t + 1];
+    }
+    int sl = 0, kk[FAST_G];
+    double ww[FAST_G];
+    if (hn) {
+        sl = S.tslot[n];
+#pragma unroll
+        for (int e = 0; e < FAST_G; ++e) {
+            kk[e] = S.pk[e * FAST_T + n];
+            ww[e] = S.pw[e * FAST_T + n];
+        }
+    }
+
+    // P1 (all threads): box slots, two per thread per pass; a slot's touched index and its 9
+    // pull values are loaded together, touched slots keep theirs
+    const ST *__restrict__ Asrc = reinterpret_cast<const ST *>(parity ? I.f[1] : I.f[0]);
+    const int nbox = I.bw * I.bh;
+    const bool interior = I.bx0 >= 1 && I.by0 >= 1 && I.bx0 + I.bw <= I.nx - 1 &&
+                          I.by0 + I.bh <= I.ny - 1;
+    for (int base = 0; base < nbox; base += 2 * NTF) {
+        int tx[2];
+        double fs[2][9];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int s0 = base + u * NTF + t;
+            tx[u] = -1;
+            if (s0 < nbox) {
+                tx[u] = S.tindex[s0];
+                const int i = I.bx0 + s0 % I.bw, j = I.by0 + s0 / I.bw;
+                if (interior)
+                    pull_interior(Asrc, I, i, j, fs[u]);
+                else
+                    fstar_at(Asrc, I, i, j, fs[u]);
+            }
+        }
+        if (base == 0) {
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                const int e = t + u * NTF;
+                if (e < K * Lb) {
+                    sh.mti[e] = st_i[u];
+                    sh.mw[e] = st_w[u];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int m = tx[u];
+            if (m >= 0) {
+                double r, u0, u1;
+#pragma unroll
+                for (int q = 0; q < 9; ++q) sh.fs[q * FAST_T + m] = fs[u][q];
+                macro9(fs[u], inc, r, u0, u1);
+                uw[m] = make_double2(u0, u1);
+                sh.rot[m] = r;
+            }
+        }
+    }
+    if (t < 2) rbits[t] = 0ull;
+    if (t == 0) lf[Lb] = make_double2(0.0, 0.0);  // target of the padded gather entries
+    __syncthreads();  // P1 done
+
+    double rho_acc = 0.0, fxs = 0.0, fys = 0.0, trq = 0.0;
+    if (marker_wave) {
+        int ti[K];
+        if (t < Lb) {
+#pragma unroll
+            for (int s = 0; s < K; ++s) ti[s] = sh.mti[s * Lb + t];
+        }
+        double R = 0.0;
+        if (t < Lb) {  // rho at the marker
+            double rs[K];
+#pragma unroll
+            for (int s = 0; s < K; ++s) rs[s] = sh.rot[ti[s]];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int s = 0; s < K; ++s) R += rs[s] * sh.mw[s * Lb + t];
+        }
+        const bool unit_dt = dt == 1.0;  // a division by 1 is exact and skipped
+        for (int it = 0; it < n_iter; ++it) {
+            double rv = 0.0;
+            if (t < Lb) {  // interpolation + force (df.py:239-263), loads batched by GRP
+                double lx = 0.0, ly = 0.0;
+#pragma unroll
+                for (int g0 = 0; g0 < K; g0 += GRP) {
+                    double2 u[GRP];
+                    double w[GRP];
+#pragma unroll
+                    for (int s = g0; s < g0 + GRP && s < K; ++s) {
+                        u[s - g0] = uw[ti[s]];
+                        w[s - g0] = sh.mw[s * Lb + t];
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int s = g0; s < g0 + GRP && s < K; ++s) {
+                        lx += u[s - g0].x * w[s - g0];
+                        ly += u[s - g0].y * w[s - g0];
+                    }
+                }
+                double Fx, Fy;
+                if (finc) {
+                    Fx = 2.0 * (udx - lx);
+                    Fy = 2.0 * (udy - ly);
+                } else {
+                    Fx = 2.0 * R * (udx - lx);
+                    Fy = 2.0 * R * (udy - ly);
+                }
+                if (!unit_dt) {
+                    Fx = Fx / dt;
+                    Fy = Fy / dt;
+                }
+                lf[t] = make_double2(Fx, Fy);
+                const double vx = udx - lx, vy = udy - ly;
+                rv = sqrt(vx * vx + vy * vy);
+            }
+            // L-inf residual: an unsigned max over the bit patterns (non-negative doubles order
+            // like their bits and every NaN pattern lies above +inf, so NaN propagates as in
+            // np.max); a marker-less lane contributes 0, the identity; two slots alternate
+            const unsigned long long wb = dpp_max_u64_lane63(
+                t < Lb ? (unsigned long long)__double_as_longlong(rv) : 0ull);
+            if ((t & 63) == 63) atomicMax(&rbits[it & 1], wb);
+            __syncthreads();  // A: forces and residual published
+            const double res = __longlong_as_double((long long)rbits[it & 1]);
+            if (t == 0) rbits[(it + 1) & 1] = 0ull;  // its last reads preceded barrier B
+            if (mdf) {
+                last = res;
+                if (it > 0 && res > prev * 1.5) {  // divergence guard, mdf.py:192-195
+                    diverged = 1;
+                    iter_count = it + 1;
+                    break;
+                }
+            }
+            __syncthreads();  // B: velocities updated
+            if (mdf) {
+                prev = res;
+                iter_count = it + 1;
+                if (it + 1 >= min_iter && res < tol) {
+                    converged = 1;
+                    break;
+                }
+            }
+        }
+        if (mdf && t == 0) {
+            I.stats[0] = iter_count;
+            I.stats[1] = last;
+            I.stats[2] = diverged;
+            I.stats[3] = converged;
+        }
+    } else {
+        ST *__restrict__ B = reinterpret_cast<ST *>(parity ? I.f[0] : I.f[1]);
+        const double omega = I.omega;
+        double r = 1.0, u0 = 0.0, u1 = 0.0;
+        if (hn) {
+            const double2 v = uw[n];
+            r = sh.rot[n];
+            u0 = v.x;
+            u1 = v.y;
+        }
+        double x = u0, y = u1, fx = 0.0, fy = 0.0;
+        const double vs = dt / (2.0 * r);  // mdf.py:117
+        for (int it = 0; it < n_iter; ++it) {
+            __syncthreads();  // A
+            const double res = __longlong_as_double((long long)rbits[it & 1]);
+            if (mdf) {
+                last = res;
+                if (it > 0 && res > prev * 1.5) {
+                    diverged = 1;
+                    iter_count = it + 1;
+                    break;
+                }
+            }
+            if (hn) {  // spread as a gather in np.add.at order (df.py:269-283)
+                double2 g[FAST_G];
+#pragma unroll
+                for (int e = 0; e < FAST_G; ++e) g[e] = lf[kk[e]];
+                __builtin_amdgcn_sched_barrier(0);
+                // padded entries point at the zero force with weight 0: they add +0.0, which
+                // leaves the sum's bits unchanged (it starts at +0.0 and so is never -0.0)
+                double sx = 0.0, sy = 0.0;
+#pragma unroll
+                for (int e = 0; e < FAST_G; ++e) {
+                    sx += g[e].x * ww[e];
+                    sy += g[e].y * ww[e];
+                }
+                if (mdf) {  // mdf.py:197-200
+                    const double gx = sx * omega, gy = sy * omega;
+                    fx = fx + gx;
+                    fy = fy + gy;
+                    x = x + gx * vs;
+                    y = y + gy * vs;
+                } else {
+                    fx = sx;
+                    fy = sy;
+                }
+                uw[n] = make_double2(x, y);
+            }
+            __syncthreads();  // B
+            if (mdf) {
+                prev = res;
+                iter_count = it + 1;
+                if (it + 1 >= min_iter && res < tol) {
+                    converged = 1;
+                    break;
+                }
+            }
+        }
+        // P3: velocity correction, equilibrium, forced collision, store
+        if (hn) {
+            const int i = I.bx0 + sl % I.bw, j = I.by0 + sl / I.bw;
+            double fs[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                fs[q] = sh.fs[q * FAST_T + n];
+                S.fs[q * I.box_cap + sl] = fs[q];  // the step's state, for downloads
+            }
+            double v0 = u0, v1 = u1;
+            if (inc) {
+                v0 = v0 + 0.5 * fx * dt;
+                v1 = v1 + 0.5 * fy * dt;
+            } else {
+                v0 = v0 + fx * dt / (2.0 * r);
+                v1 = v1 + fy * dt / (2.0 * r);
+            }
+            S.ro[sl] = r;
+            S.ux0[sl] = v0;
+            S.uy0[sl] = v1;
+            S.fibx[sl] = fx;
+            S.fiby[sl] = fy;
+            double fe[9], out[9];
+            if (MODEL != 2) feq9(r, v0, v1, inc, fe);
+            if constexpr (MODEL == 0)
+                collide_bgk<true>(I.cp, fs, fe, v0, v1, fx, fy, out);
+            else if constexpr (MODEL == 1)
+                collide_trt<true>(I.cp, fs, fe, v0, v1, fx, fy, out);
+            else if constexpr (MODEL == 2)
+                collide_cm<true>(I.cp, fs, v0, v1, fx, fy, out);
+            else
+                collide<true>(I.cp, fs, fe, v0, v1, fx, fy, out);
+            store9(B, I, i, j, out);
+            if (diag) {
+                rho_acc = r;
+                fxs = fx;
+                fys = fy;
+                trq = ((double)i - I.cxc) * fy - ((double)j - I.cyc) * fx;
+            }
+        }
+    }
+    if (diag) {
+        const double rs = fused_block_sum(rho_acc, red);
+        const double f0 = fused_block_sum(fxs, red);
+        const double f1 = fused_block_sum(fys, red);
+        const double tq = fused_block_sum(trq, red);
+        if (t == 0) {
+            I.diag[0] = rs;
+            I.diag[1] = f0;
+            I.diag[2] = f1;
+            I.diag[3] = tq;
+            I.diag[4] = 0.0;
+        }
+    }
+}
+
+template <typename ST, int MODEL>
+__device__ __attribute__((always_inline)) void ibm_fused_dispatch(Inst &I, int parity, int diag,
+                                                                 double *red, FusedShared &sh) {
+    if (I.delta == 1)
+        ibm_fused_role<ST, MODEL, 1>(I, parity, diag, red, sh);
+    else
+        ibm_fused_role<ST, MODEL, 2>(I, parity, diag, red, sh);
+}
+
+}  // namespace iblbm

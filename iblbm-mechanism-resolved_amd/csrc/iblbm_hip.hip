@@ -13,20 +13,25 @@
 #include <vector>
 
 #include "../../include/iblbm_hip.h"
-#include "ibm_block.h"
+#include "fused_step.h"
 #include "scratch_alloc.h"
 
 using namespace iblbm;
 
 // ------------------------------------------------------------------ kernels
 
-// Bulk lattice node in FP64 (reference arithmetic, bit-exact).
+// Bulk lattice node in FP64 (reference arithmetic, bit-exact), in two halves: the pull
+// (loads) and the moments + collision + store, so that a thread can have several nodes' loads
+// in flight.
+__device__ __attribute__((always_inline)) void bulk_pull(const Inst &I, int parity, int i, int j,
+                                                        double *fs) {
+    fstar_at(buf<double>(I, parity), I, i, j, fs);
+}
 template <int MODEL>
-__device__ inline double bulk_node_fp64(const Inst &I, int parity, int i, int j) {
-    const double *__restrict__ A = buf<double>(I, parity);
+__device__ __attribute__((always_inline)) double bulk_finish(const Inst &I, int parity, int i,
+                                                            int j, double *fs) {
     double *__restrict__ B = buf<double>(I, parity ^ 1);
-    double fs[9], fe[9], out[9];
-    fstar_at(A, I, i, j, fs);
+    double fe[9], out[9];
     double r, u0, u1;
     const bool inc = I.inc != 0;
     macro9(fs, inc, r, u0, u1);
@@ -46,13 +51,16 @@ __device__ inline double bulk_node_fp64(const Inst &I, int parity, int i, int j)
     return r;
 }
 
-// Bulk lattice node in FP32 deviation form.
+// The same in FP32 deviation form.
+__device__ __attribute__((always_inline)) void bulk_pull(const Inst &I, int parity, int i, int j,
+                                                        float *g) {
+    fstar_dev32(buf<float>(I, parity), I, i, j, g);
+}
 template <int MODEL>
-__device__ inline double bulk_node_fp32(const Inst &I, int parity, int i, int j) {
-    const float *__restrict__ A = buf<float>(I, parity);
+__device__ __attribute__((always_inline)) double bulk_finish(const Inst &I, int parity, int i,
+                                                            int j, float *g) {
     float *__restrict__ B = buf<float>(I, parity ^ 1);
-    float g[9], out[9];
-    fstar_dev32(A, I, i, j, g);
+    float out[9];
     float drho, ux, uy;
     const bool inc = I.inc != 0;
     moments32(g, inc, drho, ux, uy);
@@ -71,33 +79,76 @@ __device__ inline double bulk_node_fp32(const Inst &I, int parity, int i, int j)
     return 1.0 + (double)drho;
 }
 
+// Pull of a node whose 8 neighbours all lie inside the grid (no closure, no branches).
+__device__ __attribute__((always_inline)) void bulk_pull_interior(const Inst &I, int parity, int i,
+                                                                 int j, double *fs) {
+    pull_interior(buf<double>(I, parity), I, i, j, fs);
+}
+__device__ __attribute__((always_inline)) void bulk_pull_interior(const Inst &I, int parity, int i,
+                                                                 int j, float *g) {
+    const float *__restrict__ A = buf<float>(I, parity);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) g[q] = A[q * I.qs + (int64_t)(j - EYi[q]) * I.pitch + (i - EXi[q])];
+}
+
+template <typename S, int MODEL>
+__device__ __attribute__((always_inline)) double bulk_node(const Inst &I, int parity, int i,
+                                                          int j) {
+    S v[9];
+    bulk_pull(I, parity, i, j, v);
+    return bulk_finish<MODEL>(I, parity, i, j, v);
+}
+
+// The bulk work is a list of rows: row r is row r % TILE_Y of tile r / TILE_Y (TILE_X nodes,
+// one per lane of a wavefront).  A lane's node: instance, coordinates, and whether the node is
+// the bulk's (inside the grid, not touched by a marker stencil).
+struct TileNode {
+    const Inst *I;
+    int i, j;
+    bool mine;
+    bool interior;  // (uniform) every node of the row and all its neighbours are in the grid
+};
+// row is uniform over the wavefront: the instance lookup and the descriptor reads are scalar
+// loads through the constant address space (the tables are read-only during a step).
+__device__ __attribute__((always_inline)) TileNode row_node(const LaunchTable &T, int row,
+                                                           int lane) {
+    typedef __attribute__((address_space(4))) const int CInt;
+    typedef __attribute__((address_space(4))) Inst CInst;
+    row = __builtin_amdgcn_readfirstlane(row);
+    const int tb = row / TILE_Y;
+    const CInt *prefix = (const CInt *)T.tile_prefix;
+    int inst = 0;
+    while (inst + 1 < T.n_inst && prefix[inst + 1] <= tb) ++inst;
+    inst = __builtin_amdgcn_readfirstlane(inst);
+    const Inst &I = *(const Inst *)((const CInst *)T.insts + inst);
+    const int lt = tb - prefix[inst];
+    const int tx = lt % I.tiles_x, ty = lt / I.tiles_x;
+    TileNode n;
+    n.I = &I;
+    n.i = tx * TILE_X + lane;
+    n.j = ty * TILE_Y + row % TILE_Y;
+    const bool in_grid = n.i < I.nx && n.j < I.ny;
+    n.interior = tx * TILE_X >= 1 && tx * TILE_X + TILE_X <= I.nx - 1 && n.j >= 1 &&
+                 n.j <= I.ny - 2;
+    // nodes a marker stencil touches belong to the window workgroup
+    const int ibm = I.ibm, bx0 = I.bx0, by0 = I.by0, bw = I.bw, bh = I.bh;
+    const bool in_box = ibm != 0 & n.i >= bx0 & n.i < bx0 + bw & n.j >= by0 & n.j < by0 + bh;
+    bool touched = false;
+    if (in_box) touched = I.s.tindex[(n.j - by0) * bw + (n.i - bx0)] >= 0;
+    n.mine = in_grid && !touched;
+    return n;
+}
+
 // Bulk tiles of one time step: every node that no marker stencil touches.
 template <typename S, int MODEL>
 __global__ __launch_bounds__(NT_BULK) void k_bulk(LaunchTable T) {
-    __shared__ double red[NT_BULK / 64];
-    const int tb = blockIdx.x;
-    int inst = 0;
-    while (inst + 1 < T.n_inst && T.tile_prefix[inst + 1] <= tb) ++inst;
-    const Inst &I = T.insts[inst];
-    const int lt = tb - T.tile_prefix[inst];
-    const int tx = lt % I.tiles_x, ty = lt / I.tiles_x;
-    const int i = tx * TILE_X + (int)(threadIdx.x % TILE_X);
-    const int j = ty * TILE_Y + (int)(threadIdx.x / TILE_X);
+    const int row = (int)blockIdx.x * (NT_BULK / 64) + (int)(threadIdx.x >> 6);
+    const TileNode n = row_node(T, row, (int)(threadIdx.x & 63));
     double rho = 0.0;
-    const bool in_grid = i < I.nx && j < I.ny;
-    // nodes a marker stencil touches belong to the window workgroup
-    const bool touched = I.ibm != 0 && i >= I.bx0 && i < I.bx0 + I.bw && j >= I.by0 &&
-                         j < I.by0 + I.bh &&
-                         I.s.tindex[(j - I.by0) * I.bw + (i - I.bx0)] >= 0;
-    if (in_grid && !touched) {
-        if constexpr (sizeof(S) == 8)
-            rho = bulk_node_fp64<MODEL>(I, T.parity, i, j);
-        else
-            rho = bulk_node_fp32<MODEL>(I, T.parity, i, j);
-    }
-    if (T.diag) {
-        const double s = bulk_block_sum(rho, red);
-        if (threadIdx.x == 0) T.rho_partial[tb] = s;
+    if (n.mine) rho = bulk_node<S, MODEL>(*n.I, T.parity, n.i, n.j);
+    if (T.diag) {  // one FP64 partial per row: a fixed shuffle tree over the wavefront
+        const double v = wave_sum(rho);
+        if ((threadIdx.x & 63) == 0) T.rho_partial[row] = v;
     }
 }
 
@@ -105,18 +156,60 @@ __global__ __launch_bounds__(NT_BULK) void k_bulk(LaunchTable T) {
 template <typename S, int MODEL>
 __global__ __launch_bounds__(NTI) void k_ibm(LaunchTable T) {
     __shared__ double red[NTI];
-    Inst &I = T.insts[T.ibm_inst[T.n_fast + blockIdx.x]];
+    Inst &I = T.insts[T.ibm_inst[blockIdx.x]];
     ibm_role<S, MODEL>(I, T.parity, T.diag, red);
 }
 
-// The same for instances whose window takes the register-resident path.
+// One whole time step in one launch, for contexts whose windows all take the fused path:
+// workgroups [0, n_fast) run the windows (fused_step.h), the others run four bulk tiles each
+// (one node per thread).  One launch per step leaves no cross-stream wait between steps.
 template <typename S, int MODEL>
-__global__ __launch_bounds__(NTI) void k_ibm_fast(LaunchTable T) {
-    __shared__ double red[NTI];
-    __builtin_amdgcn_s_setprio(3);  // issue ahead of bulk wavefronts sharing the CU
-    const int b = blockIdx.x;
-    Inst &I = T.insts[b < 8 ? T.fast_id[b] : T.ibm_inst[b]];
-    ibm_fast_dispatch<S, MODEL>(I, T.parity, T.diag, red);
+__global__ __launch_bounds__(NTF) void k_step(LaunchTable T) {
+    __shared__ double red[NTF / 64 + 2];
+    __shared__ FusedShared sh;
+    if ((int)blockIdx.x < T.n_fast) {
+        __builtin_amdgcn_s_setprio(3);  // issue ahead of bulk wavefronts sharing the CU
+        // The instance table and descriptors are read-only during a step: reading them through
+        // the constant address space with a uniform index makes them scalar (K$) loads.
+        typedef __attribute__((address_space(4))) const int CInt;
+        typedef __attribute__((address_space(4))) Inst CInst;
+        const int id = __builtin_amdgcn_readfirstlane(((CInt *)T.ibm_inst)[blockIdx.x]);
+        Inst &I = *(Inst *)((CInst *)T.insts + id);
+        ibm_fused_dispatch<S, MODEL>(I, T.parity, T.diag, red, sh);
+        return;
+    }
+    // bulk: every wavefront walks the rows with a grid stride, two rows in flight at a time
+    // (no workgroup barrier: a wavefront moves on as soon as its rows are done)
+    const int lane = (int)threadIdx.x & 63;
+    const int nw = ((int)gridDim.x - T.n_fast) * (NTF / 64);
+    const int n_rows = T.n_tiles * TILE_Y;
+    for (int r0 = ((int)blockIdx.x - T.n_fast) * (NTF / 64) + (int)(threadIdx.x >> 6);
+         r0 < n_rows; r0 += 2 * nw) {
+        const bool v1 = r0 + nw < n_rows;
+        const int r1 = v1 ? r0 + nw : r0;  // a missing second row repeats the first, unused
+        const TileNode n0 = row_node(T, r0, lane);
+        TileNode n1 = row_node(T, r1, lane);
+        n1.mine = n1.mine && v1;
+        S a0[9], a1[9];
+        if (n0.interior && n1.interior) {  // uniform: both rows' 18 loads in one batch
+            bulk_pull_interior(*n0.I, T.parity, n0.i, n0.j, a0);
+            bulk_pull_interior(*n1.I, T.parity, n1.i, n1.j, a1);
+            __builtin_amdgcn_sched_barrier(0);
+        } else {
+            if (n0.mine) bulk_pull(*n0.I, T.parity, n0.i, n0.j, a0);
+            if (n1.mine) bulk_pull(*n1.I, T.parity, n1.i, n1.j, a1);
+        }
+        double rho0 = 0.0, rho1 = 0.0;
+        if (n0.mine) rho0 = bulk_finish<MODEL>(*n0.I, T.parity, n0.i, n0.j, a0);
+        if (n1.mine) rho1 = bulk_finish<MODEL>(*n1.I, T.parity, n1.i, n1.j, a1);
+        if (T.diag) {  // one FP64 partial per row (same tree as k_bulk)
+            const double w0 = wave_sum(rho0), w1 = wave_sum(rho1);
+            if (lane == 0) {
+                T.rho_partial[r0] = w0;
+                if (v1) T.rho_partial[r1] = w1;
+            }
+        }
+    }
 }
 
 // Window tables of one instance for newly set markers (one workgroup).
@@ -209,6 +302,7 @@ struct HostInst {
 
 struct Ctx {
     int device = 0;
+    int n_cu = 256;
     hipStream_t stream = nullptr, side = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_main = nullptr, ev_side = nullptr;
     bool concurrent = true;
@@ -246,7 +340,7 @@ int ensure_tables(Ctx &C) {
     }
     C.n_tiles = C.h_tile_prefix[n];
     C.n_ibm = (int)C.h_ibm_inst.size();
-    const int blocks = C.n_tiles + C.n_ibm;
+    const int blocks = C.n_tiles * TILE_Y + C.n_ibm;  // row partials
     if (n > C.cap_inst) {
         int cap = n < 16 ? 16 : 2 * n;
         void *p;
@@ -296,7 +390,7 @@ LaunchTable make_table(Ctx &C, int parity, int diag) {
     T.n_inst = (int)C.inst.size();
     T.n_ibm = C.n_ibm;
     T.n_fast = C.n_fast;
-    for (int b = 0; b < 8; ++b) T.fast_id[b] = b < C.n_fast ? C.h_ibm_inst[b] : 0;
+    T.n_tiles = C.n_tiles;
     T.parity = parity;
     T.diag = diag;
     return T;
@@ -305,30 +399,30 @@ LaunchTable make_table(Ctx &C, int parity, int diag) {
 typedef void (*StepKernel)(LaunchTable);
 
 template <typename S, int MODEL>
-void pick_pair(StepKernel &bulk, StepKernel &ibm, StepKernel &fast) {
+void pick_pair(StepKernel &bulk, StepKernel &ibm, StepKernel &step) {
     bulk = k_bulk<S, MODEL>;
     ibm = k_ibm<S, MODEL>;
-    fast = k_ibm_fast<S, MODEL>;
+    step = k_step<S, MODEL>;
 }
 
-void pick_kernels(const Ctx &C, StepKernel &bulk, StepKernel &ibm, StepKernel &fast) {
+void pick_kernels(const Ctx &C, StepKernel &bulk, StepKernel &ibm, StepKernel &step) {
     int model = C.inst[0].d.cp.model;
     for (const HostInst &h : C.inst)
         if (h.d.cp.model != model) model = -1;
     const bool f32 = C.inst[0].d.prec == 1;
     switch (model * 2 + (f32 ? 1 : 0)) {
-        case 0: return pick_pair<double, 0>(bulk, ibm, fast);
-        case 1: return pick_pair<float, 0>(bulk, ibm, fast);
-        case 2: return pick_pair<double, 1>(bulk, ibm, fast);
-        case 3: return pick_pair<float, 1>(bulk, ibm, fast);
-        case 4: return pick_pair<double, 2>(bulk, ibm, fast);
-        case 5: return pick_pair<float, 2>(bulk, ibm, fast);
+        case 0: return pick_pair<double, 0>(bulk, ibm, step);
+        case 1: return pick_pair<float, 0>(bulk, ibm, step);
+        case 2: return pick_pair<double, 1>(bulk, ibm, step);
+        case 3: return pick_pair<float, 1>(bulk, ibm, step);
+        case 4: return pick_pair<double, 2>(bulk, ibm, step);
+        case 5: return pick_pair<float, 2>(bulk, ibm, step);
         default: break;
     }
     if (f32)
-        pick_pair<float, -1>(bulk, ibm, fast);
+        pick_pair<float, -1>(bulk, ibm, step);
     else
-        pick_pair<double, -1>(bulk, ibm, fast);
+        pick_pair<double, -1>(bulk, ibm, step);
 }
 
 // Enqueues nsteps steps.  The bulk kernel runs on the context stream and the IBM kernel on a
@@ -336,9 +430,23 @@ void pick_kernels(const Ctx &C, StepKernel &bulk, StepKernel &ibm, StepKernel &f
 // window workgroups overlap the bulk tiles of the same step.  With concurrent = false both
 // run in order on the context stream.
 int launch_steps(Ctx &C, int nsteps, int diag_every, bool concurrent) {
-    StepKernel kb, ki, kf;
-    pick_kernels(C, kb, ki, kf);
-    const int n_gen = C.n_ibm - C.n_fast;
+    StepKernel kb, ki, ks;
+    pick_kernels(C, kb, ki, ks);
+    if (concurrent && C.n_fast == C.n_ibm) {  // every window is fused: one launch per step
+        // one bulk workgroup per CU left over (a fused workgroup fills a CU's LDS)
+        const int rows = C.n_tiles * TILE_Y;
+        const int bulk = std::max(1, std::min(C.n_cu - C.n_fast, (rows + NTF / 64 - 1) / (NTF / 64)));
+        const int blocks = C.n_fast + bulk;
+        for (int s = 0; s < nsteps; ++s) {
+            const int64_t stepno = C.step + s + 1;
+            const int diag = (diag_every > 0 && stepno % diag_every == 0) ? 1 : 0;
+            LaunchTable T = make_table(C, (C.parity + s) & 1, diag);
+            T.n_fast = C.n_fast;
+            hipLaunchKernelGGL(ks, dim3(blocks), dim3(NTF), 0, C.stream, T);
+        }
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
     const bool two = concurrent && C.n_ibm > 0;
     if (two) {  // join the side stream to everything already queued on the main stream
         HIPCHK(hipEventRecord(C.ev_main, C.stream));
@@ -353,14 +461,12 @@ int launch_steps(Ctx &C, int nsteps, int diag_every, bool concurrent) {
                 HIPCHK(hipStreamWaitEvent(C.stream, C.ev_side, 0));
                 HIPCHK(hipStreamWaitEvent(C.side, C.ev_main, 0));
             }
-            if (C.n_fast) hipLaunchKernelGGL(kf, dim3(C.n_fast), dim3(NTI), 0, C.side, T);
-            if (n_gen) hipLaunchKernelGGL(ki, dim3(n_gen), dim3(NTI), 0, C.side, T);
+            hipLaunchKernelGGL(ki, dim3(C.n_ibm), dim3(NTI), 0, C.side, T);
             hipLaunchKernelGGL(kb, dim3(C.n_tiles), dim3(NT_BULK), 0, C.stream, T);
             HIPCHK(hipEventRecord(C.ev_main, C.stream));
             HIPCHK(hipEventRecord(C.ev_side, C.side));
         } else {
-            if (C.n_fast) hipLaunchKernelGGL(kf, dim3(C.n_fast), dim3(NTI), 0, C.stream, T);
-            if (n_gen) hipLaunchKernelGGL(ki, dim3(n_gen), dim3(NTI), 0, C.stream, T);
+            if (C.n_ibm) hipLaunchKernelGGL(ki, dim3(C.n_ibm), dim3(NTI), 0, C.stream, T);
             if (C.n_tiles) hipLaunchKernelGGL(kb, dim3(C.n_tiles), dim3(NT_BULK), 0, C.stream, T);
         }
     }
@@ -399,6 +505,11 @@ int iblbm_ctx_create(int device, void **out) {
     HIPCHK(hipSetDevice(device));
     Ctx *C = new Ctx();
     C->device = device;
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+            C->n_cu = prop.multiProcessorCount;
+    }
     if (hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&C->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&C->ev0) != hipSuccess || hipEventCreate(&C->ev1) != hipSuccess ||
@@ -858,13 +969,13 @@ int iblbm_get_diag(void *p, int id, iblbm_diag *out) {
     HIPCHK(hipSetDevice(C->device));
     HIPCHK(hipStreamSynchronize(C->stream));
     const Inst &d = C->inst[id].d;
-    std::vector<double> part(C->n_tiles + C->n_ibm);
+    std::vector<double> part((size_t)C->n_tiles * TILE_Y);
     HIPCHK(hipMemcpy(part.data(), C->d_rho_partial, sizeof(double) * part.size(),
                      hipMemcpyDeviceToHost));
     double box[5] = {0, 0, 0, 0, 0};
     if (d.ibm != 0) HIPCHK(hipMemcpy(box, d.diag, sizeof box, hipMemcpyDeviceToHost));
     double rs = 0.0;
-    const int b0 = C->h_tile_prefix[id], b1 = C->h_tile_prefix[id + 1];  // bulk tile partials
+    const int b0 = TILE_Y * C->h_tile_prefix[id], b1 = TILE_Y * C->h_tile_prefix[id + 1];  // rows
     for (int b = b0; b < b1; ++b) rs += part[b];
     rs += box[0];
     out->rho_sum = rs;

@@ -36,14 +36,6 @@ struct IbmShared {
     alignas(16) int toff[MAX_T + 4];  // 16-byte copy granules
 };
 
-// LDS of the register-resident path: interleaved (x, y) pairs, read as 16-byte loads.
-struct FastShared {
-    alignas(16) double uw[2 * FAST_T];  // MDF working velocity of the touched nodes
-    alignas(16) double lf[2 * MAX_LB];  // marker forces
-    double rot[FAST_T];                 // rho of the touched nodes
-    double fs[9 * FAST_T];              // post-stream populations of the touched nodes
-};
-
 __device__ inline int64_t floor_idx(double v) { return (int64_t)floor(v); }
 
 // ------------------------------------------------------------------ block-wide primitives
@@ -674,302 +666,6 @@ __device__ void ibm_role_impl(Inst &I, int parity, int diag, double *red, IbmSha
     }
 }
 
-// ------------------------------------------------------------------ register-resident path
-
-// DF / MDF window step when the window is small enough (see ibm_fast_lists).  The first
-// ceil(Lb/64) wavefronts are marker wavefronts: thread t < Lb holds marker t's stencil in
-// registers and computes its interpolation, force and residual.  The other wavefronts are
-// node wavefronts: each thread owns up to FAST_NPT touched nodes with their gather lists and
-// MDF state in registers.  Only the working velocity, the marker forces and the post-stream
-// populations pass through LDS.  Both roles run the same barrier sequence and take the same
-// (uniform) loop exits.  Arithmetic and summation orders are those of ibm_role_impl.
-template <typename ST, int MODEL, int A>
-__device__ void ibm_fast_role(Inst &I, int parity, int diag, double *red, FastShared &sh) {
-    constexpr int K = (2 * A + 1) * (2 * A + 1);
-    const int t = threadIdx.x;
-    IbmScratch &S = I.s;
-    const bool inc = I.inc != 0;
-    const double dt = I.cp.dt;
-    const int Lb = I.Lb, nT = I.nT;
-    const int mw = (Lb + 63) >> 6;
-    const bool marker_wave = (t >> 6) < mw;
-    double2 *uw = reinterpret_cast<double2 *>(sh.uw);
-    double2 *lf = reinterpret_cast<double2 *>(sh.lf);
-    unsigned long long *rbits = reinterpret_cast<unsigned long long *>(red) + (NTI - 2);
-    const bool mdf = I.ibm == 2;
-    const bool finc = mdf ? false : inc;  // MDF sub-steps are compressible (mdf.py:177-181)
-    const int n_iter = mdf ? I.n_iter : 1;
-    const int min_iter = max(1, min(I.min_iter, n_iter));
-    const double tol = I.tol;
-    double prev = INFINITY, last = INFINITY;
-    int iter_count = 0, diverged = 0, converged = 0;
-    double rho_acc = 0.0, fxs = 0.0, fys = 0.0, trq = 0.0;
-
-    MarkerRegs<A> M;
-    if (marker_wave) load_marker_regs<A>(I, M);  // in flight during P1
-
-    // P1 (all threads): box slots in chunks of CH per thread; a slot's touched index and its
-    // 9 pull values are loaded together (no dependent table load), touched slots keep theirs.
-    // A box clear of the domain edge needs no closure (wider chunks, fewer registers).
-    const ST *__restrict__ Asrc = buf<ST>(I, parity);
-    const int nbox = I.bw * I.bh;
-    auto p1 = [&](auto chunk, auto edge) {
-        constexpr int CH = decltype(chunk)::value;
-        constexpr bool EDGE = decltype(edge)::value;
-        for (int base = 0; base < nbox; base += CH * NTI) {
-            int ti[CH];
-            double fs[CH][9];
-#pragma unroll
-            for (int u = 0; u < CH; ++u) {
-                const int sl = base + u * NTI + t;
-                ti[u] = -1;
-                if (sl < nbox) {
-                    ti[u] = S.tindex[sl];
-                    const int i = I.bx0 + sl % I.bw, j = I.by0 + sl / I.bw;
-                    if constexpr (EDGE)
-                        fstar_at(Asrc, I, i, j, fs[u]);
-                    else
-                        pull_interior(Asrc, I, i, j, fs[u]);
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < CH; ++u) {
-                const int n = ti[u];
-                if (n >= 0) {
-                    double r, u0, u1;
-#pragma unroll
-                    for (int q = 0; q < 9; ++q) sh.fs[q * FAST_T + n] = fs[u][q];
-                    macro9(fs[u], inc, r, u0, u1);
-                    uw[n] = make_double2(u0, u1);
-                    sh.rot[n] = r;
-                }
-            }
-        }
-    };
-    const bool interior = I.bx0 >= 1 && I.by0 >= 1 && I.bx0 + I.bw <= I.nx - 1 &&
-                          I.by0 + I.bh <= I.ny - 1;
-    if (interior)
-        p1(std::integral_constant<int, 4>(), std::false_type());
-    else
-        p1(std::integral_constant<int, 2>(), std::true_type());
-    if (t < 2) rbits[t] = 0ull;
-    if (t == 0) lf[Lb] = make_double2(0.0, 0.0);  // target of the padded gather entries
-    __syncthreads();  // P1 done: uw, rot, fs of the touched nodes
-
-    if (marker_wave) {
-        double R = 0.0;
-        if (t < Lb) {
-#pragma unroll
-            for (int s = 0; s < K; ++s) R += sh.rot[M.ti[s]] * M.w[s];  // rho at the marker
-        }
-        // F = 2 R (U_d - U~) / dt; a division by dt == 1 is exact and skipped
-        const bool unit_dt = dt == 1.0;
-        for (int it = 0; it < n_iter; ++it) {
-            double rv = 0.0;
-            if (t < Lb) {  // interpolation + force (df.py:239-263)
-                double lx = 0.0, ly = 0.0;
-#pragma unroll
-                for (int s = 0; s < K; ++s) {
-                    const double2 u = uw[M.ti[s]];
-                    lx += u.x * M.w[s];
-                    ly += u.y * M.w[s];
-                }
-                double Fx, Fy;
-                if (finc) {
-                    Fx = 2.0 * (M.udx - lx);
-                    Fy = 2.0 * (M.udy - ly);
-                } else {
-                    Fx = 2.0 * R * (M.udx - lx);
-                    Fy = 2.0 * R * (M.udy - ly);
-                }
-                if (!unit_dt) {
-                    Fx = Fx / dt;
-                    Fy = Fy / dt;
-                }
-                lf[t] = make_double2(Fx, Fy);
-                const double vx = M.udx - lx, vy = M.udy - ly;
-                rv = sqrt(vx * vx + vy * vy);
-            }
-            // L-inf residual: a wavefront max, then one unsigned max per wavefront over the
-            // bit patterns (non-negative doubles order like their bits and every NaN pattern
-            // lies above +inf, so NaN propagates as in np.max); two slots alternate.
-            // (a marker-less lane contributes 0, the identity; every marker wavefront has one)
-            const unsigned long long wb = dpp_max_u64_lane63(
-                t < Lb ? (unsigned long long)__double_as_longlong(rv) : 0ull);
-            if ((t & 63) == 63) atomicMax(&rbits[it & 1], wb);
-            __syncthreads();  // A: forces and residual published
-            const double res = __longlong_as_double((long long)rbits[it & 1]);
-            if (t == 0) rbits[(it + 1) & 1] = 0ull;  // its last reads preceded barrier B
-            if (mdf) {
-                last = res;
-                if (it > 0 && res > prev * 1.5) {  // divergence guard, mdf.py:192-195
-                    diverged = 1;
-                    iter_count = it + 1;
-                    break;
-                }
-            }
-            __syncthreads();  // B: velocities updated
-            if (mdf) {
-                prev = res;
-                iter_count = it + 1;
-                if (it + 1 >= min_iter && res < tol) {
-                    converged = 1;
-                    break;
-                }
-            }
-        }
-        if (mdf && t == 0) {
-            I.stats[0] = iter_count;
-            I.stats[1] = last;
-            I.stats[2] = diverged;
-            I.stats[3] = converged;
-        }
-    } else {
-        ST *__restrict__ B = buf<ST>(I, parity ^ 1);
-        const int nthr = NTI - 64 * mw, tn = t - 64 * mw;
-        const double omega = I.omega;
-        int sl[FAST_NPT], kk[FAST_NPT][FAST_G];
-        double ww[FAST_NPT][FAST_G];
-        double r[FAST_NPT], u0[FAST_NPT], u1[FAST_NPT];
-#pragma unroll
-        for (int j = 0; j < FAST_NPT; ++j) {
-            const int n = tn + j * nthr;
-            sl[j] = 0;
-            r[j] = 1.0;
-            u0[j] = 0.0;
-            u1[j] = 0.0;
-            if (n < nT) {
-                sl[j] = S.tslot[n];
-#pragma unroll
-                for (int e = 0; e < FAST_G; ++e) {
-                    kk[j][e] = S.pk[e * FAST_T + n];
-                    ww[j][e] = S.pw[e * FAST_T + n];
-                }
-                const double2 v = uw[n];
-                r[j] = sh.rot[n];
-                u0[j] = v.x;
-                u1[j] = v.y;
-            }
-        }
-        double x[FAST_NPT], y[FAST_NPT], fx[FAST_NPT], fy[FAST_NPT], vs[FAST_NPT];
-#pragma unroll
-        for (int j = 0; j < FAST_NPT; ++j) {
-            x[j] = u0[j];
-            y[j] = u1[j];
-            fx[j] = 0.0;
-            fy[j] = 0.0;
-            vs[j] = dt / (2.0 * r[j]);  // mdf.py:117
-        }
-        for (int it = 0; it < n_iter; ++it) {
-            __syncthreads();  // A
-            const double res = __longlong_as_double((long long)rbits[it & 1]);
-            if (mdf) {
-                last = res;
-                if (it > 0 && res > prev * 1.5) {
-                    diverged = 1;
-                    iter_count = it + 1;
-                    break;
-                }
-            }
-            // spread as a gather in np.add.at order (df.py:269-283)
-#pragma unroll
-            for (int j = 0; j < FAST_NPT; ++j) {
-                const int n = tn + j * nthr;
-                if (n < nT) {
-                    double sx = 0.0, sy = 0.0;
-                    // padded entries point at the zero force lf[Lb] with weight 0: they add
-                    // +0.0, which leaves the sum's bits unchanged (it starts at +0.0 and so is
-                    // never -0.0)
-#pragma unroll
-                    for (int e = 0; e < FAST_G; ++e) {
-                        const double2 f = lf[kk[j][e]];
-                        sx += f.x * ww[j][e];
-                        sy += f.y * ww[j][e];
-                    }
-                    if (mdf) {  // mdf.py:197-200
-                        const double gx = sx * omega, gy = sy * omega;
-                        fx[j] = fx[j] + gx;
-                        fy[j] = fy[j] + gy;
-                        x[j] = x[j] + gx * vs[j];
-                        y[j] = y[j] + gy * vs[j];
-                    } else {
-                        fx[j] = sx;
-                        fy[j] = sy;
-                    }
-                    uw[n] = make_double2(x[j], y[j]);
-                }
-            }
-            __syncthreads();  // B
-            if (mdf) {
-                prev = res;
-                iter_count = it + 1;
-                if (it + 1 >= min_iter && res < tol) {
-                    converged = 1;
-                    break;
-                }
-            }
-        }
-        // P3: velocity correction, equilibrium, forced collision, store
-#pragma unroll
-        for (int j = 0; j < FAST_NPT; ++j) {
-            const int n = tn + j * nthr;
-            if (n < nT) {
-                const int s0 = sl[j];
-                const int i = I.bx0 + s0 % I.bw, jj = I.by0 + s0 / I.bw;
-                double fs[9];
-#pragma unroll
-                for (int q = 0; q < 9; ++q) {
-                    fs[q] = sh.fs[q * FAST_T + n];
-                    S.fs[q * I.box_cap + s0] = fs[q];  // the step's state, for downloads
-                }
-                double v0 = u0[j], v1 = u1[j];
-                const double gx = fx[j], gy = fy[j], rr = r[j];
-                S.ro[s0] = rr;
-                if (inc) {
-                    v0 = v0 + 0.5 * gx * dt;
-                    v1 = v1 + 0.5 * gy * dt;
-                } else {
-                    v0 = v0 + gx * dt / (2.0 * rr);
-                    v1 = v1 + gy * dt / (2.0 * rr);
-                }
-                S.ux0[s0] = v0;
-                S.uy0[s0] = v1;
-                S.fibx[s0] = gx;
-                S.fiby[s0] = gy;
-                double fe[9], out[9];
-                if (MODEL != 2) feq9(rr, v0, v1, inc, fe);
-                if constexpr (MODEL == 0)
-                    collide_bgk<true>(I.cp, fs, fe, v0, v1, gx, gy, out);
-                else if constexpr (MODEL == 1)
-                    collide_trt<true>(I.cp, fs, fe, v0, v1, gx, gy, out);
-                else if constexpr (MODEL == 2)
-                    collide_cm<true>(I.cp, fs, v0, v1, gx, gy, out);
-                else
-                    collide<true>(I.cp, fs, fe, v0, v1, gx, gy, out);
-                store9(B, I, i, jj, out);
-                if (diag) {
-                    rho_acc += rr;
-                    fxs += gx;
-                    fys += gy;
-                    trq += ((double)i - I.cxc) * gy - ((double)jj - I.cyc) * gx;
-                }
-            }
-        }
-    }
-    if (diag) {
-        const double rs = block_reduce(rho_acc, red, false);
-        const double f0 = block_reduce(fxs, red, false), f1 = block_reduce(fys, red, false);
-        const double tq = block_reduce(trq, red, false);
-        if (t == 0) {
-            I.diag[0] = rs;
-            I.diag[1] = f0;
-            I.diag[2] = f1;
-            I.diag[3] = tq;
-            I.diag[4] = 0.0;
-        }
-    }
-}
-
 template <typename ST, int MODEL>
 __device__ void ibm_role(Inst &I, int parity, int diag, double *red) {
     __shared__ IbmShared sh;
@@ -987,19 +683,13 @@ __device__ void ibm_role(Inst &I, int parity, int diag, double *red) {
     }
 }
 
-template <typename ST, int MODEL>
-__device__ void ibm_fast_dispatch(Inst &I, int parity, int diag, double *red) {
-    __shared__ FastShared sh;
-    if (I.delta == 1)
-        ibm_fast_role<ST, MODEL, 1>(I, parity, diag, red, sh);
-    else
-        ibm_fast_role<ST, MODEL, 2>(I, parity, diag, red, sh);
-}
-
-// Padded entry-major gather lists for the register-resident path, and its flag.
-__device__ inline void ibm_fast_lists(Inst &I, int *redi) {
+// Tables of the fused window path (see fused_step.h) and its flag: DF/MDF, Lb <= FAST_LB,
+// one node thread per touched node, at most FAST_G contributions per node.
+template <int A>
+__device__ void ibm_fast_lists(Inst &I, int *redi) {
+    constexpr int K = (2 * A + 1) * (2 * A + 1);
     IbmScratch &S = I.s;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, Lb = I.Lb;
     const int nT = S.tindex[I.bw * I.bh];
     __syncthreads();
     if (t == 0) redi[0] = 0;
@@ -1008,17 +698,21 @@ __device__ inline void ibm_fast_lists(Inst &I, int *redi) {
     for (int ti = t; ti < nT; ti += NTI) mx = max(mx, S.toff[ti + 1] - S.toff[ti]);
     atomicMax(&redi[0], mx);
     __syncthreads();
-    const int mw = (I.Lb + 63) >> 6;  // marker wavefronts; the rest own FAST_NPT nodes each
-    const int fast = (I.ibm == 1 || I.ibm == 2) && mw < NTI / 64 &&
-                     nT <= FAST_NPT * (NTI - 64 * mw) && redi[0] <= FAST_G;
+    const int mwv = (Lb + 63) >> 6;  // marker wavefronts
+    const int fast = (I.ibm == 1 || I.ibm == 2) && Lb <= FAST_LB &&
+                     nT <= NTF - 64 * mwv && redi[0] <= FAST_G;
     if (fast) {
         for (int ti = t; ti < nT; ti += NTI) {
             const int o = S.toff[ti], n = S.toff[ti + 1] - o;
-            S.pn[ti] = n;
             for (int e = 0; e < FAST_G; ++e) {
-                S.pk[e * FAST_T + ti] = e < n ? S.gk[o + e] : I.Lb;  // padding: zero force
+                S.pk[e * FAST_T + ti] = e < n ? S.gk[o + e] : Lb;  // padding: zero force
                 S.pw[e * FAST_T + ti] = e < n ? S.gw[o + e] : 0.0;
             }
+        }
+        for (int c = t; c < K * Lb; c += NTI) {  // marker-major -> stencil-major
+            const int k = c / K, sp = c - k * K;
+            S.mti[sp * Lb + k] = S.cti[c];
+            S.mw[sp * Lb + k] = S.cw[c];
         }
     }
     if (t == 0) I.fast = fast;
@@ -1035,7 +729,10 @@ __device__ inline void ibm_tables_role(Inst &I, int *redi) {
         ibm_build_tables<2>(I, redi);
         if (I.ibm == 3) ibm_dfc_lambda<2>(I, I.s.tindex[I.bw * I.bh], I.Lb);
     }
-    ibm_fast_lists(I, redi);
+    if (I.delta == 1)
+        ibm_fast_lists<1>(I, redi);
+    else
+        ibm_fast_lists<2>(I, redi);
     for (int sl = threadIdx.x; sl < I.bw * I.bh; sl += NTI) {
         I.s.fibx[sl] = 0.0;
         I.s.fiby[sl] = 0.0;

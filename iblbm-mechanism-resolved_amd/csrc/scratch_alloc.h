@@ -62,8 +62,9 @@ inline bool alloc_ibm_scratch(Inst &d, int lb_cap, int box_cap, Get get) {
     D(&s.q, 2 * L);
     D(&s.dfl, 9 * L);
     I(&s.pk, (size_t)FAST_G * FAST_T);
-    I(&s.pn, FAST_T);
     D(&s.pw, (size_t)FAST_G * FAST_T);
+    I(&s.mti, (size_t)FAST_K * FAST_LB);
+    D(&s.mw, (size_t)FAST_K * FAST_LB);
     D(&d.stats, 4);
     D(&d.diag, 8);
     d.lb_cap = (int)L;

@@ -17,9 +17,11 @@ constexpr int NT_BULK = 256;  // bulk tile workgroup: 4 wavefronts, a 64 x 4 nod
 constexpr int TILE_X = 64;
 constexpr int TILE_Y = NT_BULK / TILE_X;
 constexpr int NTI = 512;
-constexpr int FAST_NPT = 2;                     // touched nodes per node thread (fast path)
-constexpr int FAST_T = FAST_NPT * (NTI - 64);  // touched-node capacity of the fast path
-constexpr int FAST_G = 8;                       // gather entries per touched node (fast path)      // immersed-boundary workgroup: 8 wavefronts
+constexpr int NTF = 1024;              // threads of a fused-step workgroup (16 wavefronts)
+constexpr int FAST_T = NTF - 64;       // touched-node capacity of the fused window (1 per thread)
+constexpr int FAST_G = 8;              // gather entries per touched node on that path
+constexpr int FAST_LB = 192;           // markers on that path (up to 3 marker wavefronts)
+constexpr int FAST_K = 25;             // stencil points per marker (Peskin 4-point: 5 x 5)
 
 struct IbmScratch {
     // per box slot (box_cap)
@@ -42,9 +44,10 @@ struct IbmScratch {
     // per marker (lb_cap)
     double *Lux, *Luy, *R, *Lfx, *Lfy, *lam, *q;  // q: DFC force (Lb, 2)
     double *dfl;              // DFC delta f (Lb, 9)
-    // gather lists padded to FAST_G entries, entry-major [e * FAST_T + ti] (fast path)
-    int *pk, *pn;             // marker of entry e, entries per touched node
-    double *pw;               // wx*wy*Larea of entry e (0 beyond pn)
+    // fused window path: gather lists padded to FAST_G entries, entry-major [e * FAST_T + ti]
+    // (padding: marker Lb, weight 0), and the marker stencils stencil-major [s * Lb + k]
+    int *pk, *mti;
+    double *pw, *mw;
 };
 
 struct Inst {
@@ -79,8 +82,7 @@ struct LaunchTable {
     const int *ibm_inst;     // instance ids of the window workgroups: fast-path ones first
     const int *tile_prefix;  // tile_prefix[i]: first bulk tile of instance i (n_inst + 1 entries)
     double *rho_partial;     // one FP64 partial rho sum per bulk workgroup (diagnostic steps)
-    int n_inst, n_ibm, n_fast;
-    int fast_id[8];          // the first fast-path instance ids, passed by value
+    int n_inst, n_ibm, n_fast, n_tiles;
     int parity;              // source buffer index of this step
     int diag;                // take diagnostic reductions this step
 };
@@ -212,19 +214,5 @@ __device__ inline double nanmax2(double a, double b) {
     return (a != a || a > b) ? a : b;  // np.max semantics: NaN propagates
 }
 
-// Deterministic sum over the NT_BULK threads of a bulk workgroup (fixed shuffle tree).
-__device__ inline double bulk_block_sum(double v, double *red) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) red[wv] = v;
-    __syncthreads();
-    double r = 0.0;
-    if (threadIdx.x == 0) {
-        r = red[0];
-        for (int k = 1; k < NT_BULK / 64; ++k) r = r + red[k];
-    }
-    return r;
-}
 
 }  // namespace iblbm
