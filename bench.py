@@ -23,6 +23,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+KERNEL_OF_MODE = {"fused": "k_step (window + bulk rows, one launch per step)",
+                  "two_streams": "k_ibm || k_bulk (two streams, one step)",
+                  "serial": "k_ibm then k_bulk (one stream, one step)"}
 BYTES_PER_NODE = {"fp64": 144, "fp32dev": 72}  # 9 reads + 9 writes per node update
 
 
@@ -99,6 +102,7 @@ def main():
     if args.serial:
         sim.ctx.set_concurrent(False)
     graph = not args.no_graph
+    mode = sim.ctx.step_mode()
     N = sim.st.nodenums
     for _ in range(warm // ci):
         sim.advance(ci, diag_every=ci, graph=graph)
@@ -134,14 +138,13 @@ def main():
         "data": "synthetic (seeded 1e-3 inflow perturbation, seed 1234)",
         "config": {"workload": "fixed cylinder Re=100, 1601x641, D=40, 126 markers, TRT "
                                "Lambda=1/4, MDF [5,20] tol 1e-4, Peskin-4, U=0.04",
-                   "precision": args.precision, "streams": 1 if args.serial else 2,
-                   "graph": graph,
+                   "precision": args.precision, "launch": mode, "graph": graph,
                    "nodes": N, "check_interval": ci, "Cd_last": cd, "Cl_last": cl,
                    "mdf_iter_last": iters[-1] if iters else None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None,
-                     "kernel": "k_bulk + k_ibm (one step)", "bytes_per_node": bpn,
+                     "kernel": KERNEL_OF_MODE[mode], "bytes_per_node": bpn,
                      "avg_launch_us": round(kernel_s * 1e6, 3)},
     }
     if not args.no_cpu:

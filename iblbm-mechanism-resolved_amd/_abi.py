@@ -73,6 +73,7 @@ SIGNATURES = {
     "iblbm_sync": (_i, [_VP]),
     "iblbm_graph_run": (_i, [_VP, _i, _i]),
     "iblbm_set_concurrent": (_i, [_VP, _i]),
+    "iblbm_step_mode": (_i, [_VP, _I]),
     "iblbm_get_mdf_stats": (_i, [_VP, _i, ctypes.POINTER(MdfStats)]),
     "iblbm_get_diag": (_i, [_VP, _i, ctypes.POINTER(Diag)]),
     "iblbm_download_fib": (_i, [_VP, _i, _D]),

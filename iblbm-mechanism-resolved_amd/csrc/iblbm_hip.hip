@@ -932,6 +932,16 @@ int iblbm_sync(void *p) {
     return 0;
 }
 
+int iblbm_step_mode(void *p, int *mode) {
+    Ctx *C = as_ctx(p);
+    if (!C || !mode) return fail(IBLBM_EINVAL, "step_mode: bad argument");
+    int rc = ensure_tables(*C);
+    if (rc) return rc;
+    *mode = !C->concurrent ? IBLBM_MODE_SERIAL
+                           : (C->n_fast == C->n_ibm ? IBLBM_MODE_FUSED : IBLBM_MODE_TWO_STREAMS);
+    return 0;
+}
+
 int iblbm_last_timing(void *p, double *ms, int *steps) {
     Ctx *C = as_ctx(p);
     if (!C || !ms || !steps) return fail(IBLBM_EINVAL, "last_timing: bad argument");

@@ -106,6 +106,12 @@ class Context:
     def sync(self):
         check(self.lib.iblbm_sync(self.h), "sync")
 
+    def step_mode(self) -> str:
+        """"fused" (one launch per step), "two_streams" or "serial"."""
+        m = ctypes.c_int()
+        check(self.lib.iblbm_step_mode(self.h, ctypes.byref(m)), "step_mode")
+        return ("fused", "two_streams", "serial")[m.value]
+
     def set_concurrent(self, on: bool):
         check(self.lib.iblbm_set_concurrent(self.h, int(bool(on))), "set_concurrent")
 

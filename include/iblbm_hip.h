@@ -131,9 +131,17 @@ int iblbm_sync(void *ctx);
 
 /* Captures nsteps steps into a hipGraph and replays it (the step counter advances). */
 int iblbm_graph_run(void *ctx, int nsteps, int diag_every);
-/* on = 1 (default): the marker-window workgroups run on a second stream concurrently with
- * the bulk tiles of the same step; on = 0: both kernels run in order on one stream. */
+/* on = 1 (default): a step runs as one fused launch when every marker window of the context
+ * qualifies for it, otherwise the marker-window workgroups run on a second stream concurrently
+ * with the bulk tiles of the same step; on = 0: window and bulk kernels run in order on one
+ * stream. */
 int iblbm_set_concurrent(void *ctx, int on);
+/* How the next steps will be launched: IBLBM_MODE_FUSED (one launch per step),
+ * IBLBM_MODE_TWO_STREAMS or IBLBM_MODE_SERIAL. */
+#define IBLBM_MODE_FUSED 0
+#define IBLBM_MODE_TWO_STREAMS 1
+#define IBLBM_MODE_SERIAL 2
+int iblbm_step_mode(void *ctx, int *mode);
 
 int iblbm_get_mdf_stats(void *ctx, int id, iblbm_mdf_stats *out);
 int iblbm_get_diag(void *ctx, int id, iblbm_diag *out);
