@@ -20,6 +20,8 @@ BC = {"dirichlet_neumann": 0, "settling_channel": 1}
 IBM = {None: 0, "NONE": 0, "DF": 1, "MDF": 2, "DFC": 3}
 DELTA = {"peskin4pt": 0, "hat": 1}
 DELTA_RADIUS = {"peskin4pt": 2, "hat": 1}
+PRED = {"off": 0, "collect": 1, "warm": 2, "learned": 3}
+MLP_IN, MLP_OUT, MLP_NP, MLP_NORM = 8, 2, 4866, 18
 FP64, FP32_DEV = 0, 1
 
 
@@ -55,6 +57,7 @@ class Diag(ctypes.Structure):
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
 _VP = ctypes.c_void_p
+_F = ctypes.POINTER(ctypes.c_float)
 _i, _d, _l = ctypes.c_int, ctypes.c_double, ctypes.c_int64
 
 # name -> (restype, argtypes)
@@ -94,6 +97,17 @@ SIGNATURES = {
                                            _i, _i, _d, _d, _D, ctypes.POINTER(MdfStats)]),
     "iblbm_op_dfc_lambda": (_i, [_D, _D, _i, _d, _d, _d, _d, _i, _i, _i, _D]),
     "iblbm_op_dfc_correction": (_i, [_D, _D, _D, _i, _D, _D, _d, _d, _d, _i, _i, _i, _d, _D]),
+    "iblbm_predictor_mode": (_i, [_VP, _i, _i, _l]),
+    "iblbm_predictor_samples": (_i, [_VP, _i, ctypes.POINTER(ctypes.c_int64)]),
+    "iblbm_predictor_samples_written": (_i, [_VP, _i, ctypes.POINTER(ctypes.c_int64)]),
+    "iblbm_predictor_data": (_i, [_VP, _i, _l, _l, _F]),
+    "iblbm_predictor_train": (_i, [_VP, _i, _i, _d, ctypes.c_uint32, _F]),
+    "iblbm_predictor_params": (_i, [_VP, _i, _F, _F]),
+    "iblbm_predictor_set_params": (_i, [_VP, _i, _F, _F]),
+    "iblbm_predictor_stats": (_i, [_VP, _i, _D, _i]),
+    "iblbm_predictor_previous_force": (_i, [_VP, _i, _D, _i]),
+    "iblbm_mlp_loss_grad": (_i, [_F, _F, _F, _i, _F, _F]),
+    "iblbm_mlp_infer_bf16": (_i, [_F, _F, _i, _F]),
 }
 
 _LIB = None
@@ -139,3 +153,13 @@ def dptr(a: np.ndarray):
 
 def f64(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def fptr(a: np.ndarray):
+    if a.dtype != np.float32 or not a.flags.c_contiguous:
+        raise ValueError("expected a C-contiguous float32 array")
+    return a.ctypes.data_as(_F)

@@ -14,6 +14,7 @@
 // MDF: iblbm/ibm/mdf.py:175-229, velocity correction: iblbm/ibm/dispatch.py:18-33).
 #pragma once
 #include "ibm_block.h"
+#include "mlp.h"
 
 namespace iblbm {
 
@@ -24,6 +25,7 @@ struct FusedShared {
     double mw[FAST_K * FAST_LB];                // marker stencil weights, [s * Lb + k]
     int mti[FAST_K * FAST_LB];                  // marker stencil touched indices, [s * Lb + k]
     double fs[9 * FAST_T];                      // post-stream populations of the touched nodes
+    MlpTile mlp[FAST_LB / 64];                  // predictor scratch of each marker wavefront
 };
 
 // Copies the descriptor fields the window reads into a local descriptor, in one batch at the
@@ -58,7 +60,55 @@ __device__ inline void light_copy(const Inst &G, Inst &L) {
     L.diag = G.diag;
     L.cxc = G.cxc;
     L.cyc = G.cyc;
+    L.cxd = G.cxd;
+    L.cyd = G.cyd;
+    L.dx = G.dx;
+    L.dy = G.dy;
+    L.Lx = G.Lx;
+    L.Ly = G.Ly;
+    L.pmode = G.pmode;
+    L.lb_cap = G.lb_cap;
+    L.data_cap = G.data_cap;
 }
+
+// The MDF loop decisions (mdf.py:175-229), made identically by every wavefront of the window
+// from the block-wide residual.  In the warm-start modes the loop needs only one true
+// correction; a divergence there restarts plain MDF from the uncorrected velocity once.
+struct MdfCtl {
+    bool mdf, warm;
+    int n_iter, min_iter, min_plain;
+    double tol, prev = INFINITY, last = INFINITY;
+    int iter_count = 0, diverged = 0, converged = 0, restarted = 0, applied = 0;
+    // after barrier A: 0 apply this sub-step, 1 stop (divergence guard), 2 restart plain MDF
+    __device__ int at_a(double res, int it) {
+        if (!mdf) return 0;
+        last = res;
+        if (it > 0 && res > prev * 1.5) {  // divergence guard, mdf.py:192-195
+            if (warm && !restarted) {
+                restarted = 1;
+                prev = INFINITY;
+                min_iter = min_plain;
+                return 2;
+            }
+            diverged = 1;
+            iter_count = it + 1;
+            return 1;
+        }
+        return 0;
+    }
+    // after barrier B of an applied sub-step: true when the loop has converged
+    __device__ bool at_b(double res, int it) {
+        ++applied;
+        if (!mdf) return true;
+        prev = res;
+        iter_count = it + 1;
+        if (it + 1 >= min_iter && res < tol) {
+            converged = 1;
+            return true;
+        }
+        return false;
+    }
+};
 
 // Sum over the NTF threads: fixed shuffle tree per wavefront, then the wavefronts in order
 // (the result is valid in thread 0).
@@ -76,7 +126,9 @@ __device__ inline double fused_block_sum(double v, double *red) {
     return r;
 }
 
-template <typename ST, int MODEL, int A>
+// PRED: compiled with the learned-predictor modes (instances whose mode is off run plain MDF
+// in either build; without PRED the predictor's registers stay out of the step kernel).
+template <typename ST, int MODEL, int A, bool PRED>
 __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int parity, int diag,
                                                              double *red, FusedShared &sh) {
     constexpr int K = (2 * A + 1) * (2 * A + 1);
@@ -100,8 +152,6 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
     const int n_iter = mdf ? I.n_iter : 1;
     const int min_iter = max(1, min(I.min_iter, n_iter));
     const double tol = I.tol;
-    double prev = INFINITY, last = INFINITY;
-    int iter_count = 0, diverged = 0, converged = 0;
 
     // Every global load of the prologue is issued before the first wait: the marker stencils
     // (for LDS), a marker's desired velocity, a node's gather list, the box slots of P1.
@@ -182,7 +232,107 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
     __syncthreads();  // P1 done
 
     double rho_acc = 0.0, fxs = 0.0, fys = 0.0, trq = 0.0;
+    const int pmode = (PRED && mdf) ? I.pmode : 0;  // predictor: 1 collect, 2 warm, 3 learned
+    const bool warm = pmode >= 2;
+    MdfCtl ctl;
+    ctl.mdf = mdf;
+    ctl.warm = warm;
+    ctl.n_iter = n_iter;
+    ctl.min_plain = min_iter;
+    ctl.min_iter = warm ? 1 : min_iter;  // at least one true correction after a prediction
+    ctl.tol = tol;
+    int seq = 0;  // residual slot sequence (slots alternate)
     if (marker_wave) {
+        // Predictor (MDF with a mode set), computed ahead of the loop while few registers are
+        // taken.  The forcing slip q = R (ud - U) (U interpolated before forcing, R the marker
+        // density, 1 for the incompressible form) is what the first MDF sub-step turns into a
+        // force, and a fixed number of sub-steps maps q to the converged force linearly: the
+        // change of force is a geometry-set operator on the change of q.  fprev holds per
+        // marker the previous converged force (2 Lb) and q (2 Lb after it); only the marker's
+        // own thread touches its entries.  Features: change of q since the previous step /
+        // sig, direction of q, marker direction from the body centre, sub-cell offset.  The
+        // network predicts (change of force) / sig with sig = max over the wavefront's markers
+        // of |change of q| (+ 1e-9 max |q|), exact in scale while the change decays by orders of
+        // magnitude along a run.
+        // With a warm start the prediction F = F_prev (+ sig x the network's output) is
+        // published here; the loop's first pass spreads it with weight 1 and does not count it
+        // as a sub-step.
+        const bool fprev_ok = pmode != 0 && S.pctr[1] != 0;
+        const bool record = pmode == 1 && fprev_ok;
+        float *rowp = nullptr;  // this marker's training sample (collect mode)
+        double facx = 0.0, facy = 0.0;
+        if (PRED && pmode != 0) {
+            double ax = 0.0, ay = 0.0, ex = 0.0, ey = 0.0;
+            if (t < Lb) {
+                double lx = 0.0, ly = 0.0, rm = 0.0;
+#pragma unroll 4
+                for (int s = 0; s < K; ++s) {
+                    const int at = sh.mti[s * Lb + t];
+                    const double2 u = uw[at];
+                    const double w = sh.mw[s * Lb + t];
+                    lx += u.x * w;
+                    ly += u.y * w;
+                    rm += sh.rot[at] * w;
+                }
+                if (finc) rm = 1.0;
+                ax = rm * (udx - lx);
+                ay = rm * (udy - ly);
+                double spx = 0.0, spy = 0.0;
+                if (fprev_ok) {
+                    spx = S.fprev[2 * I.lb_cap + 2 * t];
+                    spy = S.fprev[2 * I.lb_cap + 2 * t + 1];
+                }
+                S.fprev[2 * I.lb_cap + 2 * t] = ax;  // the next step's previous q
+                S.fprev[2 * I.lb_cap + 2 * t + 1] = ay;
+                ex = ax - spx;
+                ey = ay - spy;
+            }
+            const double a = sqrt(ax * ax + ay * ay);
+            const float sig =
+                (float)fmax(wave_max(sqrt(ex * ex + ey * ey)) + 1e-9 * wave_max(a), 1e-30);
+            float feat[MLP_IN];
+#pragma unroll
+            for (int q = 0; q < MLP_IN; ++q) feat[q] = 0.f;
+            double pfx = 0.0, pfy = 0.0;
+            if (t < Lb) {
+                feat[0] = (float)(ex / (double)sig);
+                feat[1] = (float)(ey / (double)sig);
+                feat[2] = a > 0.0 ? (float)(ax / a) : 0.f;
+                feat[3] = a > 0.0 ? (float)(ay / a) : 0.f;
+                const double rx = I.Lx[t] - I.cxd, ry = I.Ly[t] - I.cyd;
+                const double rr = sqrt(rx * rx + ry * ry);
+                const double gx = I.Lx[t] / I.dx + 0.5, gy = I.Ly[t] / I.dy + 0.5;
+                feat[4] = rr > 0.0 ? (float)(rx / rr) : 0.f;
+                feat[5] = rr > 0.0 ? (float)(ry / rr) : 0.f;
+                feat[6] = (float)(gx - floor(gx) - 0.5);
+                feat[7] = (float)(gy - floor(gy) - 0.5);
+                if (record) {  // features now, sig in the target's place until the end
+                    rowp = S.data + ((S.pctr[0] + t) % I.data_cap) * MLP_ROW;
+#pragma unroll
+                    for (int q = 0; q < MLP_IN; ++q) rowp[q] = feat[q];
+                    rowp[MLP_IN] = sig;
+                }
+                if (warm && fprev_ok) {
+                    pfx = S.fprev[2 * t];
+                    pfy = S.fprev[2 * t + 1];
+                }
+            }
+            if (pmode == 3) {  // every lane of the wavefront takes part
+                float xn[MLP_IN], yo[MLP_OUT];
+#pragma unroll
+                for (int q = 0; q < MLP_IN; ++q)
+                    xn[q] = (feat[q] - S.mlp_norm[q]) / S.mlp_norm[MLP_IN + q];
+                mlp_infer_wave(reinterpret_cast<const mlp_short4 *>(S.mlp_wb), S.mlp_p, xn, yo,
+                               sh.mlp[t >> 6]);
+                pfx += (double)sig * (double)(yo[0] * S.mlp_norm[2 * MLP_IN]);
+                pfy += (double)sig * (double)(yo[1] * S.mlp_norm[2 * MLP_IN + 1]);
+            }
+            if (warm) {  // published here; read by the node threads after barrier A
+                if (t < Lb) lf[t] = make_double2(pfx, pfy);
+                facx = pfx;
+                facy = pfy;
+            }
+        }
         int ti[K];
         if (t < Lb) {
 #pragma unroll
@@ -198,38 +348,44 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
             for (int s = 0; s < K; ++s) R += rs[s] * sh.mw[s * Lb + t];
         }
         const bool unit_dt = dt == 1.0;  // a division by 1 is exact and skipped
-        for (int it = 0; it < n_iter; ++it) {
-            double rv = 0.0;
-            if (t < Lb) {  // interpolation + force (df.py:239-263), loads batched by GRP
-                double lx = 0.0, ly = 0.0;
+        // interpolation + force (df.py:239-263) of this marker from uw, loads batched by GRP
+        auto interp_force = [&](double &lx, double &ly, double &Fx, double &Fy) {
+            lx = 0.0;
+            ly = 0.0;
 #pragma unroll
-                for (int g0 = 0; g0 < K; g0 += GRP) {
-                    double2 u[GRP];
-                    double w[GRP];
+            for (int g0 = 0; g0 < K; g0 += GRP) {
+                double2 u[GRP];
+                double w[GRP];
 #pragma unroll
-                    for (int s = g0; s < g0 + GRP && s < K; ++s) {
-                        u[s - g0] = uw[ti[s]];
-                        w[s - g0] = sh.mw[s * Lb + t];
-                    }
-                    __builtin_amdgcn_sched_barrier(0);
+                for (int s = g0; s < g0 + GRP && s < K; ++s) {
+                    u[s - g0] = uw[ti[s]];
+                    w[s - g0] = sh.mw[s * Lb + t];
+                }
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                    for (int s = g0; s < g0 + GRP && s < K; ++s) {
-                        lx += u[s - g0].x * w[s - g0];
-                        ly += u[s - g0].y * w[s - g0];
-                    }
+                for (int s = g0; s < g0 + GRP && s < K; ++s) {
+                    lx += u[s - g0].x * w[s - g0];
+                    ly += u[s - g0].y * w[s - g0];
                 }
-                double Fx, Fy;
-                if (finc) {
-                    Fx = 2.0 * (udx - lx);
-                    Fy = 2.0 * (udy - ly);
-                } else {
-                    Fx = 2.0 * R * (udx - lx);
-                    Fy = 2.0 * R * (udy - ly);
-                }
-                if (!unit_dt) {
-                    Fx = Fx / dt;
-                    Fy = Fy / dt;
-                }
+            }
+            if (finc) {
+                Fx = 2.0 * (udx - lx);
+                Fy = 2.0 * (udy - ly);
+            } else {
+                Fx = 2.0 * R * (udx - lx);
+                Fy = 2.0 * R * (udy - ly);
+            }
+            if (!unit_dt) {
+                Fx = Fx / dt;
+                Fy = Fy / dt;
+            }
+        };
+        for (int it = 0; it < n_iter; ++seq) {
+            const bool predict = PRED && warm && seq == 0;
+            double rv = 0.0, Fx = 0.0, Fy = 0.0;
+            if (t < Lb && !predict) {
+                double lx, ly;
+                interp_force(lx, ly, Fx, Fy);
                 lf[t] = make_double2(Fx, Fy);
                 const double vx = udx - lx, vy = udy - ly;
                 rv = sqrt(vx * vx + vy * vy);
@@ -239,33 +395,58 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
             // np.max); a marker-less lane contributes 0, the identity; two slots alternate
             const unsigned long long wb = dpp_max_u64_lane63(
                 t < Lb ? (unsigned long long)__double_as_longlong(rv) : 0ull);
-            if ((t & 63) == 63) atomicMax(&rbits[it & 1], wb);
+            if ((t & 63) == 63) atomicMax(&rbits[seq & 1], wb);
             __syncthreads();  // A: forces and residual published
-            const double res = __longlong_as_double((long long)rbits[it & 1]);
-            if (t == 0) rbits[(it + 1) & 1] = 0ull;  // its last reads preceded barrier B
-            if (mdf) {
-                last = res;
-                if (it > 0 && res > prev * 1.5) {  // divergence guard, mdf.py:192-195
-                    diverged = 1;
-                    iter_count = it + 1;
-                    break;
-                }
+            const double res = __longlong_as_double((long long)rbits[seq & 1]);
+            if (t == 0) {
+                rbits[(seq + 1) & 1] = 0ull;  // its last reads preceded barrier B
+                // first residual after the prediction (warm) or of plain MDF
+                if (PRED && pmode != 0 && seq == (warm ? 1 : 0)) S.pstat[2] += res;
+            }
+            if (predict) {
+                __syncthreads();  // B: predicted velocities applied
+                continue;
+            }
+            const int act = ctl.at_a(res, it);
+            if (act == 1) break;
+            if (act == 0) {
+                facx += I.omega * Fx;
+                facy += I.omega * Fy;
+            } else {
+                facx = 0.0;
+                facy = 0.0;
             }
             __syncthreads();  // B: velocities updated
-            if (mdf) {
-                prev = res;
-                iter_count = it + 1;
-                if (it + 1 >= min_iter && res < tol) {
-                    converged = 1;
-                    break;
-                }
+            if (act == 2) {
+                it = 0;
+                continue;
             }
+            if (ctl.at_b(res, it)) break;
+            ++it;
         }
         if (mdf && t == 0) {
-            I.stats[0] = iter_count;
-            I.stats[1] = last;
-            I.stats[2] = diverged;
-            I.stats[3] = converged;
+            I.stats[0] = ctl.iter_count;
+            I.stats[1] = ctl.last;
+            I.stats[2] = ctl.diverged;
+            I.stats[3] = ctl.converged;
+        }
+        if (pmode != 0) {
+            if (t < Lb) {
+                if (record) {  // training target: scaled change of the converged force
+                    const double sig = (double)rowp[MLP_IN];
+                    rowp[MLP_IN] = (float)((facx - S.fprev[2 * t]) / sig);
+                    rowp[MLP_IN + 1] = (float)((facy - S.fprev[2 * t + 1]) / sig);
+                }
+                S.fprev[2 * t] = facx;
+                S.fprev[2 * t + 1] = facy;
+            }
+            if (t == 0) {
+                if (record) S.pctr[0] += Lb;
+                S.pctr[1] = 1;
+                S.pstat[0] += ctl.applied;
+                S.pstat[1] += 1.0;
+                S.pstat[3] += ctl.restarted;
+            }
         }
     } else {
         ST *__restrict__ B = reinterpret_cast<ST *>(parity ? I.f[0] : I.f[1]);
@@ -279,51 +460,69 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
         }
         double x = u0, y = u1, fx = 0.0, fy = 0.0;
         const double vs = dt / (2.0 * r);  // mdf.py:117
-        for (int it = 0; it < n_iter; ++it) {
-            __syncthreads();  // A
-            const double res = __longlong_as_double((long long)rbits[it & 1]);
-            if (mdf) {
-                last = res;
-                if (it > 0 && res > prev * 1.5) {
-                    diverged = 1;
-                    iter_count = it + 1;
-                    break;
-                }
+        // spread as a gather in np.add.at order (df.py:269-283)
+        auto gather = [&](double &sx, double &sy) {
+            double2 g[FAST_G];
+#pragma unroll
+            for (int e = 0; e < FAST_G; ++e) g[e] = lf[kk[e]];
+            __builtin_amdgcn_sched_barrier(0);
+            // padded entries point at the zero force with weight 0: they add +0.0, which
+            // leaves the sum's bits unchanged (it starts at +0.0 and so is never -0.0)
+            sx = 0.0;
+            sy = 0.0;
+#pragma unroll
+            for (int e = 0; e < FAST_G; ++e) {
+                sx += g[e].x * ww[e];
+                sy += g[e].y * ww[e];
             }
-            if (hn) {  // spread as a gather in np.add.at order (df.py:269-283)
-                double2 g[FAST_G];
-#pragma unroll
-                for (int e = 0; e < FAST_G; ++e) g[e] = lf[kk[e]];
-                __builtin_amdgcn_sched_barrier(0);
-                // padded entries point at the zero force with weight 0: they add +0.0, which
-                // leaves the sum's bits unchanged (it starts at +0.0 and so is never -0.0)
-                double sx = 0.0, sy = 0.0;
-#pragma unroll
-                for (int e = 0; e < FAST_G; ++e) {
-                    sx += g[e].x * ww[e];
-                    sy += g[e].y * ww[e];
-                }
-                if (mdf) {  // mdf.py:197-200
-                    const double gx = sx * omega, gy = sy * omega;
-                    fx = fx + gx;
-                    fy = fy + gy;
-                    x = x + gx * vs;
-                    y = y + gy * vs;
-                } else {
+        };
+        for (int it = 0; it < n_iter; ++seq) {
+            __syncthreads();  // A
+            if (PRED && warm && seq == 0) {  // the predicted force and the velocity it implies
+                if (hn) {
+                    double sx, sy;
+                    gather(sx, sy);
                     fx = sx;
                     fy = sy;
+                    x = u0 + sx * vs;
+                    y = u1 + sy * vs;
+                    uw[n] = make_double2(x, y);
+                }
+                __syncthreads();  // B
+                continue;
+            }
+            const double res = __longlong_as_double((long long)rbits[seq & 1]);
+            const int act = ctl.at_a(res, it);
+            if (act == 1) break;
+            if (hn) {
+                if (act == 0) {
+                    double sx, sy;
+                    gather(sx, sy);
+                    if (mdf) {  // mdf.py:197-200
+                        const double gx = sx * omega, gy = sy * omega;
+                        fx = fx + gx;
+                        fy = fy + gy;
+                        x = x + gx * vs;
+                        y = y + gy * vs;
+                    } else {
+                        fx = sx;
+                        fy = sy;
+                    }
+                } else {  // restart from the uncorrected velocity
+                    fx = 0.0;
+                    fy = 0.0;
+                    x = u0;
+                    y = u1;
                 }
                 uw[n] = make_double2(x, y);
             }
             __syncthreads();  // B
-            if (mdf) {
-                prev = res;
-                iter_count = it + 1;
-                if (it + 1 >= min_iter && res < tol) {
-                    converged = 1;
-                    break;
-                }
+            if (act == 2) {
+                it = 0;
+                continue;
             }
+            if (ctl.at_b(res, it)) break;
+            ++it;
         }
         // P3: velocity correction, equilibrium, forced collision, store
         if (hn) {
@@ -381,13 +580,13 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
     }
 }
 
-template <typename ST, int MODEL>
+template <typename ST, int MODEL, bool PRED>
 __device__ __attribute__((always_inline)) void ibm_fused_dispatch(Inst &I, int parity, int diag,
                                                                  double *red, FusedShared &sh) {
     if (I.delta == 1)
-        ibm_fused_role<ST, MODEL, 1>(I, parity, diag, red, sh);
+        ibm_fused_role<ST, MODEL, 1, PRED>(I, parity, diag, red, sh);
     else
-        ibm_fused_role<ST, MODEL, 2>(I, parity, diag, red, sh);
+        ibm_fused_role<ST, MODEL, 2, PRED>(I, parity, diag, red, sh);
 }
 
 }  // namespace iblbm

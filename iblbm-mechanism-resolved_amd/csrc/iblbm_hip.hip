@@ -3,6 +3,7 @@
 
 #include <cmath>
 #include <cstddef>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
@@ -163,7 +164,7 @@ __global__ __launch_bounds__(NTI) void k_ibm(LaunchTable T) {
 // One whole time step in one launch, for contexts whose windows all take the fused path:
 // workgroups [0, n_fast) run the windows (fused_step.h), the others run four bulk tiles each
 // (one node per thread).  One launch per step leaves no cross-stream wait between steps.
-template <typename S, int MODEL>
+template <typename S, int MODEL, bool PRED>
 __global__ __launch_bounds__(NTF) void k_step(LaunchTable T) {
     __shared__ double red[NTF / 64 + 2];
     __shared__ FusedShared sh;
@@ -175,7 +176,7 @@ __global__ __launch_bounds__(NTF) void k_step(LaunchTable T) {
         typedef __attribute__((address_space(4))) Inst CInst;
         const int id = __builtin_amdgcn_readfirstlane(((CInt *)T.ibm_inst)[blockIdx.x]);
         Inst &I = *(Inst *)((CInst *)T.insts + id);
-        ibm_fused_dispatch<S, MODEL>(I, T.parity, T.diag, red, sh);
+        ibm_fused_dispatch<S, MODEL, PRED>(I, T.parity, T.diag, red, sh);
         return;
     }
     // bulk: every wavefront walks the rows with a grid stride, two rows in flight at a time
@@ -399,30 +400,33 @@ LaunchTable make_table(Ctx &C, int parity, int diag) {
 typedef void (*StepKernel)(LaunchTable);
 
 template <typename S, int MODEL>
-void pick_pair(StepKernel &bulk, StepKernel &ibm, StepKernel &step) {
+void pick_pair(bool pred, StepKernel &bulk, StepKernel &ibm, StepKernel &step) {
     bulk = k_bulk<S, MODEL>;
     ibm = k_ibm<S, MODEL>;
-    step = k_step<S, MODEL>;
+    step = pred ? k_step<S, MODEL, true> : k_step<S, MODEL, false>;
 }
 
 void pick_kernels(const Ctx &C, StepKernel &bulk, StepKernel &ibm, StepKernel &step) {
     int model = C.inst[0].d.cp.model;
-    for (const HostInst &h : C.inst)
+    bool pred = false;  // the step kernel with the predictor modes only when one is set
+    for (const HostInst &h : C.inst) {
         if (h.d.cp.model != model) model = -1;
+        pred = pred || h.d.pmode != 0;
+    }
     const bool f32 = C.inst[0].d.prec == 1;
     switch (model * 2 + (f32 ? 1 : 0)) {
-        case 0: return pick_pair<double, 0>(bulk, ibm, step);
-        case 1: return pick_pair<float, 0>(bulk, ibm, step);
-        case 2: return pick_pair<double, 1>(bulk, ibm, step);
-        case 3: return pick_pair<float, 1>(bulk, ibm, step);
-        case 4: return pick_pair<double, 2>(bulk, ibm, step);
-        case 5: return pick_pair<float, 2>(bulk, ibm, step);
+        case 0: return pick_pair<double, 0>(pred, bulk, ibm, step);
+        case 1: return pick_pair<float, 0>(pred, bulk, ibm, step);
+        case 2: return pick_pair<double, 1>(pred, bulk, ibm, step);
+        case 3: return pick_pair<float, 1>(pred, bulk, ibm, step);
+        case 4: return pick_pair<double, 2>(pred, bulk, ibm, step);
+        case 5: return pick_pair<float, 2>(pred, bulk, ibm, step);
         default: break;
     }
     if (f32)
-        pick_pair<float, -1>(bulk, ibm, step);
+        pick_pair<float, -1>(pred, bulk, ibm, step);
     else
-        pick_pair<double, -1>(bulk, ibm, step);
+        pick_pair<double, -1>(pred, bulk, ibm, step);
 }
 
 // Enqueues nsteps steps.  The bulk kernel runs on the context stream and the IBM kernel on a
@@ -994,6 +998,227 @@ int iblbm_get_diag(void *p, int id, iblbm_diag *out) {
     out->torque_fib = box[3];
     out->torque_q = box[4];
     out->step = C->diag_step;
+    return 0;
+}
+
+// ------------------------------------------------------------------ learned MDF predictor
+
+int iblbm_predictor_mode(void *p, int id, int mode, int64_t capacity) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id)) return fail(IBLBM_EINVAL, "predictor_mode: bad instance id");
+    if (mode < 0 || mode > 3) return fail(IBLBM_EKEY, "predictor_mode: unknown mode");
+    HostInst &h = C->inst[id];
+    Inst &d = h.d;
+    if (mode != 0 && h.c.ibm != IBLBM_IBM_MDF)
+        return fail(IBLBM_EINVAL, "predictor_mode: the predictor drives MDF instances only");
+    if (mode != 0 && d.ibm != 0 && !d.fast)
+        return fail(IBLBM_EINVAL, "predictor_mode: the window does not take the fused path");
+    HIPCHK(hipSetDevice(C->device));
+    IbmScratch &S = d.s;
+    if (mode != 0 && !S.mlp_p) {  // first use: allocate and initialise
+        if (capacity < 1) return fail(IBLBM_EINVAL, "predictor_mode: capacity must be >= 1");
+        void *q;
+        auto A = [&](size_t bytes) -> void * {
+            return C->mem.alloc(&q, bytes) == hipSuccess ? q : nullptr;
+        };
+        S.mlp_p = (float *)A(sizeof(float) * 4 * MLP_NP);  // params, m, v, grad
+        S.mlp_norm = (float *)A(sizeof(float) * MLP_NORM);
+        S.mlp_wb = (unsigned short *)A(sizeof(unsigned short) * MLP_NBLK * 256);
+        S.data = (float *)A(sizeof(float) * MLP_ROW * (size_t)capacity);
+        S.fprev = (double *)A(sizeof(double) * 4 * d.lb_cap);  // forces, then slips
+        S.pstat = (double *)A(sizeof(double) * 4);
+        S.pctr = (long long *)A(sizeof(long long) * 2);
+        if (!S.mlp_p || !S.mlp_norm || !S.mlp_wb || !S.data || !S.fprev || !S.pstat || !S.pctr)
+            return fail(IBLBM_ENOMEM, "predictor_mode: allocation failed");
+        d.data_cap = capacity > INT32_MAX ? INT32_MAX : (int)capacity;
+        // Glorot-uniform weights from a fixed seed, zero biases and Adam moments
+        std::vector<float> hp(4 * MLP_NP, 0.0f);
+        uint64_t z = 0x2545F4914F6CDD1Dull;
+        auto uni = [&]() {
+            z ^= z << 13;
+            z ^= z >> 7;
+            z ^= z << 17;
+            return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+        };
+        auto fill = [&](int off, int fin, int fout) {
+            const double a = std::sqrt(6.0 / (fin + fout));
+            for (int k = 0; k < fin * fout; ++k) hp[off + k] = (float)((2.0 * uni() - 1.0) * a);
+        };
+        fill(MLP_W1, MLP_IN, MLP_H);
+        fill(MLP_W2, MLP_H, MLP_H);
+        fill(MLP_W3, MLP_H, MLP_OUT);
+        std::vector<float> hn(MLP_NORM, 1.0f);
+        for (int k = 0; k < MLP_IN; ++k) hn[k] = 0.0f;
+        HIPCHK(hipMemcpy(S.mlp_p, hp.data(), sizeof(float) * hp.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(S.mlp_norm, hn.data(), sizeof(float) * MLP_NORM, hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(S.pstat, 0, sizeof(double) * 4));
+        HIPCHK(hipMemset(S.pctr, 0, sizeof(long long) * 2));
+        hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(64), 0, C->stream, S.mlp_p, S.mlp_wb);
+        HIPCHK(hipGetLastError());
+    }
+    // The previous force is stale after the predictor was off, and a training sample needs a
+    // previous force from plain MDF: its target is the change between two plain-MDF forces.
+    if ((mode != 0 && d.pmode == 0) || (mode == 1 && d.pmode >= 2)) {
+        HIPCHK(hipMemsetAsync(S.pctr + 1, 0, sizeof(long long), C->stream));
+    }
+    if ((mode != 0) != (d.pmode != 0)) {  // captured steps may hold the other step kernel
+        HIPCHK(hipStreamSynchronize(C->stream));
+        for (auto &g : C->graphs) (void)hipGraphExecDestroy(g.second);
+        C->graphs.clear();
+    }
+    d.pmode = mode;
+    int rc = C->tables_dirty ? ensure_tables(*C) : upload_inst(*C, id);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return 0;
+}
+
+int iblbm_predictor_samples_written(void *p, int id, int64_t *n) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || !n) return fail(IBLBM_EINVAL, "predictor_samples_written: bad argument");
+    const Inst &d = C->inst[id].d;
+    *n = 0;
+    if (!d.s.pctr) return 0;
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    long long c = 0;
+    HIPCHK(hipMemcpy(&c, d.s.pctr, sizeof c, hipMemcpyDeviceToHost));
+    *n = c;
+    return 0;
+}
+
+int iblbm_predictor_samples(void *p, int id, int64_t *n) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || !n) return fail(IBLBM_EINVAL, "predictor_samples: bad argument");
+    const Inst &d = C->inst[id].d;
+    *n = 0;
+    if (!d.s.pctr) return 0;
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    long long c = 0;
+    HIPCHK(hipMemcpy(&c, d.s.pctr, sizeof c, hipMemcpyDeviceToHost));
+    *n = c < d.data_cap ? c : d.data_cap;
+    return 0;
+}
+
+int iblbm_predictor_train(void *p, int id, int steps, double lr, uint32_t seed, float *losses) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || steps < 1 || !(lr > 0.0))
+        return fail(IBLBM_EINVAL, "predictor_train: bad argument");
+    Inst &d = C->inst[id].d;
+    IbmScratch &S = d.s;
+    if (!S.mlp_p) return fail(IBLBM_EINVAL, "predictor_train: predictor not enabled");
+    int64_t n = 0;
+    int rc = iblbm_predictor_samples(p, id, &n);
+    if (rc) return rc;
+    if (n < MLP_B) return fail(IBLBM_EINVAL, "predictor_train: fewer samples than one minibatch");
+    HIPCHK(hipSetDevice(C->device));
+    float *dl = nullptr;
+    HIPCHK(hipMalloc(&dl, sizeof(float) * steps));
+    hipLaunchKernelGGL(k_mlp_norm, dim3(1), dim3(MLP_TT), 0, C->stream, S.data, (long long)n,
+                       S.mlp_norm);
+    // fresh Adam moments, then chunks of at most 256 steps per launch
+    hipError_t e = hipMemsetAsync(S.mlp_p + MLP_NP, 0, sizeof(float) * 2 * MLP_NP, C->stream);
+    for (int s0 = 0; e == hipSuccess && s0 < steps; s0 += 256) {
+        const int k = std::min(256, steps - s0);
+        hipLaunchKernelGGL(k_mlp_train, dim3(1), dim3(MLP_TT), 0, C->stream, S.mlp_p,
+                           S.mlp_p + MLP_NP, S.mlp_p + 2 * MLP_NP, S.mlp_p + 3 * MLP_NP,
+                           (const float *)S.mlp_norm, (const float *)S.data, (long long)n, k, s0,
+                           (float)lr, seed, dl + s0);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(64), 0, C->stream, S.mlp_p, S.mlp_wb);
+        e = hipStreamSynchronize(C->stream);
+    }
+    if (e == hipSuccess && losses)
+        e = hipMemcpy(losses, dl, sizeof(float) * steps, hipMemcpyDeviceToHost);
+    (void)hipFree(dl);
+    if (e != hipSuccess) return fail(IBLBM_EDEVICE, std::string("predictor_train: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int iblbm_predictor_params(void *p, int id, float *params, float *norm) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || !params || !norm) return fail(IBLBM_EINVAL, "predictor_params: bad argument");
+    const IbmScratch &S = C->inst[id].d.s;
+    if (!S.mlp_p) return fail(IBLBM_EINVAL, "predictor_params: predictor not enabled");
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    HIPCHK(hipMemcpy(params, S.mlp_p, sizeof(float) * MLP_NP, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(norm, S.mlp_norm, sizeof(float) * MLP_NORM, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int iblbm_predictor_set_params(void *p, int id, const float *params, const float *norm) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || !params || !norm)
+        return fail(IBLBM_EINVAL, "predictor_set_params: bad argument");
+    IbmScratch &S = C->inst[id].d.s;
+    if (!S.mlp_p) return fail(IBLBM_EINVAL, "predictor_set_params: predictor not enabled");
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    HIPCHK(hipMemcpy(S.mlp_p, params, sizeof(float) * MLP_NP, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(S.mlp_norm, norm, sizeof(float) * MLP_NORM, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(64), 0, C->stream, S.mlp_p, S.mlp_wb);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return 0;
+}
+
+int iblbm_predictor_data(void *p, int id, int64_t first, int64_t n, float *rows) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || !rows || first < 0 || n < 0)
+        return fail(IBLBM_EINVAL, "predictor_data: bad argument");
+    const Inst &d = C->inst[id].d;
+    if (!d.s.data) return fail(IBLBM_EINVAL, "predictor_data: predictor not enabled");
+    if (n > d.data_cap) return fail(IBLBM_EINVAL, "predictor_data: more rows than the capacity");
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    for (int64_t r = 0; r < n;) {  // the samples form a ring of data_cap rows
+        const int64_t at = (first + r) % d.data_cap, k = std::min<int64_t>(n - r, d.data_cap - at);
+        HIPCHK(hipMemcpy(rows + r * MLP_ROW, d.s.data + at * MLP_ROW, sizeof(float) * MLP_ROW * k,
+                         hipMemcpyDeviceToHost));
+        r += k;
+    }
+    return 0;
+}
+
+int iblbm_predictor_previous_force(void *p, int id, double *F, int set) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || !F) return fail(IBLBM_EINVAL, "predictor_previous_force: bad argument");
+    const HostInst &h = C->inst[id];
+    const IbmScratch &S = h.d.s;
+    if (!S.fprev) return fail(IBLBM_EINVAL, "predictor_previous_force: predictor not enabled");
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    const size_t bytes = sizeof(double) * 2 * h.Lb;
+    double *slip = S.fprev + 2 * h.d.lb_cap;
+    if (set) {
+        HIPCHK(hipMemcpy(S.fprev, F, bytes, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(slip, F + 2 * h.Lb, bytes, hipMemcpyHostToDevice));
+        const long long one = 1;
+        HIPCHK(hipMemcpy(S.pctr + 1, &one, sizeof one, hipMemcpyHostToDevice));
+    } else {
+        HIPCHK(hipMemcpy(F, S.fprev, bytes, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(F + 2 * h.Lb, slip, bytes, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+int iblbm_predictor_stats(void *p, int id, double out[4], int reset) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || !out) return fail(IBLBM_EINVAL, "predictor_stats: bad argument");
+    const IbmScratch &S = C->inst[id].d.s;
+    for (int k = 0; k < 4; ++k) out[k] = 0.0;
+    if (!S.pstat) return 0;
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    HIPCHK(hipMemcpy(out, S.pstat, sizeof(double) * 4, hipMemcpyDeviceToHost));
+    if (reset) {
+        HIPCHK(hipMemset(S.pstat, 0, sizeof(double) * 4));
+        HIPCHK(hipDeviceSynchronize());
+    }
     return 0;
 }
 

@@ -14,6 +14,7 @@
 #include "../../include/iblbm_hip.h"
 #include "ibm_block.h"
 #include "scratch_alloc.h"
+#include "mlp.h"
 
 using namespace iblbm;
 
@@ -522,6 +523,37 @@ double iblbm_omega_estimate(const double *Lx, const double *Ly, int Lb, int delt
         best = std::max(best, acc);
     }
     return 1.0 / best;
+}
+
+int iblbm_mlp_loss_grad(const float *params, const float *x, const float *y, int nb,
+                        float *grad, float *loss) {
+    if (!params || !x || !y || !grad || !loss || nb < 1 || nb > MLP_B)
+        return op_fail(IBLBM_EINVAL, "mlp_loss_grad: bad argument (1 <= nb <= 128)");
+    Dev m;
+    const float *dp = m.up(params, MLP_NP), *dx = m.up(x, (size_t)nb * MLP_IN),
+                *dy = m.up(y, (size_t)nb * MLP_OUT);
+    float *dg = m.get<float>(MLP_NP), *dl = m.get<float>(1);
+    NEED(dp && dx && dy && dg && dl);
+    hipLaunchKernelGGL(k_mlp_grad, dim3(1), dim3(MLP_TT), 0, 0, dp, dx, dy, nb, dg, dl);
+    OPCHK(hipGetLastError());
+    OPCHK(hipMemcpy(grad, dg, sizeof(float) * MLP_NP, hipMemcpyDeviceToHost));
+    OPCHK(hipMemcpy(loss, dl, sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int iblbm_mlp_infer_bf16(const float *params, const float *x, int n, float *out) {
+    if (!params || !x || !out || n < 1) return op_fail(IBLBM_EINVAL, "mlp_infer_bf16: bad argument");
+    Dev m;
+    const float *dp = m.up(params, MLP_NP), *dx = m.up(x, (size_t)n * MLP_IN);
+    unsigned short *wb = m.get<unsigned short>(MLP_NBLK * 256);
+    float *dout = m.get<float>((size_t)n * MLP_OUT);
+    NEED(dp && dx && wb && dout);
+    hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(64), 0, 0, dp, wb);
+    hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, 0, wb, dp, dx,
+                       n, dout);
+    OPCHK(hipGetLastError());
+    OPCHK(hipMemcpy(out, dout, sizeof(float) * n * MLP_OUT, hipMemcpyDeviceToHost));
+    return 0;
 }
 
 }  // extern "C"

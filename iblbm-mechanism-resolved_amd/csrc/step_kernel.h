@@ -20,7 +20,7 @@ constexpr int NTI = 512;
 constexpr int NTF = 1024;              // threads of a fused-step workgroup (16 wavefronts)
 constexpr int FAST_T = NTF - 64;       // touched-node capacity of the fused window (1 per thread)
 constexpr int FAST_G = 8;              // gather entries per touched node on that path
-constexpr int FAST_LB = 192;           // markers on that path (up to 3 marker wavefronts)
+constexpr int FAST_LB = 128;           // markers on that path (up to 2 marker wavefronts)
 constexpr int FAST_K = 25;             // stencil points per marker (Peskin 4-point: 5 x 5)
 
 struct IbmScratch {
@@ -48,6 +48,14 @@ struct IbmScratch {
     // (padding: marker Lb, weight 0), and the marker stencils stencil-major [s * Lb + k]
     int *pk, *mti;
     double *pw, *mw;
+    // learned MDF force predictor (allocated when enabled): FP32 parameters, normalisation,
+    // BF16 weight blocks, training samples (features, target), previous converged marker
+    // forces, counters [samples written, previous force valid] and statistics [applied
+    // corrections, steps, first residual, fallbacks]
+    float *mlp_p, *mlp_norm, *data;
+    unsigned short *mlp_wb;
+    double *fprev, *pstat;
+    long long *pctr;
 };
 
 struct Inst {
@@ -67,6 +75,7 @@ struct Inst {
     int bx0, by0, bw, bh;
     int box_cap, n_iter, min_iter, fixed_body;
     int nT, fast;  // touched nodes; 1 when the register-resident window path applies
+    int pmode, data_cap;  // predictor mode (0 off, 1 collect, 2 warm start, 3 learned), capacity
     double dx, dy, Larea, omega, tol, dfc_scale;
     const double *Lx, *Ly, *ud;
     IbmScratch s;

@@ -194,6 +194,25 @@ class Sedimentation:
         self.d_lattice = 2.0 * st.lattice_r
         self.termination_reason = "max_steps"
 
+    @property
+    def predictor(self):
+        """The learned MDF force predictor of this instance (``iblbm_amd.predictor``)."""
+        from .predictor import Predictor
+        return Predictor(self.ctx, self.k)
+
+    def snapshot(self):
+        """Device populations and host particle state, for ``restore``."""
+        import copy
+        return self.ctx.download_f(self.k), copy.deepcopy(self.particles), self.ttt
+
+    def restore(self, snap):
+        import copy
+        f, parts, ttt = snap
+        self.particles = copy.deepcopy(parts)
+        self.ttt = ttt
+        self._push_markers()
+        self.ctx.upload_f(self.k, f)
+
     def _push_markers(self):
         Lx = np.concatenate([p.Lx for p in self.particles])
         Ly = np.concatenate([p.Ly for p in self.particles])

@@ -217,6 +217,12 @@ class FixedCylinder:
             self.ctx.step(n, diag_every, graph)
             self.ttt += n
 
+    @property
+    def predictor(self):
+        """The learned MDF force predictor of this instance (``iblbm_amd.predictor``)."""
+        from .predictor import Predictor
+        return Predictor(self.ctx, self.k)
+
     def cd_cl(self):
         """Cd, Cl of the last diagnostic step (``iblbm/diagnostics/core.py:53-113``)."""
         d = self.ctx.diag(self.k)
