@@ -21,6 +21,7 @@ IBM = {None: 0, "NONE": 0, "DF": 1, "MDF": 2, "DFC": 3}
 DELTA = {"peskin4pt": 0, "hat": 1}
 DELTA_RADIUS = {"peskin4pt": 2, "hat": 1}
 PRED = {"off": 0, "collect": 1, "warm": 2, "learned": 3}
+MODES = {"auto": -1, "fused": 0, "two_streams": 1, "serial": 2}
 MLP_IN, MLP_OUT, MLP_NP, MLP_NORM = 8, 2, 4866, 18
 FP64, FP32_DEV = 0, 1
 
@@ -40,6 +41,12 @@ class Case(ctypes.Structure):
         ("lb_capacity", ctypes.c_int32), ("box_capacity", ctypes.c_int32),
         ("fixed_body", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
+
+
+class Schedule(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int32), ("bulk_threads", ctypes.c_int32),
+                ("bulk_rows", ctypes.c_int32), ("bulk_waves", ctypes.c_int32),
+                ("step_rows", ctypes.c_int32)]
 
 
 class MdfStats(ctypes.Structure):
@@ -77,6 +84,10 @@ SIGNATURES = {
     "iblbm_graph_run": (_i, [_VP, _i, _i]),
     "iblbm_set_concurrent": (_i, [_VP, _i]),
     "iblbm_step_mode": (_i, [_VP, _I]),
+    "iblbm_get_schedule": (_i, [_VP, ctypes.POINTER(Schedule)]),
+    "iblbm_set_schedule": (_i, [_VP, ctypes.POINTER(Schedule)]),
+    "iblbm_autotune": (_i, [_VP, _i, _D, _I]),
+    "iblbm_init_flow": (_i, [_VP, _i, _d, _d, _d, _d, ctypes.c_uint64]),
     "iblbm_get_mdf_stats": (_i, [_VP, _i, ctypes.POINTER(MdfStats)]),
     "iblbm_get_diag": (_i, [_VP, _i, ctypes.POINTER(Diag)]),
     "iblbm_download_fib": (_i, [_VP, _i, _D]),

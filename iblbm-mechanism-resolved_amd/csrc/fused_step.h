@@ -188,11 +188,12 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
     const int nbox = I.bw * I.bh;
     const bool interior = I.bx0 >= 1 && I.by0 >= 1 && I.bx0 + I.bw <= I.nx - 1 &&
                           I.by0 + I.bh <= I.ny - 1;
-    for (int base = 0; base < nbox; base += 2 * NTF) {
-        int tx[2];
-        double fs[2][9];
+    constexpr int PU = 2;
+    for (int base = 0; base < nbox; base += PU * NTF) {
+        int tx[PU];
+        double fs[PU][9];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < PU; ++u) {
             const int s0 = base + u * NTF + t;
             tx[u] = -1;
             if (s0 < nbox) {
@@ -215,7 +216,7 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
             }
         }
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < PU; ++u) {
             const int m = tx[u];
             if (m >= 0) {
                 double r, u0, u1;

@@ -118,9 +118,15 @@ __device__ __attribute__((always_inline)) TileNode row_node(const LaunchTable &T
     row = __builtin_amdgcn_readfirstlane(row);
     const int tb = row / TILE_Y;
     const CInt *prefix = (const CInt *)T.tile_prefix;
-    int inst = 0;
-    while (inst + 1 < T.n_inst && prefix[inst + 1] <= tb) ++inst;
-    inst = __builtin_amdgcn_readfirstlane(inst);
+    int lo = 0, hi = T.n_inst - 1;  // the last instance whose first tile is <= tb
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= tb)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int inst = __builtin_amdgcn_readfirstlane(lo);
     const Inst &I = *(const Inst *)((const CInst *)T.insts + inst);
     const int lt = tb - prefix[inst];
     const int tx = lt % I.tiles_x, ty = lt / I.tiles_x;
@@ -140,17 +146,66 @@ __device__ __attribute__((always_inline)) TileNode row_node(const LaunchTable &T
     return n;
 }
 
-// Bulk tiles of one time step: every node that no marker stencil touches.
-template <typename S, int MODEL>
-__global__ __launch_bounds__(NT_BULK) void k_bulk(LaunchTable T) {
-    const int row = (int)blockIdx.x * (NT_BULK / 64) + (int)(threadIdx.x >> 6);
-    const TileNode n = row_node(T, row, (int)(threadIdx.x & 63));
-    double rho = 0.0;
-    if (n.mine) rho = bulk_node<S, MODEL>(*n.I, T.parity, n.i, n.j);
-    if (T.diag) {  // one FP64 partial per row: a fixed shuffle tree over the wavefront
-        const double v = wave_sum(rho);
-        if ((threadIdx.x & 63) == 0) T.rho_partial[row] = v;
+// Bulk rows of one time step (every node that no marker stencil touches): a wavefront takes
+// rows first, first + nw, ...; with TWO it keeps two rows' loads in flight at a time.  Each
+// row's FP64 partial rho sum is a fixed shuffle tree over the wavefront.
+template <typename S, int MODEL, bool TWO>
+__device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, int first, int nw) {
+    const int lane = (int)threadIdx.x & 63;
+    const int n_rows = T.n_tiles * TILE_Y;
+    if constexpr (TWO) {
+        for (int r0 = first; r0 < n_rows; r0 += 2 * nw) {
+            const bool v1 = r0 + nw < n_rows;
+            const int r1 = v1 ? r0 + nw : r0;  // a missing second row repeats the first, unused
+            const TileNode n0 = row_node(T, r0, lane);
+            TileNode n1 = row_node(T, r1, lane);
+            n1.mine = n1.mine && v1;
+            S a0[9], a1[9];
+            if (n0.interior && n1.interior) {  // uniform: both rows' 18 loads in one batch
+                bulk_pull_interior(*n0.I, T.parity, n0.i, n0.j, a0);
+                bulk_pull_interior(*n1.I, T.parity, n1.i, n1.j, a1);
+                __builtin_amdgcn_sched_barrier(0);
+            } else {
+                if (n0.mine) bulk_pull(*n0.I, T.parity, n0.i, n0.j, a0);
+                if (n1.mine) bulk_pull(*n1.I, T.parity, n1.i, n1.j, a1);
+            }
+            double rho0 = 0.0, rho1 = 0.0;
+            if (n0.mine) rho0 = bulk_finish<MODEL>(*n0.I, T.parity, n0.i, n0.j, a0);
+            if (n1.mine) rho1 = bulk_finish<MODEL>(*n1.I, T.parity, n1.i, n1.j, a1);
+            if (T.diag) {
+                const double w0 = wave_sum(rho0), w1 = wave_sum(rho1);
+                if (lane == 0) {
+                    T.rho_partial[r0] = w0;
+                    if (v1) T.rho_partial[r1] = w1;
+                }
+            }
+        }
+    } else {
+        for (int r0 = first; r0 < n_rows; r0 += nw) {
+            const TileNode n0 = row_node(T, r0, lane);
+            S a0[9];
+            if (n0.interior)
+                bulk_pull_interior(*n0.I, T.parity, n0.i, n0.j, a0);
+            else if (n0.mine)
+                bulk_pull(*n0.I, T.parity, n0.i, n0.j, a0);
+            double rho0 = 0.0;
+            if (n0.mine) rho0 = bulk_finish<MODEL>(*n0.I, T.parity, n0.i, n0.j, a0);
+            if (T.diag) {
+                const double w0 = wave_sum(rho0);
+                if (lane == 0) T.rho_partial[r0] = w0;
+            }
+        }
     }
+}
+
+// Bulk kernel of the two-kernel step.  Any workgroup size up to 1024 (a tile of 64 x
+// blockDim / 64 rows per workgroup); the grid either covers every row once or walks them
+// with a grid stride (the schedule picks both).
+template <typename S, int MODEL, bool TWO>
+__global__ __launch_bounds__(1024) void k_bulk(LaunchTable T) {
+    const int wpb = (int)blockDim.x >> 6;
+    bulk_rows<S, MODEL, TWO>(T, (int)blockIdx.x * wpb + (int)(threadIdx.x >> 6),
+                             (int)gridDim.x * wpb);
 }
 
 // Immersed-boundary workgroups of one time step (one per instance with markers).
@@ -179,37 +234,42 @@ __global__ __launch_bounds__(NTF) void k_step(LaunchTable T) {
         ibm_fused_dispatch<S, MODEL, PRED>(I, T.parity, T.diag, red, sh);
         return;
     }
-    // bulk: every wavefront walks the rows with a grid stride, two rows in flight at a time
-    // (no workgroup barrier: a wavefront moves on as soon as its rows are done)
-    const int lane = (int)threadIdx.x & 63;
+    // bulk: every wavefront walks the rows with a grid stride (no workgroup barrier: a
+    // wavefront moves on as soon as its rows are done)
+    const int first = ((int)blockIdx.x - T.n_fast) * (NTF / 64) + (int)(threadIdx.x >> 6);
     const int nw = ((int)gridDim.x - T.n_fast) * (NTF / 64);
-    const int n_rows = T.n_tiles * TILE_Y;
-    for (int r0 = ((int)blockIdx.x - T.n_fast) * (NTF / 64) + (int)(threadIdx.x >> 6);
-         r0 < n_rows; r0 += 2 * nw) {
-        const bool v1 = r0 + nw < n_rows;
-        const int r1 = v1 ? r0 + nw : r0;  // a missing second row repeats the first, unused
-        const TileNode n0 = row_node(T, r0, lane);
-        TileNode n1 = row_node(T, r1, lane);
-        n1.mine = n1.mine && v1;
-        S a0[9], a1[9];
-        if (n0.interior && n1.interior) {  // uniform: both rows' 18 loads in one batch
-            bulk_pull_interior(*n0.I, T.parity, n0.i, n0.j, a0);
-            bulk_pull_interior(*n1.I, T.parity, n1.i, n1.j, a1);
-            __builtin_amdgcn_sched_barrier(0);
-        } else {
-            if (n0.mine) bulk_pull(*n0.I, T.parity, n0.i, n0.j, a0);
-            if (n1.mine) bulk_pull(*n1.I, T.parity, n1.i, n1.j, a1);
-        }
-        double rho0 = 0.0, rho1 = 0.0;
-        if (n0.mine) rho0 = bulk_finish<MODEL>(*n0.I, T.parity, n0.i, n0.j, a0);
-        if (n1.mine) rho1 = bulk_finish<MODEL>(*n1.I, T.parity, n1.i, n1.j, a1);
-        if (T.diag) {  // one FP64 partial per row (same tree as k_bulk)
-            const double w0 = wave_sum(rho0), w1 = wave_sum(rho1);
-            if (lane == 0) {
-                T.rho_partial[r0] = w0;
-                if (v1) T.rho_partial[r1] = w1;
-            }
-        }
+    if (T.rows2)
+        bulk_rows<S, MODEL, true>(T, first, nw);
+    else
+        bulk_rows<S, MODEL, false>(T, first, nw);
+}
+
+// Seeded synthetic initial state f = f_eq(rho, u(1 + amp xi)) in both population buffers;
+// xi in (-1, 1) per node and component from a splitmix64 hash of (seed, node, component).
+__device__ inline double hash_sym(unsigned long long seed, long long key) {
+    unsigned long long z = seed * 0xD1B54A32D192ED03ull + (unsigned long long)key;
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * 0x1.0p-52 - 1.0;  // [-1, 1)
+}
+template <typename S>
+__global__ __launch_bounds__(256) void k_init_flow(Inst I, double rho, double ux, double uy,
+                                                   double amp, unsigned long long seed) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (int64_t)I.nx * I.ny) return;
+    const int i = (int)(n % I.nx), j = (int)(n / I.nx);
+    const double u0 = ux * (1.0 + amp * hash_sym(seed, 2 * n));
+    const double u1 = uy * (1.0 + amp * hash_sym(seed, 2 * n + 1));
+    double fe[9];
+    feq9(rho, u0, u1, I.inc != 0, fe);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const int64_t at = q * I.qs + (int64_t)j * I.pitch + i;
+        const auto v = Store<S>::st(fe[q], q);
+        reinterpret_cast<S *>(I.f[0])[at] = v;
+        reinterpret_cast<S *>(I.f[1])[at] = v;
     }
 }
 
@@ -322,7 +382,28 @@ struct Ctx {
     int last_steps = 0;
     int64_t diag_step = -1;
     std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
+    iblbm_schedule sched{IBLBM_MODE_AUTO, 256, 1, 0, 2};
 };
+
+void clear_graphs(Ctx &C) {
+    for (auto &g : C.graphs) (void)hipGraphExecDestroy(g.second);
+    C.graphs.clear();
+}
+
+// The launch mode a step takes under the context's schedule: fused needs every window on
+// the fast path and the concurrent setting; an explicit mode is honoured where it applies.
+int effective_mode(const Ctx &C) {
+    const bool fused_ok = C.n_fast == C.n_ibm;
+    if (!C.concurrent) return IBLBM_MODE_SERIAL;
+    switch (C.sched.mode) {
+        case IBLBM_MODE_FUSED: return fused_ok ? IBLBM_MODE_FUSED : IBLBM_MODE_TWO_STREAMS;
+        case IBLBM_MODE_TWO_STREAMS: return C.n_ibm > 0 ? IBLBM_MODE_TWO_STREAMS : IBLBM_MODE_SERIAL;
+        case IBLBM_MODE_SERIAL: return IBLBM_MODE_SERIAL;
+        default: break;
+    }
+    if (fused_ok) return IBLBM_MODE_FUSED;
+    return C.n_ibm > 0 ? IBLBM_MODE_TWO_STREAMS : IBLBM_MODE_SERIAL;
+}
 
 int ensure_tables(Ctx &C) {
     if (!C.tables_dirty) return 0;
@@ -369,8 +450,7 @@ int ensure_tables(Ctx &C) {
         HIPCHK(hipMemcpyAsync(C.d_ibm_inst, C.h_ibm_inst.data(), sizeof(int) * C.n_ibm,
                               hipMemcpyHostToDevice, C.stream));
     HIPCHK(hipStreamSynchronize(C.stream));
-    for (auto &g : C.graphs) (void)hipGraphExecDestroy(g.second);
-    C.graphs.clear();
+    clear_graphs(C);
     C.tables_dirty = false;
     return 0;
 }
@@ -394,19 +474,21 @@ LaunchTable make_table(Ctx &C, int parity, int diag) {
     T.n_tiles = C.n_tiles;
     T.parity = parity;
     T.diag = diag;
+    T.rows2 = 0;
     return T;
 }
 
 typedef void (*StepKernel)(LaunchTable);
 
 template <typename S, int MODEL>
-void pick_pair(bool pred, StepKernel &bulk, StepKernel &ibm, StepKernel &step) {
-    bulk = k_bulk<S, MODEL>;
+void pick_pair(bool pred, bool two, StepKernel &bulk, StepKernel &ibm, StepKernel &step) {
+    bulk = two ? k_bulk<S, MODEL, true> : k_bulk<S, MODEL, false>;
     ibm = k_ibm<S, MODEL>;
     step = pred ? k_step<S, MODEL, true> : k_step<S, MODEL, false>;
 }
 
 void pick_kernels(const Ctx &C, StepKernel &bulk, StepKernel &ibm, StepKernel &step) {
+    const bool two = C.sched.bulk_rows == 2;
     int model = C.inst[0].d.cp.model;
     bool pred = false;  // the step kernel with the predictor modes only when one is set
     for (const HostInst &h : C.inst) {
@@ -415,43 +497,51 @@ void pick_kernels(const Ctx &C, StepKernel &bulk, StepKernel &ibm, StepKernel &s
     }
     const bool f32 = C.inst[0].d.prec == 1;
     switch (model * 2 + (f32 ? 1 : 0)) {
-        case 0: return pick_pair<double, 0>(pred, bulk, ibm, step);
-        case 1: return pick_pair<float, 0>(pred, bulk, ibm, step);
-        case 2: return pick_pair<double, 1>(pred, bulk, ibm, step);
-        case 3: return pick_pair<float, 1>(pred, bulk, ibm, step);
-        case 4: return pick_pair<double, 2>(pred, bulk, ibm, step);
-        case 5: return pick_pair<float, 2>(pred, bulk, ibm, step);
+        case 0: return pick_pair<double, 0>(pred, two, bulk, ibm, step);
+        case 1: return pick_pair<float, 0>(pred, two, bulk, ibm, step);
+        case 2: return pick_pair<double, 1>(pred, two, bulk, ibm, step);
+        case 3: return pick_pair<float, 1>(pred, two, bulk, ibm, step);
+        case 4: return pick_pair<double, 2>(pred, two, bulk, ibm, step);
+        case 5: return pick_pair<float, 2>(pred, two, bulk, ibm, step);
         default: break;
     }
     if (f32)
-        pick_pair<float, -1>(pred, bulk, ibm, step);
+        pick_pair<float, -1>(pred, two, bulk, ibm, step);
     else
-        pick_pair<double, -1>(pred, bulk, ibm, step);
+        pick_pair<double, -1>(pred, two, bulk, ibm, step);
 }
 
-// Enqueues nsteps steps.  The bulk kernel runs on the context stream and the IBM kernel on a
-// second stream; both depend on both kernels of the previous step through events, so the
-// window workgroups overlap the bulk tiles of the same step.  With concurrent = false both
-// run in order on the context stream.
+// Enqueues nsteps steps under the context's schedule.
+//  fused: one k_step launch per step; the window workgroups come first, then one bulk
+//    workgroup per remaining CU (a fused workgroup fills a CU's LDS) walking the rows.
+//  two streams: the window kernel on a second stream, the bulk kernel on the context stream;
+//    both depend on both kernels of the previous step through events, so the window
+//    workgroups overlap the bulk tiles of the same step.
+//  serial: window kernel then bulk kernel on the context stream.
 int launch_steps(Ctx &C, int nsteps, int diag_every, bool concurrent) {
     StepKernel kb, ki, ks;
     pick_kernels(C, kb, ki, ks);
-    if (concurrent && C.n_fast == C.n_ibm) {  // every window is fused: one launch per step
-        // one bulk workgroup per CU left over (a fused workgroup fills a CU's LDS)
-        const int rows = C.n_tiles * TILE_Y;
+    const int rows = C.n_tiles * TILE_Y;
+    const int mode = concurrent ? effective_mode(C) : IBLBM_MODE_SERIAL;
+    if (mode == IBLBM_MODE_FUSED) {
         const int bulk = std::max(1, std::min(C.n_cu - C.n_fast, (rows + NTF / 64 - 1) / (NTF / 64)));
         const int blocks = C.n_fast + bulk;
         for (int s = 0; s < nsteps; ++s) {
             const int64_t stepno = C.step + s + 1;
             const int diag = (diag_every > 0 && stepno % diag_every == 0) ? 1 : 0;
             LaunchTable T = make_table(C, (C.parity + s) & 1, diag);
-            T.n_fast = C.n_fast;
+            T.rows2 = C.sched.step_rows == 2 ? 1 : 0;
             hipLaunchKernelGGL(ks, dim3(blocks), dim3(NTF), 0, C.stream, T);
         }
         HIPCHK(hipGetLastError());
         return 0;
     }
-    const bool two = concurrent && C.n_ibm > 0;
+    const int bt = C.sched.bulk_threads, wpb = bt / 64;
+    const int full = (rows + wpb - 1) / wpb;  // one row per wavefront
+    const int bblocks = C.sched.bulk_waves > 0
+                            ? std::max(1, std::min(full, (C.sched.bulk_waves + wpb - 1) / wpb))
+                            : full;
+    const bool two = mode == IBLBM_MODE_TWO_STREAMS;
     if (two) {  // join the side stream to everything already queued on the main stream
         HIPCHK(hipEventRecord(C.ev_main, C.stream));
         HIPCHK(hipStreamWaitEvent(C.side, C.ev_main, 0));
@@ -460,18 +550,19 @@ int launch_steps(Ctx &C, int nsteps, int diag_every, bool concurrent) {
         const int64_t stepno = C.step + s + 1;
         const int diag = (diag_every > 0 && stepno % diag_every == 0) ? 1 : 0;
         LaunchTable T = make_table(C, (C.parity + s) & 1, diag);
+        T.rows2 = C.sched.bulk_rows == 2 ? 1 : 0;
         if (two) {
             if (s > 0) {
                 HIPCHK(hipStreamWaitEvent(C.stream, C.ev_side, 0));
                 HIPCHK(hipStreamWaitEvent(C.side, C.ev_main, 0));
             }
             hipLaunchKernelGGL(ki, dim3(C.n_ibm), dim3(NTI), 0, C.side, T);
-            hipLaunchKernelGGL(kb, dim3(C.n_tiles), dim3(NT_BULK), 0, C.stream, T);
+            if (rows) hipLaunchKernelGGL(kb, dim3(bblocks), dim3(bt), 0, C.stream, T);
             HIPCHK(hipEventRecord(C.ev_main, C.stream));
             HIPCHK(hipEventRecord(C.ev_side, C.side));
         } else {
             if (C.n_ibm) hipLaunchKernelGGL(ki, dim3(C.n_ibm), dim3(NTI), 0, C.stream, T);
-            if (C.n_tiles) hipLaunchKernelGGL(kb, dim3(C.n_tiles), dim3(NT_BULK), 0, C.stream, T);
+            if (rows) hipLaunchKernelGGL(kb, dim3(bblocks), dim3(bt), 0, C.stream, T);
         }
     }
     if (two) HIPCHK(hipStreamWaitEvent(C.stream, C.ev_side, 0));  // join back
@@ -941,8 +1032,142 @@ int iblbm_step_mode(void *p, int *mode) {
     if (!C || !mode) return fail(IBLBM_EINVAL, "step_mode: bad argument");
     int rc = ensure_tables(*C);
     if (rc) return rc;
-    *mode = !C->concurrent ? IBLBM_MODE_SERIAL
-                           : (C->n_fast == C->n_ibm ? IBLBM_MODE_FUSED : IBLBM_MODE_TWO_STREAMS);
+    *mode = effective_mode(*C);
+    return 0;
+}
+
+int iblbm_get_schedule(void *p, iblbm_schedule *out) {
+    Ctx *C = as_ctx(p);
+    if (!C || !out) return fail(IBLBM_EINVAL, "get_schedule: bad argument");
+    *out = C->sched;
+    return 0;
+}
+
+int iblbm_set_schedule(void *p, const iblbm_schedule *in) {
+    Ctx *C = as_ctx(p);
+    if (!C || !in) return fail(IBLBM_EINVAL, "set_schedule: bad argument");
+    if (in->mode < IBLBM_MODE_AUTO || in->mode > IBLBM_MODE_SERIAL)
+        return fail(IBLBM_EINVAL, "set_schedule: unknown mode");
+    if (in->bulk_threads < 64 || in->bulk_threads > 1024 || in->bulk_threads % 64)
+        return fail(IBLBM_EINVAL, "set_schedule: bulk_threads must be a multiple of 64 in [64, 1024]");
+    if ((in->bulk_rows != 1 && in->bulk_rows != 2) || (in->step_rows != 1 && in->step_rows != 2))
+        return fail(IBLBM_EINVAL, "set_schedule: rows in flight must be 1 or 2");
+    if (in->bulk_waves < 0) return fail(IBLBM_EINVAL, "set_schedule: bulk_waves < 0");
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    C->sched = *in;
+    clear_graphs(*C);
+    return 0;
+}
+
+int iblbm_autotune(void *p, int steps, double *best_ms, int *n_tried) {
+    Ctx *C = as_ctx(p);
+    if (!C || steps < 1) return fail(IBLBM_EINVAL, "autotune: bad argument");
+    if (C->inst.empty()) return fail(IBLBM_EINVAL, "autotune: no instance");
+    for (const HostInst &h : C->inst)
+        if (h.d.pmode != 0)
+            return fail(IBLBM_EINVAL, "autotune: switch the learned predictor off first");
+    HIPCHK(hipSetDevice(C->device));
+    int rc = ensure_tables(*C);
+    if (rc) return rc;
+    // candidates: the fused launch where every window qualifies, and the two-kernel launch
+    // over workgroup sizes, rows in flight and grid (every row once, or 16 / 32 wavefronts
+    // per CU walking the rows)
+    std::vector<iblbm_schedule> cand;
+    if (C->n_fast == C->n_ibm)
+        for (int r = 1; r <= 2; ++r) cand.push_back({IBLBM_MODE_FUSED, 256, 1, 0, r});
+    const int mode2 = C->n_ibm > 0 ? IBLBM_MODE_TWO_STREAMS : IBLBM_MODE_SERIAL;
+    for (int bt : {256, 512, 1024})
+        for (int r = 1; r <= 2; ++r)
+            for (int wv : {0, 16 * C->n_cu, 32 * C->n_cu}) cand.push_back({mode2, bt, r, wv, 2});
+    // the trial steps start from a copy of every instance's source populations
+    const int n = (int)C->inst.size();
+    std::vector<void *> backup(n, nullptr);
+    auto release = [&]() {
+        for (void *b : backup)
+            if (b) (void)hipFree(b);
+    };
+    auto fbytes = [&](const Inst &d) {
+        return (d.prec == 1 ? sizeof(float) : sizeof(double)) * 9 * (size_t)d.qs;
+    };
+    for (int k = 0; k < n; ++k) {
+        const Inst &d = C->inst[k].d;
+        if (hipMalloc(&backup[k], fbytes(d)) != hipSuccess) {
+            backup[k] = nullptr;
+            release();
+            return fail(IBLBM_ENOMEM, "autotune: no room for a copy of the populations");
+        }
+        HIPCHK(hipMemcpyAsync(backup[k], d.f[C->parity], fbytes(d), hipMemcpyDeviceToDevice,
+                              C->stream));
+    }
+    const int64_t step0 = C->step;
+    const int parity0 = C->parity;
+    auto restore = [&]() -> int {
+        for (int k = 0; k < n; ++k) {
+            const Inst &d = C->inst[k].d;
+            HIPCHK(hipMemcpyAsync(d.f[parity0], backup[k], fbytes(d), hipMemcpyDeviceToDevice,
+                                  C->stream));
+        }
+        C->step = step0;
+        C->parity = parity0;
+        return 0;
+    };
+    const iblbm_schedule keep = C->sched;
+    const bool conc = C->concurrent;
+    C->concurrent = true;
+    double best = 1e300;
+    iblbm_schedule pick = keep;
+    const int warm = 2, timed = steps + (steps & 1);  // even: the trial ends on the same parity
+    for (const iblbm_schedule &sc : cand) {
+        C->sched = sc;
+        rc = restore();
+        if (!rc) rc = launch_steps(*C, warm, 0, true);
+        C->parity = parity0;
+        if (!rc) HIPCHK(hipEventRecord(C->ev0, C->stream));
+        if (!rc) rc = launch_steps(*C, timed, 0, true);
+        if (!rc) HIPCHK(hipEventRecord(C->ev1, C->stream));
+        if (!rc) HIPCHK(hipEventSynchronize(C->ev1));
+        if (rc) {
+            release();
+            C->sched = keep;
+            C->concurrent = conc;
+            return rc;
+        }
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, C->ev0, C->ev1));
+        if (ms / timed < best) {
+            best = ms / timed;
+            pick = sc;
+        }
+    }
+    rc = restore();
+    HIPCHK(hipStreamSynchronize(C->stream));
+    release();
+    C->concurrent = conc;
+    C->sched = pick;
+    clear_graphs(*C);
+    if (rc) return rc;
+    if (best_ms) *best_ms = best;
+    if (n_tried) *n_tried = (int)cand.size();
+    return 0;
+}
+
+int iblbm_init_flow(void *p, int id, double rho, double ux, double uy, double amp,
+                    uint64_t seed) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id)) return fail(IBLBM_EINVAL, "init_flow: bad instance id");
+    HIPCHK(hipSetDevice(C->device));
+    const Inst &d = C->inst[id].d;
+    const int64_t N = (int64_t)d.nx * d.ny;
+    const int blocks = (int)((N + 255) / 256);
+    if (d.prec == 1)
+        hipLaunchKernelGGL(k_init_flow<float>, dim3(blocks), dim3(256), 0, C->stream, d, rho, ux,
+                           uy, amp, (unsigned long long)seed);
+    else
+        hipLaunchKernelGGL(k_init_flow<double>, dim3(blocks), dim3(256), 0, C->stream, d, rho, ux,
+                           uy, amp, (unsigned long long)seed);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(C->stream));
     return 0;
 }
 

@@ -94,6 +94,7 @@ struct LaunchTable {
     int n_inst, n_ibm, n_fast, n_tiles;
     int parity;              // source buffer index of this step
     int diag;                // take diagnostic reductions this step
+    int rows2;               // bulk wavefronts keep two rows in flight (else one)
 };
 
 // ---------------------------------------------------------------- storage traits

@@ -85,12 +85,14 @@ def grid_of(cfg):
     return nx, ny, dx, dy
 
 
-def initialize(cfg, perturb=None) -> HostState:
+def initialize(cfg, perturb=None, fields: bool = True) -> HostState:
     """Initial state of ``iblbm/init.py:116-265``.
 
     ``perturb=(seed, amp)`` multiplies the initial velocity by (1 + amp*xi), xi ~ U(-1, 1)
     i.i.d. per node and component from ``numpy.random.default_rng(seed)``, before the
     equilibrium is formed (the seeded-perturbation inputs of the Re=100 and batched cases).
+    ``fields=False`` leaves out the full-grid U0 / feq0 arrays (set to None), for instances
+    whose initial state is formed on the device.
     """
     if cfg.scenario_type == "taylor_green":
         raise ValueError("taylor_green is outside the GPU hot path of this package")
@@ -104,15 +106,17 @@ def initialize(cfg, perturb=None) -> HostState:
     lattice_r = lattice_D / 2.0
     lattice_nu = cfg.lattice_u * 2.0 * lattice_r / cfg.Re
     tau = 3.0 * lattice_nu + 0.5
-    U = np.zeros((N, 2))
-    if cfg.motion_type is None:
-        U[:, 0] = cfg.inflow_u
-    ro = 1.0 * np.ones(N)
-    if perturb is not None:
-        seed, amp = perturb
-        xi = np.random.default_rng(seed).uniform(-1.0, 1.0, size=U.shape)
-        U = U * (1.0 + amp * xi)
-    feq0 = feq_host(ro, U, cfg.incompressible_lbgk)
+    U = feq0 = None
+    if fields:
+        U = np.zeros((N, 2))
+        if cfg.motion_type is None:
+            U[:, 0] = cfg.inflow_u
+        ro = 1.0 * np.ones(N)
+        if perturb is not None:
+            seed, amp = perturb
+            xi = np.random.default_rng(seed).uniform(-1.0, 1.0, size=U.shape)
+            U = U * (1.0 + amp * xi)
+        feq0 = feq_host(ro, U, cfg.incompressible_lbgk)
     cx, cy = cfg.cylinder_center
     if cx - r < 0 or cx + r > cfg.xmax or cy - r < 0 or cy + r > cfg.ymax:
         raise ValueError(f"cylinder (cx={cx}, cy={cy}, r={r}) outside [0,{cfg.xmax}]x[0,{cfg.ymax}]")

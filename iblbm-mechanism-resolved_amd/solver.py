@@ -112,6 +112,34 @@ class Context:
         check(self.lib.iblbm_step_mode(self.h, ctypes.byref(m)), "step_mode")
         return ("fused", "two_streams", "serial")[m.value]
 
+    def schedule(self) -> dict:
+        """The launch schedule (``iblbm_get_schedule``): mode and bulk work distribution."""
+        sc = _abi.Schedule()
+        check(self.lib.iblbm_get_schedule(self.h, ctypes.byref(sc)), "get_schedule")
+        mode = {v: k for k, v in _abi.MODES.items()}[sc.mode]
+        return {"mode": mode, "bulk_threads": sc.bulk_threads, "bulk_rows": sc.bulk_rows,
+                "bulk_waves": sc.bulk_waves, "step_rows": sc.step_rows}
+
+    def set_schedule(self, mode="auto", bulk_threads=256, bulk_rows=1, bulk_waves=0,
+                     step_rows=2):
+        sc = _abi.Schedule(_abi.MODES[mode], bulk_threads, bulk_rows, bulk_waves, step_rows)
+        check(self.lib.iblbm_set_schedule(self.h, ctypes.byref(sc)), "set_schedule")
+
+    def autotune(self, steps: int = 20) -> dict:
+        """Times the candidate schedules on the current state (which is restored) and keeps
+        the fastest; returns it with its time per step."""
+        ms, n = ctypes.c_double(), ctypes.c_int()
+        check(self.lib.iblbm_autotune(self.h, int(steps), ctypes.byref(ms), ctypes.byref(n)),
+              "autotune")
+        out = self.schedule()
+        out.update(ms_per_step=ms.value, candidates=n.value)
+        return out
+
+    def init_flow(self, k, rho, ux, uy, amp=0.0, seed=0):
+        """Seeded synthetic equilibrium state on the device (``iblbm_init_flow``)."""
+        check(self.lib.iblbm_init_flow(self.h, k, float(rho), float(ux), float(uy), float(amp),
+                                       int(seed) & 0xFFFFFFFFFFFFFFFF), "init_flow")
+
     def set_concurrent(self, on: bool):
         check(self.lib.iblbm_set_concurrent(self.h, int(bool(on))), "set_concurrent")
 
@@ -189,14 +217,19 @@ class FixedCylinder:
     (interpolation, forces, MDF residual, reductions) in FP64.
     """
 
-    def __init__(self, cfg, perturb=None, device=0, ctx=None, precision="fp64"):
+    def __init__(self, cfg, perturb=None, device=0, ctx=None, precision="fp64", init="host"):
         if cfg.motion_type is not None:
             raise ValueError("FixedCylinder runs motion_type=None; use the sedimentation driver")
         if precision not in PRECISIONS:
             raise KeyError(f"precision={precision!r} (expected one of {sorted(PRECISIONS)})")
+        if init not in ("host", "device"):
+            raise KeyError(f"init={init!r} (expected 'host' or 'device')")
         self.cfg = cfg
         self.precision = precision
-        self.st = initialize(cfg, perturb)
+        # "host": the reference's initial state (numpy perturbation, pre-loop collision);
+        # "device": the same equilibrium formed on the device from a hash-seeded perturbation,
+        # for ensembles too large to set up on the host
+        self.st = initialize(cfg, perturb if init == "host" else None, fields=init == "host")
         self.ctx = ctx or Context(device)
         case = make_case(cfg, self.st, PRECISIONS[precision])
         self.omega = None
@@ -209,7 +242,11 @@ class FixedCylinder:
                              self.st.Larea)
         cx, cy = cfg.cylinder_center
         self.ctx.set_torque_center(self.k, cx, cy)
-        self.ctx.upload_f(self.k, pre_collide(cfg, self.st))
+        if init == "host":
+            self.ctx.upload_f(self.k, pre_collide(cfg, self.st))
+        else:
+            seed, amp = perturb if perturb is not None else (0, 0.0)
+            self.ctx.init_flow(self.k, 1.0, cfg.inflow_u, 0.0, amp, seed)
         self.ttt = 0
 
     def advance(self, n, diag_every=0, graph=False):

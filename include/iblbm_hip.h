@@ -138,10 +138,41 @@ int iblbm_graph_run(void *ctx, int nsteps, int diag_every);
 int iblbm_set_concurrent(void *ctx, int on);
 /* How the next steps will be launched: IBLBM_MODE_FUSED (one launch per step),
  * IBLBM_MODE_TWO_STREAMS or IBLBM_MODE_SERIAL. */
+#define IBLBM_MODE_AUTO (-1)
 #define IBLBM_MODE_FUSED 0
 #define IBLBM_MODE_TWO_STREAMS 1
 #define IBLBM_MODE_SERIAL 2
 int iblbm_step_mode(void *ctx, int *mode);
+
+/* --- launch schedule and autotuner ---
+ * No counterpart in the reference (its kernels use fixed 256-thread blocks,
+ * iblbm/gpu_kernels.py:733-736).  A schedule only changes how the same per-node arithmetic
+ * is distributed over workgroups, so every schedule gives bit-identical results. */
+typedef struct {
+    int32_t mode;         /* IBLBM_MODE_AUTO (fused where eligible) or an explicit mode */
+    int32_t bulk_threads; /* bulk workgroup size of the two-kernel modes: 64..1024, x64;
+                           * a workgroup covers a 64 x (bulk_threads / 64) tile of rows */
+    int32_t bulk_rows;    /* rows a bulk wavefront keeps in flight: 1 or 2 */
+    int32_t bulk_waves;   /* 0: one row per wavefront (grid covers every row); > 0: that
+                           * many wavefronts walk the rows with a grid stride */
+    int32_t step_rows;    /* rows in flight of a bulk wavefront in the fused launch: 1 or 2 */
+} iblbm_schedule;
+int iblbm_get_schedule(void *ctx, iblbm_schedule *out);
+int iblbm_set_schedule(void *ctx, const iblbm_schedule *in);
+/* Times every candidate schedule for `steps` steps (after 2 warm-up steps) on the context's
+ * current state, restores the populations and the step counter after each trial, and keeps
+ * the fastest.  best_ms (may be NULL) receives its time per step; n_tried the number of
+ * candidates.  Refused while a learned-predictor mode is on (trial steps would record
+ * samples).  The last step's diagnostics and MDF statistics afterwards are a trial's. */
+int iblbm_autotune(void *ctx, int steps, double *best_ms, int *n_tried);
+
+/* Device-side initial state for large ensembles: f = f_eq(rho, u) with
+ * u = (ux (1 + amp xi0), uy (1 + amp xi1)), xi uniform in (-1, 1) per node and component from
+ * a counter-based hash of (seed, node, component) -- a seeded synthetic field, not numpy's
+ * stream (iblbm/init.py:116-265 forms the same equilibrium on the host).  Both population
+ * buffers are written. */
+int iblbm_init_flow(void *ctx, int id, double rho, double ux, double uy, double amp,
+                    uint64_t seed);
 
 int iblbm_get_mdf_stats(void *ctx, int id, iblbm_mdf_stats *out);
 int iblbm_get_diag(void *ctx, int id, iblbm_diag *out);
