@@ -19,6 +19,8 @@ import subprocess
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -40,6 +42,13 @@ def workload_cfg(max_steps=200_000):
 
 
 PERTURB = (1234, 1e-3)
+
+# HBM bytes per k_step launch of the FP64 fused step on this workload, from rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE (kB per dispatch), each scaled by what the counter reads for a
+# 1 GiB device copy (FETCH_SIZE reads 0.5x, WRITE_SIZE 1.0x of the bytes moved); the summary
+# is profiles/k_step_fp64_trt_mdf_p4_1601x641_traffic.md.
+TRAFFIC_FP64_C2 = {"nodes": 1026241, "bytes": (2 * 38470.41 + 72435.52) * 1024,
+                   "source": "profiles/k_step_fp64_trt_mdf_p4_1601x641_traffic.md"}
 
 
 def cpu_info():
@@ -87,12 +96,22 @@ def main():
     ap.add_argument("--serial", action="store_true",
                     help="run the window and bulk kernels in order on one stream")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--surrogate-steps", type=int, default=2000,
+                    help="steps of the plain-vs-learned-predictor MDF comparison (0: skip)")
+    ap.add_argument("--variant-steps", type=int, default=2000,
+                    help="steps of the FP32-deviation comparison of an fp64 run (0: skip)")
+    ap.add_argument("--workload", choices=["cylinder", "matrix72"], default="cylinder",
+                    help="cylinder: Re=100 1601x641 (default); matrix72: the batched 72-case "
+                         "Re x method x kernel x collision matrix (about 300 M nodes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the lattice fits one MI355X and this build has no "
               "multi-GPU path (see DESIGN.md for how it would shard)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "matrix72":
+        print(json.dumps(matrix_bench(args)), flush=True)
+        return
     from iblbm_amd.solver import FixedCylinder
     cfg = workload_cfg()
     ci = cfg.check_interval
@@ -147,9 +166,149 @@ def main():
                      "kernel": KERNEL_OF_MODE[mode], "bytes_per_node": bpn,
                      "avg_launch_us": round(kernel_s * 1e6, 3)},
     }
+    if N == TRAFFIC_FP64_C2["nodes"] and args.precision == "fp64" and mode == "fused":
+        out["roofline"]["traffic"] = TRAFFIC_FP64_C2["bytes"]
+        out["roofline"]["traffic_source"] = TRAFFIC_FP64_C2["source"]
+    if args.surrogate_steps > 0:
+        out["mdf_step"] = surrogate_leg(sim, args.surrogate_steps, graph)
+    if args.precision == "fp64" and args.variant_steps > 0:
+        out["fp32dev"] = fp32dev_leg(cfg, args.variant_steps, warm, graph)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def matrix_bench(args):
+    """The 72-case matrix in one context: FP32-deviation storage, device-formed seeded initial
+    states, launch schedule from the autotuner.  Also timed: the same matrix batched in FP64,
+    and the plain FP64 path case by case (one context per case, default schedule), whose
+    aggregate is the baseline of the batched form."""
+    from iblbm_amd.ensemble import Ensemble, case_matrix
+    from iblbm_amd.solver import FixedCylinder
+    cases = case_matrix()
+    steps = max(10, args.steps // 40)
+    warm = 20
+    res = {}
+    for prec in ("fp32dev", "fp64"):
+        ens = Ensemble(cases, precision=prec, init="device")
+        ens.advance(warm, graph=True)
+        tuned = ens.autotune(10)
+        us = _timed(ens, steps, True, chunk=10)
+        cdcl = ens.cd_cl()
+        res[prec] = {"MLUPS": round(ens.nodes / us, 2), "ms_per_step": round(us / 1e3, 4),
+                     "schedule": tuned, "cd_finite": bool(np.isfinite(cdcl).all()),
+                     "nodes": ens.nodes}
+        ens.close()
+        if args.variant_steps <= 0:
+            break
+    seq_nodes, seq_time = 0, 0.0
+    if args.variant_steps > 0:
+        for k, cfg in enumerate(cases):  # the plain FP64 path, one case at a time
+            sim = FixedCylinder(cfg, perturb=(1000 + k, 1e-3), precision="fp64", init="device")
+            sim.advance(warm, graph=True)
+            n = max(10, steps // 2)
+            us = _timed(sim, n, True, chunk=10)
+            seq_nodes += sim.st.nodenums
+            seq_time += us / 1e6
+            sim.ctx.close()
+    b = res["fp32dev"]
+    step_s = b["ms_per_step"] / 1e3
+    achieved = BYTES_PER_NODE["fp32dev"] * b["nodes"] / step_s / 1e9
+    out = {
+        "metric": "MLUPS (fp32dev batched 72-case fixed-cylinder matrix, one launch schedule)",
+        "value": b["MLUPS"], "unit": "MLUPS", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": b["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (deviation storage; FP64 IBM)",
+        "data": "synthetic (device-formed equilibria, hash-seeded 1e-3 velocity perturbation "
+                "per case, seeds 1000..1071)",
+        "config": {"workload": "72 fixed cylinders: Re {20,40,100,200} x {DF,MDF,DFC} x "
+                               "{hat,P4} x {BGK,TRT,CM-MRT}; Ny 641/961/1281/1921 by Re, "
+                               "Nx = 2.5 (Ny - 1) + 1, D = (Ny - 1) / 16, U = 0.04",
+                   "nodes": b["nodes"], "schedule": b["schedule"], "cd_finite": b["cd_finite"]},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_ibm || k_bulk, bytes over the whole step time",
+                     "bytes_per_node": BYTES_PER_NODE["fp32dev"]},
+    }
+    if "fp64" in res:
+        out["fp64_batched"] = res["fp64"]
+    if seq_time > 0:
+        plain = seq_nodes / seq_time / 1e6
+        out["fp64_plain_case_by_case"] = {"MLUPS": round(plain, 2)}
+        out["speedup_vs_fp64_plain"] = round(b["MLUPS"] / plain, 3)
+    return out
+
+
+def _timed(sim, n, graph, chunk=50):
+    """Time per step (us) of n steps replayed in chunks whose graph is captured first
+    (chunk + n + 2 steps run in all); the diagnostics are taken on the last step, after the
+    timed region."""
+    sim.advance(chunk, graph=graph)
+    sim.ctx.sync()
+    reps = max(1, n // chunk)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sim.advance(chunk, graph=graph)
+    sim.ctx.sync()
+    us = (time.perf_counter() - t0) / (reps * chunk) * 1e6
+    sim.advance(2, diag_every=sim.ctx.step_counter() + 2)
+    return us
+
+
+def surrogate_leg(sim, n, graph):
+    """MDF step time, plain against the learned force predictor, from one state.
+
+    The predictor is trained on the device on 2000 plain-MDF steps of this run; then both
+    forms run n steps from the same populations and previous forces.  Reported: time per
+    step, true MDF sub-steps per step, and the relative Cd difference at the last step."""
+    pred = sim.predictor
+    pred.mode("collect", capacity=1 << 19)
+    sim.advance(2000, graph=graph)
+    losses = pred.train(steps=3000, lr=2e-3, seed=11)
+    f0 = sim.ctx.download_f(sim.k)
+    F0 = pred.previous_force(sim.st.Lb)
+    res = {}
+    for mode in ("off", "learned"):
+        sim.ctx.upload_f(sim.k, f0)
+        pred.mode(mode)
+        if mode != "off":
+            pred.set_previous_force(F0)
+            pred.stats(reset=True)
+        us = _timed(sim, n, graph)  # Cd of both forms after the same steps from f0
+        cd = sim.cd_cl()[0]
+        sub = (pred.stats()["substeps_per_step"] if mode != "off"
+               else float(sim.ctx.mdf_stats(sim.k)["iter_count"]))
+        res[mode] = (us, cd, sub, pred.stats()["restarts"] if mode != "off" else 0)
+    pred.mode("off")
+    (up, cdp, sp, _), (ul, cdl, sl, rl) = res["off"], res["learned"]
+    return {"steps": n, "plain_us": round(up, 3), "surrogate_us": round(ul, 3),
+            "speedup": round(up / ul, 4), "substeps_plain": sp,
+            "substeps_surrogate": round(sl, 3), "restarts": rl,
+            "dCd_rel": abs(cdl - cdp) / abs(cdp), "Cd_plain": cdp, "Cd_surrogate": cdl,
+            "train": {"samples": pred.samples(), "adam_steps": len(losses),
+                      "loss_first100": float(losses[:100].mean()),
+                      "loss_last100": float(losses[-100:].mean())}}
+
+
+def fp32dev_leg(cfg, n, warm, graph):
+    """The same workload with FP32-deviation storage from the same initial state: MLUPS and
+    the Cd / Cl difference against the FP64 run after the same number of steps."""
+    from iblbm_amd.solver import FixedCylinder
+    sim = FixedCylinder(cfg, perturb=PERTURB, precision="fp32dev")
+    sim.advance(warm, graph=graph)
+    us = _timed(sim, n, graph)
+    cd32, cl32 = sim.cd_cl()
+    t = sim.ttt
+    ref_sim = FixedCylinder(cfg, perturb=PERTURB, precision="fp64")
+    ref_sim.advance(t, diag_every=t, graph=graph)
+    cd64, cl64 = ref_sim.cd_cl()
+    ref_sim.ctx.close()
+    out = {"MLUPS": round(sim.st.nodenums / us, 2), "ms_per_step": round(us / 1e3, 5),
+           "launch": sim.ctx.step_mode(), "steps": n, "compared_at_step": t,
+           "dCd_rel_vs_fp64": abs(cd32 - cd64) / abs(cd64),
+           "dCl_vs_fp64_over_Cd": abs(cl32 - cl64) / abs(cd64)}
+    sim.ctx.close()
+    return out
 
 
 if __name__ == "__main__":

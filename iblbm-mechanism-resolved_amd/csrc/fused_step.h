@@ -25,7 +25,9 @@ struct FusedShared {
     double mw[FAST_K * FAST_LB];                // marker stencil weights, [s * Lb + k]
     int mti[FAST_K * FAST_LB];                  // marker stencil touched indices, [s * Lb + k]
     double fs[9 * FAST_T];                      // post-stream populations of the touched nodes
-    MlpTile mlp[FAST_LB / 64];                  // predictor scratch of each marker wavefront
+    alignas(16) unsigned short mlpx[FAST_LB * 16];  // predictor feature rows (bf16)
+    float mlpy[FAST_LB * MLP_OUT];                   // predictor outputs
+    MlpLds mlpw;                                     // predictor weights (learned mode)
 };
 
 // Copies the descriptor fields the window reads into a local descriptor, in one batch at the
@@ -124,6 +126,12 @@ __device__ inline double fused_block_sum(double v, double *red) {
     }
     __syncthreads();
     return r;
+}
+
+// One 16-marker tile of the learned predictor's inference on the calling wavefront.
+__device__ inline void mlp_infer_tile(FusedShared &sh, int tile) {
+    mlp_tile16(sh.mlpw.blk, sh.mlpw.bias, sh.mlpw.bias + MLP_H, sh.mlpw.bias + 2 * MLP_H,
+               sh.mlpx + 256 * tile, sh.mlpy + 16 * MLP_OUT * tile);
 }
 
 // PRED: compiled with the learned-predictor modes (instances whose mode is off run plain MDF
@@ -228,6 +236,14 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
             }
         }
     }
+    if (PRED && I.ibm == 2 && I.pmode == 3) {  // learned predictor: weights and biases to LDS
+        const mlp_short4 *wb = reinterpret_cast<const mlp_short4 *>(S.mlp_wb);
+        for (int e = t; e < MLP_NBLK * 64; e += NTF) sh.mlpw.blk[e] = wb[e];
+        if (t < 2 * MLP_H + MLP_OUT)
+            sh.mlpw.bias[t] =
+                S.mlp_p[t < MLP_H ? MLP_B1 + t
+                                  : (t < 2 * MLP_H ? MLP_B2 + t - MLP_H : MLP_B3 + t - 2 * MLP_H)];
+    }
     if (t < 2) rbits[t] = 0ull;
     if (t == 0) lf[Lb] = make_double2(0.0, 0.0);  // target of the padded gather entries
     __syncthreads();  // P1 done
@@ -263,10 +279,30 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
         float *rowp = nullptr;  // this marker's training sample (collect mode)
         double facx = 0.0, facy = 0.0;
         if (PRED && pmode != 0) {
+            // this marker's global inputs first, in one round trip: previous q and force,
+            // position, normalisation
+            double spx = 0.0, spy = 0.0, pfx = 0.0, pfy = 0.0, mx = 0.0, my = 0.0;
+            if (t < Lb) {
+                if (fprev_ok) {
+                    spx = S.fprev[2 * I.lb_cap + 2 * t];
+                    spy = S.fprev[2 * I.lb_cap + 2 * t + 1];
+                    if (warm) {
+                        pfx = S.fprev[2 * t];
+                        pfy = S.fprev[2 * t + 1];
+                    }
+                }
+                mx = I.Lx[t];
+                my = I.Ly[t];
+            }
+            float nrm[MLP_NORM];
+            if (pmode == 3) {
+#pragma unroll
+                for (int q = 0; q < MLP_NORM; ++q) nrm[q] = S.mlp_norm[q];
+            }
             double ax = 0.0, ay = 0.0, ex = 0.0, ey = 0.0;
             if (t < Lb) {
                 double lx = 0.0, ly = 0.0, rm = 0.0;
-#pragma unroll 4
+#pragma unroll 5
                 for (int s = 0; s < K; ++s) {
                     const int at = sh.mti[s * Lb + t];
                     const double2 u = uw[at];
@@ -278,11 +314,6 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
                 if (finc) rm = 1.0;
                 ax = rm * (udx - lx);
                 ay = rm * (udy - ly);
-                double spx = 0.0, spy = 0.0;
-                if (fprev_ok) {
-                    spx = S.fprev[2 * I.lb_cap + 2 * t];
-                    spy = S.fprev[2 * I.lb_cap + 2 * t + 1];
-                }
                 S.fprev[2 * I.lb_cap + 2 * t] = ax;  // the next step's previous q
                 S.fprev[2 * I.lb_cap + 2 * t + 1] = ay;
                 ex = ax - spx;
@@ -294,15 +325,14 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
             float feat[MLP_IN];
 #pragma unroll
             for (int q = 0; q < MLP_IN; ++q) feat[q] = 0.f;
-            double pfx = 0.0, pfy = 0.0;
             if (t < Lb) {
                 feat[0] = (float)(ex / (double)sig);
                 feat[1] = (float)(ey / (double)sig);
                 feat[2] = a > 0.0 ? (float)(ax / a) : 0.f;
                 feat[3] = a > 0.0 ? (float)(ay / a) : 0.f;
-                const double rx = I.Lx[t] - I.cxd, ry = I.Ly[t] - I.cyd;
+                const double rx = mx - I.cxd, ry = my - I.cyd;
                 const double rr = sqrt(rx * rx + ry * ry);
-                const double gx = I.Lx[t] / I.dx + 0.5, gy = I.Ly[t] / I.dy + 0.5;
+                const double gx = mx / I.dx + 0.5, gy = my / I.dy + 0.5;
                 feat[4] = rr > 0.0 ? (float)(rx / rr) : 0.f;
                 feat[5] = rr > 0.0 ? (float)(ry / rr) : 0.f;
                 feat[6] = (float)(gx - floor(gx) - 0.5);
@@ -313,20 +343,19 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
                     for (int q = 0; q < MLP_IN; ++q) rowp[q] = feat[q];
                     rowp[MLP_IN] = sig;
                 }
-                if (warm && fprev_ok) {
-                    pfx = S.fprev[2 * t];
-                    pfy = S.fprev[2 * t + 1];
-                }
             }
-            if (pmode == 3) {  // every lane of the wavefront takes part
-                float xn[MLP_IN], yo[MLP_OUT];
+            if (pmode == 3) {  // inference: one 16-marker tile per wavefront (barriers X, Y)
+                float xn[MLP_IN];
 #pragma unroll
-                for (int q = 0; q < MLP_IN; ++q)
-                    xn[q] = (feat[q] - S.mlp_norm[q]) / S.mlp_norm[MLP_IN + q];
-                mlp_infer_wave(reinterpret_cast<const mlp_short4 *>(S.mlp_wb), S.mlp_p, xn, yo,
-                               sh.mlp[t >> 6]);
-                pfx += (double)sig * (double)(yo[0] * S.mlp_norm[2 * MLP_IN]);
-                pfy += (double)sig * (double)(yo[1] * S.mlp_norm[2 * MLP_IN + 1]);
+                for (int q = 0; q < MLP_IN; ++q) xn[q] = (feat[q] - nrm[q]) / nrm[MLP_IN + q];
+                mlp_put_row(sh.mlpx + 16 * t, xn);  // rows from Lb on are computed, unused
+                __syncthreads();  // X: feature rows published
+                if ((t >> 6) < ((Lb + 15) >> 4)) mlp_infer_tile(sh, t >> 6);
+                __syncthreads();  // Y: outputs published
+                if (t < Lb) {
+                    pfx += (double)sig * (double)(sh.mlpy[MLP_OUT * t] * nrm[2 * MLP_IN]);
+                    pfy += (double)sig * (double)(sh.mlpy[MLP_OUT * t + 1] * nrm[2 * MLP_IN + 1]);
+                }
             }
             if (warm) {  // published here; read by the node threads after barrier A
                 if (t < Lb) lf[t] = make_double2(pfx, pfy);
@@ -477,6 +506,11 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
                 sy += g[e].y * ww[e];
             }
         };
+        if (PRED && pmode == 3) {  // the node wavefronts take the other inference tiles
+            __syncthreads();  // X
+            if ((t >> 6) < ((Lb + 15) >> 4)) mlp_infer_tile(sh, t >> 6);
+            __syncthreads();  // Y
+        }
         for (int it = 0; it < n_iter; ++seq) {
             __syncthreads();  // A
             if (PRED && warm && seq == 0) {  // the predicted force and the velocity it implies

@@ -1208,14 +1208,15 @@ int iblbm_get_diag(void *p, int id, iblbm_diag *out) {
     HIPCHK(hipSetDevice(C->device));
     HIPCHK(hipStreamSynchronize(C->stream));
     const Inst &d = C->inst[id].d;
-    std::vector<double> part((size_t)C->n_tiles * TILE_Y);
-    HIPCHK(hipMemcpy(part.data(), C->d_rho_partial, sizeof(double) * part.size(),
-                     hipMemcpyDeviceToHost));
+    const int b0 = TILE_Y * C->h_tile_prefix[id], b1 = TILE_Y * C->h_tile_prefix[id + 1];  // rows
+    std::vector<double> part((size_t)(b1 - b0));
+    if (b1 > b0)
+        HIPCHK(hipMemcpy(part.data(), C->d_rho_partial + b0, sizeof(double) * part.size(),
+                         hipMemcpyDeviceToHost));
     double box[5] = {0, 0, 0, 0, 0};
     if (d.ibm != 0) HIPCHK(hipMemcpy(box, d.diag, sizeof box, hipMemcpyDeviceToHost));
     double rs = 0.0;
-    const int b0 = TILE_Y * C->h_tile_prefix[id], b1 = TILE_Y * C->h_tile_prefix[id + 1];  // rows
-    for (int b = b0; b < b1; ++b) rs += part[b];
+    for (double v : part) rs += v;
     rs += box[0];
     out->rho_sum = rs;
     out->force[0] = box[1];

@@ -67,100 +67,70 @@ __device__ inline float mlp_tanh_bf16(float x) {
     return 1.f - 2.f * __builtin_amdgcn_rcpf(e + 1.f);
 }
 
-// Inference of one wavefront's 64 rows (row = lane): x (MLP_IN normalised features of this
-// lane's row) -> out (MLP_OUT).  tile: this wavefront's LDS scratch (16 x 16 + 2 x 16 x 64
-// bf16 and 16 x 2 floats).  Every lane of the wavefront must call it.
-struct MlpTile {
-    unsigned short x[16 * 16];
-    unsigned short h1[16 * MLP_H];
-    unsigned short h2[16 * MLP_H];
-    float y[16 * MLP_OUT];
-};
+// LDS writes of a wavefront made visible to its other lanes.
+__device__ inline void mlp_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
-__device__ inline void mlp_infer_wave(const mlp_short4 *__restrict__ wblk,
-                                      const float *__restrict__ par, const float *x, float *out,
-                                      MlpTile &tile) {
-    // The weight pointers pass through an empty asm here, so the weight loads stay where
-    // inference runs: a caller that infers once inside a loop does not get them hoisted out
-    // of the loop and held in registers across it.
-    asm volatile("" : "+s"(wblk), "+s"(par));
+// Inference of one 16-row tile on one wavefront, as transposed products H^T = W^T X^T.
+// With v_mfma_f32_16x16x16bf16_1k lane l = 16 g + r holds A[r][4 g + j], B[4 g + j][r] and
+// D[4 g + j][r] (j = 0..3).  Taking the weights as the A operand (a packed block holds
+// W[16 kb + 4 g + j][16 nb + r], which is W^T[16 nb + r][16 kb + 4 g + j]) and the activations
+// as the B operand, a layer's output block (hidden 4 g + j of block ib, tile row r) is in
+// exactly the lane and slot the next layer's B operand needs: the hidden activations never
+// leave registers.  x16: 16 rows x 16 bf16 (features, then zeros) in LDS; y16: 16 rows x
+// MLP_OUT floats.  Every lane of the wavefront must call it.
+__device__ inline void mlp_tile16(const mlp_short4 *__restrict__ wblk,
+                                  const float *__restrict__ b1, const float *__restrict__ b2,
+                                  const float *__restrict__ b3, const unsigned short *x16,
+                                  float *y16) {
     const int l = threadIdx.x & 63, r = l % 16, g = l / 16;
-    // The tile and block loops stay rolled: inference runs inside the caller's MDF loop, where
-    // most registers are taken, and costs little next to the sub-steps it saves.
-#pragma unroll 1
-    for (int tsub = 0; tsub < 4; ++tsub) {  // rows 16 tsub .. 16 tsub + 15 of the wavefront
-        // the 16 rows of this tile write their features (lane 16 tsub + r owns row r)
-        if (g == tsub) {
+    const mlp_short4 xb = *reinterpret_cast<const mlp_short4 *>(x16 + r * 16 + 4 * g);
+    mlp_short4 h1[MLP_H / 16], h2[MLP_H / 16];
 #pragma unroll
-            for (int k = 0; k < 16; ++k) tile.x[r * 16 + k] = k < MLP_IN ? f32_to_bf16(x[k]) : 0;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        mlp_short4 a;
+    for (int ib = 0; ib < MLP_H / 16; ++ib) {  // layer 1
+        mlp_float4 c = {0.f, 0.f, 0.f, 0.f};
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wblk[(MLP_BLK1 + ib) * 64 + l], xb, c, 0,
+                                                       0, 0);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) a[j] = (short)tile.x[r * 16 + 4 * g + j];
-#pragma unroll 1
-        for (int nb = 0; nb < MLP_H / 16; ++nb) {  // layer 1
-            mlp_float4 c = {0.f, 0.f, 0.f, 0.f};
-            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, wblk[(MLP_BLK1 + nb) * 64 + l], c,
-                                                           0, 0, 0);
+        for (int j = 0; j < 4; ++j)
+            h1[ib][j] = (short)f32_to_bf16(mlp_tanh_bf16(c[j] + b1[16 * ib + 4 * g + j]));
+    }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int row = 4 * g + j, col = 16 * nb + r;
-                tile.h1[row * MLP_H + col] = f32_to_bf16(mlp_tanh_bf16(c[j] + par[MLP_B1 + col]));
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll 1
-        for (int nb = 0; nb < MLP_H / 16; ++nb) {  // layer 2
-            mlp_float4 c = {0.f, 0.f, 0.f, 0.f};
+    for (int ib = 0; ib < MLP_H / 16; ++ib) {  // layer 2
+        mlp_float4 c = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int kb = 0; kb < MLP_H / 16; ++kb) {
-                mlp_short4 a2;
+        for (int kb = 0; kb < MLP_H / 16; ++kb)
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wblk[(MLP_BLK2 + kb * 4 + ib) * 64 + l],
+                                                           h1[kb], c, 0, 0, 0);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) a2[j] = (short)tile.h1[r * MLP_H + 16 * kb + 4 * g + j];
-                c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
-                    a2, wblk[(MLP_BLK2 + kb * 4 + nb) * 64 + l], c, 0, 0, 0);
-            }
+        for (int j = 0; j < 4; ++j)
+            h2[ib][j] = (short)f32_to_bf16(mlp_tanh_bf16(c[j] + b2[16 * ib + 4 * g + j]));
+    }
+    mlp_float4 c = {0.f, 0.f, 0.f, 0.f};  // layer 3: outputs 0, 1 in lanes g = 0
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int row = 4 * g + j, col = 16 * nb + r;
-                tile.h2[row * MLP_H + col] = f32_to_bf16(mlp_tanh_bf16(c[j] + par[MLP_B2 + col]));
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        {  // layer 3
-            mlp_float4 c = {0.f, 0.f, 0.f, 0.f};
+    for (int kb = 0; kb < MLP_H / 16; ++kb)
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wblk[(MLP_BLK3 + kb) * 64 + l], h2[kb], c,
+                                                       0, 0, 0);
+    if (g == 0) {
 #pragma unroll
-            for (int kb = 0; kb < MLP_H / 16; ++kb) {
-                mlp_short4 a3;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) a3[j] = (short)tile.h2[r * MLP_H + 16 * kb + 4 * g + j];
-                c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3, wblk[(MLP_BLK3 + kb) * 64 + l],
-                                                               c, 0, 0, 0);
-            }
-            if (r < MLP_OUT) {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) tile.y[(4 * g + j) * MLP_OUT + r] = c[j] + par[MLP_B3 + r];
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (g == tsub) {
-#pragma unroll
-            for (int o = 0; o < MLP_OUT; ++o) out[o] = tile.y[r * MLP_OUT + o];
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int o = 0; o < MLP_OUT; ++o) y16[r * MLP_OUT + o] = c[o] + b3[o];
     }
 }
+
+// bf16 feature row of the tiles: MLP_IN normalised features, then zeros up to 16.
+__device__ inline void mlp_put_row(unsigned short *row16, const float *xn) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) row16[k] = k < MLP_IN ? f32_to_bf16(xn[k]) : (unsigned short)0;
+}
+
+// LDS copy of the inference weights: the BF16 blocks and the three biases.
+struct MlpLds {
+    mlp_short4 blk[MLP_NBLK * 64];
+    float bias[2 * MLP_H + MLP_OUT];  // b1, b2, b3
+};
 
 // ------------------------------------------------------------------ FP32 training
 
@@ -411,20 +381,26 @@ static __global__ void k_mlp_pack(const float *__restrict__ p, unsigned short *_
         mlp_pack_block(p + MLP_W3, MLP_H, MLP_OUT, b - MLP_BLK3, 0, wb + 256 * b);
 }
 
-// BF16 MFMA inference of n rows of normalised features (64 rows per workgroup).
+// BF16 MFMA inference of n rows of normalised features (64 rows per workgroup, four tiles).
 static __global__ __launch_bounds__(64) void k_mlp_infer(const unsigned short *__restrict__ wb,
                                                  const float *__restrict__ p,
                                                  const float *__restrict__ x, int n,
                                                  float *__restrict__ out) {
-    __shared__ MlpTile tile;
+    __shared__ alignas(16) unsigned short xs[64 * 16];
+    __shared__ float ys[64 * MLP_OUT];
     const int row = blockIdx.x * 64 + threadIdx.x;
-    float xi[MLP_IN], yo[MLP_OUT];
+    float xi[MLP_IN];
 #pragma unroll
     for (int q = 0; q < MLP_IN; ++q) xi[q] = row < n ? x[row * MLP_IN + q] : 0.f;
-    mlp_infer_wave(reinterpret_cast<const mlp_short4 *>(wb), p, xi, yo, tile);
+    mlp_put_row(xs + threadIdx.x * 16, xi);
+    mlp_wave_sync();
+    for (int tile = 0; tile < 4; ++tile)
+        mlp_tile16(reinterpret_cast<const mlp_short4 *>(wb), p + MLP_B1, p + MLP_B2, p + MLP_B3,
+                   xs + 256 * tile, ys + 16 * MLP_OUT * tile);
+    mlp_wave_sync();
     if (row < n) {
 #pragma unroll
-        for (int o = 0; o < MLP_OUT; ++o) out[row * MLP_OUT + o] = yo[o];
+        for (int o = 0; o < MLP_OUT; ++o) out[row * MLP_OUT + o] = ys[threadIdx.x * MLP_OUT + o];
     }
 }
 

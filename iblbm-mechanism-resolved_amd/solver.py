@@ -106,6 +106,10 @@ class Context:
     def sync(self):
         check(self.lib.iblbm_sync(self.h), "sync")
 
+    def step_counter(self) -> int:
+        """Steps run by the context so far (diagnostics fire on multiples of diag_every)."""
+        return int(self.lib.iblbm_step_counter(self.h))
+
     def step_mode(self) -> str:
         """"fused" (one launch per step), "two_streams" or "serial"."""
         m = ctypes.c_int()
