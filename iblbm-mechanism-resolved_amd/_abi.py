@@ -51,8 +51,8 @@ class Schedule(ctypes.Structure):
 
 class MdfStats(ctypes.Structure):
     _fields_ = [("iter_count", ctypes.c_int32), ("diverged", ctypes.c_int32),
-                ("converged", ctypes.c_int32), ("reserved", ctypes.c_int32),
-                ("last_residual", ctypes.c_double)]
+                ("converged", ctypes.c_int32), ("guard_trips", ctypes.c_int32),
+                ("last_residual", ctypes.c_double), ("omega_used", ctypes.c_double)]
 
 
 class Diag(ctypes.Structure):
@@ -78,6 +78,9 @@ SIGNATURES = {
     "iblbm_download_f": (_i, [_VP, _i, _D]),
     "iblbm_set_markers": (_i, [_VP, _i, _D, _D, _D, _i, _d]),
     "iblbm_set_torque_center": (_i, [_VP, _i, _d, _d]),
+    "iblbm_set_particles": (_i, [_VP, _i, _i, _I, _D]),
+    "iblbm_download_particle_fib": (_i, [_VP, _i, _i, _D, _D, _I]),
+    "iblbm_download_marker_force": (_i, [_VP, _i, _D]),
     "iblbm_marker_box": (_i, [_D, _D, _i, _d, _d, _i, _i, _i, _I]),
     "iblbm_step": (_i, [_VP, _i, _i]),
     "iblbm_sync": (_i, [_VP]),

@@ -359,6 +359,7 @@ struct HostInst {
     Inst d;  // shadow of the device descriptor
     int Lb = 0;
     double Larea = 0.0;
+    int pfb_np = 0;  // particles the per-particle window fields are sized for
 };
 
 struct Ctx {
@@ -695,6 +696,7 @@ int iblbm_instance_add(void *p, const iblbm_case *c, int *id) {
     d.n_iter = c->mdf_iterations;
     d.min_iter = c->mdf_min_iterations;
     d.fixed_body = c->fixed_body;
+    d.np_part = 1;
     void *ptr;
     const size_t fbytes = (d.prec == 1 ? sizeof(float) : sizeof(double)) * 9 * (size_t)d.qs;
     for (int k = 0; k < 2; ++k) {
@@ -770,6 +772,8 @@ int iblbm_set_markers(void *p, int id, const double *Lx, const double *Ly, const
     if (bx0 < 0 || by0 < 0 || bx0 + bw > d.nx || by0 + bh > d.ny)
         return fail(IBLBM_EINVAL, "set_markers: window box outside the grid");
     if (!(Larea > 0.0)) return fail(IBLBM_EINVAL, "set_markers: Larea must be positive");
+    if (d.np_part > 1 && d.poff[d.np_part] != Lb)
+        return fail(IBLBM_EINVAL, "set_markers: Lb differs from the particles' marker total");
     HIPCHK(hipSetDevice(C->device));
     HIPCHK(hipMemcpyAsync((void *)d.Lx, Lx, sizeof(double) * Lb, hipMemcpyHostToDevice, C->stream));
     HIPCHK(hipMemcpyAsync((void *)d.Ly, Ly, sizeof(double) * Lb, hipMemcpyHostToDevice, C->stream));
@@ -808,6 +812,77 @@ int iblbm_set_markers(void *p, int id, const double *Lx, const double *Ly, const
         int rc = ensure_tables(*C);
         if (rc) return rc;
     }
+    return 0;
+}
+
+int iblbm_set_particles(void *p, int id, int np, const int *lb, const double *larea) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id)) return fail(IBLBM_EINVAL, "set_particles: bad instance id");
+    if (np < 1 || np > IBM_MAXP || !lb || !larea)
+        return fail(IBLBM_EINVAL, "set_particles: need 1 <= np <= 8 and both arrays");
+    HostInst &h = C->inst[id];
+    Inst &d = h.d;
+    int off = 0;
+    for (int q = 0; q < np; ++q) {
+        if (lb[q] < 1 || !(larea[q] > 0.0))
+            return fail(IBLBM_EINVAL, "set_particles: markers and Larea must be positive");
+        off += lb[q];
+    }
+    if (off > d.lb_cap) return fail(IBLBM_EINVAL, "set_particles: markers exceed lb_capacity");
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    if (np > 1 && (h.c.ibm == IBLBM_IBM_DF || h.c.ibm == IBLBM_IBM_MDF) && np > h.pfb_np) {
+        void *q = nullptr;
+        const size_t bytes = sizeof(double) * 2 * (size_t)np * d.box_cap;
+        if (C->mem.alloc(&q, bytes) != hipSuccess)
+            return fail(IBLBM_ENOMEM, "set_particles: per-particle window fields do not fit");
+        HIPCHK(hipMemsetAsync(q, 0, bytes, C->stream));
+        d.s.pfb = (double *)q;
+        h.pfb_np = np;
+    }
+    d.np_part = np;
+    d.poff[0] = 0;
+    for (int q = 0; q < np; ++q) {
+        d.poff[q + 1] = d.poff[q] + lb[q];
+        d.plarea[q] = larea[q];
+    }
+    C->tables_dirty = true;
+    return ensure_tables(*C);
+}
+
+int iblbm_download_particle_fib(void *p, int id, int part, double *fibx, double *fiby,
+                                int box[4]) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || !fibx || !fiby || !box)
+        return fail(IBLBM_EINVAL, "download_particle_fib: bad argument");
+    const HostInst &h = C->inst[id];
+    const Inst &d = h.d;
+    if (d.ibm != IBLBM_IBM_MDF || d.np_part < 2 || part < 0 || part >= d.np_part || !d.s.pfb)
+        return fail(IBLBM_EINVAL, "download_particle_fib: needs a coupled MDF window and a "
+                                  "particle index below np");
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    const int nb = d.bw * d.bh;
+    HIPCHK(hipMemcpy(fibx, d.s.pfb + (size_t)(2 * part) * d.box_cap, sizeof(double) * nb,
+                     hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(fiby, d.s.pfb + (size_t)(2 * part + 1) * d.box_cap, sizeof(double) * nb,
+                     hipMemcpyDeviceToHost));
+    box[0] = d.bx0;
+    box[1] = d.by0;
+    box[2] = d.bw;
+    box[3] = d.bh;
+    return 0;
+}
+
+int iblbm_download_marker_force(void *p, int id, double *F) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || !F) return fail(IBLBM_EINVAL, "download_marker_force: bad argument");
+    const Inst &d = C->inst[id].d;
+    if (d.ibm != IBLBM_IBM_DF || d.np_part < 2)
+        return fail(IBLBM_EINVAL, "download_marker_force: needs a DF window of several particles");
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    HIPCHK(hipMemcpy(F, d.s.mf, sizeof(double) * 2 * d.Lb, hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -1192,13 +1267,15 @@ int iblbm_get_mdf_stats(void *p, int id, iblbm_mdf_stats *out) {
     if (!valid_id(C, id) || !out) return fail(IBLBM_EINVAL, "mdf_stats: bad argument");
     HIPCHK(hipSetDevice(C->device));
     HIPCHK(hipStreamSynchronize(C->stream));
-    double s[4];
+    double s[6];
     HIPCHK(hipMemcpy(s, C->inst[id].d.stats, sizeof s, hipMemcpyDeviceToHost));
     out->iter_count = (int32_t)s[0];
     out->last_residual = s[1];
     out->diverged = (int32_t)s[2];
     out->converged = (int32_t)s[3];
-    out->reserved = 0;
+    const bool coupled = C->inst[id].d.np_part > 1;
+    out->guard_trips = coupled ? (int32_t)s[4] : 0;
+    out->omega_used = coupled ? s[5] : C->inst[id].d.omega;
     return 0;
 }
 

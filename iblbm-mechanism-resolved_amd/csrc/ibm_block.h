@@ -131,6 +131,14 @@ __device__ inline void block_scan_int(const int *in, int *out, int n, int *redi)
 
 // Contribution tables (marker-major c_m = k*K + s), touched-node compaction and per-node
 // gather lists in np.add.at order.
+// Particle of marker k in a window of several particles (markers in particle order).
+__device__ inline int particle_of(const Inst &I, int k) {
+    int p = 0;
+#pragma unroll
+    for (int q = 1; q < IBM_MAXP; ++q) p += (q < I.np_part && k >= I.poff[q]) ? 1 : 0;
+    return p;
+}
+
 template <int A>
 __device__ void ibm_build_tables(Inst &I, int *redi) {
     constexpr int side = 2 * A + 1, K = side * side;
@@ -186,13 +194,18 @@ __device__ void ibm_build_tables(Inst &I, int *redi) {
     __threadfence_block();
     __syncthreads();
     block_scan_int(S.toff, S.toff, nT, redi);
+    // np.add.at order key s * Lb + k; a DF / MDF window of several particles spreads each
+    // particle on its own (multi_particle.py:139-160, :400-421), so its key leads with the
+    // particle: (p * K + s) * Lb + k orders by particle, then as one particle's np.add.at
+    const bool by_part = (I.ibm == 1 || I.ibm == 2) && I.np_part > 1;
     for (int cm = t; cm < nc; cm += NTI) {
         const int sl = S.cslot[cm];
         const int ti = S.tindex[sl];
         S.cti[cm] = ti;
         const int k = cm / K, s = cm - k * K;
+        const int p = by_part ? particle_of(I, k) : 0;
         const int pos = atomicAdd(&S.cur[sl], 1);
-        S.gc[S.toff[ti] + pos] = s * Lb + k;  // np.add.at order key
+        S.gc[S.toff[ti] + pos] = (p * K + s) * Lb + k;
     }
     __threadfence_block();
     __syncthreads();
@@ -209,9 +222,9 @@ __device__ void ibm_build_tables(Inst &I, int *redi) {
             L[y + 1] = v;
         }
         for (int x = 0; x < n; ++x) {  // (marker, w*Larea) per entry
-            const int c = L[x], s = c / Lb, k = c - s * Lb;
+            const int c = L[x], s = (c / Lb) % K, k = c % Lb;
             S.gk[S.toff[ti] + x] = k;
-            S.gw[S.toff[ti] + x] = S.cw[k * K + s] * I.Larea;
+            S.gw[S.toff[ti] + x] = S.cw[k * K + s] * (by_part ? I.plarea[particle_of(I, k)] : I.Larea);
         }
     }
     __threadfence_block();
@@ -365,6 +378,176 @@ __device__ inline void ibm_touched_init(const Inst &I, int nT, const IbmPtrs &P)
 // (DF/MDF), or S.fs of the touched slots (DFC).  DF/MDF leave fib of the touched nodes in
 // P.fbx/P.fby; DFC corrects S.fs and recomputes S.ro, S.ux0, S.uy0 of the touched slots and
 // leaves the marker forces in S.q.
+// DF / MDF of a window holding several particles (iblbm/ibm/multi_particle.py).
+//  DF (multi_particle_df, :347-421): each particle's closure from the same velocity; the
+//    window fib is the particles' spreads added in particle order; the marker forces of the
+//    pass are kept (the host takes each particle's force and torque from them,
+//    forcing.py:263-275).
+//  MDF (multi_particle_mdf_coupled, :62-316): the particles share the working velocity;
+//    omega starts at the host's min(0.8 omega_est, 0.8); an iteration whose residual exceeds
+//    1.2 x the previous one is dropped and halves omega (floor omega / 8, reaching it ends the
+//    loop as diverged); each particle's accumulated field is kept for its force and torque.
+template <int A>
+__device__ void ibm_core_multi(Inst &I, int nT, const IbmPtrs &P, double *red,
+                               const MarkerRegs<A> &M) {
+    constexpr int K = (2 * A + 1) * (2 * A + 1);
+    const int t = threadIdx.x, Lb = I.Lb, bc = I.box_cap;
+    IbmScratch &S = I.s;
+    const bool inc = I.inc != 0;
+    const double dt = I.cp.dt;
+    const bool mdf = I.ibm == 2;
+    const int npart = I.np_part;
+    for (int ti = t; ti < nT; ti += NTI) {
+        P.fbx[ti] = 0.0;
+        P.fby[ti] = 0.0;
+        if (mdf) {
+            const int sl = S.tslot[ti];
+            for (int p = 0; p < npart; ++p) {
+                S.pfb[(2 * p) * bc + sl] = 0.0;
+                S.pfb[(2 * p + 1) * bc + sl] = 0.0;
+            }
+        }
+    }
+    const bool regs = Lb <= NTI;
+    double R = 0.0;
+    if (regs && t < Lb) {
+#pragma unroll
+        for (int s = 0; s < K; ++s) R += P.rot[M.ti[s]] * M.w[s];
+    }
+    if (!regs) {
+        for (int k = t; k < Lb; k += NTI) {
+            double r = 0.0;
+#pragma unroll
+            for (int s = 0; s < K; ++s) r += P.rot[S.cti[k * K + s]] * S.cw[k * K + s];
+            S.R[k] = r;
+        }
+    }
+    __threadfence_block();
+    __syncthreads();
+    const int n_iter = mdf ? I.n_iter : 1;
+    const int min_iter = max(1, min(I.min_iter, n_iter));
+    const double tol = I.tol;
+    double omega = I.omega;
+    const double omega_min = omega * 0.125;
+    double prev = INFINITY, last = INFINITY;
+    int iter_count = 0, diverged = 0, converged = 0, trips = 0;
+    for (int it = 0; it < n_iter; ++it) {
+        double rmax = -INFINITY;
+        for (int k = t; k < Lb; k += NTI) {  // interpolation + force (df.py:239-263)
+            double lx = 0.0, ly = 0.0, Rk, udx, udy;
+            if (regs) {
+#pragma unroll
+                for (int s = 0; s < K; ++s) {
+                    lx += P.uwx[M.ti[s]] * M.w[s];
+                    ly += P.uwy[M.ti[s]] * M.w[s];
+                }
+                Rk = R;
+                udx = M.udx;
+                udy = M.udy;
+            } else {
+#pragma unroll
+                for (int s = 0; s < K; ++s) {
+                    const int ti = S.cti[k * K + s];
+                    const double w = S.cw[k * K + s];
+                    lx += P.uwx[ti] * w;
+                    ly += P.uwy[ti] * w;
+                }
+                Rk = S.R[k];
+                udx = I.ud[2 * k];
+                udy = I.ud[2 * k + 1];
+            }
+            if (inc) {
+                P.Lfx[k] = 2.0 * (udx - lx) / dt;
+                P.Lfy[k] = 2.0 * (udy - ly) / dt;
+            } else {
+                P.Lfx[k] = 2.0 * Rk * (udx - lx) / dt;
+                P.Lfy[k] = 2.0 * Rk * (udy - ly) / dt;
+            }
+            if (!mdf) {
+                S.mf[2 * k] = P.Lfx[k];
+                S.mf[2 * k + 1] = P.Lfy[k];
+            }
+            const double vx = udx - lx, vy = udy - ly;
+            rmax = nanmax2(sqrt(vx * vx + vy * vy), rmax);
+        }
+        __threadfence_block();
+        const double res = block_reduce(rmax, red, true);  // its barrier publishes Lf
+        if (mdf && it > 0 && res > prev * 1.2) {  // multi_particle.py:152-162
+            ++trips;
+            omega = fmax(omega * 0.5, omega_min);
+            if (omega <= omega_min) {
+                diverged = 1;
+                break;
+            }
+            prev = res;
+            continue;
+        }
+        // spread: each particle's contributions summed on their own, particles added in order
+        for (int ti = t; ti < nT; ti += NTI) {
+            const int sl = S.tslot[ti];
+            double tx = 0.0, ty = 0.0;
+            int x = P.toff[ti];
+            const int end = P.toff[ti + 1];
+            while (x < end) {
+                const int p = particle_of(I, P.gk[x]);
+                double ex = 0.0, ey = 0.0;
+                for (; x < end && particle_of(I, P.gk[x]) == p; ++x) {
+                    const int k = P.gk[x];
+                    const double w = P.gw[x];
+                    ex += P.Lfx[k] * w;
+                    ey += P.Lfy[k] * w;
+                }
+                if (mdf) {
+                    const double fx = ex * omega, fy = ey * omega;
+                    tx = tx + fx;
+                    ty = ty + fy;
+                    S.pfb[(2 * p) * bc + sl] = S.pfb[(2 * p) * bc + sl] + fx;
+                    S.pfb[(2 * p + 1) * bc + sl] = S.pfb[(2 * p + 1) * bc + sl] + fy;
+                } else {
+                    tx = tx + ex;
+                    ty = ty + ey;
+                }
+            }
+            if (mdf) {
+                P.fbx[ti] = P.fbx[ti] + tx;
+                P.fby[ti] = P.fby[ti] + ty;
+                if (inc) {
+                    P.uwx[ti] = P.uwx[ti] + 0.5 * tx * dt;
+                    P.uwy[ti] = P.uwy[ti] + 0.5 * ty * dt;
+                } else {
+                    const double vs = P.vs[ti];
+                    P.uwx[ti] = P.uwx[ti] + tx * vs;
+                    P.uwy[ti] = P.uwy[ti] + ty * vs;
+                }
+            } else {
+                P.fbx[ti] = tx;
+                P.fby[ti] = ty;
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+        if (mdf) {
+            prev = res;
+            ++iter_count;
+            last = res;
+            if (it + 1 >= min_iter && res < tol) {
+                converged = 1;
+                break;
+            }
+        }
+    }
+    if (mdf && t == 0) {
+        I.stats[0] = iter_count;
+        I.stats[1] = last;
+        I.stats[2] = diverged;
+        I.stats[3] = converged;
+        I.stats[4] = trips;
+        I.stats[5] = omega;
+    }
+    __threadfence_block();
+    __syncthreads();
+}
+
 template <int A>
 __device__ void ibm_core(Inst &I, int nT, const IbmPtrs &P, double *red, bool store_lu,
                          const MarkerRegs<A> &M) {
@@ -373,7 +556,9 @@ __device__ void ibm_core(Inst &I, int nT, const IbmPtrs &P, double *red, bool st
     IbmScratch &S = I.s;
     const bool inc = I.inc != 0;
     const double dt = I.cp.dt;
-    if (I.ibm == 1 || I.ibm == 2) {
+    if ((I.ibm == 1 || I.ibm == 2) && I.np_part > 1) {
+        ibm_core_multi<A>(I, nT, P, red, M);
+    } else if (I.ibm == 1 || I.ibm == 2) {
         for (int ti = t; ti < nT; ti += NTI) {
             P.fbx[ti] = 0.0;
             P.fby[ti] = 0.0;
@@ -699,7 +884,7 @@ __device__ void ibm_fast_lists(Inst &I, int *redi) {
     atomicMax(&redi[0], mx);
     __syncthreads();
     const int mwv = (Lb + 63) >> 6;  // marker wavefronts
-    const int fast = (I.ibm == 1 || I.ibm == 2) && Lb <= FAST_LB &&
+    const int fast = (I.ibm == 1 || I.ibm == 2) && I.np_part <= 1 && Lb <= FAST_LB &&
                      nT <= NTF - 64 * mwv && redi[0] <= FAST_G;
     if (fast) {
         for (int ti = t; ti < nT; ti += NTI) {
@@ -737,6 +922,10 @@ __device__ inline void ibm_tables_role(Inst &I, int *redi) {
         I.s.fibx[sl] = 0.0;
         I.s.fiby[sl] = 0.0;
     }
+    if (I.s.pfb && I.np_part > 1 && I.ibm == 2)  // per-particle fields of the coupled MDF
+        for (int p = 0; p < 2 * I.np_part; ++p)
+            for (int sl = threadIdx.x; sl < I.bw * I.bh; sl += NTI)
+                I.s.pfb[(size_t)p * I.box_cap + sl] = 0.0;
 }
 
 }  // namespace iblbm

@@ -428,7 +428,8 @@ int iblbm_op_multi_direct_forcing(const double *Lx, const double *Ly, const doub
         stats->last_residual = s[1];
         stats->diverged = (int32_t)s[2];
         stats->converged = (int32_t)s[3];
-        stats->reserved = 0;
+        stats->guard_trips = 0;
+        stats->omega_used = omega;
     }
     return 0;
 }

@@ -61,11 +61,13 @@ inline bool alloc_ibm_scratch(Inst &d, int lb_cap, int box_cap, Get get) {
     D(&s.lam, L);
     D(&s.q, 2 * L);
     D(&s.dfl, 9 * L);
+    D(&s.mf, 2 * L);
+    s.pfb = nullptr;  // several particles only (iblbm_set_particles)
     I(&s.pk, (size_t)FAST_G * FAST_T);
     D(&s.pw, (size_t)FAST_G * FAST_T);
     I(&s.mti, (size_t)FAST_K * FAST_LB);
     D(&s.mw, (size_t)FAST_K * FAST_LB);
-    D(&d.stats, 4);
+    D(&d.stats, 6);
     D(&d.diag, 8);
     d.lb_cap = (int)L;
     d.box_cap = (int)B;

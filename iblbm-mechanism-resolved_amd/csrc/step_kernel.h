@@ -23,6 +23,8 @@ constexpr int FAST_G = 8;              // gather entries per touched node on tha
 constexpr int FAST_LB = 128;           // markers on that path (up to 2 marker wavefronts)
 constexpr int FAST_K = 25;             // stencil points per marker (Peskin 4-point: 5 x 5)
 
+constexpr int IBM_MAXP = 8;  // particles of one DF / MDF window
+
 struct IbmScratch {
     // per box slot (box_cap)
     double *fs;               // 9 planes: post-stream/boundary (and DFC-corrected) fstar
@@ -43,6 +45,9 @@ struct IbmScratch {
     double *gw;               // wx*wy*Larea
     // per marker (lb_cap)
     double *Lux, *Luy, *R, *Lfx, *Lfy, *lam, *q;  // q: DFC force (Lb, 2)
+    // several particles' DF / MDF windows: per particle window fib [p][x|y][box_cap]
+    // (coupled MDF) and the marker forces of the last pass (Lb, 2)
+    double *pfb, *mf;
     double *dfl;              // DFC delta f (Lb, 9)
     // fused window path: gather lists padded to FAST_G entries, entry-major [e * FAST_T + ti]
     // (padding: marker Lb, weight 0), and the marker stencils stencil-major [s * Lb + k]
@@ -78,9 +83,14 @@ struct Inst {
     int pmode, data_cap;  // predictor mode (0 off, 1 collect, 2 warm start, 3 learned), capacity
     double dx, dy, Larea, omega, tol, dfc_scale;
     const double *Lx, *Ly, *ud;
+    // particles of a DF / MDF window (markers concatenated in particle order); with more
+    // than one, contributions are spread per particle and the MDF iteration is coupled
+    int np_part, pad_np;
+    int poff[IBM_MAXP + 1];     // first marker of each particle
+    double plarea[IBM_MAXP];    // arc length per marker of each particle
     IbmScratch s;
     // outputs of the IBM workgroup
-    double *stats;  // [iter_count, last_residual, diverged, converged]
+    double *stats;  // [iter_count, last_residual, diverged, converged, guard_trips, omega]
     double *diag;   // [rho_box, force_x, force_y, torque_fib, torque_q]
     double cxc, cyc;  // torque centre in lattice units (cx/dx, cy/dy)
     double cxd, cyd;  // torque centre in domain units

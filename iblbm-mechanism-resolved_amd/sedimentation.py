@@ -158,15 +158,16 @@ class Sedimentation:
         self.g = cfg.gravity
         self.multi = cfg.particles_config is not None
         if self.multi:
-            if cfg.ibm_method != "DFC":
-                raise ValueError("multi-particle sedimentation on the GPU path runs DFC")
             self.particles = [Particle(pc["center"][0], pc["center"][1], pc["rho_ratio"],
                                        st.lattice_r, st.r, st.dx, cfg.marker_spacing_factor,
                                        cfg.retraction_dx) for pc in cfg.particles_config]
+            if len(self.particles) > 8:
+                raise ValueError("the GPU window holds at most 8 particles")
             larea = self.particles[0].Larea
-            for p in self.particles:
-                if abs(float(p.Larea) - larea) > 1e-12:
-                    raise ValueError("multi-particle DFC needs one Larea for every particle")
+            if cfg.ibm_method == "DFC":  # strategies/dfc.py:56-58
+                for p in self.particles:
+                    if abs(float(p.Larea) - larea) > 1e-12:
+                        raise ValueError("multi-particle DFC needs one Larea for every particle")
         else:
             cx, cy = cfg.cylinder_center
             self.particles = [Particle(cx, cy, cfg.rho_ratio, st.lattice_r, st.r, st.dx,
@@ -178,14 +179,19 @@ class Sedimentation:
         case.lb_capacity = sum(p.Lb for p in self.particles)
         case.box_capacity = st.nx * st.ny
         self.omega = None
-        if cfg.ibm_method == "MDF":  # strategies/mdf.py:27-41
+        if cfg.ibm_method == "MDF":  # strategies/mdf.py:27-41 (the ring at cylinder_center)
             om = gpu_kernels.estimate_optimal_omega_from_type(st.Lx, st.Ly, cfg.delta_type,
                                                               st.Larea, st.dx)
             if cfg.enable_rotation:
                 om = min(float(om), 1.0)
             self.omega = om
-            case.mdf_omega = om
+            # several particles iterate coupled from min(0.8 omega, 0.8)
+            # (multi_particle.py:32-36, :85-86)
+            case.mdf_omega = min(om * 0.8, 0.8) if self.multi else om
         self.k = self.ctx.add(case)
+        if self.multi and cfg.ibm_method in ("DF", "MDF"):
+            self.ctx.set_particles(self.k, [p.Lb for p in self.particles],
+                                   [float(p.Larea) for p in self.particles])
         self._push_markers()
         self.ctx.upload_f(self.k, pre_collide(cfg, st))
         self.ttt = 0
@@ -263,7 +269,33 @@ class Sedimentation:
                     T = float(np.sum(rx * fq[:, 1] - ry * fq[:, 0]))
                 out.append((F, T))
             return out
-        fx, fy, box = self.ctx.download_window(self.k)
+        if self.multi and cfg.ibm_method == "DF":  # forcing.py:263-275, per particle
+            Lbt = sum(p.Lb for p in self.particles)
+            Lf = self.ctx.download_marker_force(self.k, Lbt)
+            c = 0
+            for p in self.particles:
+                Lfx, Lfy = Lf[c:c + p.Lb, 0], Lf[c:c + p.Lb, 1]
+                c += p.Lb
+                la = float(p.Larea)
+                F = np.array([-float(np.sum(Lfx * la)), -float(np.sum(Lfy * la))])
+                T = 0.0
+                if cfg.enable_rotation:
+                    rx = (p.Lx - float(p.pos[0])) / st.dx
+                    ry = (p.Ly - float(p.pos[1])) / st.dy
+                    T = -float(np.sum((rx * Lfy - ry * Lfx) * la))
+                out.append((F, T))
+            return out
+        if self.multi:  # coupled MDF: each particle's own field (multi_particle.py:302-316)
+            for i, p in enumerate(self.particles):
+                out.append(self._window_force(*self.ctx.download_particle_fib(self.k, i), p))
+            return out
+        out.append(self._window_force(*self.ctx.download_window(self.k), self.particles[0]))
+        return out
+
+    def _window_force(self, fx, fy, box, p):
+        """-(grid sums) of a window fib and its torque about particle p, in NumPy's order
+        over the full (N,) / (ny, nx) arrays (strategies/mdf.py:86-101)."""
+        cfg, st = self.cfg, self.st
         x0, y0, bw, bh = box
         nx, N = st.nx, st.nodenums
         start = y0 * nx
@@ -276,15 +308,13 @@ class Sedimentation:
         F = -np.array([np_sum_sparse(seg_x, start, N), np_sum_sparse(seg_y, start, N)])
         T = 0.0
         if cfg.enable_rotation:  # strategies/mdf.py:94-101
-            p = self.particles[0]
             cx_lat = float(p.pos[0]) / st.dx
             cy_lat = float(p.pos[1]) / st.dy
             II = np.tile(np.arange(nx, dtype=float), bh)
             JJ = np.repeat(np.arange(y0, y0 + bh, dtype=float), nx)
             term = (II - cx_lat) * seg_y - (JJ - cy_lat) * seg_x
             T = -np_sum_sparse(term, start, N)
-        out.append((F, T))
-        return out
+        return F, T
 
     def _full_step(self, forces):
         cfg, st = self.cfg, self.st

@@ -151,7 +151,32 @@ class Context:
         st = _abi.MdfStats()
         check(self.lib.iblbm_get_mdf_stats(self.h, k, ctypes.byref(st)), "mdf_stats")
         return {"iter_count": st.iter_count, "last_residual": st.last_residual,
-                "diverged": bool(st.diverged), "converged": bool(st.converged)}
+                "diverged": bool(st.diverged), "converged": bool(st.converged),
+                "guard_trips": st.guard_trips, "omega_used": st.omega_used}
+
+    def set_particles(self, k, lb, larea):
+        """Several particles in one DF / MDF window (markers concatenated in this order)."""
+        lb = np.ascontiguousarray(lb, dtype=np.int32)
+        la = f64(larea)
+        check(self.lib.iblbm_set_particles(self.h, k, len(lb),
+                                           lb.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                           dptr(la)), "set_particles")
+
+    def download_particle_fib(self, k, part):
+        """Coupled MDF: particle ``part``'s accumulated window fib and the window box."""
+        nx, ny = self.n[k]
+        fx, fy = np.empty(nx * ny), np.empty(nx * ny)
+        box = (ctypes.c_int * 4)()
+        check(self.lib.iblbm_download_particle_fib(self.h, k, int(part), dptr(fx), dptr(fy), box),
+              "download_particle_fib")
+        nb = box[2] * box[3]
+        return fx[:nb], fy[:nb], (box[0], box[1], box[2], box[3])
+
+    def download_marker_force(self, k, Lb):
+        """DF of several particles: the marker forces (Lb, 2) of the last step."""
+        F = np.empty((int(Lb), 2))
+        check(self.lib.iblbm_download_marker_force(self.h, k, dptr(F)), "download_marker_force")
+        return F
 
     def diag(self, k):
         d = _abi.Diag()

@@ -81,13 +81,15 @@ typedef struct iblbm_case {
     int32_t reserved;
 } iblbm_case;
 
-/* Per-step IBM statistics (iblbm/ibm/mdf.py:224-229). */
+/* Per-step IBM statistics (iblbm/ibm/mdf.py:224-229; coupled iteration of several particles,
+ * iblbm/ibm/multi_particle.py:214-221, adds guard_trips and omega_used). */
 typedef struct iblbm_mdf_stats {
     int32_t iter_count;
     int32_t diverged;
     int32_t converged;
-    int32_t reserved;
+    int32_t guard_trips;  /* coupled: rejected iterations (omega halved); else 0 */
     double last_residual;
+    double omega_used;    /* coupled: omega at the end of the step; else the instance's omega */
 } iblbm_mdf_stats;
 
 /* Diagnostic reductions of the most recent step that requested them (FP64). */
@@ -117,6 +119,21 @@ int iblbm_download_f(void *ctx, int id, double *f_aos);
  * Replaces the per-step H2D of iblbm/runtime/scenarios/sedimentation_single.py:156-160. */
 int iblbm_set_markers(void *ctx, int id, const double *Lx, const double *Ly, const double *ud,
                       int Lb, double Larea);
+/* Several particles in one DF / MDF window (replaces the per-particle loops of
+ * iblbm/ibm/multi_particle.py:62-455): the markers passed to iblbm_set_markers are the
+ * particles' rings concatenated in order, lb[p] markers of arc length larea[p] each.  DF spreads
+ * each particle's closure on its own and adds them in particle order (:347-421); MDF runs the
+ * coupled iteration (:62-316) from the instance's mdf_omega, which the caller sets to
+ * min(0.8 omega_est, 0.8).  DFC windows ignore this (their rings are spread as one).  Call it
+ * before iblbm_set_markers; np <= 8. */
+int iblbm_set_particles(void *ctx, int id, int np, const int *lb, const double *larea);
+/* Coupled MDF: particle `part`'s accumulated window fib of the last step, box as in
+ * iblbm_download_window (its force and torque are grid sums of it, multi_particle.py:199-211). */
+int iblbm_download_particle_fib(void *ctx, int id, int part, double *fibx, double *fiby,
+                                int box[4]);
+/* DF of several particles: the marker forces (Lb, 2) of the last step (a particle's force is
+ * -sum(Lf * Larea) over its markers, iblbm/ibm/forcing.py:263-275). */
+int iblbm_download_marker_force(void *ctx, int id, double *F);
 /* Centre (domain units) about which the diagnostic torque sums are taken. */
 int iblbm_set_torque_center(void *ctx, int id, double cx, double cy);
 /* Window box [box[0], box[0]+box[2]) x [box[1], box[1]+box[3]) covering every clamped

@@ -361,6 +361,10 @@ def _run_sed(name, cfg_kwargs):
                 row += [float(p.pos[0]), float(p.pos[1]), float(p.vel[0]), float(p.vel[1]),
                         float(p.omega), float(p.angle), float(p.force[0]), float(p.force[1]),
                         float(p.torque)]
+            st = getattr(state, "_mdf_iter_stats", None)
+            if st is not None and "guard_trips" in st:  # coupled MDF of several particles
+                row += [st["iter_count"], st["last_residual"], st["guard_trips"],
+                        st["omega_used"]]
         else:
             row = [step, float(state.particle_pos[0]), float(state.particle_pos[1]),
                    float(state.particle_vel[0]), float(state.particle_vel[1]),
@@ -420,6 +424,32 @@ def case_sed_multi_dfc_cm_hat():
         gravity=2e-4, enable_rotation=True, settling_inertia_model="explicit_history",
         particles_config=[{"center": (0.5, 3.0), "rho_ratio": 1.5},
                           {"center": (0.53, 2.55), "rho_ratio": 1.5}]))
+
+
+def case_sed_multi_df_bgk_hat():
+    # two discs whose hat stencils share nodes across a 0.8-lattice gap
+    _run_sed("sed_multi_df_bgk_hat", dict(
+        Re=2.0, NN=25, xmax=1.0, ymax=4.0, cylinder_center=(0.5, 3.0), cylinder_D_ratio=0.25,
+        lattice_u=0.02, inflow_u=0.0, bc_type="settling_channel", delta_type="hat",
+        ibm_method="DF", collision_model="BGK", marker_spacing_factor=1.0, max_steps=150,
+        check_interval=1, use_convergence=False, motion_type="sedimentation", rho_ratio=1.5,
+        gravity=2e-4, enable_rotation=True, settling_inertia_model="explicit_history",
+        particles_config=[{"center": (0.5, 3.0), "rho_ratio": 1.5},
+                          {"center": (0.53, 2.72), "rho_ratio": 1.3}]))
+
+
+def case_sed_multi_mdf_cm_p4():
+    # coupled MDF: two D = 12 discs whose Peskin-4 stencils share nodes (2.4-lattice gap)
+    _run_sed("sed_multi_mdf_cm_p4", dict(
+        Re=20.0, NN=49, xmax=1.0, ymax=3.0, cylinder_center=(0.5, 2.2), cylinder_D_ratio=0.25,
+        lattice_u=0.02, inflow_u=0.0, bc_type="settling_channel", delta_type="peskin4pt",
+        ibm_method="MDF", collision_model="CM_MRT", mdf_iterations=20, mdf_min_iterations=5,
+        mdf_tolerance=1e-4, marker_spacing_factor=1.0, max_steps=40, check_interval=1,
+        use_convergence=False, motion_type="sedimentation", rho_ratio=1.5, gravity=5e-5,
+        enable_rotation=True, rotation_coupling="indirect",
+        settling_inertia_model="explicit_history", time_integrator="verlet",
+        particles_config=[{"center": (0.5, 2.2), "rho_ratio": 1.5},
+                          {"center": (0.52, 1.9), "rho_ratio": 1.3}]))
 
 
 CASES = {k[5:]: v for k, v in dict(globals()).items() if k.startswith("case_")}

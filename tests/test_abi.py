@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 def test_case_struct_layout_matches_header():
     # 8 int32 + 2 doubles + 9 doubles + 5 doubles + 2 int32 + 2 doubles + 4 int32
     assert ctypes.sizeof(_abi.Case) == 8 * 4 + 16 * 8 + 2 * 4 + 2 * 8 + 4 * 4
-    assert ctypes.sizeof(_abi.MdfStats) == 24
+    assert ctypes.sizeof(_abi.MdfStats) == 4 * 4 + 2 * 8
     assert ctypes.sizeof(_abi.Diag) == 48
 
 

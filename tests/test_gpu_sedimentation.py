@@ -1,4 +1,6 @@
-"""Settling particles on the GPU path against the reference's own runs (tests/golden/sed_*).
+"""Settling particles on the GPU path against the reference's own runs (tests/golden/sed_*):
+one particle (DF, MDF) and two (DFC spread as one ring; DF and coupled MDF spread per particle,
+with stencils that share nodes).
 
 Trajectories (position, velocity, angular velocity, angle, force, torque per step) and the final
 populations are compared; the bar is 1e-10 relative for trajectories (the measured error is
@@ -14,7 +16,8 @@ TRAJ_TOL = 1e-10
 
 
 @pytest.mark.parametrize("case", ["sed_single_df_bgk_hat", "sed_single_mdf_cm_p4",
-                                  "sed_multi_dfc_cm_hat"])
+                                  "sed_multi_dfc_cm_hat", "sed_multi_df_bgk_hat",
+                                  "sed_multi_mdf_cm_p4"])
 def test_sedimentation_matches_reference(case):
     g, sc = golden(case)
     cfg = golden_cfg(sc)
@@ -26,8 +29,11 @@ def test_sedimentation_matches_reference(case):
         for p in sim.particles:
             row += [p.pos[0], p.pos[1], p.vel[0], p.vel[1], p.omega, p.angle, p.force[0],
                     p.force[1], p.torque]
-        if sim.mdf_stats is not None and not sim.multi:
-            row += [sim.mdf_stats["iter_count"], sim.mdf_stats["last_residual"]]
+        if sim.mdf_stats is not None:
+            st = sim.mdf_stats
+            row += [st["iter_count"], st["last_residual"]]
+            if sim.multi:  # the coupled iteration of several particles
+                row += [st["guard_trips"], st["omega_used"]]
         rows.append(row)
     got = np.array(rows, dtype=np.float64)
     ref = g["particles"]
