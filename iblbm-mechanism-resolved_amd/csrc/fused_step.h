@@ -28,6 +28,11 @@ struct FusedShared {
     alignas(16) unsigned short mlpx[FAST_LB * 16];  // predictor feature rows (bf16)
     float mlpy[FAST_LB * MLP_OUT];                   // predictor outputs
     MlpLds mlpw;                                     // predictor weights (learned mode)
+    // predictor inputs, copied from global memory straight into LDS during P1: per marker
+    // the previous converged force and forcing slip (fprev), the position, and the
+    // normalisation
+    alignas(16) double pf[2 * FAST_LB], pq[2 * FAST_LB], plx[FAST_LB], ply[FAST_LB];
+    alignas(16) float pnrm[MLP_NORM + 2];
 };
 
 // Copies the descriptor fields the window reads into a local descriptor, in one batch at the
@@ -128,6 +133,23 @@ __device__ inline double fused_block_sum(double v, double *red) {
     return r;
 }
 
+// Largest value over the wavefront (every lane gets it).
+__device__ inline float wave_maxf(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// 16 bytes per lane from global memory straight into LDS (global_load_lds_dwordx4): lane l of
+// the calling wavefront writes lds_wave + 16 l, where lds_wave is the same for the wavefront.
+// The data arrive with the wavefront's other vector loads; the compiler waits for them before
+// a barrier that follows.
+__device__ inline void lds_dma16(const void *src, void *lds_wave) {
+    typedef __attribute__((address_space(1))) void GVoid;
+    typedef __attribute__((address_space(3))) void LVoid;
+    __builtin_amdgcn_global_load_lds((GVoid *)src, (LVoid *)lds_wave, 16, 0, 0);
+}
+
 // One 16-marker tile of the learned predictor's inference on the calling wavefront.
 __device__ inline void mlp_infer_tile(FusedShared &sh, int tile) {
     mlp_tile16(sh.mlpw.blk, sh.mlpw.bias, sh.mlpw.bias + MLP_H, sh.mlpw.bias + 2 * MLP_H,
@@ -175,6 +197,36 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
         }
     }
     double udx = 0.0, udy = 0.0;
+    // predictor inputs straight to LDS (no registers held across P1): marker wavefronts take
+    // their markers' previous force and q; wavefront 0 the positions and normalisation; in
+    // learned mode wavefronts 2.. the weight blocks and 3 more the biases
+    long long pctr0 = 0, pctr1 = 0;  // the predictor's counters, read before P1's loads
+    if (PRED && I.ibm == 2 && I.pmode != 0) {
+        pctr0 = S.pctr[0];
+        pctr1 = S.pctr[1];
+        const int wv = t >> 6, ln = t & 63;
+        if (t < Lb) {
+            lds_dma16(S.fprev + 2 * t, sh.pf + 128 * wv);
+            lds_dma16(S.fprev + 2 * I.lb_cap + 2 * t, sh.pq + 128 * wv);
+        }
+        if (wv == 0 && 2 * ln < Lb) {  // two positions per lane
+            lds_dma16(I.Lx + 2 * ln, sh.plx);
+            lds_dma16(I.Ly + 2 * ln, sh.ply);
+        }
+        if (wv == 0 && ln < (MLP_NORM + 3) / 4) lds_dma16(S.mlp_norm + 4 * ln, sh.pnrm);
+        if (I.pmode == 3) {
+            constexpr int NB = MLP_NBLK * 64 * 8 / 1024;  // 1 KB per wavefront instruction
+            if (wv >= 2 && wv < 2 + NB)
+                lds_dma16(reinterpret_cast<const char *>(S.mlp_wb) + 1024 * (wv - 2) + 16 * ln,
+                          reinterpret_cast<char *>(sh.mlpw.blk) + 1024 * (wv - 2));
+            // biases: b1 from lanes 0-15 of one wavefront; b2 from lanes 0-15 and b3 from lane
+            // 16 of the next (lane 16 lands 64 floats after b2's start, where b3 belongs)
+            static_assert(2 + NB + 1 < NTF / 64, "bias wavefronts");
+            if (wv == 2 + NB && ln < 16) lds_dma16(S.mlp_p + MLP_B1 + 4 * ln, sh.mlpw.bias);
+            if (wv == 3 + NB && ln <= 16)
+                lds_dma16(S.mlp_p + (ln < 16 ? MLP_B2 + 4 * ln : MLP_B3), sh.mlpw.bias + MLP_H);
+        }
+    }
     if (t < Lb) {
         udx = I.ud[2 * t];
         udy = I.ud[2 * t + 1];
@@ -236,16 +288,11 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
             }
         }
     }
-    if (PRED && I.ibm == 2 && I.pmode == 3) {  // learned predictor: weights and biases to LDS
-        const mlp_short4 *wb = reinterpret_cast<const mlp_short4 *>(S.mlp_wb);
-        for (int e = t; e < MLP_NBLK * 64; e += NTF) sh.mlpw.blk[e] = wb[e];
-        if (t < 2 * MLP_H + MLP_OUT)
-            sh.mlpw.bias[t] =
-                S.mlp_p[t < MLP_H ? MLP_B1 + t
-                                  : (t < 2 * MLP_H ? MLP_B2 + t - MLP_H : MLP_B3 + t - 2 * MLP_H)];
-    }
     if (t < 2) rbits[t] = 0ull;
     if (t == 0) lf[Lb] = make_double2(0.0, 0.0);  // target of the padded gather entries
+    // the predictor's copies into LDS come from other wavefronts too: every wavefront's own
+    // have landed before it reaches the barrier
+    if (PRED && I.ibm == 2 && I.pmode != 0) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();  // P1 done
 
     double rho_acc = 0.0, fxs = 0.0, fys = 0.0, trq = 0.0;
@@ -274,95 +321,10 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
         // With a warm start the prediction F = F_prev (+ sig x the network's output) is
         // published here; the loop's first pass spreads it with weight 1 and does not count it
         // as a sub-step.
-        const bool fprev_ok = pmode != 0 && S.pctr[1] != 0;
+        const bool fprev_ok = pmode != 0 && pctr1 != 0;
         const bool record = pmode == 1 && fprev_ok;
         float *rowp = nullptr;  // this marker's training sample (collect mode)
         double facx = 0.0, facy = 0.0;
-        if (PRED && pmode != 0) {
-            // this marker's global inputs first, in one round trip: previous q and force,
-            // position, normalisation
-            double spx = 0.0, spy = 0.0, pfx = 0.0, pfy = 0.0, mx = 0.0, my = 0.0;
-            if (t < Lb) {
-                if (fprev_ok) {
-                    spx = S.fprev[2 * I.lb_cap + 2 * t];
-                    spy = S.fprev[2 * I.lb_cap + 2 * t + 1];
-                    if (warm) {
-                        pfx = S.fprev[2 * t];
-                        pfy = S.fprev[2 * t + 1];
-                    }
-                }
-                mx = I.Lx[t];
-                my = I.Ly[t];
-            }
-            float nrm[MLP_NORM];
-            if (pmode == 3) {
-#pragma unroll
-                for (int q = 0; q < MLP_NORM; ++q) nrm[q] = S.mlp_norm[q];
-            }
-            double ax = 0.0, ay = 0.0, ex = 0.0, ey = 0.0;
-            if (t < Lb) {
-                double lx = 0.0, ly = 0.0, rm = 0.0;
-#pragma unroll 5
-                for (int s = 0; s < K; ++s) {
-                    const int at = sh.mti[s * Lb + t];
-                    const double2 u = uw[at];
-                    const double w = sh.mw[s * Lb + t];
-                    lx += u.x * w;
-                    ly += u.y * w;
-                    rm += sh.rot[at] * w;
-                }
-                if (finc) rm = 1.0;
-                ax = rm * (udx - lx);
-                ay = rm * (udy - ly);
-                S.fprev[2 * I.lb_cap + 2 * t] = ax;  // the next step's previous q
-                S.fprev[2 * I.lb_cap + 2 * t + 1] = ay;
-                ex = ax - spx;
-                ey = ay - spy;
-            }
-            const double a = sqrt(ax * ax + ay * ay);
-            const float sig =
-                (float)fmax(wave_max(sqrt(ex * ex + ey * ey)) + 1e-9 * wave_max(a), 1e-30);
-            float feat[MLP_IN];
-#pragma unroll
-            for (int q = 0; q < MLP_IN; ++q) feat[q] = 0.f;
-            if (t < Lb) {
-                feat[0] = (float)(ex / (double)sig);
-                feat[1] = (float)(ey / (double)sig);
-                feat[2] = a > 0.0 ? (float)(ax / a) : 0.f;
-                feat[3] = a > 0.0 ? (float)(ay / a) : 0.f;
-                const double rx = mx - I.cxd, ry = my - I.cyd;
-                const double rr = sqrt(rx * rx + ry * ry);
-                const double gx = mx / I.dx + 0.5, gy = my / I.dy + 0.5;
-                feat[4] = rr > 0.0 ? (float)(rx / rr) : 0.f;
-                feat[5] = rr > 0.0 ? (float)(ry / rr) : 0.f;
-                feat[6] = (float)(gx - floor(gx) - 0.5);
-                feat[7] = (float)(gy - floor(gy) - 0.5);
-                if (record) {  // features now, sig in the target's place until the end
-                    rowp = S.data + ((S.pctr[0] + t) % I.data_cap) * MLP_ROW;
-#pragma unroll
-                    for (int q = 0; q < MLP_IN; ++q) rowp[q] = feat[q];
-                    rowp[MLP_IN] = sig;
-                }
-            }
-            if (pmode == 3) {  // inference: one 16-marker tile per wavefront (barriers X, Y)
-                float xn[MLP_IN];
-#pragma unroll
-                for (int q = 0; q < MLP_IN; ++q) xn[q] = (feat[q] - nrm[q]) / nrm[MLP_IN + q];
-                mlp_put_row(sh.mlpx + 16 * t, xn);  // rows from Lb on are computed, unused
-                __syncthreads();  // X: feature rows published
-                if ((t >> 6) < ((Lb + 15) >> 4)) mlp_infer_tile(sh, t >> 6);
-                __syncthreads();  // Y: outputs published
-                if (t < Lb) {
-                    pfx += (double)sig * (double)(sh.mlpy[MLP_OUT * t] * nrm[2 * MLP_IN]);
-                    pfy += (double)sig * (double)(sh.mlpy[MLP_OUT * t + 1] * nrm[2 * MLP_IN + 1]);
-                }
-            }
-            if (warm) {  // published here; read by the node threads after barrier A
-                if (t < Lb) lf[t] = make_double2(pfx, pfy);
-                facx = pfx;
-                facy = pfy;
-            }
-        }
         int ti[K];
         if (t < Lb) {
 #pragma unroll
@@ -410,6 +372,81 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
                 Fy = Fy / dt;
             }
         };
+        if (PRED && pmode != 0) {
+            // this marker's inputs, already in LDS: previous q and force, position,
+            // normalisation
+            double spx = 0.0, spy = 0.0, pfx = 0.0, pfy = 0.0, mx = 0.0, my = 0.0;
+            if (t < Lb) {
+                if (fprev_ok) {
+                    spx = sh.pq[2 * t];
+                    spy = sh.pq[2 * t + 1];
+                    if (warm) {
+                        pfx = sh.pf[2 * t];
+                        pfy = sh.pf[2 * t + 1];
+                    }
+                }
+                mx = sh.plx[t];
+                my = sh.ply[t];
+            }
+            const float *nrm = sh.pnrm;
+            double ax = 0.0, ay = 0.0, ex = 0.0, ey = 0.0;
+            if (t < Lb) {  // the loop's own interpolation (register stencils, batched loads)
+                double lx, ly, Fx, Fy;
+                interp_force(lx, ly, Fx, Fy);
+                const double rm = finc ? 1.0 : R;
+                ax = rm * (udx - lx);
+                ay = rm * (udy - ly);
+                S.fprev[2 * I.lb_cap + 2 * t] = ax;  // the next step's previous q
+                S.fprev[2 * I.lb_cap + 2 * t + 1] = ay;
+                ex = ax - spx;
+                ey = ay - spy;
+            }
+            // the features feed a BF16 network: FP32 arithmetic from here
+            const float fex = (float)ex, fey = (float)ey, fax = (float)ax, fay = (float)ay;
+            const float a = sqrtf(fax * fax + fay * fay);
+            const float sig =
+                fmaxf(wave_maxf(sqrtf(fex * fex + fey * fey)) + 1e-9f * wave_maxf(a), 1e-30f);
+            float feat[MLP_IN];
+#pragma unroll
+            for (int q = 0; q < MLP_IN; ++q) feat[q] = 0.f;
+            if (t < Lb) {
+                feat[0] = fex / sig;
+                feat[1] = fey / sig;
+                feat[2] = a > 0.f ? fax / a : 0.f;
+                feat[3] = a > 0.f ? fay / a : 0.f;
+                const float rx = (float)(mx - I.cxd), ry = (float)(my - I.cyd);
+                const float rr = sqrtf(rx * rx + ry * ry);
+                const double gx = mx / I.dx + 0.5, gy = my / I.dy + 0.5;
+                feat[4] = rr > 0.f ? rx / rr : 0.f;
+                feat[5] = rr > 0.f ? ry / rr : 0.f;
+                feat[6] = (float)(gx - floor(gx) - 0.5);
+                feat[7] = (float)(gy - floor(gy) - 0.5);
+                if (record) {  // features now, sig in the target's place until the end
+                    rowp = S.data + ((pctr0 + t) % I.data_cap) * MLP_ROW;
+#pragma unroll
+                    for (int q = 0; q < MLP_IN; ++q) rowp[q] = feat[q];
+                    rowp[MLP_IN] = sig;
+                }
+            }
+            if (pmode == 3) {  // inference: one 16-marker tile per wavefront (barriers X, Y)
+                float xn[MLP_IN];
+#pragma unroll
+                for (int q = 0; q < MLP_IN; ++q) xn[q] = (feat[q] - nrm[q]) / nrm[MLP_IN + q];
+                mlp_put_row(sh.mlpx + 16 * t, xn);  // rows from Lb on are computed, unused
+                __syncthreads();  // X: feature rows published
+                if ((t >> 6) < ((Lb + 15) >> 4)) mlp_infer_tile(sh, t >> 6);
+                __syncthreads();  // Y: outputs published
+                if (t < Lb) {
+                    pfx += (double)sig * (double)(sh.mlpy[MLP_OUT * t] * nrm[2 * MLP_IN]);
+                    pfy += (double)sig * (double)(sh.mlpy[MLP_OUT * t + 1] * nrm[2 * MLP_IN + 1]);
+                }
+            }
+            if (warm) {  // published here; read by the node threads after barrier A
+                if (t < Lb) lf[t] = make_double2(pfx, pfy);
+                facx = pfx;
+                facy = pfy;
+            }
+        }
         for (int it = 0; it < n_iter; ++seq) {
             const bool predict = PRED && warm && seq == 0;
             double rv = 0.0, Fx = 0.0, Fy = 0.0;
@@ -471,7 +508,7 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
                 S.fprev[2 * t + 1] = facy;
             }
             if (t == 0) {
-                if (record) S.pctr[0] += Lb;
+                if (record) S.pctr[0] = pctr0 + Lb;
                 S.pctr[1] = 1;
                 S.pstat[0] += ctl.applied;
                 S.pstat[1] += 1.0;

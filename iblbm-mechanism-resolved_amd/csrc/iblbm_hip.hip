@@ -1325,7 +1325,7 @@ int iblbm_predictor_mode(void *p, int id, int mode, int64_t capacity) {
             return C->mem.alloc(&q, bytes) == hipSuccess ? q : nullptr;
         };
         S.mlp_p = (float *)A(sizeof(float) * 4 * MLP_NP);  // params, m, v, grad
-        S.mlp_norm = (float *)A(sizeof(float) * MLP_NORM);
+        S.mlp_norm = (float *)A(sizeof(float) * (MLP_NORM + 2));  // 16-byte copies
         S.mlp_wb = (unsigned short *)A(sizeof(unsigned short) * MLP_NBLK * 256);
         S.data = (float *)A(sizeof(float) * MLP_ROW * (size_t)capacity);
         S.fprev = (double *)A(sizeof(double) * 4 * d.lb_cap);  // forces, then slips

@@ -128,8 +128,8 @@ __device__ inline void mlp_put_row(unsigned short *row16, const float *xn) {
 
 // LDS copy of the inference weights: the BF16 blocks and the three biases.
 struct MlpLds {
-    mlp_short4 blk[MLP_NBLK * 64];
-    float bias[2 * MLP_H + MLP_OUT];  // b1, b2, b3
+    alignas(16) mlp_short4 blk[MLP_NBLK * 64];
+    alignas(16) float bias[2 * MLP_H + 4];  // b1, b2, b3 (b3 lands with 2 trailing floats)
 };
 
 // ------------------------------------------------------------------ FP32 training

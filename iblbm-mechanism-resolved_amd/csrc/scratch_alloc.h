@@ -23,8 +23,8 @@ inline bool alloc_ibm_scratch(Inst &d, int lb_cap, int box_cap, Get get) {
     };
     IbmScratch &s = d.s;
     double *lx, *ly, *ud;
-    D(&lx, L);
-    D(&ly, L);
+    D(&lx, L + 2);  // the predictor copies positions two at a time (16 bytes)
+    D(&ly, L + 2);
     D(&ud, 2 * L);
     d.Lx = lx;
     d.Ly = ly;
