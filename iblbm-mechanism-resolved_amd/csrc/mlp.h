@@ -27,6 +27,13 @@ constexpr int MLP_BLK1 = 0, MLP_BLK2 = 4, MLP_BLK3 = 20, MLP_NBLK = 24;
 
 typedef short mlp_short4 __attribute__((ext_vector_type(4)));
 typedef float mlp_float4 __attribute__((ext_vector_type(4)));
+typedef __bf16 mlp_bf16x4 __attribute__((ext_vector_type(4)));
+
+// Four floats to BF16 (round to nearest even) with gfx950's v_cvt_pk_bf16_f32.
+__device__ inline mlp_short4 mlp_pack_bf16(float a, float b, float c, float d) {
+    const mlp_bf16x4 v = {(__bf16)a, (__bf16)b, (__bf16)c, (__bf16)d};
+    return __builtin_bit_cast(mlp_short4, v);
+}
 
 __host__ __device__ inline unsigned short f32_to_bf16(float f) {  // round to nearest even
     union {
@@ -94,9 +101,9 @@ __device__ inline void mlp_tile16(const mlp_short4 *__restrict__ wblk,
         mlp_float4 c = {0.f, 0.f, 0.f, 0.f};
         c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wblk[(MLP_BLK1 + ib) * 64 + l], xb, c, 0,
                                                        0, 0);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            h1[ib][j] = (short)f32_to_bf16(mlp_tanh_bf16(c[j] + b1[16 * ib + 4 * g + j]));
+        const mlp_float4 bb = *reinterpret_cast<const mlp_float4 *>(b1 + 16 * ib + 4 * g);
+        h1[ib] = mlp_pack_bf16(mlp_tanh_bf16(c[0] + bb[0]), mlp_tanh_bf16(c[1] + bb[1]),
+                               mlp_tanh_bf16(c[2] + bb[2]), mlp_tanh_bf16(c[3] + bb[3]));
     }
 #pragma unroll
     for (int ib = 0; ib < MLP_H / 16; ++ib) {  // layer 2
@@ -105,9 +112,9 @@ __device__ inline void mlp_tile16(const mlp_short4 *__restrict__ wblk,
         for (int kb = 0; kb < MLP_H / 16; ++kb)
             c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wblk[(MLP_BLK2 + kb * 4 + ib) * 64 + l],
                                                            h1[kb], c, 0, 0, 0);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            h2[ib][j] = (short)f32_to_bf16(mlp_tanh_bf16(c[j] + b2[16 * ib + 4 * g + j]));
+        const mlp_float4 bb = *reinterpret_cast<const mlp_float4 *>(b2 + 16 * ib + 4 * g);
+        h2[ib] = mlp_pack_bf16(mlp_tanh_bf16(c[0] + bb[0]), mlp_tanh_bf16(c[1] + bb[1]),
+                               mlp_tanh_bf16(c[2] + bb[2]), mlp_tanh_bf16(c[3] + bb[3]));
     }
     mlp_float4 c = {0.f, 0.f, 0.f, 0.f};  // layer 3: outputs 0, 1 in lanes g = 0
 #pragma unroll
@@ -122,8 +129,12 @@ __device__ inline void mlp_tile16(const mlp_short4 *__restrict__ wblk,
 
 // bf16 feature row of the tiles: MLP_IN normalised features, then zeros up to 16.
 __device__ inline void mlp_put_row(unsigned short *row16, const float *xn) {
-#pragma unroll
-    for (int k = 0; k < 16; ++k) row16[k] = k < MLP_IN ? f32_to_bf16(xn[k]) : (unsigned short)0;
+    static_assert(MLP_IN == 8, "two packed quads of features");
+    mlp_short4 *r = reinterpret_cast<mlp_short4 *>(row16);
+    r[0] = mlp_pack_bf16(xn[0], xn[1], xn[2], xn[3]);
+    r[1] = mlp_pack_bf16(xn[4], xn[5], xn[6], xn[7]);
+    r[2] = mlp_short4{0, 0, 0, 0};
+    r[3] = mlp_short4{0, 0, 0, 0};
 }
 
 // LDS copy of the inference weights: the BF16 blocks and the three biases.
