@@ -96,6 +96,8 @@ def main():
     ap.add_argument("--serial", action="store_true",
                     help="run the window and bulk kernels in order on one stream")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch schedule instead of the autotuner's")
     ap.add_argument("--surrogate-steps", type=int, default=2000,
                     help="steps of the plain-vs-learned-predictor MDF comparison (0: skip)")
     ap.add_argument("--variant-steps", type=int, default=2000,
@@ -125,6 +127,12 @@ def main():
     N = sim.st.nodenums
     for _ in range(warm // ci):
         sim.advance(ci, diag_every=ci, graph=graph)
+        sim.cd_cl()
+    tuned = None
+    if not args.no_tune and not args.serial:  # launch schedule from the autotuner
+        tuned = sim.ctx.autotune(20)
+        mode = sim.ctx.step_mode()
+        sim.advance(ci, diag_every=ci, graph=graph)  # the chunk's graph, outside the timing
         sim.cd_cl()
     sim.ctx.sync()
     ev_ms = 0.0
@@ -158,6 +166,7 @@ def main():
         "config": {"workload": "fixed cylinder Re=100, 1601x641, D=40, 126 markers, TRT "
                                "Lambda=1/4, MDF [5,20] tol 1e-4, Peskin-4, U=0.04",
                    "precision": args.precision, "launch": mode, "graph": graph,
+                   "schedule": tuned,
                    "nodes": N, "check_interval": ci, "Cd_last": cd, "Cl_last": cl,
                    "mdf_iter_last": iters[-1] if iters else None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -279,12 +288,27 @@ def surrogate_leg(sim, n, graph):
         sub = (pred.stats()["substeps_per_step"] if mode != "off"
                else float(sim.ctx.mdf_stats(sim.k)["iter_count"]))
         res[mode] = (us, cd, sub, pred.stats()["restarts"] if mode != "off" else 0)
+    # Cd over the same window, sampled every 100 steps, and each form's last marker slip
+    means, slips = {}, {}
+    for mode in ("off", "learned"):
+        sim.ctx.upload_f(sim.k, f0)
+        pred.mode(mode)
+        if mode != "off":
+            pred.set_previous_force(F0)
+        cds = []
+        for _ in range(max(1, n // 100)):
+            sim.advance(100, diag_every=sim.ctx.step_counter() + 100, graph=graph)
+            cds.append(sim.cd_cl()[0])
+        means[mode] = float(np.mean(cds))
+        slips[mode] = sim.ctx.mdf_stats(sim.k)["last_residual"]
     pred.mode("off")
     (up, cdp, sp, _), (ul, cdl, sl, rl) = res["off"], res["learned"]
     return {"steps": n, "plain_us": round(up, 3), "surrogate_us": round(ul, 3),
             "speedup": round(up / ul, 4), "substeps_plain": sp,
             "substeps_surrogate": round(sl, 3), "restarts": rl,
             "dCd_rel": abs(cdl - cdp) / abs(cdp), "Cd_plain": cdp, "Cd_surrogate": cdl,
+            "dCd_mean_rel": abs(means["learned"] - means["off"]) / abs(means["off"]),
+            "slip_residual_plain": slips["off"], "slip_residual_surrogate": slips["learned"],
             "train": {"samples": pred.samples(), "adam_steps": len(losses),
                       "loss_first100": float(losses[:100].mean()),
                       "loss_last100": float(losses[-100:].mean())}}
@@ -296,6 +320,7 @@ def fp32dev_leg(cfg, n, warm, graph):
     from iblbm_amd.solver import FixedCylinder
     sim = FixedCylinder(cfg, perturb=PERTURB, precision="fp32dev")
     sim.advance(warm, graph=graph)
+    tuned = sim.ctx.autotune(20)
     us = _timed(sim, n, graph)
     cd32, cl32 = sim.cd_cl()
     t = sim.ttt
@@ -304,7 +329,7 @@ def fp32dev_leg(cfg, n, warm, graph):
     cd64, cl64 = ref_sim.cd_cl()
     ref_sim.ctx.close()
     out = {"MLUPS": round(sim.st.nodenums / us, 2), "ms_per_step": round(us / 1e3, 5),
-           "launch": sim.ctx.step_mode(), "steps": n, "compared_at_step": t,
+           "launch": sim.ctx.step_mode(), "schedule": tuned, "steps": n, "compared_at_step": t,
            "dCd_rel_vs_fp64": abs(cd32 - cd64) / abs(cd64),
            "dCl_vs_fp64_over_Cd": abs(cl32 - cl64) / abs(cd64)}
     sim.ctx.close()

@@ -147,12 +147,14 @@ __device__ __attribute__((always_inline)) TileNode row_node(const LaunchTable &T
 }
 
 // Bulk rows of one time step (every node that no marker stencil touches): a wavefront takes
-// rows first, first + nw, ...; with TWO it keeps two rows' loads in flight at a time.  Each
-// row's FP64 partial rho sum is a fixed shuffle tree over the wavefront.
+// rows first, first + nw, ... below stop (all rows when stop < 0); with TWO it keeps two rows'
+// loads in flight at a time.  Each row's FP64 partial rho sum is a fixed shuffle tree over the
+// wavefront.
 template <typename S, int MODEL, bool TWO>
-__device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, int first, int nw) {
+__device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, int first, int nw,
+                                                        int stop = -1) {
     const int lane = (int)threadIdx.x & 63;
-    const int n_rows = T.n_tiles * TILE_Y;
+    const int n_rows = stop >= 0 ? stop : T.n_tiles * TILE_Y;
     if constexpr (TWO) {
         for (int r0 = first; r0 < n_rows; r0 += 2 * nw) {
             const bool v1 = r0 + nw < n_rows;
@@ -238,10 +240,19 @@ __global__ __launch_bounds__(NTF) void k_step(LaunchTable T) {
     // wavefront moves on as soon as its rows are done)
     const int first = ((int)blockIdx.x - T.n_fast) * (NTF / 64) + (int)(threadIdx.x >> 6);
     const int nw = ((int)gridDim.x - T.n_fast) * (NTF / 64);
-    if (T.rows2)
+    if (T.blocked) {  // one block of adjacent rows per wavefront (an even count)
+        const int n_rows = T.n_tiles * TILE_Y;
+        const int per = ((n_rows + nw - 1) / nw + 1) & ~1;
+        const int r0 = min(n_rows, first * per), r1 = min(n_rows, r0 + per);
+        if (T.rows2)
+            bulk_rows<S, MODEL, true>(T, r0, 1, r1);
+        else
+            bulk_rows<S, MODEL, false>(T, r0, 1, r1);
+    } else if (T.rows2) {
         bulk_rows<S, MODEL, true>(T, first, nw);
-    else
+    } else {
         bulk_rows<S, MODEL, false>(T, first, nw);
+    }
 }
 
 // Seeded synthetic initial state f = f_eq(rho, u(1 + amp xi)) in both population buffers;
@@ -383,7 +394,7 @@ struct Ctx {
     int last_steps = 0;
     int64_t diag_step = -1;
     std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
-    iblbm_schedule sched{IBLBM_MODE_AUTO, 256, 1, 0, 2};
+    iblbm_schedule sched{IBLBM_MODE_AUTO, 256, 1, 0, 2, 0};
 };
 
 void clear_graphs(Ctx &C) {
@@ -476,6 +487,7 @@ LaunchTable make_table(Ctx &C, int parity, int diag) {
     T.parity = parity;
     T.diag = diag;
     T.rows2 = 0;
+    T.blocked = 0;
     return T;
 }
 
@@ -532,6 +544,7 @@ int launch_steps(Ctx &C, int nsteps, int diag_every, bool concurrent) {
             const int diag = (diag_every > 0 && stepno % diag_every == 0) ? 1 : 0;
             LaunchTable T = make_table(C, (C.parity + s) & 1, diag);
             T.rows2 = C.sched.step_rows == 2 ? 1 : 0;
+            T.blocked = C.sched.step_order;
             hipLaunchKernelGGL(ks, dim3(blocks), dim3(NTF), 0, C.stream, T);
         }
         HIPCHK(hipGetLastError());
@@ -1128,6 +1141,8 @@ int iblbm_set_schedule(void *p, const iblbm_schedule *in) {
     if ((in->bulk_rows != 1 && in->bulk_rows != 2) || (in->step_rows != 1 && in->step_rows != 2))
         return fail(IBLBM_EINVAL, "set_schedule: rows in flight must be 1 or 2");
     if (in->bulk_waves < 0) return fail(IBLBM_EINVAL, "set_schedule: bulk_waves < 0");
+    if (in->step_order != 0 && in->step_order != 1)
+        return fail(IBLBM_EINVAL, "set_schedule: step_order must be 0 or 1");
     HIPCHK(hipSetDevice(C->device));
     HIPCHK(hipStreamSynchronize(C->stream));
     C->sched = *in;
@@ -1150,11 +1165,13 @@ int iblbm_autotune(void *p, int steps, double *best_ms, int *n_tried) {
     // per CU walking the rows)
     std::vector<iblbm_schedule> cand;
     if (C->n_fast == C->n_ibm)
-        for (int r = 1; r <= 2; ++r) cand.push_back({IBLBM_MODE_FUSED, 256, 1, 0, r});
+        for (int order = 0; order <= 1; ++order)
+            for (int r = 1; r <= 2; ++r) cand.push_back({IBLBM_MODE_FUSED, 256, 1, 0, r, order});
     const int mode2 = C->n_ibm > 0 ? IBLBM_MODE_TWO_STREAMS : IBLBM_MODE_SERIAL;
     for (int bt : {256, 512, 1024})
         for (int r = 1; r <= 2; ++r)
-            for (int wv : {0, 16 * C->n_cu, 32 * C->n_cu}) cand.push_back({mode2, bt, r, wv, 2});
+            for (int wv : {0, 16 * C->n_cu, 32 * C->n_cu})
+                cand.push_back({mode2, bt, r, wv, 2, 0});
     // the trial steps start from a copy of every instance's source populations
     const int n = (int)C->inst.size();
     std::vector<void *> backup(n, nullptr);

@@ -105,6 +105,7 @@ struct LaunchTable {
     int parity;              // source buffer index of this step
     int diag;                // take diagnostic reductions this step
     int rows2;               // bulk wavefronts keep two rows in flight (else one)
+    int blocked;             // fused launch: each bulk wavefront takes one block of rows
 };
 
 // ---------------------------------------------------------------- storage traits

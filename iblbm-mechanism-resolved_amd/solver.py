@@ -122,11 +122,13 @@ class Context:
         check(self.lib.iblbm_get_schedule(self.h, ctypes.byref(sc)), "get_schedule")
         mode = {v: k for k, v in _abi.MODES.items()}[sc.mode]
         return {"mode": mode, "bulk_threads": sc.bulk_threads, "bulk_rows": sc.bulk_rows,
-                "bulk_waves": sc.bulk_waves, "step_rows": sc.step_rows}
+                "bulk_waves": sc.bulk_waves, "step_rows": sc.step_rows,
+                "step_order": "blocked" if sc.step_order else "stride"}
 
     def set_schedule(self, mode="auto", bulk_threads=256, bulk_rows=1, bulk_waves=0,
-                     step_rows=2):
-        sc = _abi.Schedule(_abi.MODES[mode], bulk_threads, bulk_rows, bulk_waves, step_rows)
+                     step_rows=2, step_order="stride"):
+        sc = _abi.Schedule(_abi.MODES[mode], bulk_threads, bulk_rows, bulk_waves, step_rows,
+                           {"stride": 0, "blocked": 1}[step_order])
         check(self.lib.iblbm_set_schedule(self.h, ctypes.byref(sc)), "set_schedule")
 
     def autotune(self, steps: int = 20) -> dict:

@@ -173,6 +173,8 @@ typedef struct {
     int32_t bulk_waves;   /* 0: one row per wavefront (grid covers every row); > 0: that
                            * many wavefronts walk the rows with a grid stride */
     int32_t step_rows;    /* rows in flight of a bulk wavefront in the fused launch: 1 or 2 */
+    int32_t step_order;   /* fused launch: 0 rows by grid stride, 1 one block of adjacent rows
+                           * per bulk wavefront */
 } iblbm_schedule;
 int iblbm_get_schedule(void *ctx, iblbm_schedule *out);
 int iblbm_set_schedule(void *ctx, const iblbm_schedule *in);
