@@ -93,6 +93,10 @@ SIGNATURES = {
     "iblbm_init_flow": (_i, [_VP, _i, _d, _d, _d, _d, ctypes.c_uint64]),
     "iblbm_get_mdf_stats": (_i, [_VP, _i, ctypes.POINTER(MdfStats)]),
     "iblbm_get_diag": (_i, [_VP, _i, ctypes.POINTER(Diag)]),
+    "iblbm_diag_fetch": (_i, [_VP, _i]),
+    "iblbm_get_diag_slot": (_i, [_VP, _i, _i, ctypes.POINTER(Diag)]),
+    "iblbm_event_record": (_i, [_VP, _i]),
+    "iblbm_event_elapsed": (_i, [_VP, _i, _i, _D]),
     "iblbm_download_fib": (_i, [_VP, _i, _D]),
     "iblbm_download_state": (_i, [_VP, _i, _D, _D, _D]),
     "iblbm_download_dfc_force": (_i, [_VP, _i, _D]),

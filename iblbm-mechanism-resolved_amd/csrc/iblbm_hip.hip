@@ -240,10 +240,16 @@ __global__ __launch_bounds__(NTF) void k_step(LaunchTable T) {
     // wavefront moves on as soon as its rows are done)
     const int first = ((int)blockIdx.x - T.n_fast) * (NTF / 64) + (int)(threadIdx.x >> 6);
     const int nw = ((int)gridDim.x - T.n_fast) * (NTF / 64);
-    if (T.blocked) {  // one block of adjacent rows per wavefront (an even count)
+    if (T.blocked) {  // one block of adjacent rows per wavefront
         const int n_rows = T.n_tiles * TILE_Y;
-        const int per = ((n_rows + nw - 1) / nw + 1) & ~1;
-        const int r0 = min(n_rows, first * per), r1 = min(n_rows, r0 + per);
+        int r0, r1;
+        if (T.blocked == 1) {  // every wavefront, block sizes differing by one
+            r0 = (int)((int64_t)first * n_rows / nw);
+            r1 = (int)((int64_t)(first + 1) * n_rows / nw);
+        } else {  // blocks of T.blocked rows; the wavefronts past the last block stay idle
+            r0 = (int)min((int64_t)n_rows, (int64_t)first * T.blocked);
+            r1 = min(n_rows, r0 + T.blocked);
+        }
         if (T.rows2)
             bulk_rows<S, MODEL, true>(T, r0, 1, r1);
         else
@@ -395,6 +401,13 @@ struct Ctx {
     int64_t diag_step = -1;
     std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
     iblbm_schedule sched{IBLBM_MODE_AUTO, 256, 1, 0, 2, 0};
+    // asynchronous diagnostics: two pinned host slots [row partials | per-instance box sums]
+    // filled by copies queued behind the steps, each with its completion event
+    double *h_diag[2] = {nullptr, nullptr};
+    size_t h_diag_cap = 0;
+    hipEvent_t ev_diag[2] = {nullptr, nullptr};
+    int64_t diag_step_slot[2] = {-1, -1};
+    hipEvent_t ev_mark[4] = {nullptr, nullptr, nullptr, nullptr};  // caller's timing marks
 };
 
 void clear_graphs(Ctx &C) {
@@ -622,6 +635,10 @@ int iblbm_ctx_create(int device, void **out) {
     if (hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&C->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&C->ev0) != hipSuccess || hipEventCreate(&C->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&C->ev_diag[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&C->ev_diag[1], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreate(&C->ev_mark[0]) != hipSuccess || hipEventCreate(&C->ev_mark[1]) != hipSuccess ||
+        hipEventCreate(&C->ev_mark[2]) != hipSuccess || hipEventCreate(&C->ev_mark[3]) != hipSuccess ||
         hipEventCreateWithFlags(&C->ev_main, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&C->ev_side, hipEventDisableTiming) != hipSuccess) {
         delete C;
@@ -640,6 +657,11 @@ int iblbm_ctx_destroy(void *p) {
     C->mem.release();
     (void)hipEventDestroy(C->ev0);
     (void)hipEventDestroy(C->ev1);
+    for (int k = 0; k < 2; ++k) {
+        (void)hipEventDestroy(C->ev_diag[k]);
+        if (C->h_diag[k]) (void)hipHostFree(C->h_diag[k]);
+    }
+    for (int k = 0; k < 4; ++k) (void)hipEventDestroy(C->ev_mark[k]);
     (void)hipEventDestroy(C->ev_main);
     (void)hipEventDestroy(C->ev_side);
     (void)hipStreamDestroy(C->side);
@@ -1141,8 +1163,8 @@ int iblbm_set_schedule(void *p, const iblbm_schedule *in) {
     if ((in->bulk_rows != 1 && in->bulk_rows != 2) || (in->step_rows != 1 && in->step_rows != 2))
         return fail(IBLBM_EINVAL, "set_schedule: rows in flight must be 1 or 2");
     if (in->bulk_waves < 0) return fail(IBLBM_EINVAL, "set_schedule: bulk_waves < 0");
-    if (in->step_order != 0 && in->step_order != 1)
-        return fail(IBLBM_EINVAL, "set_schedule: step_order must be 0 or 1");
+    if (in->step_order < 0 || in->step_order > 64)
+        return fail(IBLBM_EINVAL, "set_schedule: step_order must be in [0, 64]");
     HIPCHK(hipSetDevice(C->device));
     HIPCHK(hipStreamSynchronize(C->stream));
     C->sched = *in;
@@ -1165,7 +1187,7 @@ int iblbm_autotune(void *p, int steps, double *best_ms, int *n_tried) {
     // per CU walking the rows)
     std::vector<iblbm_schedule> cand;
     if (C->n_fast == C->n_ibm)
-        for (int order = 0; order <= 1; ++order)
+        for (int order : {0, 1, 6, 8})
             for (int r = 1; r <= 2; ++r) cand.push_back({IBLBM_MODE_FUSED, 256, 1, 0, r, order});
     const int mode2 = C->n_ibm > 0 ? IBLBM_MODE_TWO_STREAMS : IBLBM_MODE_SERIAL;
     for (int bt : {256, 512, 1024})
@@ -1277,6 +1299,84 @@ int iblbm_last_timing(void *p, double *ms, int *steps) {
 int64_t iblbm_step_counter(void *p) {
     Ctx *C = as_ctx(p);
     return C ? C->step : -1;
+}
+
+int iblbm_diag_fetch(void *p, int slot) {
+    Ctx *C = as_ctx(p);
+    if (!C || slot < 0 || slot > 1) return fail(IBLBM_EINVAL, "diag_fetch: bad argument");
+    HIPCHK(hipSetDevice(C->device));
+    int rc = ensure_tables(*C);
+    if (rc) return rc;
+    const size_t rows = (size_t)C->n_tiles * TILE_Y, n = C->inst.size();
+    const size_t need = rows + 8 * n;
+    if (need > C->h_diag_cap) {  // (re)size both slots once no copy into them is pending
+        HIPCHK(hipStreamSynchronize(C->stream));
+        for (int k = 0; k < 2; ++k) {
+            if (C->h_diag[k]) HIPCHK(hipHostFree(C->h_diag[k]));
+            C->h_diag[k] = nullptr;
+            HIPCHK(hipHostMalloc((void **)&C->h_diag[k], sizeof(double) * need,
+                                 hipHostMallocDefault));
+        }
+        C->h_diag_cap = need;
+    }
+    HIPCHK(hipEventSynchronize(C->ev_diag[slot]));  // the slot's previous copy has landed
+    double *h = C->h_diag[slot];
+    if (rows)
+        HIPCHK(hipMemcpyAsync(h, C->d_rho_partial, sizeof(double) * rows,
+                              hipMemcpyDeviceToHost, C->stream));
+    for (size_t k = 0; k < n; ++k) {
+        const Inst &d = C->inst[k].d;
+        if (d.ibm != 0)
+            HIPCHK(hipMemcpyAsync(h + rows + 8 * k, d.diag, sizeof(double) * 5,
+                                  hipMemcpyDeviceToHost, C->stream));
+        else
+            for (int q = 0; q < 5; ++q) h[rows + 8 * k + q] = 0.0;
+    }
+    HIPCHK(hipEventRecord(C->ev_diag[slot], C->stream));
+    C->diag_step_slot[slot] = C->diag_step;
+    return 0;
+}
+
+int iblbm_get_diag_slot(void *p, int id, int slot, iblbm_diag *out) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || !out || slot < 0 || slot > 1 || !C->h_diag[slot])
+        return fail(IBLBM_EINVAL, "get_diag_slot: bad argument or slot never fetched");
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipEventSynchronize(C->ev_diag[slot]));
+    const size_t rows = (size_t)C->n_tiles * TILE_Y;
+    const double *h = C->h_diag[slot];
+    const int b0 = TILE_Y * C->h_tile_prefix[id], b1 = TILE_Y * C->h_tile_prefix[id + 1];
+    double rs = 0.0;
+    for (int b = b0; b < b1; ++b) rs += h[b];  // the order of iblbm_get_diag
+    const double *box = h + rows + 8 * (size_t)id;
+    rs += box[0];
+    out->rho_sum = rs;
+    out->force[0] = box[1];
+    out->force[1] = box[2];
+    out->torque_fib = box[3];
+    out->torque_q = box[4];
+    out->step = C->diag_step_slot[slot];
+    return 0;
+}
+
+int iblbm_event_record(void *p, int k) {
+    Ctx *C = as_ctx(p);
+    if (!C || k < 0 || k > 3) return fail(IBLBM_EINVAL, "event_record: bad argument");
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipEventRecord(C->ev_mark[k], C->stream));
+    return 0;
+}
+
+int iblbm_event_elapsed(void *p, int a, int b, double *ms) {
+    Ctx *C = as_ctx(p);
+    if (!C || a < 0 || a > 3 || b < 0 || b > 3 || !ms)
+        return fail(IBLBM_EINVAL, "event_elapsed: bad argument");
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipEventSynchronize(C->ev_mark[b]));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, C->ev_mark[a], C->ev_mark[b]));
+    *ms = t;
+    return 0;
 }
 
 int iblbm_get_mdf_stats(void *p, int id, iblbm_mdf_stats *out) {

@@ -123,12 +123,14 @@ class Context:
         mode = {v: k for k, v in _abi.MODES.items()}[sc.mode]
         return {"mode": mode, "bulk_threads": sc.bulk_threads, "bulk_rows": sc.bulk_rows,
                 "bulk_waves": sc.bulk_waves, "step_rows": sc.step_rows,
-                "step_order": "blocked" if sc.step_order else "stride"}
+                "step_order": {0: "stride", 1: "blocked"}.get(sc.step_order, sc.step_order)}
 
     def set_schedule(self, mode="auto", bulk_threads=256, bulk_rows=1, bulk_waves=0,
                      step_rows=2, step_order="stride"):
+        """step_order: "stride", "blocked" (one block per wavefront) or a block size k >= 2."""
+        order = {"stride": 0, "blocked": 1}.get(step_order, step_order)
         sc = _abi.Schedule(_abi.MODES[mode], bulk_threads, bulk_rows, bulk_waves, step_rows,
-                           {"stride": 0, "blocked": 1}[step_order])
+                           int(order))
         check(self.lib.iblbm_set_schedule(self.h, ctypes.byref(sc)), "set_schedule")
 
     def autotune(self, steps: int = 20) -> dict:
@@ -180,10 +182,30 @@ class Context:
         check(self.lib.iblbm_download_marker_force(self.h, k, dptr(F)), "download_marker_force")
         return F
 
-    def diag(self, k):
+    def diag(self, k, slot=None):
+        """Diagnostics of the last diagnostic step; with ``slot``, those fetched into that
+        pinned slot by ``diag_fetch`` (waits for that copy only)."""
         d = _abi.Diag()
-        check(self.lib.iblbm_get_diag(self.h, k, ctypes.byref(d)), "get_diag")
+        if slot is None:
+            check(self.lib.iblbm_get_diag(self.h, k, ctypes.byref(d)), "get_diag")
+        else:
+            check(self.lib.iblbm_get_diag_slot(self.h, k, int(slot), ctypes.byref(d)),
+                  "get_diag_slot")
         return d
+
+    def diag_fetch(self, slot):
+        """Queue the copy of the last diagnostic step's sums into pinned slot 0 or 1."""
+        check(self.lib.iblbm_diag_fetch(self.h, int(slot)), "diag_fetch")
+
+    def mark(self, k):
+        """Record timing mark k (0..3) on the context stream."""
+        check(self.lib.iblbm_event_record(self.h, int(k)), "event_record")
+
+    def elapsed_ms(self, a, b):
+        ms = ctypes.c_double()
+        check(self.lib.iblbm_event_elapsed(self.h, int(a), int(b), ctypes.byref(ms)),
+              "event_elapsed")
+        return ms.value
 
     def last_timing(self):
         ms, n = ctypes.c_double(), ctypes.c_int()
@@ -291,9 +313,10 @@ class FixedCylinder:
         from .predictor import Predictor
         return Predictor(self.ctx, self.k)
 
-    def cd_cl(self):
-        """Cd, Cl of the last diagnostic step (``iblbm/diagnostics/core.py:53-113``)."""
-        d = self.ctx.diag(self.k)
+    def cd_cl(self, slot=None):
+        """Cd, Cl of the last diagnostic step (``iblbm/diagnostics/core.py:53-113``), or of
+        the one fetched into ``slot``."""
+        d = self.ctx.diag(self.k, slot)
         ro_mean = d.rho_sum / self.st.nodenums
         denom = ro_mean * self.st.in_u ** 2 * self.st.lattice_r
         return float(abs(d.force[0]) / denom), float(d.force[1] / denom)

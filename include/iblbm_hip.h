@@ -173,8 +173,10 @@ typedef struct {
     int32_t bulk_waves;   /* 0: one row per wavefront (grid covers every row); > 0: that
                            * many wavefronts walk the rows with a grid stride */
     int32_t step_rows;    /* rows in flight of a bulk wavefront in the fused launch: 1 or 2 */
-    int32_t step_order;   /* fused launch: 0 rows by grid stride, 1 one block of adjacent rows
-                           * per bulk wavefront */
+    int32_t step_order;   /* fused launch: 0 rows by grid stride; 1 one block of adjacent rows
+                           * per bulk wavefront, all wavefronts; k >= 2 blocks of k rows (the
+                           * wavefronts past the last block idle, leaving memory bandwidth to
+                           * the windows); at most 64 */
 } iblbm_schedule;
 int iblbm_get_schedule(void *ctx, iblbm_schedule *out);
 int iblbm_set_schedule(void *ctx, const iblbm_schedule *in);
@@ -195,6 +197,16 @@ int iblbm_init_flow(void *ctx, int id, double rho, double ux, double uy, double 
 
 int iblbm_get_mdf_stats(void *ctx, int id, iblbm_mdf_stats *out);
 int iblbm_get_diag(void *ctx, int id, iblbm_diag *out);
+/* Asynchronous diagnostics (no counterpart in the reference, whose loop.py:113-114 reads
+ * them synchronously): iblbm_diag_fetch queues copies of the last diagnostic step's sums into
+ * pinned host slot 0 or 1 behind the steps already queued; iblbm_get_diag_slot waits for that
+ * slot only and returns what iblbm_get_diag would have returned then.  A host loop can queue
+ * the next steps before reading a slot, so the device never waits for the host. */
+int iblbm_diag_fetch(void *ctx, int slot);
+int iblbm_get_diag_slot(void *ctx, int id, int slot, iblbm_diag *out);
+/* Timing marks k = 0..3 on the context stream; elapsed(a, b) waits for b. */
+int iblbm_event_record(void *ctx, int k);
+int iblbm_event_elapsed(void *ctx, int a, int b, double *ms);
 /* Window fib of the last step scattered into a zero (N,2) array. */
 int iblbm_download_fib(void *ctx, int id, double *fib_aos);
 /* Post-stream/boundary/IBM populations fstar, rho and U (velocity-corrected for DF/MDF) of
