@@ -22,14 +22,16 @@ struct CollParams32 {
     float S[9];
 };
 
+// FP32 arithmetic may contract a * b + c into one fused multiply-add (the FP32 path is held
+// to a tolerance against FP64, not to bit equality); terms with a zero lattice-velocity
+// coordinate are left out.
 __device__ inline void moments32(const float *g, bool inc, float &drho, float &ux, float &uy) {
-    float s = 0.f, jx = 0.f, jy = 0.f;
+#pragma clang fp contract(fast)
+    float s = g[0];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        s += g[i];
-        jx += g[i] * EXf(i);
-        jy += g[i] * EYf(i);
-    }
+    for (int i = 1; i < 9; ++i) s += g[i];
+    const float jx = g[1] - g[3] + g[5] - g[6] - g[7] + g[8];
+    const float jy = g[2] - g[4] + g[5] + g[6] - g[7] - g[8];
     drho = s;
     if (inc) {
         ux = jx;
@@ -41,14 +43,21 @@ __device__ inline void moments32(const float *g, bool inc, float &drho, float &u
     }
 }
 
+// Deviation equilibrium; directions 3, 4, 7, 8 reuse the terms of their opposites 1, 2, 5, 6
+// (eU changes sign, eU^2 does not).
 __device__ inline void geq32(float drho, float ux, float uy, bool inc, float *ge) {
-    const float u2 = ux * ux + uy * uy;
+#pragma clang fp contract(fast)
+    const float a = 1.5f * (ux * ux + uy * uy);
     const float rho = 1.f + drho;
+    ge[0] = Wf(0) * (drho + (inc ? -a : rho * -a));
+    const float E[4] = {ux, uy, ux + uy, uy - ux};
+    const int PI[4] = {1, 2, 5, 6}, MI[4] = {3, 4, 7, 8};
 #pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        const float eu = EXf(i) * ux + EYf(i) * uy;
-        const float X = 3.f * eu + 4.5f * eu * eu - 1.5f * u2;
-        ge[i] = Wf(i) * (drho + (inc ? X : rho * X));
+    for (int k = 0; k < 4; ++k) {
+        const float pe = 3.f * E[k], t = 4.5f * E[k] * E[k] - a;
+        const float Xp = pe + t, Xm = t - pe;
+        ge[PI[k]] = Wf(PI[k]) * (drho + (inc ? Xp : rho * Xp));
+        ge[MI[k]] = Wf(MI[k]) * (drho + (inc ? Xm : rho * Xm));
     }
 }
 
@@ -56,6 +65,7 @@ __device__ inline void geq32(float drho, float ux, float uy, bool inc, float *ge
 template <int MODEL>
 __device__ inline void collide32(const CollParams32 &p, const float *g, float drho, float ux,
                                  float uy, bool inc, float *out) {
+#pragma clang fp contract(fast)
     if constexpr (MODEL == 0) {
         float ge[9];
         geq32(drho, ux, uy, inc, ge);
@@ -64,12 +74,16 @@ __device__ inline void collide32(const CollParams32 &p, const float *g, float dr
     } else if constexpr (MODEL == 1) {
         float ge[9];
         geq32(drho, ux, uy, inc, ge);
+        out[0] = g[0] - p.s_plus * (g[0] - ge[0]);
+        const int PI[4] = {1, 2, 5, 6}, MI[4] = {3, 4, 7, 8};
 #pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            const int o = OPP[i];
+        for (int k = 0; k < 4; ++k) {  // the pair (i, OPP[i]) shares gp - ep, negates gm - em
+            const int i = PI[k], o = MI[k];
             const float gp = 0.5f * (g[i] + g[o]), gm = 0.5f * (g[i] - g[o]);
             const float ep = 0.5f * (ge[i] + ge[o]), em = 0.5f * (ge[i] - ge[o]);
-            out[i] = g[i] - (p.s_plus * (gp - ep) + p.s_minus * (gm - em));
+            const float A = p.s_plus * (gp - ep), B = p.s_minus * (gm - em);
+            out[i] = g[i] - (A + B);
+            out[o] = g[o] - (A - B);
         }
     } else {
         // central moments of g about u

@@ -21,34 +21,92 @@ using namespace iblbm;
 
 // ------------------------------------------------------------------ kernels
 
+// The instance that a wavefront's current rows belong to, with every field the bulk path needs
+// to address a row, held in scalar registers for as long as consecutive rows stay in that
+// instance.  Rows of one wavefront are mostly consecutive (one block, or a stride through one
+// lattice), so the instance lookup and the descriptor reads happen about once per wavefront
+// instead of once per row, and nothing on a row's address chain waits for a scalar load.
+struct RowRun {
+    const Inst *I;
+    int tb_lo, tb_hi;  // the instance's bulk tiles [tb_lo, tb_hi)
+    int tiles_x, nx, ny, ibm, bx0, by0, bw, bh, pitch;
+    int64_t qs;
+    const int *tindex;
+    const void *A;  // source planes of this step
+    void *B;        // destination planes
+};
+
+// Reads are scalar loads through the constant address space: the tables are read-only
+// during a step.
+__device__ __attribute__((always_inline)) void row_run_load(const LaunchTable &T, int tb,
+                                                           RowRun &R) {
+    typedef __attribute__((address_space(4))) const int CInt;
+    typedef __attribute__((address_space(4))) Inst CInst;
+    const CInt *prefix = (const CInt *)T.tile_prefix;
+    int lo = 0, hi = T.n_inst - 1;  // the last instance whose first tile is <= tb
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= tb)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int inst = __builtin_amdgcn_readfirstlane(lo);
+    const CInst *C = (const CInst *)T.insts + inst;
+    R.I = (const Inst *)C;
+    R.tb_lo = prefix[inst];
+    R.tb_hi = prefix[inst + 1];
+    R.tiles_x = C->tiles_x;
+    R.nx = C->nx;
+    R.ny = C->ny;
+    R.ibm = C->ibm;
+    R.bx0 = C->bx0;
+    R.by0 = C->by0;
+    R.bw = C->bw;
+    R.bh = C->bh;
+    R.pitch = C->pitch;
+    R.qs = C->qs;
+    R.tindex = C->s.tindex;
+    R.A = C->f[T.parity];
+    R.B = C->f[T.parity ^ 1];
+}
+
 // Bulk lattice node in FP64 (reference arithmetic, bit-exact), in two halves: the pull
 // (loads) and the moments + collision + store, so that a thread can have several nodes' loads
-// in flight.
+// in flight.  The second half stores only when `own` holds: on an interior row every lane runs
+// it and only the store is masked, so no branch can leave a load outstanding at the end of a
+// row.  (gfx9 counts loads and stores in one vmcnt: a load left outstanding on some path makes
+// the compiler wait for vmcnt(0) before the next row's loads, i.e. for the previous row's
+// stores to be acknowledged.)
 __device__ __attribute__((always_inline)) void bulk_pull(const Inst &I, int parity, int i, int j,
                                                         double *fs) {
     fstar_at(buf<double>(I, parity), I, i, j, fs);
 }
 template <int MODEL>
-__device__ __attribute__((always_inline)) double bulk_finish(const Inst &I, int parity, int i,
-                                                            int j, double *fs) {
-    double *__restrict__ B = buf<double>(I, parity ^ 1);
-    double fe[9], out[9];
-    double r, u0, u1;
+__device__ __attribute__((always_inline)) double bulk_finish(const Inst &I, const RowRun &R, int i,
+                                                            int j, double *fs, bool own = true) {
+    double *__restrict__ B = (double *)R.B;
+    double out[9];
+    double r;
     const bool inc = I.inc != 0;
-    macro9(fs, inc, r, u0, u1);
-    if constexpr (MODEL == 0) {
-        feq9(r, u0, u1, inc, fe);
-        collide_bgk<false>(I.cp, fs, fe, u0, u1, 0.0, 0.0, out);
-    } else if constexpr (MODEL == 1) {
-        feq9(r, u0, u1, inc, fe);
-        collide_trt<false>(I.cp, fs, fe, u0, u1, 0.0, 0.0, out);
-    } else if constexpr (MODEL == 2) {
-        collide_cm<false>(I.cp, fs, u0, u1, 0.0, 0.0, out);
+    if constexpr (MODEL == 0 || MODEL == 1) {
+        r = bulk_collide64<MODEL>(I.cp, fs, inc, out);
     } else {
-        if (I.cp.model != 2) feq9(r, u0, u1, inc, fe);
-        collide<false>(I.cp, fs, fe, u0, u1, 0.0, 0.0, out);
+        const int model = MODEL == 2 ? 2 : I.cp.model;
+        if (model == 0) {
+            r = bulk_collide64<0>(I.cp, fs, inc, out);
+        } else if (model == 1) {
+            r = bulk_collide64<1>(I.cp, fs, inc, out);
+        } else {
+            double u0, u1;
+            macro9(fs, inc, r, u0, u1);
+            collide_cm<false>(I.cp, fs, u0, u1, 0.0, 0.0, out);
+        }
     }
-    store9(B, I, i, j, out);
+    if (own) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) B[q * R.qs + (int64_t)j * R.pitch + i] = out[q];
+    }
     return r;
 }
 
@@ -58,9 +116,9 @@ __device__ __attribute__((always_inline)) void bulk_pull(const Inst &I, int pari
     fstar_dev32(buf<float>(I, parity), I, i, j, g);
 }
 template <int MODEL>
-__device__ __attribute__((always_inline)) double bulk_finish(const Inst &I, int parity, int i,
-                                                            int j, float *g) {
-    float *__restrict__ B = buf<float>(I, parity ^ 1);
+__device__ __attribute__((always_inline)) double bulk_finish(const Inst &I, const RowRun &R, int i,
+                                                            int j, float *g, bool own = true) {
+    float *__restrict__ B = (float *)R.B;
     float out[9];
     float drho, ux, uy;
     const bool inc = I.inc != 0;
@@ -75,73 +133,54 @@ __device__ __attribute__((always_inline)) double bulk_finish(const Inst &I, int 
         else
             collide32<2>(I.cp32, g, drho, ux, uy, inc, out);
     }
+    if (own) {
 #pragma unroll
-    for (int q = 0; q < 9; ++q) B[q * I.qs + (int64_t)j * I.pitch + i] = out[q];
+        for (int q = 0; q < 9; ++q) B[q * R.qs + (int64_t)j * R.pitch + i] = out[q];
+    }
     return 1.0 + (double)drho;
 }
 
 // Pull of a node whose 8 neighbours all lie inside the grid (no closure, no branches).
-__device__ __attribute__((always_inline)) void bulk_pull_interior(const Inst &I, int parity, int i,
-                                                                 int j, double *fs) {
-    pull_interior(buf<double>(I, parity), I, i, j, fs);
-}
-__device__ __attribute__((always_inline)) void bulk_pull_interior(const Inst &I, int parity, int i,
-                                                                 int j, float *g) {
-    const float *__restrict__ A = buf<float>(I, parity);
+__device__ __attribute__((always_inline)) void bulk_pull_interior(const RowRun &R, int i, int j,
+                                                                 double *fs) {
+    const double *__restrict__ A = (const double *)R.A;
 #pragma unroll
-    for (int q = 0; q < 9; ++q) g[q] = A[q * I.qs + (int64_t)(j - EYi[q]) * I.pitch + (i - EXi[q])];
+    for (int q = 0; q < 9; ++q) fs[q] = A[q * R.qs + (int64_t)(j - EYi[q]) * R.pitch + (i - EXi[q])];
 }
-
-template <typename S, int MODEL>
-__device__ __attribute__((always_inline)) double bulk_node(const Inst &I, int parity, int i,
-                                                          int j) {
-    S v[9];
-    bulk_pull(I, parity, i, j, v);
-    return bulk_finish<MODEL>(I, parity, i, j, v);
+__device__ __attribute__((always_inline)) void bulk_pull_interior(const RowRun &R, int i, int j,
+                                                                 float *g) {
+    const float *__restrict__ A = (const float *)R.A;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) g[q] = A[q * R.qs + (int64_t)(j - EYi[q]) * R.pitch + (i - EXi[q])];
 }
 
 // The bulk work is a list of rows: row r is row r % TILE_Y of tile r / TILE_Y (TILE_X nodes,
-// one per lane of a wavefront).  A lane's node: instance, coordinates, and whether the node is
-// the bulk's (inside the grid, not touched by a marker stencil).
+// one per lane of a wavefront).  A lane's node: coordinates, and whether the node is the
+// bulk's (inside the grid, not touched by a marker stencil).
 struct TileNode {
-    const Inst *I;
     int i, j;
     bool mine;
     bool interior;  // (uniform) every node of the row and all its neighbours are in the grid
 };
-// row is uniform over the wavefront: the instance lookup and the descriptor reads are scalar
-// loads through the constant address space (the tables are read-only during a step).
-__device__ __attribute__((always_inline)) TileNode row_node(const LaunchTable &T, int row,
-                                                           int lane) {
-    typedef __attribute__((address_space(4))) const int CInt;
-    typedef __attribute__((address_space(4))) Inst CInst;
+// row is uniform over the wavefront; R is reloaded only when the row leaves R's instance.
+__device__ __attribute__((always_inline)) TileNode row_node(const LaunchTable &T, RowRun &R,
+                                                           int row, int lane) {
     row = __builtin_amdgcn_readfirstlane(row);
     const int tb = row / TILE_Y;
-    const CInt *prefix = (const CInt *)T.tile_prefix;
-    int lo = 0, hi = T.n_inst - 1;  // the last instance whose first tile is <= tb
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (prefix[mid] <= tb)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    const int inst = __builtin_amdgcn_readfirstlane(lo);
-    const Inst &I = *(const Inst *)((const CInst *)T.insts + inst);
-    const int lt = tb - prefix[inst];
-    const int tx = lt % I.tiles_x, ty = lt / I.tiles_x;
+    if (tb < R.tb_lo || tb >= R.tb_hi) row_run_load(T, tb, R);
+    const int lt = tb - R.tb_lo;
+    const int tx = lt % R.tiles_x, ty = lt / R.tiles_x;
     TileNode n;
-    n.I = &I;
     n.i = tx * TILE_X + lane;
     n.j = ty * TILE_Y + row % TILE_Y;
-    const bool in_grid = n.i < I.nx && n.j < I.ny;
-    n.interior = tx * TILE_X >= 1 && tx * TILE_X + TILE_X <= I.nx - 1 && n.j >= 1 &&
-                 n.j <= I.ny - 2;
+    const bool in_grid = n.i < R.nx && n.j < R.ny;
+    n.interior = tx * TILE_X >= 1 && tx * TILE_X + TILE_X <= R.nx - 1 && n.j >= 1 &&
+                 n.j <= R.ny - 2;
     // nodes a marker stencil touches belong to the window workgroup
-    const int ibm = I.ibm, bx0 = I.bx0, by0 = I.by0, bw = I.bw, bh = I.bh;
-    const bool in_box = ibm != 0 & n.i >= bx0 & n.i < bx0 + bw & n.j >= by0 & n.j < by0 + bh;
+    const bool in_box = R.ibm != 0 & n.i >= R.bx0 & n.i < R.bx0 + R.bw & n.j >= R.by0 &
+                        n.j < R.by0 + R.bh;
     bool touched = false;
-    if (in_box) touched = I.s.tindex[(n.j - by0) * bw + (n.i - bx0)] >= 0;
+    if (in_box) touched = R.tindex[(n.j - R.by0) * R.bw + (n.i - R.bx0)] >= 0;
     n.mine = in_grid && !touched;
     return n;
 }
@@ -155,25 +194,39 @@ __device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, i
                                                         int stop = -1) {
     const int lane = (int)threadIdx.x & 63;
     const int n_rows = stop >= 0 ? stop : T.n_tiles * TILE_Y;
+    RowRun R;  // empty tile range: the first row loads it
+    R.tb_lo = 1;
+    R.tb_hi = 0;
     if constexpr (TWO) {
+        RowRun R1;
+        R1.tb_lo = 1;
+        R1.tb_hi = 0;
         for (int r0 = first; r0 < n_rows; r0 += 2 * nw) {
             const bool v1 = r0 + nw < n_rows;
             const int r1 = v1 ? r0 + nw : r0;  // a missing second row repeats the first, unused
-            const TileNode n0 = row_node(T, r0, lane);
-            TileNode n1 = row_node(T, r1, lane);
+            const TileNode n0 = row_node(T, R, r0, lane);
+            TileNode n1 = row_node(T, R1, r1, lane);
             n1.mine = n1.mine && v1;
             S a0[9], a1[9];
-            if (n0.interior && n1.interior) {  // uniform: both rows' 18 loads in one batch
-                bulk_pull_interior(*n0.I, T.parity, n0.i, n0.j, a0);
-                bulk_pull_interior(*n1.I, T.parity, n1.i, n1.j, a1);
-                __builtin_amdgcn_sched_barrier(0);
-            } else {
-                if (n0.mine) bulk_pull(*n0.I, T.parity, n0.i, n0.j, a0);
-                if (n1.mine) bulk_pull(*n1.I, T.parity, n1.i, n1.j, a1);
-            }
             double rho0 = 0.0, rho1 = 0.0;
-            if (n0.mine) rho0 = bulk_finish<MODEL>(*n0.I, T.parity, n0.i, n0.j, a0);
-            if (n1.mine) rho1 = bulk_finish<MODEL>(*n1.I, T.parity, n1.i, n1.j, a1);
+            if (n0.interior && n1.interior) {  // uniform: both rows' 18 loads in one batch
+                bulk_pull_interior(R, n0.i, n0.j, a0);
+                bulk_pull_interior(R1, n1.i, n1.j, a1);
+                __builtin_amdgcn_sched_barrier(0);
+                const double r0v = bulk_finish<MODEL>(*R.I, R, n0.i, n0.j, a0, n0.mine);
+                const double r1v = bulk_finish<MODEL>(*R1.I, R1, n1.i, n1.j, a1, n1.mine);
+                rho0 = n0.mine ? r0v : 0.0;
+                rho1 = n1.mine ? r1v : 0.0;
+            } else {
+                if (n0.mine) {
+                    bulk_pull(*R.I, T.parity, n0.i, n0.j, a0);
+                    rho0 = bulk_finish<MODEL>(*R.I, R, n0.i, n0.j, a0);
+                }
+                if (n1.mine) {
+                    bulk_pull(*R1.I, T.parity, n1.i, n1.j, a1);
+                    rho1 = bulk_finish<MODEL>(*R1.I, R1, n1.i, n1.j, a1);
+                }
+            }
             if (T.diag) {
                 const double w0 = wave_sum(rho0), w1 = wave_sum(rho1);
                 if (lane == 0) {
@@ -184,14 +237,17 @@ __device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, i
         }
     } else {
         for (int r0 = first; r0 < n_rows; r0 += nw) {
-            const TileNode n0 = row_node(T, r0, lane);
+            const TileNode n0 = row_node(T, R, r0, lane);
             S a0[9];
-            if (n0.interior)
-                bulk_pull_interior(*n0.I, T.parity, n0.i, n0.j, a0);
-            else if (n0.mine)
-                bulk_pull(*n0.I, T.parity, n0.i, n0.j, a0);
             double rho0 = 0.0;
-            if (n0.mine) rho0 = bulk_finish<MODEL>(*n0.I, T.parity, n0.i, n0.j, a0);
+            if (n0.interior) {
+                bulk_pull_interior(R, n0.i, n0.j, a0);
+                const double r = bulk_finish<MODEL>(*R.I, R, n0.i, n0.j, a0, n0.mine);
+                rho0 = n0.mine ? r : 0.0;
+            } else if (n0.mine) {
+                bulk_pull(*R.I, T.parity, n0.i, n0.j, a0);
+                rho0 = bulk_finish<MODEL>(*R.I, R, n0.i, n0.j, a0);
+            }
             if (T.diag) {
                 const double w0 = wave_sum(rho0);
                 if (lane == 0) T.rho_partial[r0] = w0;

@@ -241,6 +241,72 @@ __host__ __device__ inline void collide_cm(const CollParams &p, const double *fs
     out[8] = 0.25 * (-k5 - k6 + k7 + k8);
 }
 
+// Unforced bulk node update: moments, equilibrium and BGK (MODEL 0) or TRT (MODEL 1)
+// collision, equal bit for bit to macro9 + feq9 + collide_bgk / collide_trt<false> for finite,
+// normal populations, with about half of their arithmetic.  Two exact identities do it:
+//  - a term with a zero lattice-velocity coordinate (0 * x) is an exact zero, and adding an
+//    exact zero changes nothing but the sign of an exactly-zero sum, which every later step
+//    adds to a nonzero term;
+//  - the opposite directions i and OPP[i] share their sums and negate their differences
+//    (a + b = b + a and a - b = -(b - a) exactly in round-to-nearest; so do 3 * (-e) = -(3 e)
+//    and (4.5 (-e)) (-e) = (4.5 e) e), so each of the 4 pairs is evaluated once.  The rest
+//    direction needs no pair: 0.5 (f0 + f0) = f0 exactly.
+// Returns rho.
+template <int MODEL>
+__host__ __device__ inline double bulk_collide64(const CollParams &p, const double *f, bool inc,
+                                                 double *out) {
+    // macroscopic.py:35-48, sequential over i = 0..8, zero terms left out
+    double s = f[0];
+#pragma unroll
+    for (int i = 1; i < 9; ++i) s += f[i];
+    double jx = f[1] - f[3];
+    jx = jx + f[5];
+    jx = jx - f[6];
+    jx = jx - f[7];
+    jx = jx + f[8];
+    double jy = f[2] - f[4];
+    jy = jy + f[5];
+    jy = jy + f[6];
+    jy = jy - f[7];
+    jy = jy - f[8];
+    const double u0 = inc ? jx : jx / s, u1 = inc ? jy : jy / s;
+    // equilibrium.py:33-41: (c + 3 eU + 4.5 eU eU - 1.5 u2) scaled by rho w (c = 1) or by w
+    // (incompressible, c = rho)
+    const double a = 1.5 * (u0 * u0 + u1 * u1);
+    const double c = inc ? s : 1.0;
+    const double rw0 = inc ? W(0) : s * W(0), rw1 = inc ? W(1) : s * W(1),
+                 rw5 = inc ? W(5) : s * W(5);
+    double e[9];
+    e[0] = rw0 * (c - a);
+    // eU of directions 1, 2, 5, 6; their opposites 3, 4, 7, 8 have -eU
+    const double E[4] = {u0, u1, u0 + u1, u1 - u0};
+    const int PI[4] = {1, 2, 5, 6}, MI[4] = {3, 4, 7, 8};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double pe = 3.0 * E[k];
+        const double t = 4.5 * E[k] * E[k];
+        const double rw = k < 2 ? rw1 : rw5;
+        e[PI[k]] = rw * (c + pe + t - a);
+        e[MI[k]] = rw * (c - pe + t - a);
+    }
+    if constexpr (MODEL == 0) {  // bgk.py:25-40
+#pragma unroll
+        for (int i = 0; i < 9; ++i) out[i] = f[i] - p.inv_tau * (f[i] - e[i]);
+    } else {  // trt.py:38-67
+        out[0] = f[0] - p.s_plus * (f[0] - e[0]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = PI[k], o = MI[k];
+            const double fp = 0.5 * (f[i] + f[o]), fm = 0.5 * (f[i] - f[o]);
+            const double ep = 0.5 * (e[i] + e[o]), em = 0.5 * (e[i] - e[o]);
+            const double A = p.s_plus * (fp - ep), B = p.s_minus * (fm - em);
+            out[i] = f[i] - (A + B);
+            out[o] = f[o] - (A - B);
+        }
+    }
+    return s;
+}
+
 template <bool FORCED>
 __host__ __device__ inline void collide(const CollParams &p, const double *fs, const double *fe,
                                         double u0, double u1, double fx, double fy,
