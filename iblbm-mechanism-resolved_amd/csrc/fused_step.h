@@ -183,8 +183,11 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
     const int min_iter = max(1, min(I.min_iter, n_iter));
     const double tol = I.tol;
 
-    // Every global load of the prologue is issued before the first wait: the marker stencils
-    // (for LDS), a marker's desired velocity, a node's gather list, the box slots of P1.
+    // Every global load of the prologue is issued before the first wait: a node's box slot
+    // (first: P1's pulls wait for it alone), the marker stencils (for LDS), a marker's desired
+    // velocity, a node's gather list.
+    int sl = 0;
+    if (hn) sl = S.tslot[n];
     constexpr int SU = (FAST_K * FAST_LB + NTF - 1) / NTF;  // stencil entries per thread
     int st_i[SU];
     double st_w[SU];
@@ -231,10 +234,9 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
         udx = I.ud[2 * t];
         udy = I.ud[2 * t + 1];
     }
-    int sl = 0, kk[FAST_G];
+    int kk[FAST_G];
     double ww[FAST_G];
     if (hn) {
-        sl = S.tslot[n];
 #pragma unroll
         for (int e = 0; e < FAST_G; ++e) {
             kk[e] = S.pk[e * FAST_T + n];
@@ -242,50 +244,31 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
         }
     }
 
-    // P1 (all threads): box slots, two per thread per pass; a slot's touched index and its 9
-    // pull values are loaded together, touched slots keep theirs
+    // P1: each node thread pulls its own touched node (its box slot came with the prologue's
+    // loads), and the stencil tables go to LDS
     const ST *__restrict__ Asrc = reinterpret_cast<const ST *>(parity ? I.f[1] : I.f[0]);
-    const int nbox = I.bw * I.bh;
     const bool interior = I.bx0 >= 1 && I.by0 >= 1 && I.bx0 + I.bw <= I.nx - 1 &&
                           I.by0 + I.bh <= I.ny - 1;
-    constexpr int PU = 2;
-    for (int base = 0; base < nbox; base += PU * NTF) {
-        int tx[PU];
-        double fs[PU][9];
+    if (hn) {
+        const int i = I.bx0 + sl % I.bw, j = I.by0 + sl / I.bw;
+        double fsn[9];
+        if (interior)
+            pull_interior(Asrc, I, i, j, fsn);
+        else
+            fstar_at(Asrc, I, i, j, fsn);
+        double r, u0, u1;
 #pragma unroll
-        for (int u = 0; u < PU; ++u) {
-            const int s0 = base + u * NTF + t;
-            tx[u] = -1;
-            if (s0 < nbox) {
-                tx[u] = S.tindex[s0];
-                const int i = I.bx0 + s0 % I.bw, j = I.by0 + s0 / I.bw;
-                if (interior)
-                    pull_interior(Asrc, I, i, j, fs[u]);
-                else
-                    fstar_at(Asrc, I, i, j, fs[u]);
-            }
-        }
-        if (base == 0) {
+        for (int q = 0; q < 9; ++q) sh.fs[q * FAST_T + n] = fsn[q];
+        macro9(fsn, inc, r, u0, u1);
+        uw[n] = make_double2(u0, u1);
+        sh.rot[n] = r;
+    }
 #pragma unroll
-            for (int u = 0; u < SU; ++u) {
-                const int e = t + u * NTF;
-                if (e < K * Lb) {
-                    sh.mti[e] = st_i[u];
-                    sh.mw[e] = st_w[u];
-                }
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < PU; ++u) {
-            const int m = tx[u];
-            if (m >= 0) {
-                double r, u0, u1;
-#pragma unroll
-                for (int q = 0; q < 9; ++q) sh.fs[q * FAST_T + m] = fs[u][q];
-                macro9(fs[u], inc, r, u0, u1);
-                uw[m] = make_double2(u0, u1);
-                sh.rot[m] = r;
-            }
+    for (int u = 0; u < SU; ++u) {
+        const int e = t + u * NTF;
+        if (e < K * Lb) {
+            sh.mti[e] = st_i[u];
+            sh.mw[e] = st_w[u];
         }
     }
     if (t < 2) rbits[t] = 0ull;
