@@ -47,7 +47,7 @@ PERTURB = (1234, 1e-3)
 # FETCH_SIZE / WRITE_SIZE (kB per dispatch), each scaled by what the counter reads for a
 # 1 GiB device copy (FETCH_SIZE reads 0.5x, WRITE_SIZE 1.0x of the bytes moved); the summary
 # is profiles/k_step_fp64_trt_mdf_p4_1601x641_traffic.md.
-TRAFFIC_FP64_C2 = {"nodes": 1026241, "bytes": (2 * 38470.41 + 72435.52) * 1024,
+TRAFFIC_FP64_C2 = {"nodes": 1026241, "bytes": (2 * 37856.9 + 72436.5) * 1024,
                    "source": "profiles/k_step_fp64_trt_mdf_p4_1601x641_traffic.md"}
 
 
@@ -96,15 +96,21 @@ def main():
     ap.add_argument("--serial", action="store_true",
                     help="run the window and bulk kernels in order on one stream")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--schedule", default=None,
+                    help="pin the launch schedule instead of autotuning: mode,bulk_threads,"
+                         "bulk_rows,bulk_waves,step_rows,step_order (e.g. "
+                         "fused,256,1,0,1,blocked), for profiling one schedule's kernels")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch schedule instead of the autotuner's")
     ap.add_argument("--surrogate-steps", type=int, default=2000,
                     help="steps of the plain-vs-learned-predictor MDF comparison (0: skip)")
     ap.add_argument("--variant-steps", type=int, default=2000,
                     help="steps of the FP32-deviation comparison of an fp64 run (0: skip)")
-    ap.add_argument("--workload", choices=["cylinder", "matrix72"], default="cylinder",
+    ap.add_argument("--workload", choices=["cylinder", "matrix72", "copy"], default="cylinder",
                     help="cylinder: Re=100 1601x641 (default); matrix72: the batched 72-case "
-                         "Re x method x kernel x collision matrix (about 300 M nodes)")
+                         "Re x method x kernel x collision matrix (about 300 M nodes); copy: "
+                         "a 1 GiB FP64 device copy, five times (the reference of the HBM "
+                         "traffic counters and of a plain copy's bandwidth)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the lattice fits one MI355X and this build has no "
@@ -113,6 +119,9 @@ def main():
 
     if args.workload == "matrix72":
         print(json.dumps(matrix_bench(args)), flush=True)
+        return
+    if args.workload == "copy":
+        print(json.dumps(copy_bench()), flush=True)
         return
     from iblbm_amd.solver import FixedCylinder
     cfg = workload_cfg()
@@ -129,7 +138,16 @@ def main():
         sim.advance(ci, diag_every=ci, graph=graph)
         sim.cd_cl()
     tuned = None
-    if not args.no_tune and not args.serial:  # launch schedule from the autotuner
+    if args.schedule:
+        f = args.schedule.split(",")
+        order = f[5] if f[5] in ("stride", "blocked") else int(f[5])
+        sim.ctx.set_schedule(mode=f[0], bulk_threads=int(f[1]), bulk_rows=int(f[2]),
+                             bulk_waves=int(f[3]), step_rows=int(f[4]), step_order=order)
+        tuned = sim.ctx.schedule()
+        mode = sim.ctx.step_mode()
+        sim.advance(ci, diag_every=ci, graph=graph)  # the chunk's graph, outside the timing
+        sim.cd_cl()
+    elif not args.no_tune and not args.serial:  # launch schedule from the autotuner
         tuned = sim.ctx.autotune(60)
         mode = sim.ctx.step_mode()
         sim.advance(ci, diag_every=ci, graph=graph)  # the chunk's graph, outside the timing
@@ -196,6 +214,24 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def copy_bench(reps=5):
+    """A 1 GiB FP64 device-to-device copy, `reps` times: the bytes the HBM counters must
+    report for it are known exactly, which calibrates them, and its rate is a plain copy's."""
+    import torch
+    n = 1 << 27
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    return {"metric": "GB/s (1 GiB FP64 device copy, read + write)", "value": round(2 * 8 * n / dt / 1e9, 1),
+            "unit": "GB/s", "reps": reps, "ms_per_copy": round(dt * 1e3, 4)}
 
 
 def matrix_bench(args):
