@@ -352,6 +352,48 @@ __global__ __launch_bounds__(NTI) void k_tables(Inst *Ip) {
     ibm_tables_role(*Ip, redi);
 }
 
+// Batched marker update: workgroup b copies instance ids[b]'s markers out of the staging
+// buffer [Lx (total) | Ly (total) | ud (2 total)] at marker offset off[b].
+__global__ __launch_bounds__(256) void k_scatter_markers(const Inst *insts, const int *ids,
+                                                         const int *off, const double *stage,
+                                                         int total) {
+    const Inst &I = insts[ids[blockIdx.x]];
+    const int o = off[blockIdx.x], lb = off[blockIdx.x + 1] - o;
+    double *Lx = const_cast<double *>(I.Lx), *Ly = const_cast<double *>(I.Ly);
+    double *ud = const_cast<double *>(I.ud);
+    for (int k = threadIdx.x; k < lb; k += blockDim.x) {
+        Lx[k] = stage[o + k];
+        Ly[k] = stage[total + o + k];
+        ud[2 * k] = stage[2 * total + 2 * (o + k)];
+        ud[2 * k + 1] = stage[2 * total + 2 * (o + k) + 1];
+    }
+}
+
+// Window tables of several instances, one workgroup each; the touched-node count and the
+// path flag of each go to nt_fast[2b], nt_fast[2b + 1] for the host descriptors.
+__global__ __launch_bounds__(NTI) void k_tables_batch(Inst *insts, const int *ids,
+                                                      int *nt_fast) {
+    __shared__ int redi[NTI];
+    Inst &I = insts[ids[blockIdx.x]];
+    ibm_tables_role(I, redi);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        nt_fast[2 * blockIdx.x] = I.nT;
+        nt_fast[2 * blockIdx.x + 1] = I.fast;
+    }
+}
+
+// Batched force read: workgroup b copies the (Lb, 2) marker forces of instance ids[b] (the
+// DFC force q, or the last pass's marker forces of a several-particle DF window) to out at
+// marker offset off[b].
+__global__ __launch_bounds__(256) void k_gather_forces(const Inst *insts, const int *ids,
+                                                       const int *off, double *out) {
+    const Inst &I = insts[ids[blockIdx.x]];
+    const double *src = I.ibm == IBLBM_IBM_DFC ? I.s.q : I.s.mf;
+    const int o = off[blockIdx.x], n2 = 2 * (off[blockIdx.x + 1] - o);
+    for (int k = threadIdx.x; k < n2; k += blockDim.x) out[2 * o + k] = src[k];
+}
+
 // state.fstar / ro / U of the last step (source buffer A = that step's input); touched nodes
 // come from the IBM scratch (DFC-corrected fstar, velocity-corrected U).
 template <typename S>
@@ -464,6 +506,11 @@ struct Ctx {
     hipEvent_t ev_diag[2] = {nullptr, nullptr};
     int64_t diag_step_slot[2] = {-1, -1};
     hipEvent_t ev_mark[4] = {nullptr, nullptr, nullptr, nullptr};  // caller's timing marks
+    // batched marker updates and force reads of many instances: pinned host staging and its
+    // device twin (doubles: markers or forces; ints: instance ids, offsets, nT/fast pairs)
+    double *h_bd = nullptr, *d_bd = nullptr;
+    int *h_bi = nullptr, *d_bi = nullptr;
+    size_t bd_cap = 0, bi_cap = 0;
 };
 
 void clear_graphs(Ctx &C) {
@@ -718,6 +765,8 @@ int iblbm_ctx_destroy(void *p) {
         if (C->h_diag[k]) (void)hipHostFree(C->h_diag[k]);
     }
     for (int k = 0; k < 4; ++k) (void)hipEventDestroy(C->ev_mark[k]);
+    if (C->h_bd) (void)hipHostFree(C->h_bd);
+    if (C->h_bi) (void)hipHostFree(C->h_bi);
     (void)hipEventDestroy(C->ev_main);
     (void)hipEventDestroy(C->ev_side);
     (void)hipStreamDestroy(C->side);
@@ -903,6 +952,177 @@ int iblbm_set_markers(void *p, int id, const double *Lx, const double *Ly, const
         int rc = ensure_tables(*C);
         if (rc) return rc;
     }
+    return 0;
+}
+
+namespace {
+
+// Pinned host and device staging of the batched calls, grown geometrically.  Every batched
+// call ends with a stream synchronisation, so no copy from or into it is pending here.
+int grow_batch(Ctx &C, size_t nd, size_t ni) {
+    if (nd > C.bd_cap) {
+        const size_t cap = nd < 4096 ? 4096 : 2 * nd;
+        if (C.h_bd) HIPCHK(hipHostFree(C.h_bd));
+        C.h_bd = nullptr;
+        HIPCHK(hipHostMalloc((void **)&C.h_bd, sizeof(double) * cap, hipHostMallocDefault));
+        void *q;
+        if (C.mem.alloc(&q, sizeof(double) * cap) != hipSuccess)
+            return fail(IBLBM_ENOMEM, "batch staging does not fit");
+        C.d_bd = (double *)q;
+        C.bd_cap = cap;
+    }
+    if (ni > C.bi_cap) {
+        const size_t cap = ni < 1024 ? 1024 : 2 * ni;
+        if (C.h_bi) HIPCHK(hipHostFree(C.h_bi));
+        C.h_bi = nullptr;
+        HIPCHK(hipHostMalloc((void **)&C.h_bi, sizeof(int) * cap, hipHostMallocDefault));
+        void *q;
+        if (C.mem.alloc(&q, sizeof(int) * cap) != hipSuccess)
+            return fail(IBLBM_ENOMEM, "batch staging does not fit");
+        C.d_bi = (int *)q;
+        C.bi_cap = cap;
+    }
+    return 0;
+}
+
+}  // namespace
+
+int iblbm_set_markers_batch(void *p, int n, const int *ids, const int *lb, const double *Lx,
+                            const double *Ly, const double *ud, const double *larea,
+                            const double *center) {
+    Ctx *C = as_ctx(p);
+    if (!C || n < 1 || !ids || !lb || !Lx || !Ly || !ud || !larea)
+        return fail(IBLBM_EINVAL, "set_markers_batch: need n >= 1 and every array");
+    // every instance is checked before any state changes
+    std::vector<int> box(4 * (size_t)n), off((size_t)n + 1, 0);
+    std::vector<char> seen(C->inst.size(), 0);
+    for (int b = 0; b < n; ++b) {
+        const int id = ids[b];
+        if (!valid_id(C, id)) return fail(IBLBM_EINVAL, "set_markers_batch: bad instance id");
+        if (seen[id]) return fail(IBLBM_EINVAL, "set_markers_batch: instance listed twice");
+        seen[id] = 1;
+        const HostInst &h = C->inst[id];
+        const Inst &d = h.d;
+        if (h.c.ibm == 0)
+            return fail(IBLBM_EINVAL, "set_markers_batch: instance has no IBM method");
+        if (lb[b] < 1 || lb[b] > d.lb_cap)
+            return fail(IBLBM_EINVAL, "set_markers_batch: Lb outside [1, lb_capacity]");
+        if (d.np_part > 1 && d.poff[d.np_part] != lb[b])
+            return fail(IBLBM_EINVAL,
+                        "set_markers_batch: Lb differs from the particles' marker total");
+        if (!(larea[b] > 0.0)) return fail(IBLBM_EINVAL, "set_markers_batch: Larea must be positive");
+        int *bx = &box[4 * (size_t)b];
+        int brc = iblbm_marker_box(Lx + off[b], Ly + off[b], lb[b], d.dx, d.dy, d.nx, d.ny,
+                                   d.delta, bx);
+        if (brc) return brc;
+        if (bx[2] < 1 || bx[3] < 1 || (int64_t)bx[2] * bx[3] > d.box_cap)
+            return fail(IBLBM_EINVAL, "set_markers_batch: window box exceeds box_capacity");
+        if (bx[0] < 0 || bx[1] < 0 || bx[0] + bx[2] > d.nx || bx[1] + bx[3] > d.ny)
+            return fail(IBLBM_EINVAL, "set_markers_batch: window box outside the grid");
+        off[b + 1] = off[b] + lb[b];
+    }
+    const int total = off[n];
+    HIPCHK(hipSetDevice(C->device));
+    int rc = grow_batch(*C, 4 * (size_t)total, 3 * (size_t)n + 1);
+    if (rc) return rc;
+    std::memcpy(C->h_bd, Lx, sizeof(double) * total);
+    std::memcpy(C->h_bd + total, Ly, sizeof(double) * total);
+    std::memcpy(C->h_bd + 2 * total, ud, sizeof(double) * 2 * total);
+    std::memcpy(C->h_bi, ids, sizeof(int) * n);
+    std::memcpy(C->h_bi + n, off.data(), sizeof(int) * (n + 1));
+    bool activated = false;
+    for (int b = 0; b < n; ++b) {
+        HostInst &h = C->inst[ids[b]];
+        Inst &d = h.d;
+        activated |= d.ibm == 0;
+        d.ibm = h.c.ibm;
+        d.Lb = lb[b];
+        d.Larea = larea[b];
+        d.bx0 = box[4 * b];
+        d.by0 = box[4 * b + 1];
+        d.bw = box[4 * b + 2];
+        d.bh = box[4 * b + 3];
+        h.Lb = lb[b];
+        h.Larea = larea[b];
+        if (center) {
+            d.cxd = center[2 * b];
+            d.cyd = center[2 * b + 1];
+            d.cxc = d.cxd / d.dx;
+            d.cyc = d.cyd / d.dy;
+        }
+    }
+    if (activated) C->tables_dirty = true;
+    std::vector<Inst> hd;  // the pageable source lives until the synchronisation below
+    if (C->tables_dirty) {
+        rc = ensure_tables(*C);  // uploads every descriptor
+        if (rc) return rc;
+    } else {
+        hd.resize(C->inst.size());
+        for (size_t k = 0; k < hd.size(); ++k) hd[k] = C->inst[k].d;
+        HIPCHK(hipMemcpyAsync(C->d_insts, hd.data(), sizeof(Inst) * hd.size(),
+                              hipMemcpyHostToDevice, C->stream));
+    }
+    HIPCHK(hipMemcpyAsync(C->d_bd, C->h_bd, sizeof(double) * 4 * total, hipMemcpyHostToDevice,
+                          C->stream));
+    HIPCHK(hipMemcpyAsync(C->d_bi, C->h_bi, sizeof(int) * (2 * n + 1), hipMemcpyHostToDevice,
+                          C->stream));
+    int *d_ntf = C->d_bi + 2 * n + 1;
+    hipLaunchKernelGGL(k_scatter_markers, dim3(n), dim3(256), 0, C->stream, C->d_insts, C->d_bi,
+                       C->d_bi + n, C->d_bd, total);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_tables_batch, dim3(n), dim3(NTI), 0, C->stream, C->d_insts, C->d_bi,
+                       d_ntf);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(C->h_bi + 2 * n + 1, d_ntf, sizeof(int) * 2 * n, hipMemcpyDeviceToHost,
+                          C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    bool path_changed = false;
+    for (int b = 0; b < n; ++b) {
+        Inst &d = C->inst[ids[b]].d;
+        const int fast_before = d.fast;
+        d.nT = C->h_bi[2 * n + 1 + 2 * b];
+        d.fast = C->h_bi[2 * n + 2 + 2 * b];
+        path_changed |= d.fast != fast_before;
+    }
+    if (path_changed) {  // a window workgroup changes kernel: rebuild the launch lists
+        C->tables_dirty = true;
+        rc = ensure_tables(*C);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int iblbm_download_marker_forces_batch(void *p, int n, const int *ids, double *F) {
+    Ctx *C = as_ctx(p);
+    if (!C || n < 1 || !ids || !F)
+        return fail(IBLBM_EINVAL, "download_marker_forces_batch: need n >= 1 and every array");
+    std::vector<int> off((size_t)n + 1, 0);
+    for (int b = 0; b < n; ++b) {
+        if (!valid_id(C, ids[b]))
+            return fail(IBLBM_EINVAL, "download_marker_forces_batch: bad instance id");
+        const Inst &d = C->inst[ids[b]].d;
+        const bool dfc = d.ibm == IBLBM_IBM_DFC;
+        const bool df_multi = d.ibm == IBLBM_IBM_DF && d.np_part >= 2;
+        if (!dfc && !df_multi)
+            return fail(IBLBM_EINVAL, "download_marker_forces_batch: needs DFC instances or DF "
+                                      "windows of several particles");
+        off[b + 1] = off[b] + d.Lb;
+    }
+    const int total = off[n];
+    HIPCHK(hipSetDevice(C->device));
+    int rc = grow_batch(*C, 2 * (size_t)total, 2 * (size_t)n + 1);
+    if (rc) return rc;
+    std::memcpy(C->h_bi, ids, sizeof(int) * n);
+    std::memcpy(C->h_bi + n, off.data(), sizeof(int) * (n + 1));
+    HIPCHK(hipMemcpyAsync(C->d_bi, C->h_bi, sizeof(int) * (2 * n + 1), hipMemcpyHostToDevice,
+                          C->stream));
+    hipLaunchKernelGGL(k_gather_forces, dim3(n), dim3(256), 0, C->stream, C->d_insts, C->d_bi,
+                       C->d_bi + n, C->d_bd);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(C->h_bd, C->d_bd, sizeof(double) * 2 * total, hipMemcpyDeviceToHost,
+                          C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    std::memcpy(F, C->h_bd, sizeof(double) * 2 * total);
     return 0;
 }
 

@@ -133,10 +133,100 @@ class Particle:
         self.omega_half = 0.0
 
 
+def make_particles(cfg, st):
+    """The particles of a settling run: ``particles_config`` (``sedimentation_multi.py``) or
+    the one disc at ``cylinder_center`` with the initial marker ring."""
+    if cfg.particles_config is not None:
+        parts = [Particle(pc["center"][0], pc["center"][1], pc["rho_ratio"], st.lattice_r, st.r,
+                          st.dx, cfg.marker_spacing_factor, cfg.retraction_dx)
+                 for pc in cfg.particles_config]
+        if len(parts) > 8:
+            raise ValueError("the GPU window holds at most 8 particles")
+        larea = parts[0].Larea
+        if cfg.ibm_method == "DFC":  # strategies/dfc.py:56-58
+            for p in parts:
+                if abs(float(p.Larea) - larea) > 1e-12:
+                    raise ValueError("multi-particle DFC needs one Larea for every particle")
+        return parts
+    cx, cy = cfg.cylinder_center
+    return [Particle(cx, cy, cfg.rho_ratio, st.lattice_r, st.r, st.dx, cfg.marker_spacing_factor,
+                     cfg.retraction_dx, Lx=st.Lx, Ly=st.Ly, Lb=st.Lb, Larea=st.Larea)]
+
+
+def half_step_particles(particles, cfg, st, g, multi):
+    """The Verlet half step of every particle, its moved markers and desired velocities:
+    ``_multi_particle_half_step`` (``sedimentation_multi.py:49-94``) and the single-particle
+    ``pre_step`` (``sedimentation_single.py:91-160``)."""
+    for p in particles:
+        pos_new, vel_half = verlet_half(cfg.settling_inertia_model, p.pos, p.vel, p.vel_prev,
+                                        p.force, p.mass, p.rho_ratio, p.lattice_r, g,
+                                        st.dt, st.dx, cfg.gravity_direction)
+        p.pos = pos_new
+        if cfg.enable_rotation:  # rigid_body.py:65-74
+            p.omega_half = p.omega + 0.5 * st.dt * p.torque / p.I
+            p.angle += st.dt * p.omega_half
+        else:
+            p.omega_half = 0.0
+        p.Lx, p.Ly = update_markers(p.Lx_c, p.Ly_c, p.cx_init, p.cy_init, pos_new,
+                                    angle=p.angle if cfg.enable_rotation else 0.0)
+        if cfg.rotation_coupling == "semi_implicit" and cfg.enable_rotation:
+            omega = p.omega_half if not multi else p.omega
+            p.desired = desired_velocity(vel_half, p.Lb, omega=omega, Lx=p.Lx, Ly=p.Ly,
+                                         cx=float(p.pos[0]), cy=float(p.pos[1]), dx=st.dx)
+        else:
+            p.desired = desired_velocity(vel_half, p.Lb)
+        p.vel_half = vel_half
+
+
+def full_step_particles(particles, forces, cfg, st, g):
+    """The Verlet full step of every particle from its (force, torque):
+    ``_multi_particle_full_step`` (``sedimentation_multi.py:97-124``) and the single-particle
+    ``post_ibm`` (``sedimentation_single.py:200-293``)."""
+    for p, (F, T) in zip(particles, forces):
+        vel_new = verlet_full(cfg.settling_inertia_model, p.vel_half, p.vel, p.vel_prev, F,
+                              p.mass, p.rho_ratio, p.lattice_r, g, st.dt,
+                              cfg.gravity_direction)
+        if cfg.enable_rotation:  # rigid_body.py:77-82
+            p.omega = p.omega_half + 0.5 * st.dt * T / p.I
+            p.torque = T
+        p.vel_prev = p.vel.copy()
+        p.vel = vel_new
+        p.force = F
+
+
+def marker_forces_to_particles(particles, q, cfg, st):
+    """(force, torque) per particle from the marker forces of its ring (rings concatenated in
+    particle order): DFC sums q (``strategies/dfc.py:129-142``), several-particle DF sums
+    -Lf * Larea (``forcing.py:263-275``)."""
+    out = []
+    c = 0
+    for p in particles:
+        fq = q[c:c + p.Lb]
+        c += p.Lb
+        if cfg.ibm_method == "DFC":
+            F = np.array([float(fq[:, 0].sum()), float(fq[:, 1].sum())])
+            T = 0.0
+            if cfg.enable_rotation:
+                rx = (p.Lx - float(p.pos[0])) / st.dx
+                ry = (p.Ly - float(p.pos[1])) / st.dy
+                T = float(np.sum(rx * fq[:, 1] - ry * fq[:, 0]))
+        else:
+            Lfx, Lfy = fq[:, 0], fq[:, 1]
+            la = float(p.Larea)
+            F = np.array([-float(np.sum(Lfx * la)), -float(np.sum(Lfy * la))])
+            T = 0.0
+            if cfg.enable_rotation:
+                rx = (p.Lx - float(p.pos[0])) / st.dx
+                ry = (p.Ly - float(p.pos[1])) / st.dy
+                T = -float(np.sum((rx * Lfy - ry * Lfx) * la))
+        out.append((F, T))
+    return out
+
+
 class Sedimentation:
     """One settling-particle instance on the device, stepped one time step at a time."""
 
-    def __init__(self, cfg, device=0, precision="fp64", ctx=None):
+    def __init__(self, cfg, device=0, precision="fp64", ctx=None, f0_cache=None):
         if cfg.motion_type != "sedimentation":
             raise ValueError("Sedimentation needs motion_type='sedimentation'")
         if cfg.settling_inertia_model not in ("none", "explicit_history"):
@@ -153,26 +243,21 @@ class Sedimentation:
         if cfg.rho_ratio <= 1.0:
             raise ValueError(f"sedimentation needs rho_ratio > 1.0 (got {cfg.rho_ratio})")
         self.cfg = cfg
-        st = initialize(cfg)
+        # the pre-collided rest state depends on the lattice and collision only, so members of
+        # an ensemble on one grid share it (f0_cache) and keep no full-grid host fields
+        st = initialize(cfg, fields=False)
+        key = (st.nx, st.ny, st.tau, st.tau_minus, cfg.collision_model,
+               None if st.S_diag is None else tuple(float(v) for v in st.S_diag),
+               bool(cfg.incompressible_lbgk))
+        f0 = None if f0_cache is None else f0_cache.get(key)
+        if f0 is None:
+            f0 = pre_collide(cfg, initialize(cfg))
+            if f0_cache is not None:
+                f0_cache[key] = f0
         self.st = st
         self.g = cfg.gravity
         self.multi = cfg.particles_config is not None
-        if self.multi:
-            self.particles = [Particle(pc["center"][0], pc["center"][1], pc["rho_ratio"],
-                                       st.lattice_r, st.r, st.dx, cfg.marker_spacing_factor,
-                                       cfg.retraction_dx) for pc in cfg.particles_config]
-            if len(self.particles) > 8:
-                raise ValueError("the GPU window holds at most 8 particles")
-            larea = self.particles[0].Larea
-            if cfg.ibm_method == "DFC":  # strategies/dfc.py:56-58
-                for p in self.particles:
-                    if abs(float(p.Larea) - larea) > 1e-12:
-                        raise ValueError("multi-particle DFC needs one Larea for every particle")
-        else:
-            cx, cy = cfg.cylinder_center
-            self.particles = [Particle(cx, cy, cfg.rho_ratio, st.lattice_r, st.r, st.dx,
-                                       cfg.marker_spacing_factor, cfg.retraction_dx,
-                                       Lx=st.Lx, Ly=st.Ly, Lb=st.Lb, Larea=st.Larea)]
+        self.particles = make_particles(cfg, st)
         self.ctx = ctx or Context(device)
         case = make_case(cfg, st, PRECISIONS[precision])
         case.fixed_body = 0
@@ -193,7 +278,7 @@ class Sedimentation:
             self.ctx.set_particles(self.k, [p.Lb for p in self.particles],
                                    [float(p.Larea) for p in self.particles])
         self._push_markers()
-        self.ctx.upload_f(self.k, pre_collide(cfg, st))
+        self.ctx.upload_f(self.k, f0)
         self.ttt = 0
         self.history = []
         self.mdf_stats = None
@@ -229,26 +314,7 @@ class Sedimentation:
 
     # -- the reference's pre_step / post_ibm hooks ------------------------------------------
     def _half_step(self):
-        cfg, st = self.cfg, self.st
-        for p in self.particles:
-            pos_new, vel_half = verlet_half(cfg.settling_inertia_model, p.pos, p.vel, p.vel_prev,
-                                            p.force, p.mass, p.rho_ratio, p.lattice_r, self.g,
-                                            st.dt, st.dx, cfg.gravity_direction)
-            p.pos = pos_new
-            if cfg.enable_rotation:  # rigid_body.py:65-74
-                p.omega_half = p.omega + 0.5 * st.dt * p.torque / p.I
-                p.angle += st.dt * p.omega_half
-            else:
-                p.omega_half = 0.0
-            p.Lx, p.Ly = update_markers(p.Lx_c, p.Ly_c, p.cx_init, p.cy_init, pos_new,
-                                        angle=p.angle if cfg.enable_rotation else 0.0)
-            if cfg.rotation_coupling == "semi_implicit" and cfg.enable_rotation:
-                omega = p.omega_half if not self.multi else p.omega
-                p.desired = desired_velocity(vel_half, p.Lb, omega=omega, Lx=p.Lx, Ly=p.Ly,
-                                             cx=float(p.pos[0]), cy=float(p.pos[1]), dx=st.dx)
-            else:
-                p.desired = desired_velocity(vel_half, p.Lb)
-            p.vel_half = vel_half
+        half_step_particles(self.particles, self.cfg, self.st, self.g, self.multi)
 
     def _forces(self):
         """Hydrodynamic force and torque per particle (strategies/*.py extract_force/torque)."""
@@ -256,35 +322,12 @@ class Sedimentation:
         out = []
         if cfg.ibm_method == "DFC":
             Lbt = sum(p.Lb for p in self.particles)
-            q = self.ctx.download_dfc_force(self.k, Lbt)
-            c = 0
-            for p in self.particles:
-                fq = q[c:c + p.Lb]
-                c += p.Lb
-                F = np.array([float(fq[:, 0].sum()), float(fq[:, 1].sum())])
-                T = 0.0
-                if cfg.enable_rotation:
-                    rx = (p.Lx - float(p.pos[0])) / st.dx
-                    ry = (p.Ly - float(p.pos[1])) / st.dy
-                    T = float(np.sum(rx * fq[:, 1] - ry * fq[:, 0]))
-                out.append((F, T))
-            return out
+            return marker_forces_to_particles(self.particles,
+                                              self.ctx.download_dfc_force(self.k, Lbt), cfg, st)
         if self.multi and cfg.ibm_method == "DF":  # forcing.py:263-275, per particle
             Lbt = sum(p.Lb for p in self.particles)
-            Lf = self.ctx.download_marker_force(self.k, Lbt)
-            c = 0
-            for p in self.particles:
-                Lfx, Lfy = Lf[c:c + p.Lb, 0], Lf[c:c + p.Lb, 1]
-                c += p.Lb
-                la = float(p.Larea)
-                F = np.array([-float(np.sum(Lfx * la)), -float(np.sum(Lfy * la))])
-                T = 0.0
-                if cfg.enable_rotation:
-                    rx = (p.Lx - float(p.pos[0])) / st.dx
-                    ry = (p.Ly - float(p.pos[1])) / st.dy
-                    T = -float(np.sum((rx * Lfy - ry * Lfx) * la))
-                out.append((F, T))
-            return out
+            return marker_forces_to_particles(self.particles,
+                                              self.ctx.download_marker_force(self.k, Lbt), cfg, st)
         if self.multi:  # coupled MDF: each particle's own field (multi_particle.py:302-316)
             for i, p in enumerate(self.particles):
                 out.append(self._window_force(*self.ctx.download_particle_fib(self.k, i), p))
@@ -317,17 +360,7 @@ class Sedimentation:
         return F, T
 
     def _full_step(self, forces):
-        cfg, st = self.cfg, self.st
-        for p, (F, T) in zip(self.particles, forces):
-            vel_new = verlet_full(cfg.settling_inertia_model, p.vel_half, p.vel, p.vel_prev, F,
-                                  p.mass, p.rho_ratio, p.lattice_r, self.g, st.dt,
-                                  cfg.gravity_direction)
-            if cfg.enable_rotation:  # rigid_body.py:77-82
-                p.omega = p.omega_half + 0.5 * st.dt * T / p.I
-                p.torque = T
-            p.vel_prev = p.vel.copy()
-            p.vel = vel_new
-            p.force = F
+        full_step_particles(self.particles, forces, self.cfg, self.st, self.g)
 
     def step(self):
         self.ttt += 1

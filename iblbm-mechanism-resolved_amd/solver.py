@@ -182,6 +182,36 @@ class Context:
         check(self.lib.iblbm_download_marker_force(self.h, k, dptr(F)), "download_marker_force")
         return F
 
+    def set_markers_batch(self, ids, lb, Lx, Ly, ud, larea, center=None):
+        """``set_markers`` (and ``set_torque_center``) of several instances in one call:
+        instance ``ids[b]`` takes the next ``lb[b]`` markers of the concatenated arrays."""
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        lb = np.ascontiguousarray(lb, dtype=np.int32)
+        Lx, Ly, ud, la = f64(Lx), f64(Ly), f64(ud), f64(larea)
+        n, tot = len(ids), int(lb.sum())
+        if len(lb) != n or len(la) != n or Lx.size != tot or Ly.size != tot or ud.size != 2 * tot:
+            raise ValueError("set_markers_batch: array sizes do not match the marker counts")
+        cp = None
+        if center is not None:
+            c = f64(center)
+            if c.size != 2 * n:
+                raise ValueError("set_markers_batch: center needs (n, 2) entries")
+            cp = dptr(c)
+        ip = ctypes.POINTER(ctypes.c_int)
+        check(self.lib.iblbm_set_markers_batch(self.h, n, ids.ctypes.data_as(ip),
+                                               lb.ctypes.data_as(ip), dptr(Lx), dptr(Ly),
+                                               dptr(ud), dptr(la), cp), "set_markers_batch")
+
+    def download_marker_forces_batch(self, ids, lb_total):
+        """DFC forces q (or several-particle DF marker forces) of several instances,
+        concatenated in ``ids`` order: (lb_total, 2)."""
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        F = np.empty((int(lb_total), 2))
+        check(self.lib.iblbm_download_marker_forces_batch(
+            self.h, len(ids), ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), dptr(F)),
+            "download_marker_forces_batch")
+        return F
+
     def diag(self, k, slot=None):
         """Diagnostics of the last diagnostic step; with ``slot``, those fetched into that
         pinned slot by ``diag_fetch`` (waits for that copy only)."""

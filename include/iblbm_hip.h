@@ -136,6 +136,23 @@ int iblbm_download_particle_fib(void *ctx, int id, int part, double *fibx, doubl
 int iblbm_download_marker_force(void *ctx, int id, double *F);
 /* Centre (domain units) about which the diagnostic torque sums are taken. */
 int iblbm_set_torque_center(void *ctx, int id, double cx, double cy);
+/* iblbm_set_markers (and, with center != NULL, iblbm_set_torque_center) for n instances of
+ * one context at once: one host-to-device copy, one scatter launch and one table launch for
+ * all, one synchronisation.  Instance ids[b] gets lb[b] markers taken in order from the
+ * concatenated Lx, Ly (sum lb) and ud (sum lb, 2), arc length larea[b] and torque centre
+ * center[2b], center[2b+1].  Every instance is checked before any changes.  Replaces the
+ * per-particle marker uploads of a batched settling ensemble, each member as
+ * iblbm/runtime/scenarios/sedimentation_multi.py:49-94 (pre_step) and
+ * iblbm/runtime/scenarios/sedimentation_single.py:156-160 set them. */
+int iblbm_set_markers_batch(void *ctx, int n, const int *ids, const int *lb, const double *Lx,
+                            const double *Ly, const double *ud, const double *larea,
+                            const double *center);
+/* The marker forces of n instances in one read, concatenated in ids order, (Lb, 2) each: the
+ * DFC force q of a DFC instance (iblbm/ibm/dfc.py:232-253, as iblbm_download_dfc_force) or the
+ * marker forces of a several-particle DF window (as iblbm_download_marker_force).  Replaces the
+ * per-member force reads of iblbm/ibm/strategies/dfc.py:129-150 and
+ * iblbm/ibm/forcing.py:263-275 in a batched settling ensemble. */
+int iblbm_download_marker_forces_batch(void *ctx, int n, const int *ids, double *F);
 /* Window box [box[0], box[0]+box[2]) x [box[1], box[1]+box[3]) covering every clamped
  * stencil node of the markers (stencil radius 2 for peskin4pt, 1 for hat). */
 int iblbm_marker_box(const double *Lx, const double *Ly, int Lb, double dx, double dy, int nx,

@@ -1,0 +1,133 @@
+"""Settling ensembles on the GPU: members stepped together in one context (one batched marker
+upload, one step launch and one batched force read per step) against their own single runs,
+bit for bit, and against the reference's settling fixtures (tests/golden/sed_*)."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from conftest import golden, golden_cfg, rel_linf
+from iblbm_amd.sed_ensemble import SedimentationEnsemble, two_disc_cfgs
+from iblbm_amd.sedimentation import Sedimentation
+
+pytestmark = pytest.mark.gpu
+TRAJ_TOL = 1e-10
+FIXTURES = ["sed_single_df_bgk_hat", "sed_single_mdf_cm_p4", "sed_multi_dfc_cm_hat",
+            "sed_multi_df_bgk_hat", "sed_multi_mdf_cm_p4"]
+
+
+def _single(cfg, n, precision="fp64"):
+    sim = Sedimentation(cfg, precision=precision)
+    rows = []
+    for _ in range(n):
+        sim.step()
+        row = []
+        for p in sim.particles:
+            row += [p.pos[0], p.pos[1], p.vel[0], p.vel[1], p.omega, p.angle, p.force[0],
+                    p.force[1], p.torque]
+        rows.append(row)
+    f = sim.ctx.download_f(sim.k)
+    sim.ctx.close()
+    return np.array(rows, dtype=np.float64), f
+
+
+def _ensemble(cfgs, n, precision="fp64"):
+    ens = SedimentationEnsemble(cfgs, precision=precision)
+    rows = [[] for _ in cfgs]
+    for _ in range(n):
+        ens.step()
+        for mi in range(len(cfgs)):
+            rows[mi].append(ens.trajectory_row(mi))
+    fs = [ens.ctx.download_f(m.k) for m in ens.members]
+    batched = ens.batched
+    ens.close()
+    return [np.array(r, dtype=np.float64) for r in rows], fs, batched
+
+
+def _check_fixture(name, rows, n):
+    g, sc = golden(name)
+    ref = g["particles"][:n]
+    for p in range(rows.shape[1] // 9):
+        cols = [9 * p + c for c in range(4)]
+        err = rel_linf(rows[:, cols], ref[:, [1 + c for c in cols]])
+        print(f"{name}: particle {p} position/velocity rel Linf vs reference {err:.2e}",
+              flush=True)
+        assert err <= TRAJ_TOL
+
+
+def test_mixed_ensemble_matches_single_runs_and_fixtures():
+    """The five settling fixtures (grids 25x73 .. 129x193; DF, MDF, DFC; one and two
+    particles; hat and Peskin-4) as one ensemble for 40 steps."""
+    n = 40
+    cfgs = [golden_cfg(golden(name)[1]) for name in FIXTURES]
+    print("mixed settling ensemble: 5 members, 40 steps, fp64", flush=True)
+    rows, fs, batched = _ensemble(cfgs, n)
+    assert not batched  # window-force members are read one by one
+    for name, cfg, r, f in zip(FIXTURES, cfgs, rows, fs):
+        r1, f1 = _single(cfg, n)
+        assert np.array_equal(r, r1), name
+        assert np.array_equal(f, f1), name
+        _check_fixture(name, r, n)
+
+
+def _dfc_variants():
+    base = golden_cfg(golden("sed_multi_dfc_cm_hat")[1])
+    out = [base]
+    for dx, dy in [(0.02, -0.1), (-0.03, 0.05)]:
+        pc = [dict(p) for p in base.particles_config]
+        c = pc[1]["center"]
+        pc[1]["center"] = [c[0] + dx, c[1] + dy]
+        out.append(dataclasses.replace(base, particles_config=pc))
+    return out
+
+
+def test_dfc_ensemble_batched_reads_match_single_runs():
+    """Three two-disc DFC members with different gaps: markers and forces of all members
+    move in one batched call each per step."""
+    n = 150
+    cfgs = _dfc_variants()
+    print("two-disc DFC ensemble: 3 members, 150 steps, fp64", flush=True)
+    rows, fs, batched = _ensemble(cfgs, n)
+    assert batched
+    for k, (cfg, r, f) in enumerate(zip(cfgs, rows, fs)):
+        r1, f1 = _single(cfg, n)
+        assert np.array_equal(r, r1), k
+        assert np.array_equal(f, f1), k
+    _check_fixture("sed_multi_dfc_cm_hat", rows[0], n)
+    g, _ = golden("sed_multi_dfc_cm_hat")
+    assert rel_linf(fs[0], g["f_post"]) <= 1e-10
+
+
+def test_fp32dev_ensemble_tracks_fp64():
+    """FP32-deviation storage against FP64 for the same two-disc DFC members: the settling
+    velocity and the positions after 150 steps."""
+    n = 150
+    cfgs = _dfc_variants()
+    print("two-disc DFC ensemble: 3 members, 150 steps, fp32dev vs fp64", flush=True)
+    r64, _, _ = _ensemble(cfgs, n, "fp64")
+    r32, _, _ = _ensemble(cfgs, n, "fp32dev")
+    for k in range(len(cfgs)):
+        for p in range(2):
+            vy64, vy32 = r64[k][:, 9 * p + 3], r32[k][:, 9 * p + 3]
+            e_v = rel_linf(vy32, vy64)
+            e_y = abs(r32[k][-1, 9 * p + 1] - r64[k][-1, 9 * p + 1]) / abs(r64[k][-1, 9 * p + 1])
+            print(f"member {k} particle {p}: settling velocity rel Linf {e_v:.2e}, "
+                  f"final y rel {e_y:.2e}", flush=True)
+            assert e_v <= 1e-3
+            assert e_y <= 1e-6
+
+
+def test_two_disc_ensemble_runs_and_keeps_members_apart():
+    """Four full-size two-disc cases (193x1921, D = 24, 2 x 75 markers, DFC + hat + CM-MRT,
+    FP32 deviations) for 200 steps: the batched path runs, members with different gaps
+    settle differently, and the member with the smallest gap matches its single run."""
+    cfgs = two_disc_cfgs(4, seed=3, check_interval=100)
+    print("two-disc ensemble 4 x 193x1921, 200 steps, fp32dev", flush=True)
+    rows, fs, batched = _ensemble(cfgs, 200, "fp32dev")
+    assert batched
+    vy = [r[-1, 3] for r in rows]
+    assert all(v < 0.0 for v in vy)  # settling
+    assert len({float(v) for v in vy}) == 4
+    r1, f1 = _single(cfgs[0], 200, "fp32dev")
+    assert np.array_equal(rows[0], r1)
+    assert np.array_equal(fs[0], f1)
