@@ -320,6 +320,7 @@ def sed_bench(args):
     for prec in ("fp32dev", "fp64"):
         ens = SedimentationEnsemble(cfgs, precision=prec)
         ens.run(warm)
+        tuned = ens.ctx.autotune(10) if not args.no_tune else ens.ctx.schedule()
         ens.ctx.sync()
         t0 = time.perf_counter()
         ens.run(steps)
@@ -346,7 +347,7 @@ def sed_bench(args):
         res[prec] = {"MLUPS": round(ens.nodes * steps / wall / 1e6, 2),
                      "ms_per_step": round(wall * 1e3 / steps, 4),
                      "step_launch_us": round(k_us, 2), "nodes": ens.nodes,
-                     "mode": ens.ctx.step_mode(), "re_history": re_hist,
+                     "mode": ens.ctx.step_mode(), "schedule": tuned, "re_history": re_hist,
                      "member0_discs_x_y_vx_vy": traj,
                      "stopped": sum(r is not None for r in ens.reasons)}
         ens.close()
@@ -381,7 +382,8 @@ def sed_bench(args):
         "config": {"workload": "64 x two discs D=24, rho_ratio 1.01, channel 8D x 80D "
                                "(193x1921), DFC + hat + CM-MRT, explicit-history Verlet "
                                "with rotation, g=1.5e-3, nu=0.04",
-                   "nodes": b["nodes"], "mode": b["mode"], "stopped": b["stopped"]},
+                   "nodes": b["nodes"], "mode": b["mode"], "schedule": b["schedule"],
+                   "stopped": b["stopped"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "step launch (k_ibm || k_bulk), markers held",

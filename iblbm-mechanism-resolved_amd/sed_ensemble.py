@@ -124,6 +124,13 @@ class ParticleBank:
                 "Ly": np.array([parts[r].Ly for r in rows]),
                 "dfc": np.array([self.methods[owner[r]] == "DFC" for r in rows]),
             })
+            g = self.groups[-1]
+            flat = g["dst"].ravel()
+            # a group whose rings are consecutive in the batch layout is one slice of it
+            g["span"] = ((int(flat[0]), int(flat[-1]) + 1)
+                         if np.array_equal(flat, np.arange(flat[0], flat[0] + flat.size))
+                         else None)
+            g["all_dfc"], g["no_dfc"] = bool(g["dfc"].all()), not bool(g["dfc"].any())
         self.Lx = np.empty(self.n_markers)
         self.Ly = np.empty(self.n_markers)
         self.ud = np.empty((self.n_markers, 2))
@@ -172,11 +179,19 @@ class ParticleBank:
                 on = om != 0.0
                 d0 = np.where(on, d0 + (-om * (Ly - py)) / dxp, d0)
                 d1 = np.where(on, d1 + (om * (Lx - px)) / dxp, d1)
-            dst = g["dst"]
-            self.Lx[dst] = Lx
-            self.Ly[dst] = Ly
-            self.ud[dst, 0] = d0
-            self.ud[dst, 1] = d1
+            if g["span"] is not None:
+                a, b = g["span"]
+                self.Lx[a:b] = Lx.ravel()
+                self.Ly[a:b] = Ly.ravel()
+                ud = self.ud[a:b].reshape(Lx.shape + (2,))
+                ud[:, :, 0] = d0
+                ud[:, :, 1] = d1
+            else:
+                dst = g["dst"]
+                self.Lx[dst] = Lx
+                self.Ly[dst] = Ly
+                self.ud[dst, 0] = d0
+                self.ud[dst, 1] = d1
         return self.Lx, self.Ly, self.ud
 
     def forces_from_markers(self, q):
@@ -187,20 +202,30 @@ class ParticleBank:
         T = np.zeros(self.P)
         for g in self.groups:
             r = g["rows"]
-            Q = q[g["dst"]]  # (P_g, Lb, 2)
+            if g["span"] is not None:
+                a, b = g["span"]
+                Q = q[a:b].reshape(g["dst"].shape + (2,))
+            else:
+                Q = q[g["dst"]]  # (P_g, Lb, 2)
             qx = np.ascontiguousarray(Q[:, :, 0])
             qy = np.ascontiguousarray(Q[:, :, 1])
             dfc = g["dfc"]
             la = self.larea[r][:, None]
-            fx = np.where(dfc, qx.sum(axis=1), -(qx * la).sum(axis=1))
-            fy = np.where(dfc, qy.sum(axis=1), -(qy * la).sum(axis=1))
-            F[r, 0], F[r, 1] = fx, fy
+
+            def pick(f_dfc, f_df):  # DFC sums q; several-particle DF sums -Lf * Larea
+                if g["all_dfc"]:
+                    return f_dfc()
+                if g["no_dfc"]:
+                    return f_df()
+                return np.where(dfc, f_dfc(), f_df())
+
+            F[r, 0] = pick(lambda: qx.sum(axis=1), lambda: -(qx * la).sum(axis=1))
+            F[r, 1] = pick(lambda: qy.sum(axis=1), lambda: -(qy * la).sum(axis=1))
             if self.rotation:
                 rx = (g["Lx"] - self.pos[r, 0][:, None]) / self.dx[r][:, None]
                 ry = (g["Ly"] - self.pos[r, 1][:, None]) / self.dy[r][:, None]
-                t_dfc = (rx * qy - ry * qx).sum(axis=1)
-                t_df = -((rx * qy - ry * qx) * la).sum(axis=1)
-                T[r] = np.where(dfc, t_dfc, t_df)
+                T[r] = pick(lambda: (rx * qy - ry * qx).sum(axis=1),
+                            lambda: -((rx * qy - ry * qx) * la).sum(axis=1))
         return F, T
 
     def full_step(self, F, T):
