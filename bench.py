@@ -106,10 +106,12 @@ def main():
                     help="steps of the plain-vs-learned-predictor MDF comparison (0: skip)")
     ap.add_argument("--variant-steps", type=int, default=2000,
                     help="steps of the FP32-deviation comparison of an fp64 run (0: skip)")
-    ap.add_argument("--workload", choices=["cylinder", "matrix72", "sed64", "copy"],
+    ap.add_argument("--workload", choices=["cylinder", "matrix72", "sed1", "sed64", "copy"],
                     default="cylinder",
                     help="cylinder: Re=100 1601x641 (default); matrix72: the batched 72-case "
-                         "Re x method x kernel x collision matrix (about 300 M nodes); sed64: "
+                         "Re x method x kernel x collision matrix (about 300 M nodes); sed1: "
+                         "one settling disc (257x1921, CM-MRT + MDF + Peskin-4), plain MDF "
+                         "against the learned predictor; sed64: "
                          "64 two-disc settling cases (193x1921, DFC + hat + CM-MRT) stepped "
                          "as one ensemble; copy: "
                          "a 1 GiB FP64 device copy, five times (the reference of the HBM "
@@ -128,6 +130,9 @@ def main():
         return
     if args.workload == "sed64":
         print(json.dumps(sed_bench(args)), flush=True)
+        return
+    if args.workload == "sed1":
+        print(json.dumps(sed1_bench(args)), flush=True)
         return
     from iblbm_amd.solver import FixedCylinder
     cfg = workload_cfg()
@@ -301,58 +306,153 @@ def matrix_bench(args):
     return out
 
 
+def sed1_cfg():
+    """One settling disc: D = 32 (101 markers), density ratio 1.5, channel 8D x 60D (257x1921)
+    with no-slip walls, CM-MRT + MDF [5, 20] tol 1e-4 + Peskin-4, explicit-history Verlet with
+    rotation; nu = 0.02 and g = 4e-5, for a terminal Reynolds number near 40."""
+    from iblbm_amd import SimConfig
+    return SimConfig(Re=38.4, NN=257, xmax=1.0, ymax=7.5, cylinder_center=(0.5, 7.0),
+                     cylinder_D_ratio=0.125, lattice_u=0.024, rho_ratio=1.5, gravity=4e-5,
+                     motion_type="sedimentation", bc_type="settling_channel",
+                     collision_model="CM_MRT", ibm_method="MDF", delta_type="peskin4pt",
+                     mdf_iterations=20, mdf_min_iterations=5, mdf_tolerance=1e-4,
+                     marker_spacing_factor=1.0, settling_inertia_model="explicit_history",
+                     enable_rotation=True, max_steps=200_000, check_interval=200)
+
+
+def sed1_bench(args):
+    """The settling disc in FP64: plain MDF against the learned force predictor trained on
+    this run's own MDF iterates.  After 1,000 steps from rest, 1,000 plain steps record
+    samples and the network trains on the device; then both forms run 1,000 steps from one
+    snapshot.  The device time of each step launch (HIP events; the markers move every step,
+    so each step is set up and read back by the host) and the whole step are timed."""
+    from iblbm_amd.sedimentation import Sedimentation
+    cfg = sed1_cfg()
+    sim = Sedimentation(cfg, precision="fp64")
+    pred = sim.predictor
+    n = max(200, args.surrogate_steps // 2) if args.surrogate_steps > 0 else 1000
+    for _ in range(1000):
+        sim.step()
+    pred.mode("collect", capacity=1 << 20)
+    for _ in range(1000):
+        sim.step()
+    losses = pred.train(steps=3000, lr=2e-3, seed=3)
+    snap = sim.snapshot()
+    F0 = pred.previous_force(sim.particles[0].Lb)
+    res = {}
+    for mode in ("off", "learned"):
+        sim.restore(snap)
+        pred.set_previous_force(F0)
+        pred.mode(mode)
+        pred.stats(reset=True)
+        dev_ms = 0.0
+        t0 = time.perf_counter()
+        for _ in range(n):  # Sedimentation.step with the launch bracketed by events
+            sim.ttt += 1
+            sim._half_step()
+            sim._push_markers()
+            sim.ctx.mark(0)
+            sim.ctx.step(1, diag_every=1 if sim.ttt % cfg.check_interval == 0 else 0)
+            sim.ctx.mark(1)
+            forces = sim._forces()
+            dev_ms += sim.ctx.elapsed_ms(0, 1)
+            sim._full_step(forces)
+        wall = time.perf_counter() - t0
+        p0 = sim.particles[0]
+        st = pred.stats()
+        res[mode] = {"launch_us": round(dev_ms * 1e3 / n, 3),
+                     "whole_step_us": round(wall * 1e6 / n, 2),
+                     "vy": float(p0.vel[1]), "y": float(p0.pos[1]),
+                     "substeps_per_step": round(st["substeps_per_step"], 3),
+                     "restarts": st["restarts"], "mode": sim.ctx.step_mode()}
+    pl, le = res["off"], res["learned"]
+    nu = (sim.st.tau - 0.5) / 3.0
+    re_t = abs(pl["vy"]) * 2.0 * sim.st.lattice_r / nu
+    nodes = sim.st.nx * sim.st.ny
+    return {
+        "metric": "MDF step time, plain vs learned predictor (fp64 settling disc)",
+        "value": round(nodes / pl["launch_us"], 2), "unit": "MLUPS", "n_gpus": 1, "steps": n,
+        "warmup": 2000, "ms_per_step": round(pl["whole_step_us"] / 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (disc released from rest)",
+        "config": {"workload": "settling disc D=32 (101 markers), rho_ratio 1.5, channel "
+                               "8D x 60D (257x1921), CM-MRT + MDF [5,20] tol 1e-4 + Peskin-4, "
+                               "explicit-history Verlet with rotation, nu=0.02, g=4e-5",
+                   "nodes": nodes, "launch": pl["mode"]},
+        "plain": pl, "learned": le,
+        "launch_speedup": round(pl["launch_us"] / le["launch_us"], 4),
+        "dvy_rel": abs(le["vy"] - pl["vy"]) / abs(pl["vy"]),
+        "Re_particle": round(re_t, 3),
+        "train": {"samples": pred.samples(), "adam_steps": 3000,
+                  "loss_first100": float(losses[:100].mean()),
+                  "loss_last100": float(losses[-100:].mean())},
+    }
+
+
 def sed_bench(args):
     """64 two-disc settling cases (193x1921 each, discs D = 24 of density ratio 1.01, seeded
-    gap and offset; DFC + hat + CM-MRT, explicit-history Verlet with rotation) as one
-    ensemble in FP32-deviation storage.  A step is the whole job: the vectorised particle
-    half step of all 128 discs, one batched marker upload, one step launch, one batched force
-    read and the full step; the wall clock around K such steps gives ``value``.  The step
-    launch alone (markers held) is timed with HIP events for the roofline.  Also timed: the
-    same ensemble in FP64, and the plain FP64 path case by case (one context per case)."""
-    from iblbm_amd.sed_ensemble import SedimentationEnsemble, two_disc_cfgs
+    gap and offset; DFC + hat + CM-MRT, explicit-history Verlet with rotation) in FP32-
+    deviation storage.  A step is the whole job: the vectorised particle half step of all
+    128 discs, the batched marker upload, the step launch, the batched force read and the
+    full step; the wall clock around K such steps gives ``value``.  Two forms: one context
+    for all members, and the members split over two contexts whose steps are pipelined (the
+    host moves one half's particles while the device steps the other).  The step launch
+    alone (one context, markers held) is timed with HIP events for the roofline.  Also
+    timed: the one-context ensemble in FP64 and the plain FP64 path case by case."""
+    from iblbm_amd.sed_ensemble import PipelinedEnsembles, SedimentationEnsemble, two_disc_cfgs
     from iblbm_amd.sedimentation import Sedimentation
     n_inst = 64
     ci = 200
     cfgs = two_disc_cfgs(n_inst, seed=2024, check_interval=ci)
     steps = max(ci, (args.steps // 4 // ci) * ci)
     warm = 40
+    legs = [("fp32dev", 1), ("fp32dev", 2)]
+    if args.variant_steps > 0:
+        legs.append(("fp64", 1))
     res = {}
-    for prec in ("fp32dev", "fp64"):
-        ens = SedimentationEnsemble(cfgs, precision=prec)
-        ens.run(warm)
-        tuned = ens.ctx.autotune(10) if not args.no_tune else ens.ctx.schedule()
-        ens.ctx.sync()
+    for prec, parts in legs:
+        if parts == 1:
+            ens = SedimentationEnsemble(cfgs, precision=prec)
+            ens.run(warm)
+            tuned = ens.ctx.autotune(10) if not args.no_tune else ens.ctx.schedule()
+        else:
+            ens = PipelinedEnsembles(cfgs, parts=parts, precision=prec)
+            ens.run(warm)
+            tuned = ens.autotune(10) if not args.no_tune else [e.ctx.schedule()
+                                                               for e in ens.parts]
+        ens.sync() if parts > 1 else ens.ctx.sync()
         t0 = time.perf_counter()
         ens.run(steps)
-        ens.ctx.sync()
         wall = time.perf_counter() - t0
-        # the step launch alone, markers held where they are
-        n_k = 200
-        ens.ctx.mark(0)
-        ens.ctx.step(n_k)
-        ens.ctx.mark(1)
-        ens.ctx.sync()
-        k_us = ens.ctx.elapsed_ms(0, 1) * 1e3 / n_k
-        re_hist = []
-        for i in range(len(ens.histories[0])):
-            re = []
-            for mi, h in enumerate(ens.histories):
-                if i < len(h):
-                    nu = (ens.members[mi].st.tau - 0.5) / 3.0
-                    re += [float(np.hypot(r["vx"], r["vy"])) * 24.0 / nu for r in h[i]]
-            re_hist.append([ens.histories[0][i][0]["step"], round(float(np.mean(re)), 4),
-                            round(float(np.max(re)), 4)])
-        traj = [[round(float(v), 6) for v in ens.trajectory_row(0)[:4]],
+        leg = {"MLUPS": round(ens.nodes * steps / wall / 1e6, 2),
+               "ms_per_step": round(wall * 1e3 / steps, 4), "contexts": parts,
+               "nodes": ens.nodes, "schedule": tuned}
+        if parts == 1:
+            n_k = 200  # the step launch alone, markers held where they are
+            ens.ctx.mark(0)
+            ens.ctx.step(n_k)
+            ens.ctx.mark(1)
+            ens.ctx.sync()
+            leg["step_launch_us"] = round(ens.ctx.elapsed_ms(0, 1) * 1e3 / n_k, 2)
+            leg["mode"] = ens.ctx.step_mode()
+            re_hist = []
+            for i in range(len(ens.histories[0])):
+                re = []
+                for mi, h in enumerate(ens.histories):
+                    if i < len(h):
+                        nu = (ens.members[mi].st.tau - 0.5) / 3.0
+                        re += [float(np.hypot(r["vx"], r["vy"])) * 24.0 / nu for r in h[i]]
+                re_hist.append([ens.histories[0][i][0]["step"], round(float(np.mean(re)), 4),
+                                round(float(np.max(re)), 4)])
+            leg["re_history"] = re_hist
+            leg["member0_discs_x_y_vx_vy"] = [
+                [round(float(v), 6) for v in ens.trajectory_row(0)[:4]],
                 [round(float(v), 6) for v in ens.trajectory_row(0)[9:13]]]
-        res[prec] = {"MLUPS": round(ens.nodes * steps / wall / 1e6, 2),
-                     "ms_per_step": round(wall * 1e3 / steps, 4),
-                     "step_launch_us": round(k_us, 2), "nodes": ens.nodes,
-                     "mode": ens.ctx.step_mode(), "schedule": tuned, "re_history": re_hist,
-                     "member0_discs_x_y_vx_vy": traj,
-                     "stopped": sum(r is not None for r in ens.reasons)}
+            leg["stopped"] = sum(r is not None for r in ens.reasons)
+        else:
+            leg["whole_step_launches_us"] = round(ens.step_launch_us(100), 2)
+        res[(prec, parts)] = leg
         ens.close()
-        if args.variant_steps <= 0:
-            break
     plain = None
     if args.variant_steps > 0:  # the plain FP64 path, one case at a time, 8 of the 64 cases
         cache, nodes, t_all = {}, 0, 0.0
@@ -369,34 +469,36 @@ def sed_bench(args):
             nodes += sim.st.nx * sim.st.ny * 100
             sim.ctx.close()
         plain = nodes / t_all / 1e6
-    b = res["fp32dev"]
-    achieved = BYTES_PER_NODE["fp32dev"] * b["nodes"] / (b["step_launch_us"] * 1e-6) / 1e9
+    one, two = res[("fp32dev", 1)], res[("fp32dev", 2)]
+    best = two if two["MLUPS"] > one["MLUPS"] else one
+    achieved = BYTES_PER_NODE["fp32dev"] * one["nodes"] / (one["step_launch_us"] * 1e-6) / 1e9
     out = {
         "metric": "MLUPS (fp32dev batched 64-member two-disc settling ensemble, whole step "
                   "incl. host particle update)",
-        "value": b["MLUPS"], "unit": "MLUPS", "n_gpus": 1, "steps": steps, "warmup": warm,
-        "ms_per_step": b["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "value": best["MLUPS"], "unit": "MLUPS", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": best["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 (deviation storage; FP64 IBM and particles)",
         "data": "synthetic (fluid at rest; seeded gaps in [2D,4D] and offsets in [0,0.1D], "
                 "numpy default_rng(2024))",
         "config": {"workload": "64 x two discs D=24, rho_ratio 1.01, channel 8D x 80D "
                                "(193x1921), DFC + hat + CM-MRT, explicit-history Verlet "
                                "with rotation, g=1.5e-3, nu=0.04",
-                   "nodes": b["nodes"], "mode": b["mode"], "schedule": b["schedule"],
-                   "stopped": b["stopped"]},
+                   "nodes": one["nodes"], "contexts": best["contexts"], "mode": one["mode"],
+                   "stopped": one["stopped"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "step launch (k_ibm || k_bulk), markers held",
+                     "kernel": "step launch (k_ibm || k_bulk), one context, markers held",
                      "bytes_per_node": BYTES_PER_NODE["fp32dev"],
-                     "avg_launch_us": b["step_launch_us"]},
-        "re_history": b["re_history"],
-        "member0_discs_x_y_vx_vy": b["member0_discs_x_y_vx_vy"],
+                     "avg_launch_us": one["step_launch_us"]},
+        "one_context": {k: v for k, v in one.items() if k != "re_history"},
+        "two_contexts_pipelined": two,
+        "re_history": one["re_history"],
     }
-    if "fp64" in res:
-        out["fp64_batched"] = {k: v for k, v in res["fp64"].items() if k != "re_history"}
+    if ("fp64", 1) in res:
+        out["fp64_batched"] = {k: v for k, v in res[("fp64", 1)].items() if k != "re_history"}
     if plain is not None:
         out["fp64_plain_case_by_case"] = {"MLUPS": round(plain, 2), "cases": 8, "steps": 100}
-        out["speedup_vs_fp64_plain"] = round(b["MLUPS"] / plain, 3)
+        out["speedup_vs_fp64_plain"] = round(best["MLUPS"] / plain, 3)
     return out
 
 

@@ -306,17 +306,27 @@ class SedimentationEnsemble:
                 T[k0 + j] = t
         return F, T
 
-    def step(self):
-        """One time step of every member."""
+    def begin_step(self):
+        """First half of a time step: particle half step, batched marker upload and the step
+        launch, which runs on while the host returns."""
         self.ttt += 1
         Lx, Ly, ud = self.bank.half_step()
         center = self.bank.pos[self.first]
         self.ctx.set_markers_batch(self.ids, self.lb, Lx, Ly, ud, self.larea, center)
         self.ctx.step(1, diag_every=1 if self.ttt % self.check_interval == 0 else 0)
+
+    def end_step(self):
+        """Second half: the forces (waits for this context's step), the full step, the
+        checks at the check interval."""
         F, T = self._forces()
         self.bank.full_step(F, T)
         if self.ttt % self.check_interval == 0:
             self._check()
+
+    def step(self):
+        """One time step of every member."""
+        self.begin_step()
+        self.end_step()
 
     def _check(self):
         self.bank.to_particles()
@@ -363,6 +373,71 @@ class SedimentationEnsemble:
 
     def close(self):
         self.ctx.close()
+
+
+class PipelinedEnsembles:
+    """Members split over ``parts`` contexts (each a ``SedimentationEnsemble`` on its own
+    streams) whose steps are interleaved: while one part's step runs on the device, the host
+    reads the forces of the part before it and moves its particles, so the host work of one
+    part hides behind the device work of the others.  Members are independent, so every
+    member's results are those of its single run."""
+
+    def __init__(self, cfgs, parts=2, precision="fp32dev", device=0):
+        idx = np.array_split(np.arange(len(cfgs)), max(1, min(int(parts), len(cfgs))))
+        self.parts = [SedimentationEnsemble([cfgs[i] for i in ix], precision=precision,
+                                            device=device) for ix in idx]
+        self.index = [ix for ix in idx]
+        self.nodes = sum(e.nodes for e in self.parts)
+        self.ttt = 0
+        self._open = False
+
+    def run(self, n):
+        """n time steps of every member; each part's step is launched before the previous
+        part's forces are read."""
+        n = int(n)
+        if n <= 0:
+            return
+        for e in self.parts:
+            e.begin_step()
+        for s in range(n):
+            last = s == n - 1
+            for e in self.parts:
+                e.end_step()
+                if not last:
+                    e.begin_step()
+        self.ttt += n
+        self.sync()
+
+    def sync(self):
+        for e in self.parts:
+            e.ctx.sync()
+
+    def autotune(self, steps=10):
+        return [e.ctx.autotune(steps) for e in self.parts]
+
+    def step_launch_us(self, n=100):
+        """Device time per step launch of all parts together, markers held (HIP events)."""
+        for e in self.parts:
+            e.ctx.sync()
+        import time
+        t0 = time.perf_counter()
+        for _ in range(int(n)):
+            for e in self.parts:
+                e.ctx.step(1)
+        self.sync()
+        return (time.perf_counter() - t0) * 1e6 / n
+
+    def member(self, i):
+        """(part, index within the part) of member i."""
+        for p, ix in enumerate(self.index):
+            hit = np.nonzero(ix == i)[0]
+            if hit.size:
+                return self.parts[p], int(hit[0])
+        raise IndexError(i)
+
+    def close(self):
+        for e in self.parts:
+            e.close()
 
 
 def two_disc_cfgs(n, seed=0, D=24, width_d=8, height_d=80, rho_ratio=1.01, gravity=1.5e-3,

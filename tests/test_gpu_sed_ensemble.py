@@ -131,3 +131,21 @@ def test_two_disc_ensemble_runs_and_keeps_members_apart():
     r1, f1 = _single(cfgs[0], 200, "fp32dev")
     assert np.array_equal(rows[0], r1)
     assert np.array_equal(fs[0], f1)
+
+
+def test_pipelined_parts_match_single_runs():
+    """The same three two-disc DFC members split over two contexts whose steps interleave."""
+    from iblbm_amd.sed_ensemble import PipelinedEnsembles
+    n = 60
+    cfgs = _dfc_variants()
+    print("pipelined two-disc DFC ensemble: 3 members over 2 contexts, 60 steps, fp64",
+          flush=True)
+    pe = PipelinedEnsembles(cfgs, parts=2, precision="fp64")
+    pe.run(n // 2)
+    pe.run(n - n // 2)
+    for i, cfg in enumerate(cfgs):
+        e, k = pe.member(i)
+        r1, f1 = _single(cfg, n)
+        assert np.array_equal(np.array(e.trajectory_row(k)), r1[-1]), i
+        assert np.array_equal(e.ctx.download_f(e.members[k].k), f1), i
+    pe.close()
