@@ -29,7 +29,7 @@ using namespace iblbm;
 struct RowRun {
     const Inst *I;
     int tb_lo, tb_hi;  // the instance's bulk tiles [tb_lo, tb_hi)
-    int tiles_x, nx, ny, ibm, bx0, by0, bw, bh, pitch;
+    int tiles_x, nx, ny, ibm, bx0, by0, bw, bh, pitch, bc;
     int64_t qs;
     const int *tindex;
     const void *A;  // source planes of this step
@@ -65,6 +65,7 @@ __device__ __attribute__((always_inline)) void row_run_load(const LaunchTable &T
     R.bw = C->bw;
     R.bh = C->bh;
     R.pitch = C->pitch;
+    R.bc = C->bc;
     R.qs = C->qs;
     R.tindex = C->s.tindex;
     R.A = C->f[T.parity];
@@ -154,6 +155,96 @@ __device__ __attribute__((always_inline)) void bulk_pull_interior(const RowRun &
     for (int q = 0; q < 9; ++q) g[q] = A[q * R.qs + (int64_t)(j - EYi[q]) * R.pitch + (i - EXi[q])];
 }
 
+// Pull of a row that touches the grid's boundary or runs past it.  Every lane loads its 9
+// values from addresses clamped into the grid (a node of the Dirichlet-Neumann outlet column
+// loads its inner neighbour's, whose sources all lie inside), so a row's loads issue together
+// with no branch around them; lanes on the boundary then take the closure's values in
+// registers, exactly as fstar_at / fstar_dev32 form them.  Lanes past the grid load valid
+// addresses and are masked by the caller.
+template <typename S, typename V>
+__device__ __attribute__((always_inline)) void edge_loads(const RowRun &R, int i, int j, V *v) {
+    const S *__restrict__ A = (const S *)R.A;
+    const int nx = R.nx, ny = R.ny;
+    int ci = i < nx - 1 ? i : nx - 1;
+    int cj = j < ny - 1 ? j : ny - 1;
+    if (R.bc == 0 && ci == nx - 1) {  // dirichlet_neumann.py:39-118: pull of (nx-2, jj)
+        ci = nx - 2;
+        cj = cj == 0 ? 1 : (cj == ny - 1 ? ny - 2 : cj);
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        int si = ci - EXi[q], sj = cj - EYi[q];
+        si = si < 0 ? 0 : (si > nx - 1 ? nx - 1 : si);
+        sj = sj < 0 ? 0 : (sj > ny - 1 ? ny - 1 : sj);
+        v[q] = A[q * R.qs + (int64_t)sj * R.pitch + si];
+    }
+}
+__device__ __attribute__((always_inline)) void edge_fix(const RowRun &R, int i, int j,
+                                                       double *fs) {
+    const int nx = R.nx, ny = R.ny;
+    if (i >= nx || j >= ny) return;
+    if (!((i == 0) | (j == 0) | (i == nx - 1) | (j == ny - 1))) return;
+    const Inst &I = *R.I;
+    if (R.bc == 0) {
+        if (i == nx - 1) {
+            double fin[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) fin[q] = fs[q];
+            neumann_copy(fin, fs);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) fs[q] = I.feq_bc[q];
+        }
+        return;
+    }
+    if ((i == 0 || i == nx - 1) && (j == 0 || j == ny - 1)) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) fs[q] = I.feq_bc[q];
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const int si = i - EXi[q], sj = j - EYi[q];
+        if (!(si >= 0 && si < nx && sj >= 0 && sj < ny)) fs[q] = 0.0;
+    }
+    settling_face(i == 0 ? 0 : (i == nx - 1 ? 1 : (j == ny - 1 ? 2 : 3)), fs);
+}
+__device__ __attribute__((always_inline)) void edge_fix(const RowRun &R, int i, int j,
+                                                       float *g) {
+    const int nx = R.nx, ny = R.ny;
+    if (i >= nx || j >= ny) return;
+    if (!((i == 0) | (j == 0) | (i == nx - 1) | (j == ny - 1))) return;
+    const Inst &I = *R.I;
+    if (R.bc == 0) {
+        if (i == nx - 1) {
+            float gin[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) gin[q] = g[q];
+            neumann_copy32(gin, g);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) g[q] = I.gbc[q];
+        }
+        return;
+    }
+    if ((i == 0 || i == nx - 1) && (j == 0 || j == ny - 1)) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) g[q] = I.gbc[q];
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const int si = i - EXi[q], sj = j - EYi[q];
+        if (!(si >= 0 && si < nx && sj >= 0 && sj < ny)) g[q] = 0.f;
+    }
+    settling_face32(i == 0 ? 0 : (i == nx - 1 ? 1 : (j == ny - 1 ? 2 : 3)), g);
+}
+template <typename V>
+__device__ __attribute__((always_inline)) void bulk_pull_edge(const RowRun &R, int i, int j,
+                                                             V *v) {
+    edge_loads<V, V>(R, i, j, v);
+}
+
 // The bulk work is a list of rows: row r is row r % TILE_Y of tile r / TILE_Y (TILE_X nodes,
 // one per lane of a wavefront).  A lane's node: coordinates, and whether the node is the
 // bulk's (inside the grid, not touched by a marker stencil).
@@ -189,7 +280,9 @@ __device__ __attribute__((always_inline)) TileNode row_node(const LaunchTable &T
 // rows first, first + nw, ... below stop (all rows when stop < 0); with TWO it keeps two rows'
 // loads in flight at a time.  Each row's FP64 partial rho sum is a fixed shuffle tree over the
 // wavefront.
-template <typename S, int MODEL, bool TWO>
+// EB: edge rows take the batched loads of edge_loads / edge_fix (else the per-lane general
+// pull); chosen per kernel where it does not raise the register count.
+template <typename S, int MODEL, bool TWO, bool EB>
 __device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, int first, int nw,
                                                         int stop = -1) {
     const int lane = (int)threadIdx.x & 63;
@@ -213,6 +306,16 @@ __device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, i
                 bulk_pull_interior(R, n0.i, n0.j, a0);
                 bulk_pull_interior(R1, n1.i, n1.j, a1);
                 __builtin_amdgcn_sched_barrier(0);
+                const double r0v = bulk_finish<MODEL>(*R.I, R, n0.i, n0.j, a0, n0.mine);
+                const double r1v = bulk_finish<MODEL>(*R1.I, R1, n1.i, n1.j, a1, n1.mine);
+                rho0 = n0.mine ? r0v : 0.0;
+                rho1 = n1.mine ? r1v : 0.0;
+            } else if constexpr (EB) {  // edge rows: the same batched loads, then the closure
+                bulk_pull_edge(R, n0.i, n0.j, a0);
+                bulk_pull_edge(R1, n1.i, n1.j, a1);
+                __builtin_amdgcn_sched_barrier(0);
+                edge_fix(R, n0.i, n0.j, a0);
+                edge_fix(R1, n1.i, n1.j, a1);
                 const double r0v = bulk_finish<MODEL>(*R.I, R, n0.i, n0.j, a0, n0.mine);
                 const double r1v = bulk_finish<MODEL>(*R1.I, R1, n1.i, n1.j, a1, n1.mine);
                 rho0 = n0.mine ? r0v : 0.0;
@@ -244,6 +347,11 @@ __device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, i
                 bulk_pull_interior(R, n0.i, n0.j, a0);
                 const double r = bulk_finish<MODEL>(*R.I, R, n0.i, n0.j, a0, n0.mine);
                 rho0 = n0.mine ? r : 0.0;
+            } else if constexpr (EB) {  // edge row: batched loads, then the closure per lane
+                bulk_pull_edge(R, n0.i, n0.j, a0);
+                edge_fix(R, n0.i, n0.j, a0);
+                const double r = bulk_finish<MODEL>(*R.I, R, n0.i, n0.j, a0, n0.mine);
+                rho0 = n0.mine ? r : 0.0;
             } else if (n0.mine) {
                 bulk_pull(*R.I, T.parity, n0.i, n0.j, a0);
                 rho0 = bulk_finish<MODEL>(*R.I, R, n0.i, n0.j, a0);
@@ -262,8 +370,9 @@ __device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, i
 template <typename S, int MODEL, bool TWO>
 __global__ __launch_bounds__(1024) void k_bulk(LaunchTable T) {
     const int wpb = (int)blockDim.x >> 6;
-    bulk_rows<S, MODEL, TWO>(T, (int)blockIdx.x * wpb + (int)(threadIdx.x >> 6),
-                             (int)gridDim.x * wpb);
+    constexpr bool EB = sizeof(S) == 4;
+    bulk_rows<S, MODEL, TWO, EB>(T, (int)blockIdx.x * wpb + (int)(threadIdx.x >> 6),
+                                 (int)gridDim.x * wpb);
 }
 
 // Immersed-boundary workgroups of one time step (one per instance with markers).
@@ -294,6 +403,7 @@ __global__ __launch_bounds__(NTF) void k_step(LaunchTable T) {
     }
     // bulk: every wavefront walks the rows with a grid stride (no workgroup barrier: a
     // wavefront moves on as soon as its rows are done)
+    constexpr bool EB = sizeof(S) == 4 || MODEL >= 0;
     const int first = ((int)blockIdx.x - T.n_fast) * (NTF / 64) + (int)(threadIdx.x >> 6);
     const int nw = ((int)gridDim.x - T.n_fast) * (NTF / 64);
     if (T.blocked) {  // one block of adjacent rows per wavefront
@@ -307,13 +417,13 @@ __global__ __launch_bounds__(NTF) void k_step(LaunchTable T) {
             r1 = min(n_rows, r0 + T.blocked);
         }
         if (T.rows2)
-            bulk_rows<S, MODEL, true>(T, r0, 1, r1);
+            bulk_rows<S, MODEL, true, EB>(T, r0, 1, r1);
         else
-            bulk_rows<S, MODEL, false>(T, r0, 1, r1);
+            bulk_rows<S, MODEL, false, EB>(T, r0, 1, r1);
     } else if (T.rows2) {
-        bulk_rows<S, MODEL, true>(T, first, nw);
+        bulk_rows<S, MODEL, true, EB>(T, first, nw);
     } else {
-        bulk_rows<S, MODEL, false>(T, first, nw);
+        bulk_rows<S, MODEL, false, EB>(T, first, nw);
     }
 }
 
