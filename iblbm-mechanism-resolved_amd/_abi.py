@@ -46,7 +46,8 @@ class Case(ctypes.Structure):
 class Schedule(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("bulk_threads", ctypes.c_int32),
                 ("bulk_rows", ctypes.c_int32), ("bulk_waves", ctypes.c_int32),
-                ("step_rows", ctypes.c_int32), ("step_order", ctypes.c_int32)]
+                ("step_rows", ctypes.c_int32), ("step_order", ctypes.c_int32),
+                ("row_layout", ctypes.c_int32)]
 
 
 class MdfStats(ctypes.Structure):
@@ -91,6 +92,7 @@ SIGNATURES = {
     "iblbm_step_mode": (_i, [_VP, _I]),
     "iblbm_get_schedule": (_i, [_VP, ctypes.POINTER(Schedule)]),
     "iblbm_set_schedule": (_i, [_VP, ctypes.POINTER(Schedule)]),
+    "iblbm_row_layout_in_use": (_i, [_VP, _I]),
     "iblbm_autotune": (_i, [_VP, _i, _D, _I]),
     "iblbm_init_flow": (_i, [_VP, _i, _d, _d, _d, _d, ctypes.c_uint64]),
     "iblbm_get_mdf_stats": (_i, [_VP, _i, ctypes.POINTER(MdfStats)]),

@@ -249,31 +249,75 @@ __device__ __attribute__((always_inline)) void bulk_pull_edge(const RowRun &R, i
 // one per lane of a wavefront).  A lane's node: coordinates, and whether the node is the
 // bulk's (inside the grid, not touched by a marker stencil).
 struct TileNode {
-    int i, j;
+    int i, j;  // j * pitch + i is the node's offset in a plane; (i, j) is the lattice node,
+               // except for a lane of a flat chunk past a row's end (see lattice_ij)
     bool mine;
     bool interior;  // (uniform) every node of the row and all its neighbours are in the grid
 };
 // row is uniform over the wavefront; R is reloaded only when the row leaves R's instance.
+// FL: the context's row layout (0: 64-column tiles of lattice rows; 1: flat chunks of 64
+// row-major nodes, every instance at least 64 nodes wide), fixed per kernel so that each
+// kernel carries one mapping.
+template <int FL>
 __device__ __attribute__((always_inline)) TileNode row_node(const LaunchTable &T, RowRun &R,
                                                            int row, int lane) {
     row = __builtin_amdgcn_readfirstlane(row);
     const int tb = row / TILE_Y;
     if (tb < R.tb_lo || tb >= R.tb_hi) row_run_load(T, tb, R);
     const int lt = tb - R.tb_lo;
-    const int tx = lt % R.tiles_x, ty = lt / R.tiles_x;
     TileNode n;
-    n.i = tx * TILE_X + lane;
-    n.j = ty * TILE_Y + row % TILE_Y;
-    const bool in_grid = n.i < R.nx && n.j < R.ny;
-    n.interior = tx * TILE_X >= 1 && tx * TILE_X + TILE_X <= R.nx - 1 && n.j >= 1 &&
-                 n.j <= R.ny - 2;
+    bool in_grid;
+    int ti, tj;  // the lane's lattice node
+    if constexpr (FL == 1) {  // chunk c holds nodes 64c .. 64c + 63 in row-major order
+        const int k0 = (lt * TILE_Y + row % TILE_Y) * TILE_X;
+        // floor(k0 / nx): a single-precision estimate (off by less than one for any grid
+        // below 2^20 rows), then one correction either way
+        int j0 = (int)((float)k0 * __builtin_amdgcn_rcpf((float)R.nx));
+        j0 -= j0 * R.nx > k0 ? 1 : 0;
+        j0 += (j0 + 1) * R.nx <= k0 ? 1 : 0;
+        j0 = __builtin_amdgcn_readfirstlane(j0);
+        const int i0 = k0 - j0 * R.nx;
+        const int il = i0 + lane;
+        const bool wrap = il >= R.nx;  // at most one wrap: nx >= 64
+        // n.j stays the chunk's first row (uniform); a wrapped lane's column is moved on by
+        // pitch - nx, so that j * pitch + i is still its node's address
+        n.i = wrap ? il - R.nx + R.pitch : il;
+        n.j = j0;
+        ti = wrap ? il - R.nx : il;
+        tj = wrap ? j0 + 1 : j0;
+        in_grid = tj < R.ny;
+        n.interior = i0 >= 1 && i0 + TILE_X <= R.nx - 1 && j0 >= 1 && j0 <= R.ny - 2;
+    } else {
+        const int tx = lt % R.tiles_x, ty = lt / R.tiles_x;
+        n.i = tx * TILE_X + lane;
+        n.j = ty * TILE_Y + row % TILE_Y;
+        ti = n.i;
+        tj = n.j;
+        in_grid = n.i < R.nx && n.j < R.ny;
+        n.interior = tx * TILE_X >= 1 && tx * TILE_X + TILE_X <= R.nx - 1 && n.j >= 1 &&
+                     n.j <= R.ny - 2;
+    }
     // nodes a marker stencil touches belong to the window workgroup
-    const bool in_box = R.ibm != 0 & n.i >= R.bx0 & n.i < R.bx0 + R.bw & n.j >= R.by0 &
-                        n.j < R.by0 + R.bh;
+    const bool in_box = R.ibm != 0 & ti >= R.bx0 & ti < R.bx0 + R.bw & tj >= R.by0 &
+                        tj < R.by0 + R.bh;
     bool touched = false;
-    if (in_box) touched = R.tindex[(n.j - R.by0) * R.bw + (n.i - R.bx0)] >= 0;
+    if (in_box) touched = R.tindex[(tj - R.by0) * R.bw + (ti - R.bx0)] >= 0;
     n.mine = in_grid && !touched;
     return n;
+}
+
+// The lattice node of a lane from its row coordinates (n.i may carry a flat chunk's wrap).
+template <int FL>
+__device__ __attribute__((always_inline)) void lattice_ij(const RowRun &R, const TileNode &n,
+                                                         int &i, int &j) {
+    if constexpr (FL == 1) {
+        const bool wrap = n.i >= R.nx;
+        i = wrap ? n.i - R.pitch : n.i;
+        j = wrap ? n.j + 1 : n.j;
+    } else {
+        i = n.i;
+        j = n.j;
+    }
 }
 
 // Bulk rows of one time step (every node that no marker stencil touches): a wavefront takes
@@ -282,7 +326,7 @@ __device__ __attribute__((always_inline)) TileNode row_node(const LaunchTable &T
 // wavefront.
 // EB: edge rows take the batched loads of edge_loads / edge_fix (else the per-lane general
 // pull); chosen per kernel where it does not raise the register count.
-template <typename S, int MODEL, bool TWO, bool EB>
+template <typename S, int MODEL, bool TWO, bool EB, int FL>
 __device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, int first, int nw,
                                                         int stop = -1) {
     const int lane = (int)threadIdx.x & 63;
@@ -297,8 +341,8 @@ __device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, i
         for (int r0 = first; r0 < n_rows; r0 += 2 * nw) {
             const bool v1 = r0 + nw < n_rows;
             const int r1 = v1 ? r0 + nw : r0;  // a missing second row repeats the first, unused
-            const TileNode n0 = row_node(T, R, r0, lane);
-            TileNode n1 = row_node(T, R1, r1, lane);
+            const TileNode n0 = row_node<FL>(T, R, r0, lane);
+            TileNode n1 = row_node<FL>(T, R1, r1, lane);
             n1.mine = n1.mine && v1;
             S a0[9], a1[9];
             double rho0 = 0.0, rho1 = 0.0;
@@ -311,22 +355,29 @@ __device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, i
                 rho0 = n0.mine ? r0v : 0.0;
                 rho1 = n1.mine ? r1v : 0.0;
             } else if constexpr (EB) {  // edge rows: the same batched loads, then the closure
-                bulk_pull_edge(R, n0.i, n0.j, a0);
-                bulk_pull_edge(R1, n1.i, n1.j, a1);
+                int i0, j0, i1, j1;
+                lattice_ij<FL>(R, n0, i0, j0);
+                lattice_ij<FL>(R1, n1, i1, j1);
+                bulk_pull_edge(R, i0, j0, a0);
+                bulk_pull_edge(R1, i1, j1, a1);
                 __builtin_amdgcn_sched_barrier(0);
-                edge_fix(R, n0.i, n0.j, a0);
-                edge_fix(R1, n1.i, n1.j, a1);
+                edge_fix(R, i0, j0, a0);
+                edge_fix(R1, i1, j1, a1);
                 const double r0v = bulk_finish<MODEL>(*R.I, R, n0.i, n0.j, a0, n0.mine);
                 const double r1v = bulk_finish<MODEL>(*R1.I, R1, n1.i, n1.j, a1, n1.mine);
                 rho0 = n0.mine ? r0v : 0.0;
                 rho1 = n1.mine ? r1v : 0.0;
             } else {
                 if (n0.mine) {
-                    bulk_pull(*R.I, T.parity, n0.i, n0.j, a0);
+                    int i0, j0;
+                    lattice_ij<FL>(R, n0, i0, j0);
+                    bulk_pull(*R.I, T.parity, i0, j0, a0);
                     rho0 = bulk_finish<MODEL>(*R.I, R, n0.i, n0.j, a0);
                 }
                 if (n1.mine) {
-                    bulk_pull(*R1.I, T.parity, n1.i, n1.j, a1);
+                    int i1, j1;
+                    lattice_ij<FL>(R1, n1, i1, j1);
+                    bulk_pull(*R1.I, T.parity, i1, j1, a1);
                     rho1 = bulk_finish<MODEL>(*R1.I, R1, n1.i, n1.j, a1);
                 }
             }
@@ -340,7 +391,7 @@ __device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, i
         }
     } else {
         for (int r0 = first; r0 < n_rows; r0 += nw) {
-            const TileNode n0 = row_node(T, R, r0, lane);
+            const TileNode n0 = row_node<FL>(T, R, r0, lane);
             S a0[9];
             double rho0 = 0.0;
             if (n0.interior) {
@@ -348,12 +399,16 @@ __device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, i
                 const double r = bulk_finish<MODEL>(*R.I, R, n0.i, n0.j, a0, n0.mine);
                 rho0 = n0.mine ? r : 0.0;
             } else if constexpr (EB) {  // edge row: batched loads, then the closure per lane
-                bulk_pull_edge(R, n0.i, n0.j, a0);
-                edge_fix(R, n0.i, n0.j, a0);
+                int i0, j0;
+                lattice_ij<FL>(R, n0, i0, j0);
+                bulk_pull_edge(R, i0, j0, a0);
+                edge_fix(R, i0, j0, a0);
                 const double r = bulk_finish<MODEL>(*R.I, R, n0.i, n0.j, a0, n0.mine);
                 rho0 = n0.mine ? r : 0.0;
             } else if (n0.mine) {
-                bulk_pull(*R.I, T.parity, n0.i, n0.j, a0);
+                int i0, j0;
+                lattice_ij<FL>(R, n0, i0, j0);
+                bulk_pull(*R.I, T.parity, i0, j0, a0);
                 rho0 = bulk_finish<MODEL>(*R.I, R, n0.i, n0.j, a0);
             }
             if (T.diag) {
@@ -367,11 +422,11 @@ __device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, i
 // Bulk kernel of the two-kernel step.  Any workgroup size up to 1024 (a tile of 64 x
 // blockDim / 64 rows per workgroup); the grid either covers every row once or walks them
 // with a grid stride (the schedule picks both).
-template <typename S, int MODEL, bool TWO>
+template <typename S, int MODEL, bool TWO, int FL>
 __global__ __launch_bounds__(1024) void k_bulk(LaunchTable T) {
     const int wpb = (int)blockDim.x >> 6;
     constexpr bool EB = sizeof(S) == 4;
-    bulk_rows<S, MODEL, TWO, EB>(T, (int)blockIdx.x * wpb + (int)(threadIdx.x >> 6),
+    bulk_rows<S, MODEL, TWO, EB, FL>(T, (int)blockIdx.x * wpb + (int)(threadIdx.x >> 6),
                                  (int)gridDim.x * wpb);
 }
 
@@ -386,7 +441,7 @@ __global__ __launch_bounds__(NTI) void k_ibm(LaunchTable T) {
 // One whole time step in one launch, for contexts whose windows all take the fused path:
 // workgroups [0, n_fast) run the windows (fused_step.h), the others run four bulk tiles each
 // (one node per thread).  One launch per step leaves no cross-stream wait between steps.
-template <typename S, int MODEL, bool PRED>
+template <typename S, int MODEL, bool PRED, int FL>
 __global__ __launch_bounds__(NTF) void k_step(LaunchTable T) {
     __shared__ double red[NTF / 64 + 2];
     __shared__ FusedShared sh;
@@ -417,13 +472,13 @@ __global__ __launch_bounds__(NTF) void k_step(LaunchTable T) {
             r1 = min(n_rows, r0 + T.blocked);
         }
         if (T.rows2)
-            bulk_rows<S, MODEL, true, EB>(T, r0, 1, r1);
+            bulk_rows<S, MODEL, true, EB, FL>(T, r0, 1, r1);
         else
-            bulk_rows<S, MODEL, false, EB>(T, r0, 1, r1);
+            bulk_rows<S, MODEL, false, EB, FL>(T, r0, 1, r1);
     } else if (T.rows2) {
-        bulk_rows<S, MODEL, true, EB>(T, first, nw);
+        bulk_rows<S, MODEL, true, EB, FL>(T, first, nw);
     } else {
-        bulk_rows<S, MODEL, false, EB>(T, first, nw);
+        bulk_rows<S, MODEL, false, EB, FL>(T, first, nw);
     }
 }
 
@@ -608,7 +663,7 @@ struct Ctx {
     int last_steps = 0;
     int64_t diag_step = -1;
     std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
-    iblbm_schedule sched{IBLBM_MODE_AUTO, 256, 1, 0, 2, 0};
+    iblbm_schedule sched{IBLBM_MODE_AUTO, 256, 1, 0, 2, 0, 0};
     // asynchronous diagnostics: two pinned host slots [row partials | per-instance box sums]
     // filled by copies queued behind the steps, each with its completion event
     double *h_diag[2] = {nullptr, nullptr};
@@ -643,9 +698,45 @@ int effective_mode(const Ctx &C) {
     return C.n_ibm > 0 ? IBLBM_MODE_TWO_STREAMS : IBLBM_MODE_SERIAL;
 }
 
+// Bulk row geometry of an instance: 64-column tiles of TILE_Y lattice rows, or (flat)
+// tiles of TILE_Y chunks of 64 consecutive row-major nodes.
+void set_row_layout(Inst &d, bool flat) {
+    d.flat = flat ? 1 : 0;
+    if (d.flat) {
+        const int64_t chunks = ((int64_t)d.nx * d.ny + TILE_X - 1) / TILE_X;
+        d.tiles_x = 1;
+        d.tiles_y = (int)((chunks + TILE_Y - 1) / TILE_Y);
+    } else {
+        d.tiles_x = (d.nx + TILE_X - 1) / TILE_X;
+        d.tiles_y = (d.ny + TILE_Y - 1) / TILE_Y;
+    }
+}
+
+// Applies the schedule's row layout to every instance (the row list changes: rebuild it).
+// The flat layout applies to the whole context when its populations are FP32 deviations and
+// every instance is at least 64 nodes wide (a chunk then wraps at most once); otherwise the
+// context keeps tiles.  (In FP64 the fused step kernel sits at its register limit, and the
+// per-lane row index of a wrapping chunk would make it spill.)
+bool flat_eligible(const Ctx &C) {
+    bool ok = !C.inst.empty();
+    for (const HostInst &h : C.inst) ok = ok && h.d.nx >= TILE_X && h.d.prec == 1;
+    return ok;
+}
+void apply_row_layout(Ctx &C) {
+    const bool flat = C.sched.row_layout == 1 && flat_eligible(C);
+    bool changed = false;
+    for (HostInst &h : C.inst) {
+        const int f = h.d.flat, tx = h.d.tiles_x, ty = h.d.tiles_y;
+        set_row_layout(h.d, flat);
+        changed |= f != h.d.flat || tx != h.d.tiles_x || ty != h.d.tiles_y;
+    }
+    if (changed) C.tables_dirty = true;
+}
+
 int ensure_tables(Ctx &C) {
     if (!C.tables_dirty) return 0;
     const int n = (int)C.inst.size();
+    if (n == 0) return 0;  // built once the first instance is added
     C.h_tile_prefix.assign(n + 1, 0);
     C.h_ibm_inst.clear();
     for (int k = 0; k < n; ++k) {
@@ -719,11 +810,19 @@ LaunchTable make_table(Ctx &C, int parity, int diag) {
 
 typedef void (*StepKernel)(LaunchTable);
 
-template <typename S, int MODEL>
-void pick_pair(bool pred, bool two, StepKernel &bulk, StepKernel &ibm, StepKernel &step) {
-    bulk = two ? k_bulk<S, MODEL, true> : k_bulk<S, MODEL, false>;
+template <typename S, int MODEL, int FL>
+void pick_layout(bool pred, bool two, StepKernel &bulk, StepKernel &ibm, StepKernel &step) {
+    bulk = two ? k_bulk<S, MODEL, true, FL> : k_bulk<S, MODEL, false, FL>;
     ibm = k_ibm<S, MODEL>;
-    step = pred ? k_step<S, MODEL, true> : k_step<S, MODEL, false>;
+    step = pred ? k_step<S, MODEL, true, FL> : k_step<S, MODEL, false, FL>;
+}
+template <typename S, int MODEL>
+void pick_pair(bool pred, bool two, bool flat, StepKernel &bulk, StepKernel &ibm,
+               StepKernel &step) {
+    if constexpr (sizeof(S) == 4) {  // FP64 contexts keep tiles (see apply_row_layout)
+        if (flat) return pick_layout<S, MODEL, 1>(pred, two, bulk, ibm, step);
+    }
+    pick_layout<S, MODEL, 0>(pred, two, bulk, ibm, step);
 }
 
 void pick_kernels(const Ctx &C, StepKernel &bulk, StepKernel &ibm, StepKernel &step) {
@@ -735,19 +834,20 @@ void pick_kernels(const Ctx &C, StepKernel &bulk, StepKernel &ibm, StepKernel &s
         pred = pred || h.d.pmode != 0;
     }
     const bool f32 = C.inst[0].d.prec == 1;
+    const bool fl = C.inst[0].d.flat != 0;  // one layout for the whole context
     switch (model * 2 + (f32 ? 1 : 0)) {
-        case 0: return pick_pair<double, 0>(pred, two, bulk, ibm, step);
-        case 1: return pick_pair<float, 0>(pred, two, bulk, ibm, step);
-        case 2: return pick_pair<double, 1>(pred, two, bulk, ibm, step);
-        case 3: return pick_pair<float, 1>(pred, two, bulk, ibm, step);
-        case 4: return pick_pair<double, 2>(pred, two, bulk, ibm, step);
-        case 5: return pick_pair<float, 2>(pred, two, bulk, ibm, step);
+        case 0: return pick_pair<double, 0>(pred, two, fl, bulk, ibm, step);
+        case 1: return pick_pair<float, 0>(pred, two, fl, bulk, ibm, step);
+        case 2: return pick_pair<double, 1>(pred, two, fl, bulk, ibm, step);
+        case 3: return pick_pair<float, 1>(pred, two, fl, bulk, ibm, step);
+        case 4: return pick_pair<double, 2>(pred, two, fl, bulk, ibm, step);
+        case 5: return pick_pair<float, 2>(pred, two, fl, bulk, ibm, step);
         default: break;
     }
     if (f32)
-        pick_pair<float, -1>(pred, two, bulk, ibm, step);
+        pick_pair<float, -1>(pred, two, fl, bulk, ibm, step);
     else
-        pick_pair<double, -1>(pred, two, bulk, ibm, step);
+        pick_pair<double, -1>(pred, two, fl, bulk, ibm, step);
 }
 
 // Enqueues nsteps steps under the context's schedule.
@@ -913,8 +1013,7 @@ int iblbm_instance_add(void *p, const iblbm_case *c, int *id) {
     d.qs = (int64_t)d.pitch * d.ny;
     d.bc = c->bc;
     d.inc = c->incompressible;
-    d.tiles_x = (c->nx + TILE_X - 1) / TILE_X;
-    d.tiles_y = (c->ny + TILE_Y - 1) / TILE_Y;
+    set_row_layout(d, false);  // the context's layout is applied once the instance is in
     CollParams &cp = d.cp;
     cp.model = c->collision;
     cp.dt = c->dt;
@@ -971,6 +1070,7 @@ int iblbm_instance_add(void *p, const iblbm_case *c, int *id) {
     h.c.ibm = c->ibm;
     C->inst.push_back(h);
     C->tables_dirty = true;
+    apply_row_layout(*C);
     *id = (int)C->inst.size() - 1;
     return 0;
 }
@@ -1539,6 +1639,13 @@ int iblbm_get_schedule(void *p, iblbm_schedule *out) {
     return 0;
 }
 
+int iblbm_row_layout_in_use(void *p, int *layout) {
+    Ctx *C = as_ctx(p);
+    if (!C || !layout) return fail(IBLBM_EINVAL, "row_layout_in_use: bad argument");
+    *layout = C->inst.empty() ? 0 : C->inst[0].d.flat;
+    return 0;
+}
+
 int iblbm_set_schedule(void *p, const iblbm_schedule *in) {
     Ctx *C = as_ctx(p);
     if (!C || !in) return fail(IBLBM_EINVAL, "set_schedule: bad argument");
@@ -1551,11 +1658,14 @@ int iblbm_set_schedule(void *p, const iblbm_schedule *in) {
     if (in->bulk_waves < 0) return fail(IBLBM_EINVAL, "set_schedule: bulk_waves < 0");
     if (in->step_order < 0 || in->step_order > 64)
         return fail(IBLBM_EINVAL, "set_schedule: step_order must be in [0, 64]");
+    if (in->row_layout != 0 && in->row_layout != 1)
+        return fail(IBLBM_EINVAL, "set_schedule: row_layout must be 0 (tiles) or 1 (flat)");
     HIPCHK(hipSetDevice(C->device));
     HIPCHK(hipStreamSynchronize(C->stream));
     C->sched = *in;
+    apply_row_layout(*C);
     clear_graphs(*C);
-    return 0;
+    return ensure_tables(*C);
 }
 
 int iblbm_autotune(void *p, int steps, double *best_ms, int *n_tried) {
@@ -1572,14 +1682,20 @@ int iblbm_autotune(void *p, int steps, double *best_ms, int *n_tried) {
     // over workgroup sizes, rows in flight and grid (every row once, or 16 / 32 wavefronts
     // per CU walking the rows)
     std::vector<iblbm_schedule> cand;
-    if (C->n_fast == C->n_ibm)
-        for (int order : {0, 1, 6, 8})
-            for (int r = 1; r <= 2; ++r) cand.push_back({IBLBM_MODE_FUSED, 256, 1, 0, r, order});
-    const int mode2 = C->n_ibm > 0 ? IBLBM_MODE_TWO_STREAMS : IBLBM_MODE_SERIAL;
-    for (int bt : {256, 512, 1024})
-        for (int r = 1; r <= 2; ++r)
-            for (int wv : {0, 16 * C->n_cu, 32 * C->n_cu})
-                cand.push_back({mode2, bt, r, wv, 2, 0});
+    // over both row layouts (the flat one where every instance is at least 64 nodes wide)
+    std::vector<int> layouts = {0};
+    if (flat_eligible(*C)) layouts.push_back(1);
+    for (int lay : layouts) {
+        if (C->n_fast == C->n_ibm)
+            for (int order : {0, 1, 6, 8})
+                for (int r = 1; r <= 2; ++r)
+                    cand.push_back({IBLBM_MODE_FUSED, 256, 1, 0, r, order, lay});
+        const int mode2 = C->n_ibm > 0 ? IBLBM_MODE_TWO_STREAMS : IBLBM_MODE_SERIAL;
+        for (int bt : {256, 512, 1024})
+            for (int r = 1; r <= 2; ++r)
+                for (int wv : {0, 16 * C->n_cu, 32 * C->n_cu})
+                    cand.push_back({mode2, bt, r, wv, 2, 0, lay});
+    }
     // the trial steps start from a copy of every instance's source populations
     const int n = (int)C->inst.size();
     std::vector<void *> backup(n, nullptr);
@@ -1620,7 +1736,9 @@ int iblbm_autotune(void *p, int steps, double *best_ms, int *n_tried) {
     const int warm = 2, timed = steps + (steps & 1);  // even: the trial ends on the same parity
     for (const iblbm_schedule &sc : cand) {
         C->sched = sc;
-        rc = restore();
+        apply_row_layout(*C);
+        rc = ensure_tables(*C);
+        if (!rc) rc = restore();
         if (!rc) rc = launch_steps(*C, warm, 0, true);
         C->parity = parity0;
         if (!rc) HIPCHK(hipEventRecord(C->ev0, C->stream));
@@ -1631,6 +1749,8 @@ int iblbm_autotune(void *p, int steps, double *best_ms, int *n_tried) {
             release();
             C->sched = keep;
             C->concurrent = conc;
+            apply_row_layout(*C);
+            (void)ensure_tables(*C);
             return rc;
         }
         float ms = 0.f;
@@ -1645,6 +1765,8 @@ int iblbm_autotune(void *p, int steps, double *best_ms, int *n_tried) {
     release();
     C->concurrent = conc;
     C->sched = pick;
+    apply_row_layout(*C);
+    if (!rc) rc = ensure_tables(*C);
     clear_graphs(*C);
     if (rc) return rc;
     if (best_ms) *best_ms = best;

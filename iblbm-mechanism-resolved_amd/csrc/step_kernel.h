@@ -69,7 +69,7 @@ struct Inst {
     int64_t qs;  // elements per direction plane
     double *f[2];  // float* when prec == 1 (FP32 deviations g = f - w)
     int bc, inc, tiles_x, tiles_y;
-    int prec, pad_p;
+    int prec, flat;  // flat: bulk rows are 64-node chunks of the row-major node order
     CollParams cp;
     CollParams32 cp32;
     float gbc[9];      // feq_bc - w in FP32 (deviation-form boundary constants)

@@ -123,15 +123,24 @@ class Context:
         mode = {v: k for k, v in _abi.MODES.items()}[sc.mode]
         return {"mode": mode, "bulk_threads": sc.bulk_threads, "bulk_rows": sc.bulk_rows,
                 "bulk_waves": sc.bulk_waves, "step_rows": sc.step_rows,
-                "step_order": {0: "stride", 1: "blocked"}.get(sc.step_order, sc.step_order)}
+                "step_order": {0: "stride", 1: "blocked"}.get(sc.step_order, sc.step_order),
+                "row_layout": ROW_LAYOUTS[sc.row_layout]}
 
     def set_schedule(self, mode="auto", bulk_threads=256, bulk_rows=1, bulk_waves=0,
-                     step_rows=2, step_order="stride"):
-        """step_order: "stride", "blocked" (one block per wavefront) or a block size k >= 2."""
+                     step_rows=2, step_order="stride", row_layout="tiles"):
+        """step_order: "stride", "blocked" (one block per wavefront) or a block size k >= 2;
+        row_layout: "tiles" (64-column tiles of lattice rows) or "flat" (64 consecutive
+        row-major nodes per bulk row)."""
         order = {"stride": 0, "blocked": 1}.get(step_order, step_order)
         sc = _abi.Schedule(_abi.MODES[mode], bulk_threads, bulk_rows, bulk_waves, step_rows,
-                           int(order))
+                           int(order), ROW_LAYOUTS.index(row_layout))
         check(self.lib.iblbm_set_schedule(self.h, ctypes.byref(sc)), "set_schedule")
+
+    def row_layout_in_use(self) -> str:
+        """The bulk row layout the instances use now ("tiles" or "flat")."""
+        v = ctypes.c_int()
+        check(self.lib.iblbm_row_layout_in_use(self.h, ctypes.byref(v)), "row_layout_in_use")
+        return ROW_LAYOUTS[v.value]
 
     def autotune(self, steps: int = 20) -> dict:
         """Times the candidate schedules on the current state (which is restored) and keeps
@@ -140,7 +149,8 @@ class Context:
         check(self.lib.iblbm_autotune(self.h, int(steps), ctypes.byref(ms), ctypes.byref(n)),
               "autotune")
         out = self.schedule()
-        out.update(ms_per_step=ms.value, candidates=n.value)
+        out.update(ms_per_step=ms.value, candidates=n.value,
+                   row_layout_in_use=self.row_layout_in_use())
         return out
 
     def init_flow(self, k, rho, ux, uy, amp=0.0, seed=0):
@@ -290,6 +300,7 @@ def pre_collide(cfg, st):
 
 
 PRECISIONS = {"fp64": _abi.FP64, "fp32dev": _abi.FP32_DEV}
+ROW_LAYOUTS = ["tiles", "flat"]
 
 
 class FixedCylinder:

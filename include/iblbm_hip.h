@@ -194,9 +194,17 @@ typedef struct {
                            * per bulk wavefront, all wavefronts; k >= 2 blocks of k rows (the
                            * wavefronts past the last block idle, leaving memory bandwidth to
                            * the windows); at most 64 */
+    int32_t row_layout;   /* 0: a bulk row is a 64-column tile of one lattice row; 1: a bulk
+                           * row is 64 consecutive nodes in row-major node order (no part-
+                           * empty tile at the end of a lattice row).  1 applies to contexts
+                           * of FP32-deviation storage whose instances are all at least 64
+                           * nodes wide; others keep tiles */
 } iblbm_schedule;
 int iblbm_get_schedule(void *ctx, iblbm_schedule *out);
 int iblbm_set_schedule(void *ctx, const iblbm_schedule *in);
+/* The row layout the context's instances use now (0 tiles, 1 flat): the schedule's, where
+ * the context qualifies for it. */
+int iblbm_row_layout_in_use(void *ctx, int *layout);
 /* Times every candidate schedule for `steps` steps (after 2 warm-up steps) on the context's
  * current state, restores the populations and the step counter after each trial, and keeps
  * the fastest.  best_ms (may be NULL) receives its time per step; n_tried the number of
