@@ -270,7 +270,10 @@ class SedimentationEnsemble:
     particles where they are from then on and records nothing more; its
     ``termination_reason`` is kept, as the single run's would be."""
 
-    def __init__(self, cfgs, precision="fp32dev", device=0):
+    def __init__(self, cfgs, precision="fp32dev", device=0, save_dir=None, first_index=0):
+        """save_dir: each member writes the reference's status.json and
+        sedimentation_history.json under save_dir/member_XXX (XXX = first_index + its
+        index) at every check interval; ``finalize_outputs`` completes them."""
         if not cfgs:
             raise ValueError("an ensemble needs at least one member")
         self.ctx = Context(device)
@@ -291,6 +294,15 @@ class SedimentationEnsemble:
         self.nodes = sum(m.st.nx * m.st.ny for m in self.members)
         self.histories = [[] for _ in self.members]
         self.reasons = [None] * len(self.members)
+        self.monitors = None
+        if save_dir:
+            import os
+            from .monitor import SedimentationMonitor
+            self.monitors = [SedimentationMonitor(
+                os.path.join(save_dir, f"member_{first_index + mi:03d}"), m.cfg)
+                for mi, m in enumerate(self.members)]
+        import time
+        self._t0 = time.time()
 
     def _forces(self):
         if self.batched:
@@ -336,6 +348,9 @@ class SedimentationEnsemble:
             m.ttt = self.ttt
             recs = m.record()
             self.histories[mi].append(recs if m.multi else recs[0])
+            if self.monitors is not None:
+                import time
+                self.monitors[mi](self.ttt, m.particles, elapsed=time.time() - self._t0)
             reason = m.should_stop()
             if reason is not None:
                 self.reasons[mi] = reason
@@ -349,6 +364,15 @@ class SedimentationEnsemble:
                 break
             self.step()
         self.ctx.sync()
+
+    def finalize_outputs(self):
+        """Marks every member's status.json complete (final step, termination reason, peak
+        Reynolds numbers) and writes its whole sedimentation_history.json."""
+        if self.monitors is None:
+            return
+        for mi, (m, mon) in enumerate(zip(self.members, self.monitors)):
+            mon.finalize(self.ttt, self.reasons[mi] or "max_steps", tau=m.st.tau,
+                         rho_ratios=[float(p.rho_ratio) for p in m.particles])
 
     def trajectory_row(self, mi):
         """[pos x, pos y, vel x, vel y, omega, angle, force x, force y, torque] of each of
@@ -382,10 +406,12 @@ class PipelinedEnsembles:
     part hides behind the device work of the others.  Members are independent, so every
     member's results are those of its single run."""
 
-    def __init__(self, cfgs, parts=2, precision="fp32dev", device=0):
+    def __init__(self, cfgs, parts=2, precision="fp32dev", device=0, save_dir=None):
         idx = np.array_split(np.arange(len(cfgs)), max(1, min(int(parts), len(cfgs))))
         self.parts = [SedimentationEnsemble([cfgs[i] for i in ix], precision=precision,
-                                            device=device) for ix in idx]
+                                            device=device, save_dir=save_dir,
+                                            first_index=int(ix[0]) if len(ix) else 0)
+                      for ix in idx]
         self.index = [ix for ix in idx]
         self.nodes = sum(e.nodes for e in self.parts)
         self.ttt = 0
@@ -414,6 +440,10 @@ class PipelinedEnsembles:
 
     def autotune(self, steps=10):
         return [e.ctx.autotune(steps) for e in self.parts]
+
+    def finalize_outputs(self):
+        for e in self.parts:
+            e.finalize_outputs()
 
     def step_launch_us(self, n=100):
         """Device time per step launch of all parts together, markers held (HIP events)."""

@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import time
 
 import numpy as np
 
@@ -419,12 +420,19 @@ def run(cfg, verbose: bool = True, callback=None, save_dir=None, device: int = 0
         precision: str = "fp64") -> dict:
     """``iblbm.run`` for sedimentation: result keys of the reference plus the history."""
     sim = Sedimentation(cfg, device=device, precision=precision)
+    mon = None
+    if save_dir:  # status.json + sedimentation_history.json in the reference's format
+        from .monitor import SedimentationMonitor
+        mon = SedimentationMonitor(save_dir, cfg)
     ci = int(cfg.check_interval)
+    t0 = time.time()
     while sim.ttt < cfg.max_steps:
         sim.step()
         if sim.ttt % ci == 0:
             recs = sim.record()
             sim.history.append(recs if sim.multi else recs[0])
+            if mon is not None:
+                mon(sim.ttt, sim.particles, sim.mdf_stats, elapsed=time.time() - t0)
             if verbose:
                 r0 = recs[0]
                 print(f"step {sim.ttt:>7d} | y*={r0['y_star']:.4f} vy*={r0['vy_star']:.6f}")
@@ -436,6 +444,9 @@ def run(cfg, verbose: bool = True, callback=None, save_dir=None, device: int = 0
                 sim.termination_reason = reason
                 break
     sim.ctx.sync()
+    if mon is not None:
+        mon.finalize(sim.ttt, sim.termination_reason, tau=sim.st.tau,
+                     rho_ratios=[float(p.rho_ratio) for p in sim.particles])
     return {"Cd_history": np.array([]), "Cl_history": np.array([]), "converged": False,
             "final_step": sim.ttt, "termination_reason": sim.termination_reason,
             "sedimentation_history": sim.history, "sim": sim}

@@ -377,6 +377,10 @@ def run(cfg, verbose: bool = True, callback=None, save_dir=None, perturb=None,
     if cfg.stream_first:
         raise ValueError("stream_first=True is outside the GPU hot path")
     sim = FixedCylinder(cfg, perturb=perturb, device=device, precision=precision)
+    mon = None
+    if save_dir:  # status.json in the reference's format
+        from .monitor import FixedCylinderMonitor
+        mon = FixedCylinderMonitor(save_dir, cfg)
     ci = int(cfg.check_interval)
     Cd_hist, Cl_hist, mdf_hist = [], [], []
     converged = False
@@ -405,6 +409,8 @@ def run(cfg, verbose: bool = True, callback=None, save_dir=None, perturb=None,
         if callback is not None:
             callback(step=ttt, Cd=Cd, Cl=Cl, error=error, state=sim, elapsed=time.time() - t0,
                      converged=converged)
+        if mon is not None:
+            mon(ttt, Cd, Cl, error=error, converged=converged, elapsed=time.time() - t0)
         if cfg.use_convergence and ttt >= cfg.convergence_start:
             _, _, U = sim.ctx.download_state(sim.k)
             if ttt == cfg.convergence_start or pre_U is None:
@@ -423,6 +429,8 @@ def run(cfg, verbose: bool = True, callback=None, save_dir=None, perturb=None,
                 converged = True
                 termination = "cd_converged"
     sim.ctx.sync()
+    if mon is not None:
+        mon.finalize(sim.ttt, termination, converged=converged, elapsed=time.time() - t0)
     return {"Cd_history": np.array(Cd_hist), "Cl_history": np.array(Cl_hist),
             "converged": converged, "final_step": sim.ttt, "termination_reason": termination,
             "mdf_stats": mdf_hist, "sim": sim, "state": sim.st}
