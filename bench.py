@@ -329,7 +329,7 @@ def sed1_bench(args):
     so each step is set up and read back by the host) and the whole step are timed."""
     from iblbm_amd.sedimentation import Sedimentation
     cfg = sed1_cfg()
-    sim = Sedimentation(cfg, precision="fp64")
+    sim = Sedimentation(cfg, precision=args.precision)
     pred = sim.predictor
     n = max(200, args.surrogate_steps // 2) if args.surrogate_steps > 0 else 1000
     for _ in range(1000):
@@ -361,20 +361,23 @@ def sed1_bench(args):
         wall = time.perf_counter() - t0
         p0 = sim.particles[0]
         st = pred.stats()
+        ms = sim.ctx.mdf_stats(sim.k)
         res[mode] = {"launch_us": round(dev_ms * 1e3 / n, 3),
                      "whole_step_us": round(wall * 1e6 / n, 2),
                      "vy": float(p0.vel[1]), "y": float(p0.pos[1]),
                      "substeps_per_step": round(st["substeps_per_step"], 3),
-                     "restarts": st["restarts"], "mode": sim.ctx.step_mode()}
+                     "restarts": st["restarts"], "mode": sim.ctx.step_mode(),
+                     "last_mdf_iter": ms["iter_count"], "last_residual": ms["last_residual"]}
     pl, le = res["off"], res["learned"]
     nu = (sim.st.tau - 0.5) / 3.0
     re_t = abs(pl["vy"]) * 2.0 * sim.st.lattice_r / nu
     nodes = sim.st.nx * sim.st.ny
     return {
-        "metric": "MDF step time, plain vs learned predictor (fp64 settling disc)",
+        "metric": f"MDF step time, plain vs learned predictor ({args.precision} settling disc)",
         "value": round(nodes / pl["launch_us"], 2), "unit": "MLUPS", "n_gpus": 1, "steps": n,
         "warmup": 2000, "ms_per_step": round(pl["whole_step_us"] / 1e3, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32 (deviation storage; FP64 IBM)",
         "data": "synthetic (disc released from rest)",
         "config": {"workload": "settling disc D=32 (101 markers), rho_ratio 1.5, channel "
                                "8D x 60D (257x1921), CM-MRT + MDF [5,20] tol 1e-4 + Peskin-4, "
