@@ -98,8 +98,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--schedule", default=None,
                     help="pin the launch schedule instead of autotuning: mode,bulk_threads,"
-                         "bulk_rows,bulk_waves,step_rows,step_order[,row_layout] (e.g. "
-                         "fused,256,1,0,1,blocked,flat), for profiling one schedule's kernels")
+                         "bulk_rows,bulk_waves,step_rows,step_order[,row_layout[,stores]] "
+                         "(e.g. fused,256,1,0,1,blocked,tiles,normal), for profiling one "
+                         "schedule's kernels")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch schedule instead of the autotuner's")
     ap.add_argument("--surrogate-steps", type=int, default=2000,
@@ -154,7 +155,8 @@ def main():
         order = f[5] if f[5] in ("stride", "blocked") else int(f[5])
         sim.ctx.set_schedule(mode=f[0], bulk_threads=int(f[1]), bulk_rows=int(f[2]),
                              bulk_waves=int(f[3]), step_rows=int(f[4]), step_order=order,
-                             row_layout=f[6] if len(f) > 6 else "tiles")
+                             row_layout=f[6] if len(f) > 6 else "tiles",
+                             stores=f[7] if len(f) > 7 else "normal")
         tuned = sim.ctx.schedule()
         mode = sim.ctx.step_mode()
         sim.advance(ci, diag_every=ci, graph=graph)  # the chunk's graph, outside the timing

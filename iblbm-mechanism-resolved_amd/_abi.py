@@ -47,7 +47,7 @@ class Schedule(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("bulk_threads", ctypes.c_int32),
                 ("bulk_rows", ctypes.c_int32), ("bulk_waves", ctypes.c_int32),
                 ("step_rows", ctypes.c_int32), ("step_order", ctypes.c_int32),
-                ("row_layout", ctypes.c_int32)]
+                ("row_layout", ctypes.c_int32), ("store_mode", ctypes.c_int32)]
 
 
 class MdfStats(ctypes.Structure):

@@ -83,7 +83,9 @@ __device__ __attribute__((always_inline)) void bulk_pull(const Inst &I, int pari
                                                         double *fs) {
     fstar_at(buf<double>(I, parity), I, i, j, fs);
 }
-template <int MODEL>
+// nt: non-temporal stores (the populations written this step are read next step only, so
+// for a working set larger than the last-level cache they bypass it)
+template <int MODEL, bool NT = false>
 __device__ __attribute__((always_inline)) double bulk_finish(const Inst &I, const RowRun &R, int i,
                                                             int j, double *fs, bool own = true) {
     double *__restrict__ B = (double *)R.B;
@@ -105,8 +107,14 @@ __device__ __attribute__((always_inline)) double bulk_finish(const Inst &I, cons
         }
     }
     if (own) {
+        if constexpr (NT) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) B[q * R.qs + (int64_t)j * R.pitch + i] = out[q];
+            for (int q = 0; q < 9; ++q)
+                __builtin_nontemporal_store(out[q], &B[q * R.qs + (int64_t)j * R.pitch + i]);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) B[q * R.qs + (int64_t)j * R.pitch + i] = out[q];
+        }
     }
     return r;
 }
@@ -116,7 +124,7 @@ __device__ __attribute__((always_inline)) void bulk_pull(const Inst &I, int pari
                                                         float *g) {
     fstar_dev32(buf<float>(I, parity), I, i, j, g);
 }
-template <int MODEL>
+template <int MODEL, bool NT = false>
 __device__ __attribute__((always_inline)) double bulk_finish(const Inst &I, const RowRun &R, int i,
                                                             int j, float *g, bool own = true) {
     float *__restrict__ B = (float *)R.B;
@@ -135,8 +143,14 @@ __device__ __attribute__((always_inline)) double bulk_finish(const Inst &I, cons
             collide32<2>(I.cp32, g, drho, ux, uy, inc, out);
     }
     if (own) {
+        if constexpr (NT) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) B[q * R.qs + (int64_t)j * R.pitch + i] = out[q];
+            for (int q = 0; q < 9; ++q)
+                __builtin_nontemporal_store(out[q], &B[q * R.qs + (int64_t)j * R.pitch + i]);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) B[q * R.qs + (int64_t)j * R.pitch + i] = out[q];
+        }
     }
     return 1.0 + (double)drho;
 }
@@ -326,7 +340,7 @@ __device__ __attribute__((always_inline)) void lattice_ij(const RowRun &R, const
 // wavefront.
 // EB: edge rows take the batched loads of edge_loads / edge_fix (else the per-lane general
 // pull); chosen per kernel where it does not raise the register count.
-template <typename S, int MODEL, bool TWO, bool EB, int FL>
+template <typename S, int MODEL, bool TWO, bool EB, int FL, bool NT = false>
 __device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, int first, int nw,
                                                         int stop = -1) {
     const int lane = (int)threadIdx.x & 63;
@@ -350,8 +364,10 @@ __device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, i
                 bulk_pull_interior(R, n0.i, n0.j, a0);
                 bulk_pull_interior(R1, n1.i, n1.j, a1);
                 __builtin_amdgcn_sched_barrier(0);
-                const double r0v = bulk_finish<MODEL>(*R.I, R, n0.i, n0.j, a0, n0.mine);
-                const double r1v = bulk_finish<MODEL>(*R1.I, R1, n1.i, n1.j, a1, n1.mine);
+                const double r0v =
+                    bulk_finish<MODEL, NT>(*R.I, R, n0.i, n0.j, a0, n0.mine);
+                const double r1v =
+                    bulk_finish<MODEL, NT>(*R1.I, R1, n1.i, n1.j, a1, n1.mine);
                 rho0 = n0.mine ? r0v : 0.0;
                 rho1 = n1.mine ? r1v : 0.0;
             } else if constexpr (EB) {  // edge rows: the same batched loads, then the closure
@@ -363,8 +379,10 @@ __device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, i
                 __builtin_amdgcn_sched_barrier(0);
                 edge_fix(R, i0, j0, a0);
                 edge_fix(R1, i1, j1, a1);
-                const double r0v = bulk_finish<MODEL>(*R.I, R, n0.i, n0.j, a0, n0.mine);
-                const double r1v = bulk_finish<MODEL>(*R1.I, R1, n1.i, n1.j, a1, n1.mine);
+                const double r0v =
+                    bulk_finish<MODEL, NT>(*R.I, R, n0.i, n0.j, a0, n0.mine);
+                const double r1v =
+                    bulk_finish<MODEL, NT>(*R1.I, R1, n1.i, n1.j, a1, n1.mine);
                 rho0 = n0.mine ? r0v : 0.0;
                 rho1 = n1.mine ? r1v : 0.0;
             } else {
@@ -372,13 +390,13 @@ __device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, i
                     int i0, j0;
                     lattice_ij<FL>(R, n0, i0, j0);
                     bulk_pull(*R.I, T.parity, i0, j0, a0);
-                    rho0 = bulk_finish<MODEL>(*R.I, R, n0.i, n0.j, a0);
+                    rho0 = bulk_finish<MODEL, NT>(*R.I, R, n0.i, n0.j, a0, true);
                 }
                 if (n1.mine) {
                     int i1, j1;
                     lattice_ij<FL>(R1, n1, i1, j1);
                     bulk_pull(*R1.I, T.parity, i1, j1, a1);
-                    rho1 = bulk_finish<MODEL>(*R1.I, R1, n1.i, n1.j, a1);
+                    rho1 = bulk_finish<MODEL, NT>(*R1.I, R1, n1.i, n1.j, a1, true);
                 }
             }
             if (T.diag) {
@@ -396,20 +414,20 @@ __device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, i
             double rho0 = 0.0;
             if (n0.interior) {
                 bulk_pull_interior(R, n0.i, n0.j, a0);
-                const double r = bulk_finish<MODEL>(*R.I, R, n0.i, n0.j, a0, n0.mine);
+                const double r = bulk_finish<MODEL, NT>(*R.I, R, n0.i, n0.j, a0, n0.mine);
                 rho0 = n0.mine ? r : 0.0;
             } else if constexpr (EB) {  // edge row: batched loads, then the closure per lane
                 int i0, j0;
                 lattice_ij<FL>(R, n0, i0, j0);
                 bulk_pull_edge(R, i0, j0, a0);
                 edge_fix(R, i0, j0, a0);
-                const double r = bulk_finish<MODEL>(*R.I, R, n0.i, n0.j, a0, n0.mine);
+                const double r = bulk_finish<MODEL, NT>(*R.I, R, n0.i, n0.j, a0, n0.mine);
                 rho0 = n0.mine ? r : 0.0;
             } else if (n0.mine) {
                 int i0, j0;
                 lattice_ij<FL>(R, n0, i0, j0);
                 bulk_pull(*R.I, T.parity, i0, j0, a0);
-                rho0 = bulk_finish<MODEL>(*R.I, R, n0.i, n0.j, a0);
+                rho0 = bulk_finish<MODEL, NT>(*R.I, R, n0.i, n0.j, a0, true);
             }
             if (T.diag) {
                 const double w0 = wave_sum(rho0);
@@ -422,11 +440,13 @@ __device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, i
 // Bulk kernel of the two-kernel step.  Any workgroup size up to 1024 (a tile of 64 x
 // blockDim / 64 rows per workgroup); the grid either covers every row once or walks them
 // with a grid stride (the schedule picks both).
-template <typename S, int MODEL, bool TWO, int FL>
+// NT: non-temporal stores (a compile-time choice: as a run-time flag the compiler merges the
+// two store forms and drops the hint)
+template <typename S, int MODEL, bool TWO, int FL, bool NT>
 __global__ __launch_bounds__(1024) void k_bulk(LaunchTable T) {
     const int wpb = (int)blockDim.x >> 6;
     constexpr bool EB = sizeof(S) == 4;
-    bulk_rows<S, MODEL, TWO, EB, FL>(T, (int)blockIdx.x * wpb + (int)(threadIdx.x >> 6),
+    bulk_rows<S, MODEL, TWO, EB, FL, NT>(T, (int)blockIdx.x * wpb + (int)(threadIdx.x >> 6),
                                  (int)gridDim.x * wpb);
 }
 
@@ -663,7 +683,7 @@ struct Ctx {
     int last_steps = 0;
     int64_t diag_step = -1;
     std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
-    iblbm_schedule sched{IBLBM_MODE_AUTO, 256, 1, 0, 2, 0, 0};
+    iblbm_schedule sched{IBLBM_MODE_AUTO, 256, 1, 0, 2, 0, 0, 0};
     // asynchronous diagnostics: two pinned host slots [row partials | per-instance box sums]
     // filled by copies queued behind the steps, each with its completion event
     double *h_diag[2] = {nullptr, nullptr};
@@ -811,18 +831,22 @@ LaunchTable make_table(Ctx &C, int parity, int diag) {
 typedef void (*StepKernel)(LaunchTable);
 
 template <typename S, int MODEL, int FL>
-void pick_layout(bool pred, bool two, StepKernel &bulk, StepKernel &ibm, StepKernel &step) {
-    bulk = two ? k_bulk<S, MODEL, true, FL> : k_bulk<S, MODEL, false, FL>;
+void pick_layout(bool pred, bool two, bool nt, StepKernel &bulk, StepKernel &ibm,
+                 StepKernel &step) {
+    if (nt)
+        bulk = two ? k_bulk<S, MODEL, true, FL, true> : k_bulk<S, MODEL, false, FL, true>;
+    else
+        bulk = two ? k_bulk<S, MODEL, true, FL, false> : k_bulk<S, MODEL, false, FL, false>;
     ibm = k_ibm<S, MODEL>;
     step = pred ? k_step<S, MODEL, true, FL> : k_step<S, MODEL, false, FL>;
 }
 template <typename S, int MODEL>
-void pick_pair(bool pred, bool two, bool flat, StepKernel &bulk, StepKernel &ibm,
+void pick_pair(bool pred, bool two, bool flat, bool nt, StepKernel &bulk, StepKernel &ibm,
                StepKernel &step) {
     if constexpr (sizeof(S) == 4) {  // FP64 contexts keep tiles (see apply_row_layout)
-        if (flat) return pick_layout<S, MODEL, 1>(pred, two, bulk, ibm, step);
+        if (flat) return pick_layout<S, MODEL, 1>(pred, two, nt, bulk, ibm, step);
     }
-    pick_layout<S, MODEL, 0>(pred, two, bulk, ibm, step);
+    pick_layout<S, MODEL, 0>(pred, two, nt, bulk, ibm, step);
 }
 
 void pick_kernels(const Ctx &C, StepKernel &bulk, StepKernel &ibm, StepKernel &step) {
@@ -835,19 +859,20 @@ void pick_kernels(const Ctx &C, StepKernel &bulk, StepKernel &ibm, StepKernel &s
     }
     const bool f32 = C.inst[0].d.prec == 1;
     const bool fl = C.inst[0].d.flat != 0;  // one layout for the whole context
+    const bool nt = C.sched.store_mode == 1;
     switch (model * 2 + (f32 ? 1 : 0)) {
-        case 0: return pick_pair<double, 0>(pred, two, fl, bulk, ibm, step);
-        case 1: return pick_pair<float, 0>(pred, two, fl, bulk, ibm, step);
-        case 2: return pick_pair<double, 1>(pred, two, fl, bulk, ibm, step);
-        case 3: return pick_pair<float, 1>(pred, two, fl, bulk, ibm, step);
-        case 4: return pick_pair<double, 2>(pred, two, fl, bulk, ibm, step);
-        case 5: return pick_pair<float, 2>(pred, two, fl, bulk, ibm, step);
+        case 0: return pick_pair<double, 0>(pred, two, fl, nt, bulk, ibm, step);
+        case 1: return pick_pair<float, 0>(pred, two, fl, nt, bulk, ibm, step);
+        case 2: return pick_pair<double, 1>(pred, two, fl, nt, bulk, ibm, step);
+        case 3: return pick_pair<float, 1>(pred, two, fl, nt, bulk, ibm, step);
+        case 4: return pick_pair<double, 2>(pred, two, fl, nt, bulk, ibm, step);
+        case 5: return pick_pair<float, 2>(pred, two, fl, nt, bulk, ibm, step);
         default: break;
     }
     if (f32)
-        pick_pair<float, -1>(pred, two, fl, bulk, ibm, step);
+        pick_pair<float, -1>(pred, two, fl, nt, bulk, ibm, step);
     else
-        pick_pair<double, -1>(pred, two, fl, bulk, ibm, step);
+        pick_pair<double, -1>(pred, two, fl, nt, bulk, ibm, step);
 }
 
 // Enqueues nsteps steps under the context's schedule.
@@ -1660,6 +1685,8 @@ int iblbm_set_schedule(void *p, const iblbm_schedule *in) {
         return fail(IBLBM_EINVAL, "set_schedule: step_order must be in [0, 64]");
     if (in->row_layout != 0 && in->row_layout != 1)
         return fail(IBLBM_EINVAL, "set_schedule: row_layout must be 0 (tiles) or 1 (flat)");
+    if (in->store_mode != 0 && in->store_mode != 1)
+        return fail(IBLBM_EINVAL, "set_schedule: store_mode must be 0 or 1 (non-temporal)");
     HIPCHK(hipSetDevice(C->device));
     HIPCHK(hipStreamSynchronize(C->stream));
     C->sched = *in;
@@ -1685,17 +1712,25 @@ int iblbm_autotune(void *p, int steps, double *best_ms, int *n_tried) {
     // over both row layouts (the flat one where every instance is at least 64 nodes wide)
     std::vector<int> layouts = {0};
     if (flat_eligible(*C)) layouts.push_back(1);
-    for (int lay : layouts) {
-        if (C->n_fast == C->n_ibm)
-            for (int order : {0, 1, 6, 8})
+    // and both store modes where the populations outgrow the 256 MB last-level cache
+    // (below that, the next step's loads hit in it after normal stores)
+    size_t pop_bytes = 0;
+    for (const HostInst &h : C->inst)
+        pop_bytes += 2 * 9 * (size_t)h.d.qs * (h.d.prec == 1 ? sizeof(float) : sizeof(double));
+    std::vector<int> stores = {0};
+    if (pop_bytes > ((size_t)256 << 20)) stores.push_back(1);
+    for (int st : stores)
+        for (int lay : layouts) {
+            if (C->n_fast == C->n_ibm && st == 0)  // the fused step stores normally
+                for (int order : {0, 1, 6, 8})
+                    for (int r = 1; r <= 2; ++r)
+                        cand.push_back({IBLBM_MODE_FUSED, 256, 1, 0, r, order, lay, st});
+            const int mode2 = C->n_ibm > 0 ? IBLBM_MODE_TWO_STREAMS : IBLBM_MODE_SERIAL;
+            for (int bt : {256, 512, 1024})
                 for (int r = 1; r <= 2; ++r)
-                    cand.push_back({IBLBM_MODE_FUSED, 256, 1, 0, r, order, lay});
-        const int mode2 = C->n_ibm > 0 ? IBLBM_MODE_TWO_STREAMS : IBLBM_MODE_SERIAL;
-        for (int bt : {256, 512, 1024})
-            for (int r = 1; r <= 2; ++r)
-                for (int wv : {0, 16 * C->n_cu, 32 * C->n_cu})
-                    cand.push_back({mode2, bt, r, wv, 2, 0, lay});
-    }
+                    for (int wv : {0, 16 * C->n_cu, 32 * C->n_cu})
+                        cand.push_back({mode2, bt, r, wv, 2, 0, lay, st});
+        }
     // the trial steps start from a copy of every instance's source populations
     const int n = (int)C->inst.size();
     std::vector<void *> backup(n, nullptr);

@@ -124,16 +124,18 @@ class Context:
         return {"mode": mode, "bulk_threads": sc.bulk_threads, "bulk_rows": sc.bulk_rows,
                 "bulk_waves": sc.bulk_waves, "step_rows": sc.step_rows,
                 "step_order": {0: "stride", 1: "blocked"}.get(sc.step_order, sc.step_order),
-                "row_layout": ROW_LAYOUTS[sc.row_layout]}
+                "row_layout": ROW_LAYOUTS[sc.row_layout],
+                "stores": STORE_MODES[sc.store_mode]}
 
     def set_schedule(self, mode="auto", bulk_threads=256, bulk_rows=1, bulk_waves=0,
-                     step_rows=2, step_order="stride", row_layout="tiles"):
+                     step_rows=2, step_order="stride", row_layout="tiles", stores="normal"):
         """step_order: "stride", "blocked" (one block per wavefront) or a block size k >= 2;
         row_layout: "tiles" (64-column tiles of lattice rows) or "flat" (64 consecutive
-        row-major nodes per bulk row)."""
+        row-major nodes per bulk row); stores: "normal" or "nontemporal"."""
         order = {"stride": 0, "blocked": 1}.get(step_order, step_order)
         sc = _abi.Schedule(_abi.MODES[mode], bulk_threads, bulk_rows, bulk_waves, step_rows,
-                           int(order), ROW_LAYOUTS.index(row_layout))
+                           int(order), ROW_LAYOUTS.index(row_layout),
+                           STORE_MODES.index(stores))
         check(self.lib.iblbm_set_schedule(self.h, ctypes.byref(sc)), "set_schedule")
 
     def row_layout_in_use(self) -> str:
@@ -301,6 +303,7 @@ def pre_collide(cfg, st):
 
 PRECISIONS = {"fp64": _abi.FP64, "fp32dev": _abi.FP32_DEV}
 ROW_LAYOUTS = ["tiles", "flat"]
+STORE_MODES = ["normal", "nontemporal"]
 
 
 class FixedCylinder:

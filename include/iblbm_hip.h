@@ -199,6 +199,10 @@ typedef struct {
                            * empty tile at the end of a lattice row).  1 applies to contexts
                            * of FP32-deviation storage whose instances are all at least 64
                            * nodes wide; others keep tiles */
+    int32_t store_mode;   /* two-kernel modes: 0: the bulk stores the new populations
+                           * normally; 1: with non-temporal stores, which leave the last-level
+                           * cache to the loads (pays once the populations outgrow it: about
+                           * 256 MB on MI355X).  The fused step always stores normally */
 } iblbm_schedule;
 int iblbm_get_schedule(void *ctx, iblbm_schedule *out);
 int iblbm_set_schedule(void *ctx, const iblbm_schedule *in);
