@@ -1725,11 +1725,15 @@ int iblbm_autotune(void *p, int steps, double *best_ms, int *n_tried) {
                 for (int order : {0, 1, 6, 8})
                     for (int r = 1; r <= 2; ++r)
                         cand.push_back({IBLBM_MODE_FUSED, 256, 1, 0, r, order, lay, st});
-            const int mode2 = C->n_ibm > 0 ? IBLBM_MODE_TWO_STREAMS : IBLBM_MODE_SERIAL;
-            for (int bt : {256, 512, 1024})
-                for (int r = 1; r <= 2; ++r)
-                    for (int wv : {0, 16 * C->n_cu, 32 * C->n_cu})
-                        cand.push_back({mode2, bt, r, wv, 2, 0, lay, st});
+            // the two-kernel launch with the windows beside the bulk (two streams) or before
+            // it (serial: no contention for the CUs, no overlap either)
+            std::vector<int> modes2 = {IBLBM_MODE_SERIAL};
+            if (C->n_ibm > 0) modes2.push_back(IBLBM_MODE_TWO_STREAMS);
+            for (int mode2 : modes2)
+                for (int bt : {256, 512, 1024})
+                    for (int r = 1; r <= 2; ++r)
+                        for (int wv : {0, 16 * C->n_cu, 32 * C->n_cu})
+                            cand.push_back({mode2, bt, r, wv, 2, 0, lay, st});
         }
     // the trial steps start from a copy of every instance's source populations
     const int n = (int)C->inst.size();
