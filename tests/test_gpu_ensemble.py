@@ -51,6 +51,8 @@ def _run_ensemble(cases, schedule):
     dict(mode="two_streams", bulk_threads=1024, bulk_rows=2, bulk_waves=0),
     dict(mode="two_streams", bulk_threads=512, bulk_rows=1, bulk_waves=40),
     dict(mode="serial", bulk_threads=64, bulk_rows=2, bulk_waves=7),
+    dict(mode="two_streams", bulk_threads=512, bulk_rows=1, bulk_waves=0, stores="nontemporal"),
+    dict(mode="serial", bulk_threads=256, bulk_rows=2, bulk_waves=0, stores="nontemporal"),
     "autotune",
 ])
 def test_mixed_ensemble_matches_reference_runs(schedule):

@@ -83,3 +83,25 @@ def test_flat_layout_settling_matches_reference():
     assert rel_linf(row, g["particles"][-1, 1:5]) <= 1e-10
     np.testing.assert_array_equal(sim.ctx.download_f(sim.k), g["f_post"])
     ctx.close()
+
+
+@pytest.mark.parametrize("stores", ["normal", "nontemporal"])
+@pytest.mark.parametrize("layout", ["tiles", "flat"])
+def test_store_modes_and_layouts_agree_on_settling_members(layout, stores):
+    """Two full-size two-disc members (193 x 1921, FP32 deviations): every layout and store
+    mode of the serial two-kernel launch gives the populations of the default schedule."""
+    from iblbm_amd.sed_ensemble import SedimentationEnsemble, two_disc_cfgs
+    cfgs = two_disc_cfgs(2, seed=9, check_interval=50)
+    out = []
+    for sched in (None, dict(mode="serial", bulk_threads=256, row_layout=layout,
+                             stores=stores)):
+        ens = SedimentationEnsemble(cfgs, precision="fp32dev")
+        if sched is not None:
+            ens.ctx.set_schedule(**sched)
+            assert ens.ctx.row_layout_in_use() == layout
+        print(f"settling members, schedule {sched}, 50 steps", flush=True)
+        ens.run(50)
+        out.append([ens.ctx.download_f(m.k) for m in ens.members])
+        ens.close()
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a, b)
