@@ -46,3 +46,46 @@ def test_thousand_steps_match_oracle(case):
     assert err <= 1e-12
     assert rel_linf(cd, ref.Cd) <= 1e-10
     assert np.max(np.abs(np.array(cl) - np.array(ref.Cl)) / np.abs(np.array(ref.Cd))) <= 1e-10
+
+
+def _c1_cfg(steps):
+    """401 x 161 channel, D = 20 at (100, 80), 63 markers, U = 0.05, Re = 20: BGK + DF + hat,
+    Dirichlet inflow / Neumann outflow."""
+    return SimConfig(Re=20.0, NN=161, xmax=2.5, ymax=1.0, cylinder_center=(0.625, 0.5),
+                     cylinder_D_ratio=0.125, lattice_u=0.05, inflow_u=0.05,
+                     bc_type="dirichlet_neumann", delta_type="hat", ibm_method="DF",
+                     collision_model="BGK", marker_spacing_factor=1.0, mdf_iterations=1,
+                     max_steps=steps, check_interval=100, use_convergence=False)
+
+
+def _c2_cfg(steps):
+    """1601 x 641 channel, D = 40, 126 markers, U = 0.04, Re = 100: TRT (Lambda = 1/4) +
+    MDF [5, 20] tol 1e-4 + Peskin-4 (the benchmark workload)."""
+    return SimConfig(Re=100.0, NN=641, xmax=2.5, ymax=1.0, cylinder_center=(0.625, 0.5),
+                     cylinder_D_ratio=1.0 / 16.0, lattice_u=0.04, inflow_u=0.04,
+                     bc_type="dirichlet_neumann", delta_type="peskin4pt", ibm_method="MDF",
+                     collision_model="TRT", trt_magic_param=0.25, marker_spacing_factor=1.0,
+                     mdf_iterations=20, mdf_min_iterations=5, mdf_tolerance=1e-4,
+                     max_steps=steps, check_interval=20, use_convergence=False)
+
+
+@pytest.mark.parametrize("which,steps,perturb", [("C1", 400, None), ("C2", 60, (1234, 1e-3))])
+def test_benchmark_configs_match_oracle(which, steps, perturb):
+    """The two fixed-cylinder configurations the benchmark is quoted on, at full size, against
+    the oracle: populations bit for bit, Cd at every check interval within 1e-10."""
+    cfg = _c1_cfg(steps) if which == "C1" else _c2_cfg(steps)
+    print(f"{which} at full size vs oracle, {steps} steps", flush=True)
+    sim = FixedCylinder(cfg, perturb=perturb)
+    assert (sim.st.nx, sim.st.ny) == ((401, 161) if which == "C1" else (1601, 641))
+    cd = []
+    for _ in range(steps // cfg.check_interval):
+        sim.advance(cfg.check_interval, diag_every=cfg.check_interval, graph=True)
+        cd.append(sim.cd_cl()[0])
+    f = sim.ctx.download_f(sim.k)
+    sim.ctx.close()
+    ref = O.FixedCylinder(cfg, perturb=perturb).run(steps)
+    err = rel_linf(f, ref.f_post)
+    print(f"{which}: populations rel Linf {err:.3e} (bit-identical: "
+          f"{np.array_equal(f, ref.f_post)}), Cd {cd[-1]:.10f}", flush=True)
+    assert err <= 1e-12
+    assert rel_linf(cd, ref.Cd) <= 1e-10
