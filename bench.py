@@ -107,7 +107,10 @@ def main():
                     help="steps of the plain-vs-learned-predictor MDF comparison (0: skip)")
     ap.add_argument("--variant-steps", type=int, default=2000,
                     help="steps of the FP32-deviation comparison of an fp64 run (0: skip)")
-    ap.add_argument("--workload", choices=["cylinder", "matrix72", "sed1", "sed64", "copy"],
+    ap.add_argument("--shedding-steps", type=int, default=200_000,
+                    help="steps of each form in --workload shedding")
+    ap.add_argument("--workload", choices=["cylinder", "matrix72", "sed1", "sed64", "shedding",
+                                           "copy"],
                     default="cylinder",
                     help="cylinder: Re=100 1601x641 (default); matrix72: the batched 72-case "
                          "Re x method x kernel x collision matrix (about 300 M nodes); sed1: "
@@ -134,6 +137,9 @@ def main():
         return
     if args.workload == "sed1":
         print(json.dumps(sed1_bench(args)), flush=True)
+        return
+    if args.workload == "shedding":
+        print(json.dumps(shedding_bench(args)), flush=True)
         return
     from iblbm_amd.solver import FixedCylinder
     cfg = workload_cfg()
@@ -307,6 +313,93 @@ def matrix_bench(args):
         out["fp64_plain_case_by_case"] = {"MLUPS": round(plain, 2)}
         out["speedup_vs_fp64_plain"] = round(b["MLUPS"] / plain, 3)
     return out
+
+
+def shedding_bench(args):
+    """The Re = 100 cylinder (1601x641, TRT + MDF + Peskin-4, seeded inflow perturbation) run
+    into periodic shedding in three forms from the same initial state: FP64 plain MDF,
+    FP32-deviation storage, and FP64 with the learned force predictor (trained on the device
+    on plain-MDF samples of the first 20,000 steps of its own run, then in learned mode).
+    Cd and Cl every 100 steps; reported per form: the mean Cd, the Cl amplitude and the
+    Strouhal number over the second half of the run (the reference's definition), the time
+    per step, and each form's relative difference from FP64 plain MDF."""
+    from iblbm_amd.diagnostics import compute_strouhal
+    from iblbm_amd.solver import FixedCylinder
+    ci = 100
+    n = max(20_000, (args.shedding_steps // ci) * ci)
+
+    def st_crossings(cl, steps, D, U):
+        """St from the mean period between upward zero crossings of the mean-free Cl (linear
+        interpolation between samples): far finer than the FFT bin of the same record."""
+        c = np.asarray(cl) - np.mean(cl)
+        t = np.asarray(steps, dtype=float)
+        up = [t[k] - c[k] * (t[k + 1] - t[k]) / (c[k + 1] - c[k])
+              for k in range(len(c) - 1) if c[k] < 0.0 <= c[k + 1]]
+        if len(up) < 3:
+            return None
+        return D / (U * float(np.mean(np.diff(up))))
+
+    res = {}
+    for form in ("fp64", "fp32dev", "fp64_learned"):
+        prec = "fp32dev" if form == "fp32dev" else "fp64"
+        sim = FixedCylinder(workload_cfg(), perturb=PERTURB, precision=prec)
+        sim.ctx.autotune(10)
+        cl, cd, steps = [], [], []
+        t_run = 0.0
+
+        def run(k):
+            nonlocal t_run
+            for _ in range(k // ci):
+                t0 = time.perf_counter()
+                sim.advance(ci, diag_every=ci, graph=True)
+                c, l = sim.cd_cl()
+                t_run += time.perf_counter() - t0
+                cd.append(c)
+                cl.append(l)
+                steps.append(sim.ttt)
+
+        if form == "fp64_learned":
+            pred = sim.predictor
+            run(10_000)
+            pred.mode("collect", capacity=1 << 20)
+            run(10_000)
+            pred.train(steps=3000, lr=2e-3, seed=11)
+            pred.mode("learned")
+            pred.stats(reset=True)
+            run(n - 20_000)
+            sub = pred.stats()["substeps_per_step"]
+        else:
+            run(n)
+            sub = None
+        half = len(cl) // 2
+        D, U = 2.0 * sim.st.lattice_r, sim.st.in_u
+        st = compute_strouhal(np.array(cl), n, D, U, ci)  # second half, FFT peak
+        res[form] = {"St": st, "St_crossings": st_crossings(cl[half:], steps[half:], D, U),
+                     "Cd_mean": float(np.mean(cd[half:])),
+                     "Cl_amplitude": float(0.5 * (np.max(cl[half:]) - np.min(cl[half:]))),
+                     "us_per_step": round(t_run * 1e6 / n, 3), "substeps_learned": sub}
+        sim.ctx.close()
+    base = res["fp64"]
+    for form in ("fp32dev", "fp64_learned"):
+        r = res[form]
+        r["dSt_rel"] = abs(r["St"] - base["St"]) / abs(base["St"]) if base["St"] else None
+        r["dSt_crossings_rel"] = (abs(r["St_crossings"] - base["St_crossings"]) /
+                                  base["St_crossings"]
+                                  if r["St_crossings"] and base["St_crossings"] else None)
+        r["dCd_mean_rel"] = abs(r["Cd_mean"] - base["Cd_mean"]) / abs(base["Cd_mean"])
+        r["dCl_amplitude_rel"] = (abs(r["Cl_amplitude"] - base["Cl_amplitude"]) /
+                                  abs(base["Cl_amplitude"]))
+    return {"metric": "Strouhal number and mean Cd, FP64 plain vs FP32-deviation vs learned "
+                      "MDF (Re=100 cylinder)",
+            "value": base["St"], "unit": "St", "n_gpus": 1, "steps": n, "warmup": 0,
+            "ms_per_step": round(base["us_per_step"] / 1e3, 5), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64 / f32 deviation",
+            "data": "synthetic (seeded 1e-3 inflow perturbation, seed 1234)",
+            "config": {"workload": "fixed cylinder Re=100, 1601x641, D=40, 126 markers, TRT "
+                                   "Lambda=1/4, MDF [5,20] tol 1e-4, Peskin-4, U=0.04; Cd/Cl "
+                                   "every 100 steps, St over the second half (FFT peak, and "
+                                   "mean zero-crossing period)"},
+            "forms": res}
 
 
 def sed1_cfg():
