@@ -1192,11 +1192,14 @@ int iblbm_set_markers(void *p, int id, const double *Lx, const double *Ly, const
 
 namespace {
 
-// Pinned host and device staging of the batched calls, grown geometrically.  Every batched
-// call ends with a stream synchronisation, so no copy from or into it is pending here.
+// Pinned host and device staging of the batched calls, grown geometrically (a superseded
+// device buffer stays with the context's allocations until it is destroyed).  Every batched
+// call ends with a stream synchronisation, so no copy from or into it is pending here.  The
+// capacity is cleared before a buffer is released, so a failed allocation leaves none.
 int grow_batch(Ctx &C, size_t nd, size_t ni) {
     if (nd > C.bd_cap) {
         const size_t cap = nd < 4096 ? 4096 : 2 * nd;
+        C.bd_cap = 0;
         if (C.h_bd) HIPCHK(hipHostFree(C.h_bd));
         C.h_bd = nullptr;
         HIPCHK(hipHostMalloc((void **)&C.h_bd, sizeof(double) * cap, hipHostMallocDefault));
@@ -1208,6 +1211,7 @@ int grow_batch(Ctx &C, size_t nd, size_t ni) {
     }
     if (ni > C.bi_cap) {
         const size_t cap = ni < 1024 ? 1024 : 2 * ni;
+        C.bi_cap = 0;
         if (C.h_bi) HIPCHK(hipHostFree(C.h_bi));
         C.h_bi = nullptr;
         HIPCHK(hipHostMalloc((void **)&C.h_bi, sizeof(int) * cap, hipHostMallocDefault));
