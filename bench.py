@@ -123,7 +123,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the lattice fits one MI355X and this build has no "
-              "multi-GPU path (see DESIGN.md for how it would shard)", file=sys.stderr)
+              "multi-GPU path; ensembles would shard by instance and a single lattice by "
+              "x-slabs with a three-population halo per face", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "matrix72":
