@@ -560,6 +560,7 @@ def sed_bench(args):
                 [round(float(v), 6) for v in ens.trajectory_row(0)[:4]],
                 [round(float(v), 6) for v in ens.trajectory_row(0)[9:13]]]
             leg["stopped"] = sum(r is not None for r in ens.reasons)
+            leg["vy_all"] = [float(ens.bank.vel[k, 1]) for k in range(ens.bank.P)]
         else:
             leg["whole_step_launches_us"] = round(ens.step_launch_us(100), 2)
         res[(prec, parts)] = leg
@@ -606,7 +607,13 @@ def sed_bench(args):
         "re_history": one["re_history"],
     }
     if ("fp64", 1) in res:
-        out["fp64_batched"] = {k: v for k, v in res[("fp64", 1)].items() if k != "re_history"}
+        r64 = res[("fp64", 1)]
+        out["fp64_batched"] = {k: v for k, v in r64.items() if k not in ("re_history", "vy_all")}
+        v32, v64 = np.array(one["vy_all"]), np.array(r64["vy_all"])
+        # the same members and steps in both precisions: settling velocity of all 128 discs
+        out["fp32dev_vs_fp64_settling_velocity_rel"] = float(np.max(np.abs(v32 - v64) /
+                                                                    np.abs(v64)))
+    out["one_context"].pop("vy_all", None)
     if plain is not None:
         out["fp64_plain_case_by_case"] = {"MLUPS": round(plain, 2), "cases": 8, "steps": 100}
         out["speedup_vs_fp64_plain"] = round(best["MLUPS"] / plain, 3)
