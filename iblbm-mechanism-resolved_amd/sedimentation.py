@@ -306,12 +306,14 @@ class Sedimentation:
         self.ctx.upload_f(self.k, f)
 
     def _push_markers(self):
+        """The moved rings, desired velocities and torque centre (particle 0) in one call
+        with one synchronisation (``iblbm_set_markers_batch`` for this one instance)."""
         Lx = np.concatenate([p.Lx for p in self.particles])
         Ly = np.concatenate([p.Ly for p in self.particles])
         ud = np.concatenate([p.desired for p in self.particles])
-        self.ctx.set_markers(self.k, Lx, Ly, ud, self.particles[0].Larea)
         p0 = self.particles[0]
-        self.ctx.set_torque_center(self.k, float(p0.pos[0]), float(p0.pos[1]))
+        self.ctx.set_markers_batch([self.k], [len(Lx)], Lx, Ly, ud, [float(p0.Larea)],
+                                   center=[float(p0.pos[0]), float(p0.pos[1])])
 
     # -- the reference's pre_step / post_ibm hooks ------------------------------------------
     def _half_step(self):
