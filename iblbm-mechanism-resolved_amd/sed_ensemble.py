@@ -346,6 +346,7 @@ class SedimentationEnsemble:
             if self.reasons[mi] is not None:
                 continue
             m.ttt = self.ttt
+            m._mdf_read = None  # the member's MDF statistics are the latest step's
             recs = m.record()
             self.histories[mi].append(recs if m.multi else recs[0])
             if self.monitors is not None:

@@ -282,7 +282,7 @@ class Sedimentation:
         self.ctx.upload_f(self.k, f0)
         self.ttt = 0
         self.history = []
-        self.mdf_stats = None
+        self._mdf_read = None
         self.d_lattice = 2.0 * st.lattice_r
         self.termination_reason = "max_steps"
 
@@ -372,8 +372,16 @@ class Sedimentation:
         ci = int(self.cfg.check_interval)
         self.ctx.step(1, diag_every=1 if self.ttt % ci == 0 else 0)
         self._full_step(self._forces())
-        if self.cfg.ibm_method == "MDF":
-            self.mdf_stats = self.ctx.mdf_stats(self.k)
+        self._mdf_read = None  # the last step's MDF statistics are read when asked for
+
+    @property
+    def mdf_stats(self):
+        """MDF statistics of the last step (``mdf.py:224-229``), or None for DF / DFC."""
+        if self.cfg.ibm_method != "MDF" or self.ttt == 0:
+            return None
+        if self._mdf_read is None:
+            self._mdf_read = self.ctx.mdf_stats(self.k)
+        return self._mdf_read
 
     def record(self):
         cfg, st = self.cfg, self.st
