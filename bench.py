@@ -52,24 +52,43 @@ TRAFFIC_FP64_C2 = {"nodes": 1026241, "bytes": (2 * 37911.6 + 72436.5) * 1024,
 
 
 def cpu_info():
-    cores = os.cpu_count() or 1
-    model = platform.processor() or "unknown"
+    """Host CPU as lscpu reports it: model, sockets, physical cores, hardware threads."""
+    info = {"model": platform.processor() or "unknown", "sockets": None,
+            "physical_cores": None, "threads": os.cpu_count() or 1}
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        kv = {}
         for line in out.splitlines():
-            if line.startswith("Model name:"):
-                model = line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        info["model"] = kv.get("Model name", info["model"])
+        if kv.get("Socket(s)", "").isdigit() and kv.get("Core(s) per socket", "").isdigit():
+            info["sockets"] = int(kv["Socket(s)"])
+            info["physical_cores"] = info["sockets"] * int(kv["Core(s) per socket"])
     except Exception:
         pass
-    return model, cores
+    return info
+
+
+def cpu_threads():
+    """OpenMP threads for the CPU baseline: the process's CPU share (OMP_NUM_THREADS where
+    set, which on a shared one-GPU host is that GPU's share of the cores), else every core
+    the process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
 
 
 def cpu_baseline(seconds=15.0, threads=None):
     """The FP64 C/OpenMP oracle on the same workload: timed steps within a bounded budget."""
     from oracle import oracle as O
-    model, cores = cpu_info()
-    n = threads or min(cores, 16)
-    used = O.set_threads(n)
+    info = cpu_info()
+    used = O.set_threads(threads or cpu_threads())
     cfg = workload_cfg()
     sim = O.FixedCylinder(cfg, PERTURB)
     N = sim.st.N
@@ -82,7 +101,24 @@ def cpu_baseline(seconds=15.0, threads=None):
     dt = time.perf_counter() - t0
     return {"value": N * steps / dt / 1e6, "unit": "MLUPS", "cores": used, "kind": "port",
             "sample": f"{steps} steps of the same 1601x641 TRT/MDF/P4 case after 1 warm-up "
-                      f"step ({dt:.1f} s, {used} OpenMP threads, {model})"}
+                      f"step ({dt:.1f} s, {used} OpenMP threads = the process's CPU share; "
+                      f"host: {info['model']}, {info['sockets']} socket(s), "
+                      f"{info['physical_cores']} physical cores, {info['threads']} hardware "
+                      f"threads)",
+            "host": info}
+
+
+def advance_aligned(sim, n, ci, graph, on_check=None):
+    """Runs exactly n steps in chunks that end on multiples of ci.  A whole chunk of ci steps
+    is replayed as a graph (the diagnostic sums fire on its last step); a partial chunk is
+    launched step by step.  on_check() runs after each chunk that ends on a check step."""
+    done = 0
+    while done < n:
+        c = min(ci - sim.ttt % ci, n - done)
+        sim.advance(c, diag_every=ci, graph=graph and c == ci)
+        done += c
+        if sim.ttt % ci == 0 and on_check is not None:
+            on_check()
 
 
 def main():
@@ -145,17 +181,15 @@ def main():
     from iblbm_amd.solver import FixedCylinder
     cfg = workload_cfg()
     ci = cfg.check_interval
-    steps = max(ci, (args.steps // ci) * ci)
-    warm = max(ci, (args.warmup // ci) * ci)
+    steps, warm = max(1, args.steps), max(0, args.warmup)
     sim = FixedCylinder(cfg, perturb=PERTURB, precision=args.precision)
     if args.serial:
         sim.ctx.set_concurrent(False)
     graph = not args.no_graph
     mode = sim.ctx.step_mode()
     N = sim.st.nodenums
-    for _ in range(warm // ci):
-        sim.advance(ci, diag_every=ci, graph=graph)
-        sim.cd_cl()
+    # set-up, before the warm-up: the launch schedule, then one check interval replayed as a
+    # graph so that the chunk's graph is captured outside the timed region
     tuned = None
     if args.schedule:
         f = args.schedule.split(",")
@@ -165,35 +199,39 @@ def main():
                              row_layout=f[6] if len(f) > 6 else "tiles",
                              stores=f[7] if len(f) > 7 else "normal")
         tuned = sim.ctx.schedule()
-        mode = sim.ctx.step_mode()
-        sim.advance(ci, diag_every=ci, graph=graph)  # the chunk's graph, outside the timing
-        sim.cd_cl()
     elif not args.no_tune and not args.serial:  # launch schedule from the autotuner
         tuned = sim.ctx.autotune(60)
-        mode = sim.ctx.step_mode()
-        sim.advance(ci, diag_every=ci, graph=graph)  # the chunk's graph, outside the timing
-        sim.cd_cl()
+    mode = sim.ctx.step_mode()
+    advance_aligned(sim, ci, ci, graph)
+    setup_steps = sim.ttt
+    advance_aligned(sim, warm, ci, graph)  # exactly `warm` untimed steps
     sim.ctx.diag_fetch(0)  # allocates the pinned slots before the timed region
     sim.ctx.diag_fetch(1)
     sim.ctx.sync()
-    # Cd and Cl every check interval, as the reference's loop takes them, but read one chunk
-    # late: each chunk's sums go to a pinned slot behind its steps, and the host reads the
-    # previous slot only after queueing the next chunk, so the device never waits for it.
+    # Exactly `steps` timed steps.  Cd and Cl every check interval, as the reference's loop
+    # takes them, but read one chunk late: each chunk's sums go to a pinned slot behind its
+    # steps, and the host reads the previous slot only after queueing the next chunk, so the
+    # device never waits for it.
     cds = []
-    slot, pending = 0, None
+    st8 = {"slot": 0, "pending": None}
+
+    def on_check():
+        sim.ctx.diag_fetch(st8["slot"])
+        if st8["pending"] is not None:
+            cds.append(sim.cd_cl(slot=st8["pending"]))
+        st8["pending"], st8["slot"] = st8["slot"], st8["slot"] ^ 1
+
     t0 = time.perf_counter()
     sim.ctx.mark(0)
-    for _ in range(steps // ci):
-        sim.advance(ci, diag_every=ci, graph=graph)
-        sim.ctx.diag_fetch(slot)
-        if pending is not None:
-            cds.append(sim.cd_cl(slot=pending))
-        pending, slot = slot, slot ^ 1
+    advance_aligned(sim, steps, ci, graph, on_check)
     sim.ctx.mark(1)
-    cds.append(sim.cd_cl(slot=pending))
+    if st8["pending"] is not None:
+        cds.append(sim.cd_cl(slot=st8["pending"]))
     sim.ctx.sync()
     wall = time.perf_counter() - t0
     ev_ms = sim.ctx.elapsed_ms(0, 1)  # device timeline of the timed steps
+    if not cds:  # no check step inside the timed region: Cd of the last check before it
+        cds.append(sim.cd_cl())
     cd, cl = cds[-1]
     iters = [sim.ctx.mdf_stats(sim.k)["iter_count"]]
     mlups = N * steps / wall / 1e6
@@ -216,7 +254,7 @@ def main():
         "config": {"workload": "fixed cylinder Re=100, 1601x641, D=40, 126 markers, TRT "
                                "Lambda=1/4, MDF [5,20] tol 1e-4, Peskin-4, U=0.04",
                    "precision": args.precision, "launch": mode, "graph": graph,
-                   "schedule": tuned,
+                   "schedule": tuned, "setup_steps": setup_steps,
                    "nodes": N, "check_interval": ci, "Cd_last": cd, "Cl_last": cl,
                    "mdf_iter_last": iters[-1] if iters else None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -83,7 +83,7 @@ SIGNATURES = {
     "iblbm_download_particle_fib": (_i, [_VP, _i, _i, _D, _D, _I]),
     "iblbm_download_marker_force": (_i, [_VP, _i, _D]),
     "iblbm_set_markers_batch": (_i, [_VP, _i, _I, _I, _D, _D, _D, _D, _D]),
-    "iblbm_download_marker_forces_batch": (_i, [_VP, _i, _I, _D]),
+    "iblbm_download_marker_forces_batch": (_i, [_VP, _i, _I, _D, _l]),
     "iblbm_marker_box": (_i, [_D, _D, _i, _d, _d, _i, _i, _i, _I]),
     "iblbm_step": (_i, [_VP, _i, _i]),
     "iblbm_sync": (_i, [_VP]),

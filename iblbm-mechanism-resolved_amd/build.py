@@ -1,7 +1,9 @@
 """In-tree build of libiblbm_hip.so for gfx950 (hipcc; no JIT cache, no CMake).
 
 -ffp-contract=off keeps every FP64 a*b+c as two IEEE roundings (the reference's CPU
-arithmetic); packed FP32 instructions are disabled for the whole library.
+arithmetic); packed FP32 instructions are disabled for the whole library.  Each source is
+compiled to its own object in parallel (every translation unit carries its own device code
+object), then the objects are linked into the shared library.
 """
 from __future__ import annotations
 
@@ -10,23 +12,40 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = [os.path.join(HERE, "csrc", f) for f in ("iblbm_hip.hip", "ops.hip")]
+SRC = [os.path.join(HERE, "csrc", f) for f in ("iblbm_hip.hip", "ops.hip", "noalloc.hip")]
 OUT = os.path.join(HERE, "libiblbm_hip.so")
+OBJ_DIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall",
          "-Wno-unused-function", "-Wno-unknown-warning-option"]
 
 
 def build(verbose: bool = False) -> str:
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    srcs = [s for s in SRC if os.path.exists(s)]
+    procs = []
+    for src in srcs:
+        obj = os.path.join(OBJ_DIR, os.path.basename(src) + ".o")
+        cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((obj, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                            stderr=subprocess.STDOUT, text=True)))
+    failed = False
+    for obj, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out)
+            failed = True
+    if failed:
+        raise RuntimeError("hipcc failed building libiblbm_hip.so")
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, *FLAGS, *SRC, "-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *[o for o, _ in procs], "-o", tmp]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libiblbm_hip.so")
+        raise RuntimeError("hipcc failed linking libiblbm_hip.so")
     os.replace(tmp, OUT)
     return OUT
 

@@ -1262,8 +1262,12 @@ int iblbm_set_markers_batch(void *p, int n, const int *ids, const int *lb, const
     }
     const int total = off[n];
     HIPCHK(hipSetDevice(C->device));
-    int rc = grow_batch(*C, 4 * (size_t)total, 3 * (size_t)n + 1);
+    // int staging: ids[n], off[n + 1], then the (nT, fast) pairs k_tables_batch writes back
+    const size_t n_int = 4 * (size_t)n + 1;
+    int rc = grow_batch(*C, 4 * (size_t)total, n_int);
     if (rc) return rc;
+    if (C->bi_cap < n_int || C->bd_cap < 4 * (size_t)total)
+        return fail(IBLBM_EDEVICE, "set_markers_batch: staging smaller than the call's layout");
     std::memcpy(C->h_bd, Lx, sizeof(double) * total);
     std::memcpy(C->h_bd + total, Ly, sizeof(double) * total);
     std::memcpy(C->h_bd + 2 * total, ud, sizeof(double) * 2 * total);
@@ -1331,7 +1335,8 @@ int iblbm_set_markers_batch(void *p, int n, const int *ids, const int *lb, const
     return 0;
 }
 
-int iblbm_download_marker_forces_batch(void *p, int n, const int *ids, double *F) {
+int iblbm_download_marker_forces_batch(void *p, int n, const int *ids, double *F,
+                                       int64_t capacity) {
     Ctx *C = as_ctx(p);
     if (!C || n < 1 || !ids || !F)
         return fail(IBLBM_EINVAL, "download_marker_forces_batch: need n >= 1 and every array");
@@ -1348,6 +1353,10 @@ int iblbm_download_marker_forces_batch(void *p, int n, const int *ids, double *F
         off[b + 1] = off[b] + d.Lb;
     }
     const int total = off[n];
+    if ((int64_t)total > capacity)
+        return fail(IBLBM_EINVAL, "download_marker_forces_batch: F holds " +
+                                      std::to_string(capacity) + " marker rows, the instances have " +
+                                      std::to_string(total));
     HIPCHK(hipSetDevice(C->device));
     int rc = grow_batch(*C, 2 * (size_t)total, 2 * (size_t)n + 1);
     if (rc) return rc;

@@ -22,7 +22,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .sedimentation import Sedimentation, compute_net_gravity
+from .sedimentation import Sedimentation, compute_net_gravity, stop_final_step
 from .solver import Context
 
 
@@ -294,6 +294,7 @@ class SedimentationEnsemble:
         self.nodes = sum(m.st.nx * m.st.ny for m in self.members)
         self.histories = [[] for _ in self.members]
         self.reasons = [None] * len(self.members)
+        self.stop_steps = [None] * len(self.members)
         self.monitors = None
         if save_dir:
             import os
@@ -351,10 +352,12 @@ class SedimentationEnsemble:
             self.histories[mi].append(recs if m.multi else recs[0])
             if self.monitors is not None:
                 import time
-                self.monitors[mi](self.ttt, m.particles, elapsed=time.time() - self._t0)
+                self.monitors[mi](self.ttt, m.particles, m.mdf_stats,
+                                  elapsed=time.time() - self._t0)
             reason = m.should_stop()
             if reason is not None:
                 self.reasons[mi] = reason
+                self.stop_steps[mi] = self.ttt
                 m.termination_reason = reason
                 k0 = int(self.first[mi])
                 self.bank.live[k0:k0 + len(m.particles)] = False
@@ -372,8 +375,15 @@ class SedimentationEnsemble:
         if self.monitors is None:
             return
         for mi, (m, mon) in enumerate(zip(self.members, self.monitors)):
-            mon.finalize(self.ttt, self.reasons[mi] or "max_steps", tau=m.st.tau,
+            mon.finalize(self.final_step(mi), self.reasons[mi] or "max_steps", tau=m.st.tau,
                          rho_ratios=[float(p.rho_ratio) for p in m.particles])
+
+    def final_step(self, mi):
+        """Member mi's ``final_step`` as the reference's loop reports it (a member stopped
+        at check step s reports s - 1; one still running reports the ensemble's step)."""
+        if self.reasons[mi] is None:
+            return int(self.ttt)
+        return stop_final_step(self.stop_steps[mi], self.reasons[mi])
 
     def trajectory_row(self, mi):
         """[pos x, pos y, vel x, vel y, omega, angle, force x, force y, torque] of each of

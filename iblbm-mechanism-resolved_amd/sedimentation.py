@@ -390,40 +390,67 @@ class Sedimentation:
         return recs
 
     def should_stop(self):
-        """``sedimentation_common.py:14-66`` (contact / offset / nan / domain bounds)."""
-        cfg, st = self.cfg, self.st
-        p0 = self.particles[0]
-        v = float(p0.vel[0]) if cfg.gravity_direction == "right" else float(p0.vel[1])
-        if not math.isfinite(v):
-            return "nan"
-        r = p0.r_domain
-        pos = p0.pos
-        if cfg.sedimentation_stop_at_contact:
-            if cfg.gravity_direction == "right":
-                if float(pos[0]) >= cfg.xmax - r:
-                    return "contact"
-            elif float(pos[1]) <= r:
+        """``sedimentation_single.py:339-351`` / ``sedimentation_multi.py:298-313``."""
+        vy_star = None
+        if not self.multi:
+            vy_star = self.record()[0]["vy_star"]
+        return stop_reason(self.cfg, self.particles, self.st.dx, self.multi, vy_star)
+
+
+def _stop_rule(pos, r_domain, cfg):
+    """``sedimentation_common.py:14-34``: "contact" / "offset" of the leading particle."""
+    if cfg.sedimentation_stop_at_contact:
+        if cfg.gravity_direction == "right":
+            if float(pos[0]) >= cfg.xmax - r_domain:
                 return "contact"
-        if cfg.sedimentation_stop_offset_d > 0.0:
-            d_dom = cfg.cylinder_D_ratio
-            if cfg.gravity_direction == "right":
-                if float(pos[0]) >= cfg.xmax - cfg.sedimentation_stop_offset_d * d_dom:
-                    return "offset"
-            elif float(pos[1]) <= cfg.sedimentation_stop_offset_d * d_dom:
+        elif float(pos[1]) <= r_domain:
+            return "contact"
+    if cfg.sedimentation_stop_offset_d > 0.0:
+        d_dom = cfg.cylinder_D_ratio
+        if cfg.gravity_direction == "right":
+            if float(pos[0]) >= cfg.xmax - cfg.sedimentation_stop_offset_d * d_dom:
                 return "offset"
-        margin = 2.0 * st.dx
-        terminal = cfg.sedimentation_stop_at_contact or cfg.sedimentation_stop_offset_d > 0.0
-        for p in self.particles:
-            x, y, rr = float(p.pos[0]), float(p.pos[1]), p.r_domain
-            if cfg.gravity_direction == "right":
-                near = (y - rr < margin or y + rr > cfg.ymax - margin or x - rr < margin or
-                        (not terminal and x + rr > cfg.xmax - margin))
-            else:
-                near = ((not terminal and y - rr < margin) or y + rr > cfg.ymax - margin or
-                        x - rr < margin or x + rr > cfg.xmax - margin)
-            if near:
-                return "domain_bounds"
-        return None
+        elif float(pos[1]) <= cfg.sedimentation_stop_offset_d * d_dom:
+            return "offset"
+    return None
+
+
+def _near_wall(pos, r_domain, cfg, dx, safety=2.0):
+    """``sedimentation_common.py:37-66``: True where the reference returns a warning (the
+    particle within ``safety*dx`` of a wall other than the stopping wall)."""
+    margin = safety * dx
+    x, y = float(pos[0]), float(pos[1])
+    terminal = cfg.sedimentation_stop_at_contact or cfg.sedimentation_stop_offset_d > 0.0
+    if cfg.gravity_direction == "right":
+        return (y - r_domain < margin or y + r_domain > cfg.ymax - margin or
+                x - r_domain < margin or (not terminal and x + r_domain > cfg.xmax - margin))
+    return ((not terminal and y - r_domain < margin) or y + r_domain > cfg.ymax - margin or
+            x - r_domain < margin or x + r_domain > cfg.xmax - margin)
+
+
+def stop_reason(cfg, particles, dx, multi, vy_star=None):
+    """The settling stop rules, in the reference's order: "nan", then "contact" / "offset" of
+    particle 0, then "domain_bounds" (any particle near a wall), else None.
+
+    One particle (``sedimentation_single.py:339-351``): "nan" when the record's ``vy_star``
+    (``-v_y / u_g``, 0 when ``u_g < 1e-15``) is not finite, and only particle 0 is checked
+    against the walls.  Several (``sedimentation_multi.py:298-313``): "nan" when particle 0's
+    velocity along gravity is not finite, and every particle is checked against the walls.
+    """
+    p0 = particles[0]
+    if multi:
+        v = float(p0.vel[0]) if cfg.gravity_direction == "right" else float(p0.vel[1])
+    else:
+        v = float(vy_star)
+    if not math.isfinite(v):
+        return "nan"
+    reason = _stop_rule(p0.pos, p0.r_domain, cfg)
+    if reason is not None:
+        return reason
+    for p in (particles if multi else particles[:1]):
+        if _near_wall(p.pos, p.r_domain, cfg, dx):
+            return "domain_bounds"
+    return None
 
 
 def run(cfg, verbose: bool = True, callback=None, save_dir=None, device: int = 0,
@@ -454,9 +481,18 @@ def run(cfg, verbose: bool = True, callback=None, save_dir=None, device: int = 0
                 sim.termination_reason = reason
                 break
     sim.ctx.sync()
+    final_step = stop_final_step(sim.ttt, sim.termination_reason)
     if mon is not None:
-        mon.finalize(sim.ttt, sim.termination_reason, tau=sim.st.tau,
+        mon.finalize(final_step, sim.termination_reason, tau=sim.st.tau,
                      rho_ratios=[float(p.rho_ratio) for p in sim.particles])
     return {"Cd_history": np.array([]), "Cl_history": np.array([]), "converged": False,
-            "final_step": sim.ttt, "termination_reason": sim.termination_reason,
+            "final_step": final_step, "termination_reason": sim.termination_reason,
             "sedimentation_history": sim.history, "sim": sim}
+
+
+def stop_final_step(ttt: int, reason) -> int:
+    """The ``final_step`` the reference reports: a stop rule breaks out of its loop before
+    the step counter advances, and ``final_step`` is the counter minus one
+    (``iblbm/runtime/loop.py:106-109``, ``:142-148``), so a run stopped at check step ``ttt``
+    reports ``ttt - 1``; a run that reaches ``max_steps`` reports ``max_steps``."""
+    return int(ttt) - 1 if reason not in (None, "max_steps") else int(ttt)

@@ -220,8 +220,8 @@ class Context:
         ids = np.ascontiguousarray(ids, dtype=np.int32)
         F = np.empty((int(lb_total), 2))
         check(self.lib.iblbm_download_marker_forces_batch(
-            self.h, len(ids), ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), dptr(F)),
-            "download_marker_forces_batch")
+            self.h, len(ids), ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), dptr(F),
+            int(lb_total)), "download_marker_forces_batch")
         return F
 
     def diag(self, k, slot=None):
@@ -366,6 +366,56 @@ class FixedCylinder:
         return float(abs(d.force[0]) / denom), float(d.force[1] / denom)
 
 
+class LoopControl:
+    """Termination bookkeeping of the reference's time loop (``iblbm/runtime/loop.py:64-70``,
+    ``:113-137``), apart from the stepping so that it can be driven on the host alone.
+
+    * The convergence exit is tested before the step bound at the top of each pass, so a run
+      that converges at its last check ends "converged".
+    * The velocity reference field is taken only at the check step equal to
+      ``convergence_start``; a start that is not a check step never converges.
+    * The Cd-window criterion sets "cd_converged" (``loop.py:132-140``).
+    """
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self.converged = False
+        self.termination = "max_steps"
+        self.error = 1.0
+        self.pre_U = None
+
+    def done(self, ttt: int) -> bool:
+        cfg = self.cfg
+        if self.converged and (cfg.use_convergence or cfg.cd_convergence_tol is not None):
+            if self.termination == "max_steps":
+                self.termination = "converged"
+            return True
+        return ttt >= cfg.max_steps
+
+    def check(self, ttt: int, Cd_hist, velocity):
+        """At a check step: ``velocity()`` returns the (N, 2) field, read only when needed."""
+        cfg = self.cfg
+        if cfg.use_convergence and (ttt == cfg.convergence_start or
+                                    (ttt > cfg.convergence_start and self.pre_U is not None)):
+            U = np.asarray(velocity())
+            if ttt == cfg.convergence_start:
+                self.pre_U = U.copy()
+            else:  # iblbm/diagnostics/core.py:116-137
+                pre = self.pre_U
+                numer = np.sum((U[:, 0] - pre[:, 0]) ** 2 + (U[:, 1] - pre[:, 1]) ** 2)
+                denom = np.sum(pre[:, 0] ** 2 + pre[:, 1] ** 2)
+                self.error = 1.0 if denom == 0.0 else float(np.sqrt(numer / denom))
+                self.pre_U = U.copy()
+                if self.error < cfg.convergence_threshold:
+                    self.converged = True
+        if (cfg.cd_convergence_tol is not None and ttt > cfg.cd_convergence_start_step
+                and len(Cd_hist) >= cfg.cd_convergence_window):
+            recent = Cd_hist[-cfg.cd_convergence_window:]
+            if max(recent) - min(recent) < cfg.cd_convergence_tol:
+                self.converged = True
+                self.termination = "cd_converged"
+
+
 def run(cfg, verbose: bool = True, callback=None, save_dir=None, perturb=None,
         device: int = 0, graph: bool = True, precision: str = "fp64") -> dict:
     """``iblbm.run`` for the fixed-cylinder scenario on the fused GPU step.
@@ -386,16 +436,9 @@ def run(cfg, verbose: bool = True, callback=None, save_dir=None, perturb=None,
         mon = FixedCylinderMonitor(save_dir, cfg)
     ci = int(cfg.check_interval)
     Cd_hist, Cl_hist, mdf_hist = [], [], []
-    converged = False
-    termination = "max_steps"
-    pre_U = None
-    error = 1.0
+    ctl = LoopControl(cfg)
     t0 = time.time()
-    while sim.ttt < cfg.max_steps:
-        if converged and (cfg.use_convergence or cfg.cd_convergence_tol is not None):
-            if termination == "max_steps":
-                termination = "converged"
-            break
+    while not ctl.done(sim.ttt):
         n = min(ci - sim.ttt % ci, cfg.max_steps - sim.ttt)
         sim.advance(n, diag_every=ci, graph=graph)
         ttt = sim.ttt
@@ -407,30 +450,15 @@ def run(cfg, verbose: bool = True, callback=None, save_dir=None, perturb=None,
         if cfg.ibm_method == "MDF":
             mdf_hist.append(sim.ctx.mdf_stats(sim.k))
         if verbose:
-            print(f"step {ttt:>7d} | Cd={Cd:.6f} Cl={Cl:.6f} | err={error:.2e} | "
+            print(f"step {ttt:>7d} | Cd={Cd:.6f} Cl={Cl:.6f} | err={ctl.error:.2e} | "
                   f"{time.time() - t0:.1f}s")
         if callback is not None:
-            callback(step=ttt, Cd=Cd, Cl=Cl, error=error, state=sim, elapsed=time.time() - t0,
-                     converged=converged)
+            callback(step=ttt, Cd=Cd, Cl=Cl, error=ctl.error, state=sim,
+                     elapsed=time.time() - t0, converged=ctl.converged)
         if mon is not None:
-            mon(ttt, Cd, Cl, error=error, converged=converged, elapsed=time.time() - t0)
-        if cfg.use_convergence and ttt >= cfg.convergence_start:
-            _, _, U = sim.ctx.download_state(sim.k)
-            if ttt == cfg.convergence_start or pre_U is None:
-                pre_U = U.copy()
-            else:  # iblbm/diagnostics/core.py:116-137
-                numer = np.sum((U[:, 0] - pre_U[:, 0]) ** 2 + (U[:, 1] - pre_U[:, 1]) ** 2)
-                denom = np.sum(pre_U[:, 0] ** 2 + pre_U[:, 1] ** 2)
-                error = 1.0 if denom == 0.0 else float(np.sqrt(numer / denom))
-                pre_U = U.copy()
-                if error < cfg.convergence_threshold:
-                    converged = True
-        if (cfg.cd_convergence_tol is not None and ttt > cfg.cd_convergence_start_step
-                and len(Cd_hist) >= cfg.cd_convergence_window):
-            recent = Cd_hist[-cfg.cd_convergence_window:]
-            if max(recent) - min(recent) < cfg.cd_convergence_tol:
-                converged = True
-                termination = "cd_converged"
+            mon(ttt, Cd, Cl, error=ctl.error, converged=ctl.converged, elapsed=time.time() - t0)
+        ctl.check(ttt, Cd_hist, lambda: sim.ctx.download_state(sim.k)[2])
+    converged, termination = ctl.converged, ctl.termination
     sim.ctx.sync()
     if mon is not None:
         mon.finalize(sim.ttt, termination, converged=converged, elapsed=time.time() - t0)

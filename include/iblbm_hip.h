@@ -152,7 +152,10 @@ int iblbm_set_markers_batch(void *ctx, int n, const int *ids, const int *lb, con
  * marker forces of a several-particle DF window (as iblbm_download_marker_force).  Replaces the
  * per-member force reads of iblbm/ibm/strategies/dfc.py:129-150 and
  * iblbm/ibm/forcing.py:263-275 in a batched settling ensemble. */
-int iblbm_download_marker_forces_batch(void *ctx, int n, const int *ids, double *F);
+int iblbm_download_marker_forces_batch(void *ctx, int n, const int *ids, double *F,
+                                       int64_t capacity);
+/* capacity: the marker rows (pairs of doubles) F holds; a call whose instances carry more
+ * markers than that fails with IBLBM_EINVAL and writes nothing. */
 /* Window box [box[0], box[0]+box[2]) x [box[1], box[1]+box[3]) covering every clamped
  * stencil node of the markers (stencil radius 2 for peskin4pt, 1 for hat). */
 int iblbm_marker_box(const double *Lx, const double *Ly, int Lb, double dx, double dy, int nx,
