@@ -117,6 +117,9 @@ SIGNATURES = {
                                      _D, _D, _D]),
     "iblbm_op_multi_direct_forcing": (_i, [_D, _D, _D, _i, _D, _D, _d, _d, _d, _d, _i, _i, _i,
                                            _i, _i, _d, _d, _D, ctypes.POINTER(MdfStats)]),
+    "iblbm_op_multi_direct_forcing_noalloc": (_i, [_VP, _VP, _VP, _i, _VP, _VP, _d, _d, _d,
+                                                   _d, _i, _i, _i, _i, _i, _d, _d, _VP,
+                                                   ctypes.POINTER(MdfStats), _VP]),
     "iblbm_op_dfc_lambda": (_i, [_D, _D, _i, _d, _d, _d, _d, _i, _i, _i, _D]),
     "iblbm_op_dfc_correction": (_i, [_D, _D, _D, _i, _D, _D, _d, _d, _d, _i, _i, _i, _d, _D]),
     "iblbm_predictor_mode": (_i, [_VP, _i, _i, _l]),

@@ -12,7 +12,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = [os.path.join(HERE, "csrc", f) for f in ("iblbm_hip.hip", "ops.hip", "noalloc.hip")]
+SRC = [os.path.join(HERE, "csrc", f) for f in ("iblbm_hip.hip", "ops.hip")]
 OUT = os.path.join(HERE, "libiblbm_hip.so")
 OBJ_DIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -23,7 +23,7 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=o
 
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ_DIR, exist_ok=True)
-    srcs = [s for s in SRC if os.path.exists(s)]
+    srcs = SRC
     procs = []
     for src in srcs:
         obj = os.path.join(OBJ_DIR, os.path.basename(src) + ".o")

@@ -6,8 +6,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -150,7 +152,7 @@ __global__ void k_op_boundary(int bc, double *fs, int nx, int ny, double in_u, d
 template <int A>
 __device__ void op_ibm_impl(Inst &I, int mode, const double *Eux, const double *Euy,
                             const double *Ro, double *fstar, double *red, int *redi,
-                            IbmShared &sh) {
+                            IbmShared &sh, int us = 1) {  // us: element stride of Eux, Euy
     IbmScratch &S = I.s;
     const int t = threadIdx.x;
     const int nbox = I.bw * I.bh;
@@ -182,8 +184,8 @@ __device__ void op_ibm_impl(Inst &I, int mode, const double *Eux, const double *
     for (int sl = t; sl < nbox; sl += NTI) {
         const int64_t n = (int64_t)(I.by0 + sl / I.bw) * I.nx + I.bx0 + sl % I.bw;
         S.ro[sl] = Ro[n];
-        S.ux0[sl] = Eux[n];
-        S.uy0[sl] = Euy[n];
+        S.ux0[sl] = Eux[us * n];
+        S.uy0[sl] = Euy[us * n];
     }
     __threadfence_block();
     __syncthreads();
@@ -273,6 +275,148 @@ struct OpIbm {
         return 0;
     }
 };
+
+}  // namespace
+
+// ------------------------------------------------------------------ no-allocation MDF
+//
+// iblbm_op_multi_direct_forcing_noalloc runs the MDF window of iblbm_op_multi_direct_forcing on
+// caller-owned device arrays, with every working array of the window (the Inst descriptor,
+// the marker copies, the contribution and gather tables, the touched-node fields and the
+// statistics) carved from the file-scope array `scratch` below instead of hipMalloc.
+constexpr int NA_LB_MAX = 512;       // markers per call
+constexpr int NA_BOX_MAX = 16384;    // window box nodes (bw * bh) per call, e.g. 128 x 128
+constexpr size_t NA_BYTES = 6u << 20;  // 6 MiB per GPU, held until the process ends
+constexpr int NA_OUT = 16;           // [stats 0..5, unused, unused, status, box 0..3, ...]
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+__global__ void k_na_zero(double *p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+// k_na_put stores the call's descriptor (array pointers into the scratch) at the scratch's
+// first slot.  k_na_mdf, one workgroup: copies the markers into the scratch, forms the window box (every clamped
+// stencil node, as iblbm_marker_box does on the host), then runs the operator's MDF window
+// and scatters the window's fib into the caller's zeroed (N, 2) array.  out[8] receives 0, or
+// 1 for a non-finite or far out-of-range marker coordinate, or 2 for a box above NA_BOX_MAX;
+// on 1 or 2 nothing else is written.
+__global__ void k_na_put(Inst v, Inst *Ip) {
+    if (threadIdx.x == 0) *Ip = v;
+}
+
+template <int A>
+__global__ __launch_bounds__(NTI) void k_na_mdf(Inst *Ip, const double *Lx, const double *Ly,
+                                                const double *ud, const double *U,
+                                                const double *ro, double *fib, double *out) {
+    __shared__ double red[NTI];
+    __shared__ int redi[NTI];
+    __shared__ IbmShared sh;
+    __shared__ int bx[4], bad;
+    const int t = threadIdx.x;
+    if (t == 0) {
+        bx[0] = INT_MAX;
+        bx[1] = INT_MIN;
+        bx[2] = INT_MAX;
+        bx[3] = INT_MIN;
+        bad = 0;
+    }
+    __syncthreads();
+    Inst &I = *Ip;
+    double *lx = (double *)I.Lx, *ly = (double *)I.Ly, *u2 = (double *)I.ud;
+    const int nx = I.nx, ny = I.ny;
+    const double dx = I.dx, dy = I.dy;
+    for (int k = t; k < I.Lb; k += NTI) {
+        const double x = Lx[k], y = Ly[k];
+        lx[k] = x;
+        ly[k] = y;
+        u2[2 * k] = ud[2 * k];
+        u2[2 * k + 1] = ud[2 * k + 1];
+        const double fx = floor(x / dx + 0.5), fy = floor(y / dy + 0.5);
+        if (!(fabs(fx) < 1.0e9) || !(fabs(fy) < 1.0e9)) {  // NaN fails both
+            atomicOr(&bad, 1);
+            continue;
+        }
+        atomicMin(&bx[0], (int)fx - A);
+        atomicMax(&bx[1], (int)fx + A);
+        atomicMin(&bx[2], (int)fy - A);
+        atomicMax(&bx[3], (int)fy + A);
+    }
+    __syncthreads();
+    const int x0 = min(max(bx[0], 0), nx - 1), x1 = min(max(bx[1], 0), nx - 1);
+    const int y0 = min(max(bx[2], 0), ny - 1), y1 = min(max(bx[3], 0), ny - 1);
+    const int bw = x1 - x0 + 1, bh = y1 - y0 + 1;
+    const int status = bad ? 1 : ((int64_t)bw * bh > NA_BOX_MAX ? 2 : 0);
+    if (t == 0) {
+        out[8] = status;
+        out[9] = x0;
+        out[10] = y0;
+        out[11] = bw;
+        out[12] = bh;
+    }
+    if (status) return;  // uniform over the workgroup
+    if (t == 0) {
+        I.bx0 = x0;
+        I.by0 = y0;
+        I.bw = bw;
+        I.bh = bh;
+    }
+    __threadfence_block();
+    __syncthreads();
+    op_ibm_impl<A>(I, 1, U, U + 1, ro, nullptr, red, redi, sh, 2);
+    __threadfence_block();
+    __syncthreads();
+    for (int sl = t; sl < bw * bh; sl += NTI) {
+        const int64_t n = (int64_t)(y0 + sl / bw) * nx + x0 + sl % bw;
+        fib[2 * n] = I.s.fibx[sl];
+        fib[2 * n + 1] = I.s.fiby[sl];
+    }
+}
+
+namespace {
+
+// Per device: the device address of `scratch` (hipGetSymbolAddress, taken once with that
+// device current) and the descriptor whose array pointers are carved from it.
+struct NaDevice {
+    bool ready = false;
+    unsigned char *base = nullptr;
+    Inst tmpl;
+};
+std::mutex g_na_mu;
+constexpr int NA_MAX_DEVICES = 64;
+NaDevice g_na[NA_MAX_DEVICES];
+
+int na_setup(int dev, NaDevice **out) {
+    if (dev < 0 || dev >= NA_MAX_DEVICES)
+        return op_fail(IBLBM_EDEVICE, "noalloc: device index out of range");
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    NaDevice &d = g_na[dev];
+    if (!d.ready) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return op_fail(IBLBM_EDEVICE, std::string("noalloc: hipGetSymbolAddress(scratch): ") +
+                                              hipGetErrorString(e));
+        d.base = (unsigned char *)ptr;
+        // bump allocation from the device address: the Inst first, then the window arrays
+        size_t used = 0;
+        auto get = [&](size_t bytes) -> void * {
+            const size_t at = (used + 255) & ~(size_t)255;
+            if (at + bytes > NA_BYTES) return nullptr;
+            used = at + bytes;
+            return d.base + at;
+        };
+        memset(&d.tmpl, 0, sizeof d.tmpl);
+        if (!get(sizeof(Inst)) || !get(sizeof(double) * NA_OUT) ||
+            !alloc_ibm_scratch(d.tmpl, NA_LB_MAX, NA_BOX_MAX, get))
+            return op_fail(IBLBM_EDEVICE, "noalloc: the window arrays do not fit the scratch");
+        d.ready = true;
+    }
+    *out = &d;
+    return 0;
+}
 
 }  // namespace
 
@@ -428,6 +572,83 @@ int iblbm_op_multi_direct_forcing(const double *Lx, const double *Ly, const doub
         stats->last_residual = s[1];
         stats->diverged = (int32_t)s[2];
         stats->converged = (int32_t)s[3];
+        stats->guard_trips = 0;
+        stats->omega_used = omega;
+    }
+    return 0;
+}
+
+int iblbm_op_multi_direct_forcing_noalloc(const double *Lx, const double *Ly, const double *ud,
+                                          int Lb, const double *U, const double *ro, double dx,
+                                          double dy, double dt, double Larea, int ny, int nx,
+                                          int n_iter, int min_iter, int delta, double omega,
+                                          double tol, double *fib_total, iblbm_mdf_stats *stats,
+                                          void *stream) {
+    if (!Lx || !Ly || !ud || !U || !ro || !fib_total)
+        return op_fail(IBLBM_EINVAL, "op_multi_direct_forcing_noalloc: null argument");
+    if (Lb < 1 || Lb > NA_LB_MAX)
+        return op_fail(IBLBM_EINVAL, "op_multi_direct_forcing_noalloc: Lb = " + std::to_string(Lb) +
+                                         " outside [1, " + std::to_string(NA_LB_MAX) + "]");
+    if (nx < 1 || ny < 1 || !(dx > 0.0) || !(dy > 0.0))
+        return op_fail(IBLBM_EINVAL, "op_multi_direct_forcing_noalloc: bad grid");
+    if (n_iter < 1)
+        return op_fail(IBLBM_EINVAL, "op_multi_direct_forcing_noalloc: n_iter must be >= 1");
+    if (delta != 0 && delta != 1)
+        return op_fail(IBLBM_EKEY, "op_multi_direct_forcing_noalloc: unknown delta type");
+    int dev = 0;
+    OPCHK(hipGetDevice(&dev));
+    NaDevice *D = nullptr;
+    int rc = na_setup(dev, &D);
+    if (rc) return rc;
+    Inst v = D->tmpl;  // array pointers into this device's scratch
+    v.nx = nx;
+    v.ny = ny;
+    v.inc = 0;  // the MDF sub-step is compressible (mdf.py:177-181)
+    v.cp.dt = dt;
+    v.delta = delta;
+    v.ibm = 2;
+    v.Lb = Lb;
+    v.dx = dx;
+    v.dy = dy;
+    v.Larea = Larea;
+    v.dfc_scale = 1.0;
+    v.n_iter = n_iter;
+    v.min_iter = min_iter;
+    v.omega = omega;
+    v.tol = tol;
+    Inst *Ip = (Inst *)D->base;  // the first scratch slot
+    double *out = (double *)(D->base + ((sizeof(Inst) + 255) & ~(size_t)255));
+    v.stats = out;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n2 = 2LL * nx * ny;
+    hipLaunchKernelGGL(k_na_zero, dim3((unsigned)std::min<int64_t>((n2 + 255) / 256, 8192)),
+                       dim3(256), 0, st, fib_total, n2);
+    OPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_na_put, dim3(1), dim3(64), 0, st, v, Ip);
+    OPCHK(hipGetLastError());
+    if (delta == 1)
+        hipLaunchKernelGGL(k_na_mdf<1>, dim3(1), dim3(NTI), 0, st, Ip, Lx, Ly, ud, U, ro,
+                           fib_total, out);
+    else
+        hipLaunchKernelGGL(k_na_mdf<2>, dim3(1), dim3(NTI), 0, st, Ip, Lx, Ly, ud, U, ro,
+                           fib_total, out);
+    OPCHK(hipGetLastError());
+    double h[NA_OUT];
+    OPCHK(hipMemcpyAsync(h, out, sizeof h, hipMemcpyDeviceToHost, st));
+    OPCHK(hipStreamSynchronize(st));
+    if (h[8] == 1.0)
+        return op_fail(IBLBM_EINVAL,
+                       "op_multi_direct_forcing_noalloc: non-finite or out-of-range marker");
+    if (h[8] == 2.0)
+        return op_fail(IBLBM_EINVAL, "op_multi_direct_forcing_noalloc: window box " +
+                                         std::to_string((int)h[11]) + " x " +
+                                         std::to_string((int)h[12]) + " exceeds " +
+                                         std::to_string(NA_BOX_MAX) + " nodes");
+    if (stats) {
+        stats->iter_count = (int32_t)h[0];
+        stats->last_residual = h[1];
+        stats->diverged = (int32_t)h[2];
+        stats->converged = (int32_t)h[3];
         stats->guard_trips = 0;
         stats->omega_used = omega;
     }

@@ -288,6 +288,24 @@ int iblbm_op_multi_direct_forcing(const double *Lx, const double *Ly, const doub
                                   double dt, double Larea, int ny, int nx, int n_iter,
                                   int min_iter, int delta, double omega, double tol,
                                   double *fib_total, iblbm_mdf_stats *stats);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the MDF
+ * window of iblbm_op_multi_direct_forcing on caller-owned DEVICE arrays -- Lx, Ly (Lb),
+ * ud (Lb, 2), U (N, 2) interleaved, ro (N) and the output fib_total (N, 2) -- with every
+ * working array (descriptor, marker copies, window tables, touched-node fields, statistics)
+ * carved from one fixed 6 MiB device array of the library per GPU, held until the process
+ * ends; the call allocates nothing.  Limits: 1 <= Lb <= 512 and a window box (every clamped
+ * stencil node) of at most 16384 nodes; a larger box, or a non-finite marker, fails with
+ * IBLBM_EINVAL after the launch and leaves fib_total zero.  Work is ordered on `stream`
+ * (a hipStream_t; NULL for the default stream) and the call waits for it before it returns
+ * the statistics.  Calls that overlap on one GPU (several host threads or streams) share
+ * that array and race. */
+int iblbm_op_multi_direct_forcing_noalloc(const double *Lx, const double *Ly, const double *ud,
+                                          int Lb, const double *U, const double *ro, double dx,
+                                          double dy, double dt, double Larea, int ny, int nx,
+                                          int n_iter, int min_iter, int delta, double omega,
+                                          double tol, double *fib_total, iblbm_mdf_stats *stats,
+                                          void *stream);
 /* compute_lambda / compute_lambda_gpu (dfc.py:126-186, gpu_kernels.py:1091) */
 int iblbm_op_dfc_lambda(const double *Lx, const double *Ly, int Lb, double rho_f, double Larea,
                         double dx, double dy, int ny, int nx, int delta, double *lambda_k);
