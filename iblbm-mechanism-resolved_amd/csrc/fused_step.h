@@ -80,16 +80,26 @@ __device__ inline void light_copy(const Inst &G, Inst &L) {
 
 // The MDF loop decisions (mdf.py:175-229), made identically by every wavefront of the window
 // from the block-wide residual.  In the warm-start modes the loop needs only one true
-// correction; a divergence there restarts plain MDF from the uncorrected velocity once.
+// correction.  It restarts plain MDF from the uncorrected velocity once when the predicted
+// start is no better than no forcing at all (the first residual after the prediction is not
+// below plain MDF's first residual, or is not finite: a wrong or corrupted network), or when
+// the divergence guard trips.
 struct MdfCtl {
     bool mdf, warm;
     int n_iter, min_iter, min_plain;
     double tol, prev = INFINITY, last = INFINITY;
+    double plain0 = INFINITY;  // warm modes: plain MDF's first residual (slip before forcing)
     int iter_count = 0, diverged = 0, converged = 0, restarted = 0, applied = 0;
     // after barrier A: 0 apply this sub-step, 1 stop (divergence guard), 2 restart plain MDF
     __device__ int at_a(double res, int it) {
         if (!mdf) return 0;
         last = res;
+        if (warm && !restarted && it == 0 && !(res < plain0)) {  // NaN compares false
+            restarted = 1;
+            prev = INFINITY;
+            min_iter = min_plain;
+            return 2;
+        }
         if (it > 0 && res > prev * 1.5) {  // divergence guard, mdf.py:192-195
             if (warm && !restarted) {
                 restarted = 1;
@@ -271,7 +281,7 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
             sh.mw[e] = st_w[u];
         }
     }
-    if (t < 2) rbits[t] = 0ull;
+    if (t < 3) rbits[t] = 0ull;
     if (t == 0) lf[Lb] = make_double2(0.0, 0.0);  // target of the padded gather entries
     // the predictor's copies into LDS come from other wavefronts too: every wavefront's own
     // have landed before it reaches the barrier
@@ -373,16 +383,24 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
             }
             const float *nrm = sh.pnrm;
             double ax = 0.0, ay = 0.0, ex = 0.0, ey = 0.0;
+            double rv0 = 0.0;  // plain MDF's first residual at this marker
             if (t < Lb) {  // the loop's own interpolation (register stencils, batched loads)
                 double lx, ly, Fx, Fy;
                 interp_force(lx, ly, Fx, Fy);
                 const double rm = finc ? 1.0 : R;
+                const double vx = udx - lx, vy = udy - ly;
+                rv0 = sqrt(vx * vx + vy * vy);
                 ax = rm * (udx - lx);
                 ay = rm * (udy - ly);
                 S.fprev[2 * I.lb_cap + 2 * t] = ax;  // the next step's previous q
                 S.fprev[2 * I.lb_cap + 2 * t + 1] = ay;
                 ex = ax - spx;
                 ey = ay - spy;
+            }
+            if (warm) {  // block max into the third residual slot, read after barrier A
+                const unsigned long long wb0 = dpp_max_u64_lane63(
+                    t < Lb ? (unsigned long long)__double_as_longlong(rv0) : 0ull);
+                if ((t & 63) == 63) atomicMax(&rbits[2], wb0);
             }
             // the features feed a BF16 network: FP32 arithmetic from here
             const float fex = (float)ex, fey = (float)ey, fax = (float)ax, fay = (float)ay;
@@ -457,6 +475,7 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
                 __syncthreads();  // B: predicted velocities applied
                 continue;
             }
+            if (warm) ctl.plain0 = __longlong_as_double((long long)rbits[2]);
             const int act = ctl.at_a(res, it);
             if (act == 1) break;
             if (act == 0) {
@@ -547,6 +566,7 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
                 continue;
             }
             const double res = __longlong_as_double((long long)rbits[seq & 1]);
+            if (warm) ctl.plain0 = __longlong_as_double((long long)rbits[2]);
             const int act = ctl.at_a(res, it);
             if (act == 1) break;
             if (hn) {

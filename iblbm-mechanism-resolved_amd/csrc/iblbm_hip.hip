@@ -463,7 +463,7 @@ __global__ __launch_bounds__(NTI) void k_ibm(LaunchTable T) {
 // (one node per thread).  One launch per step leaves no cross-stream wait between steps.
 template <typename S, int MODEL, bool PRED, int FL>
 __global__ __launch_bounds__(NTF) void k_step(LaunchTable T) {
-    __shared__ double red[NTF / 64 + 2];
+    __shared__ double red[NTF / 64 + 3];  // wavefront partials, then 3 residual slots
     __shared__ FusedShared sh;
     if ((int)blockIdx.x < T.n_fast) {
         __builtin_amdgcn_s_setprio(3);  // issue ahead of bulk wavefronts sharing the CU

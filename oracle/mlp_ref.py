@@ -65,3 +65,62 @@ def forward_bf16(p, x):
     h1 = np.tanh(bf16(x) @ bf16(w["W1"]) + w["b1"].astype(np.float32))
     h2 = np.tanh(bf16(h1) @ bf16(w["W2"]) + w["b2"].astype(np.float32))
     return bf16(h2) @ bf16(w["W3"]) + w["b3"].astype(np.float32)
+
+
+# --------------------------------------------------------------------------- training
+
+def norm_stats(rows):
+    """The normalisation the device derives from its samples: feature mean and standard
+    deviation (1 where the variance is below 1e-24), target root-mean-square (1 where below
+    1e-30).  rows: (n, IN + OUT)."""
+    r = np.asarray(rows, dtype=np.float64)
+    n = max(r.shape[0], 1)
+    mean = r.sum(0) / n
+    var = (r * r).sum(0) / n - mean * mean
+    sd = np.where(var[:IN] > 1e-24, np.sqrt(np.maximum(var[:IN], 0.0)), 1.0)
+    rms = np.sqrt((r[:, IN:] ** 2).sum(0) / n)
+    rms = np.where(rms > 1e-30, rms, 1.0)
+    return np.concatenate([mean[:IN], sd, rms])
+
+
+_M64 = (1 << 64) - 1
+
+
+def pick(seed, s, b, n):
+    """Row of minibatch slot b at training step s: splitmix64 of (seed << 32) ^ (1024 s + b),
+    modulo n (the published SplitMix64 finaliser, Steele, Lea and Flood 2014)."""
+    z = ((int(seed) << 32) ^ (int(s) * 1024 + int(b))) & _M64
+    z = (z + 0x9E3779B97F4A7C15) & _M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+    z ^= z >> 31
+    return z % int(n)
+
+
+def adam_step(p, m, v, g, t, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One bias-corrected Adam update (Kingma and Ba 2015, Algorithm 1) at step t >= 1."""
+    m = b1 * m + (1.0 - b1) * g
+    v = b2 * v + (1.0 - b2) * g * g
+    mh = m / (1.0 - b1 ** t)
+    vh = v / (1.0 - b2 ** t)
+    return p - lr * mh / (np.sqrt(vh) + eps), m, v
+
+
+def train(p0, rows, norm, steps, lr, seed, batch=128, t0=0):
+    """FP64 restatement of the device training loop: minibatches of `batch` rows picked by
+    ``pick``, normalised by ``norm``, the loss gradient of ``loss_grad`` and Adam from zero
+    moments.  Returns (parameters, minibatch losses)."""
+    rows = np.asarray(rows, dtype=np.float64)
+    n = rows.shape[0]
+    p = np.asarray(p0, dtype=np.float64).copy()
+    m, v = np.zeros_like(p), np.zeros_like(p)
+    losses = []
+    for s in range(steps):
+        idx = [pick(seed, t0 + s, b, n) for b in range(batch)]
+        mb = rows[idx]
+        x = (mb[:, :IN] - norm[:IN]) / norm[IN:2 * IN]
+        y = mb[:, IN:] / norm[2 * IN:]
+        loss, g = loss_grad(p, x, y)
+        losses.append(loss)
+        p, m, v = adam_step(p, m, v, g, t0 + s + 1, lr)
+    return p, np.array(losses)

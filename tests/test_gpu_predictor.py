@@ -165,3 +165,84 @@ def test_learned_predictor_on_a_settling_disc():
         assert abs(res[mode][0] - v_plain) <= 1e-3 * abs(v_plain)
         assert res[mode][2]["restarts"] == 0
     assert res["learned"][2]["substeps_per_step"] < 0.5 * res["collect"][2]["substeps_per_step"]
+
+
+def _collected(steps=600):
+    """A fused-window cylinder with plain-MDF samples recorded over `steps` steps."""
+    sim = FixedCylinder(_cfg(), perturb=(7, 1e-3))
+    assert sim.ctx.step_mode() == "fused"
+    pred = sim.predictor
+    sim.advance(400)
+    pred.mode("collect", capacity=1 << 18)
+    sim.advance(steps)
+    return sim, pred
+
+
+def test_device_training_matches_restated_adam():
+    """Normalisation and ten Adam steps of the device trainer against the FP64 restatement
+    (same samples, same minibatch picks, same start): the normalisation to FP32 rounding, the
+    minibatch losses within 1e-4 relative (a different pick would change them at the percent
+    level), and the parameter changes within 0.1 % of the step size lr for 99.9 % of the
+    parameters (Adam divides by |g|, so parameters whose gradient is near zero are sensitive
+    to the FP32 gradient's rounding; measured: 1.3e-4 at most)."""
+    sim, pred = _collected(300)
+    n = pred.samples()
+    rows = pred.data(0, n).astype(np.float64)
+    p0 = _glorot(21)
+    _, nrm_any = pred.params()
+    pred.set_params(p0, nrm_any)
+    lr, seed, steps = 1e-3, 5, 10
+    losses = pred.train(steps=steps, lr=lr, seed=seed)
+    p_dev, nrm_dev = pred.params()
+    nrm_ref = M.norm_stats(rows)
+    e_n = np.max(np.abs(nrm_dev - nrm_ref) / np.maximum(np.abs(nrm_ref), 1e-30))
+    p_ref, l_ref = M.train(p0, rows, nrm_dev.astype(np.float64), steps, lr, seed)
+    e_l = np.max(np.abs(losses - l_ref) / l_ref)
+    d = np.abs((p_dev - p0) - (p_ref - p0)) / lr
+    print(f"samples {n}: norm rel {e_n:.2e}, losses rel {e_l:.2e}, "
+          f"param change / lr: median {np.median(d):.2e}, 99.9% {np.quantile(d, 0.999):.2e}, "
+          f"max {d.max():.2e}")
+    assert e_n <= 1e-6
+    assert e_l <= 1e-4
+    assert np.quantile(d, 0.999) <= 1e-3
+    assert np.median(d) <= 5e-4
+    sim.ctx.close()
+
+
+@pytest.mark.parametrize("wrong", ["huge", "nan"])
+def test_wrong_network_falls_back_to_plain_mdf(wrong):
+    """A network that predicts nonsense (an output bias of 1e6, or NaN weights) is caught by
+    the first residual after its prediction: the step restarts plain MDF from the
+    uncorrected velocity and equals a plain MDF step from the same state bit for bit."""
+    sim, pred = _collected(300)
+    pred.train(steps=50, lr=2e-3, seed=1)
+    p, nrm = pred.params()
+    bad = p.copy()
+    if wrong == "huge":
+        bad[-2:] = 1e6  # b3: the predicted change of force
+    else:
+        bad[:64] = np.nan  # W1 row 0
+    f0 = sim.ctx.download_f(sim.k)
+    F0 = pred.previous_force(sim.st.Lb)
+    out = {}
+    for mode in ("off", "learned"):
+        sim.ctx.upload_f(sim.k, f0)
+        pred.set_previous_force(F0)
+        if mode == "learned":
+            pred.set_params(bad, nrm)
+        pred.mode(mode)
+        pred.stats(reset=True)
+        sim.advance(3, diag_every=sim.ctx.step_counter() + 3)
+        out[mode] = (sim.ctx.download_f(sim.k), sim.ctx.download_fib(sim.k),
+                     sim.ctx.mdf_stats(sim.k), pred.stats(), sim.cd_cl())
+    pf, pfib, pst, _, pcd = out["off"]
+    lf, lfib, lst, lstats, lcd = out["learned"]
+    print(wrong, "restarts", lstats["restarts"], "plain", pst, "learned", lst)
+    assert lstats["restarts"] == 3  # every step
+    assert np.all(np.isfinite(lf))
+    np.testing.assert_array_equal(lf, pf)
+    np.testing.assert_array_equal(lfib, pfib)
+    assert lst["iter_count"] == pst["iter_count"]
+    assert lst["last_residual"] == pst["last_residual"]
+    assert lcd == pcd
+    sim.ctx.close()
