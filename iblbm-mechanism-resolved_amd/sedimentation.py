@@ -227,7 +227,12 @@ def marker_forces_to_particles(particles, q, cfg, st):
 class Sedimentation:
     """One settling-particle instance on the device, stepped one time step at a time."""
 
-    def __init__(self, cfg, device=0, precision="fp64", ctx=None, f0_cache=None):
+    def __init__(self, cfg, device=0, precision="fp64", ctx=None, f0_cache=None,
+                 mdf_omega=None):
+        """mdf_omega: the relaxation factor the MDF window starts from each step, in place of
+        the one derived from the A-norm estimate (several particles: min(0.8 omega, 0.8),
+        ``multi_particle.py:85-91``); the reference takes it from its cached estimate
+        ``state._mdf_omega``, which a caller may preset."""
         if cfg.motion_type != "sedimentation":
             raise ValueError("Sedimentation needs motion_type='sedimentation'")
         if cfg.settling_inertia_model not in ("none", "explicit_history"):
@@ -274,6 +279,8 @@ class Sedimentation:
             # several particles iterate coupled from min(0.8 omega, 0.8)
             # (multi_particle.py:32-36, :85-86)
             case.mdf_omega = min(om * 0.8, 0.8) if self.multi else om
+            if mdf_omega is not None:
+                case.mdf_omega = float(mdf_omega)
         self.k = self.ctx.add(case)
         if self.multi and cfg.ibm_method in ("DF", "MDF"):
             self.ctx.set_particles(self.k, [p.Lb for p in self.particles],

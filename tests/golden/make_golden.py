@@ -346,7 +346,7 @@ def case_run_mdf_bgk_hat():
         use_convergence=False), record_mdf=True)
 
 
-def _run_sed(name, cfg_kwargs):
+def _run_sed(name, cfg_kwargs, extra=None):
     """Sedimentation run: final fields, per-record particle states, per-step forces."""
     import iblbm
     from iblbm.runtime import step as step_mod
@@ -392,6 +392,7 @@ def _run_sed(name, cfg_kwargs):
          "Lb": int(st.Lb), "lattice_r": float(st.lattice_r), "final_step": res["final_step"],
          "omega": getattr(st, "_mdf_omega", None), "termination_reason":
          res.get("termination_reason"), "history": hist, "gen_seconds": time.time() - t0}
+    S.update(extra or {})
     _save(name, A, S)
 
 
@@ -450,6 +451,55 @@ def case_sed_multi_mdf_cm_p4():
         settling_inertia_model="explicit_history", time_integrator="verlet",
         particles_config=[{"center": (0.5, 2.2), "rho_ratio": 1.5},
                           {"center": (0.52, 1.9), "rho_ratio": 1.3}]))
+
+
+def _run_sed_guard(name, omega_opt, steps):
+    """The coupled MDF of two particles (multi_particle.py:62-316) started from a relaxation
+    factor above its stable range, so that the 1.2x residual guard trips and halves it
+    (:155-165).  The reference derives the starting factor as min(0.8 omega_opt, 0.8) from the
+    cached estimate state._mdf_omega; here the cache is preset to omega_opt and the 0.8 cap
+    lifted, which changes only that input: the guard, the halving and the omega_min stop run
+    as the reference wrote them.  The factor used is recorded as omega_use."""
+    import iblbm.ibm.multi_particle as mp
+    orig, cap = mp.multi_particle_mdf_coupled, mp.MDF_COUPLED_OMEGA_MAX
+
+    def preset(state, cfg):
+        state._mdf_omega = omega_opt
+        orig(state, cfg)
+
+    mp.multi_particle_mdf_coupled = preset
+    mp.MDF_COUPLED_OMEGA_MAX = 1e9
+    try:
+        _run_sed(name, dict(
+            Re=20.0, NN=49, xmax=1.0, ymax=3.0, cylinder_center=(0.5, 2.2),
+            cylinder_D_ratio=0.25, lattice_u=0.02, inflow_u=0.0, bc_type="settling_channel",
+            delta_type="peskin4pt", ibm_method="MDF", collision_model="CM_MRT",
+            mdf_iterations=20, mdf_min_iterations=5, mdf_tolerance=1e-4,
+            marker_spacing_factor=1.0, max_steps=steps, check_interval=1,
+            use_convergence=False, motion_type="sedimentation", rho_ratio=1.5, gravity=5e-5,
+            enable_rotation=True, rotation_coupling="indirect",
+            settling_inertia_model="explicit_history", time_integrator="verlet",
+            particles_config=[{"center": (0.5, 2.2), "rho_ratio": 1.5},
+                              {"center": (0.52, 1.9), "rho_ratio": 1.3}]),
+            extra={"omega_opt": omega_opt, "omega_use": float(min(omega_opt * 0.8, 1e9))})
+    finally:
+        mp.multi_particle_mdf_coupled, mp.MDF_COUPLED_OMEGA_MAX = orig, cap
+
+
+def case_sed_multi_mdf_guard_trip():
+    # omega_use 6.4: one guard trip per step, then converged at 3.2
+    _run_sed_guard("sed_multi_mdf_guard_trip", 8.0, 3)
+
+
+def case_sed_multi_mdf_guard_maxiter():
+    # omega_use 10.4: a trip, then the iteration limit without convergence at 5.2
+    _run_sed_guard("sed_multi_mdf_guard_maxiter", 13.0, 3)
+
+
+def case_sed_multi_mdf_guard_stop():
+    # omega_use 32: three trips down to omega_min = omega_use / 8, which stops the loop
+    # (diverged)
+    _run_sed_guard("sed_multi_mdf_guard_stop", 40.0, 2)
 
 
 CASES = {k[5:]: v for k, v in dict(globals()).items() if k.startswith("case_")}

@@ -46,3 +46,31 @@ def test_sedimentation_matches_reference(case):
     f = sim.ctx.download_f(sim.k)
     print(f"{case}: populations rel Linf {rel_linf(f, g['f_post']):.3e}")
     np.testing.assert_array_equal(f, g["f_post"])
+
+
+@pytest.mark.parametrize("case", ["sed_multi_mdf_guard_trip", "sed_multi_mdf_guard_maxiter",
+                                  "sed_multi_mdf_guard_stop"])
+def test_coupled_mdf_guard_matches_reference(case):
+    """The coupled MDF of two particles started from a relaxation factor above its stable
+    range (omega_use 6.4, 10.4, 32; multi_particle.py:155-165): the 1.2x residual guard trips,
+    halves the factor, and either converges, runs to the iteration limit, or stops at
+    omega_min (diverged).  Every logged column and the populations bit for bit."""
+    g, sc = golden(case)
+    sim = Sedimentation(golden_cfg(sc), mdf_omega=sc["omega_use"])
+    rows = []
+    for _ in range(sc["final_step"]):
+        sim.step()
+        row = [sim.ttt]
+        for p in sim.particles:
+            row += [p.pos[0], p.pos[1], p.vel[0], p.vel[1], p.omega, p.angle, p.force[0],
+                    p.force[1], p.torque]
+        st = sim.mdf_stats
+        row += [st["iter_count"], st["last_residual"], st["guard_trips"], st["omega_used"]]
+        rows.append(row)
+        print(case, "step", sim.ttt, st, flush=True)
+    got = np.array(rows, dtype=np.float64)
+    ref = g["particles"]
+    assert ref[:, -2].min() >= 1  # the guard tripped on every step
+    bad = np.flatnonzero(np.any(got != ref, axis=0))
+    assert bad.size == 0, f"columns {bad.tolist()} differ"
+    np.testing.assert_array_equal(sim.ctx.download_f(sim.k), g["f_post"])
