@@ -121,6 +121,12 @@ SIGNATURES = {
                                                    _d, _i, _i, _i, _i, _i, _d, _d, _VP,
                                                    ctypes.POINTER(MdfStats), _VP]),
     "iblbm_op_dfc_lambda": (_i, [_D, _D, _i, _d, _d, _d, _d, _i, _i, _i, _D]),
+    "iblbm_op_dfc_correction_field": (_i, [_D, _D, _D, _i, _D, _D, _d, _d, _d, _i, _i, _i,
+                                           _D, _D]),
+    "iblbm_op_velocity_correction": (_i, [_D, _D, _D, _l, _d, _i]),
+    "iblbm_op_interp_force": (_i, [_D, _D, _D, _i, _D, _D, _D, _d, _d, _d, _i, _i, _i, _i,
+                                   _D, _D, _D, _D]),
+    "iblbm_op_spread": (_i, [_D, _D, _D, _D, _i, _d, _d, _d, _i, _i, _i, _D, _D]),
     "iblbm_op_dfc_correction": (_i, [_D, _D, _D, _i, _D, _D, _d, _d, _d, _i, _i, _i, _d, _D]),
     "iblbm_predictor_mode": (_i, [_VP, _i, _i, _l]),
     "iblbm_predictor_samples": (_i, [_VP, _i, ctypes.POINTER(ctypes.c_int64)]),

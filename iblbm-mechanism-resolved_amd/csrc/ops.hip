@@ -147,12 +147,15 @@ __global__ void k_op_boundary(int bc, double *fs, int nx, int ny, double in_u, d
 // ------------------------------------------------------------------ IBM operators
 
 // mode 0: DF fields, 1: MDF, 2: DFC lambda only, 3: DFC correction (lambda recomputed),
-// 4: DFC correction with the caller's lambda.  One workgroup; the window machinery is the
-// fused step's (ibm_build_tables / ibm_core).
+// 4: DFC correction with the caller's lambda, 5: DFC correction field alone into ef (N, 9)
+// with the caller's lambda, 6: the same with lambda recomputed, 7: spread of the marker forces
+// the host placed in S.Lfx, S.Lfy.  One workgroup; the window machinery is the fused step's
+// (ibm_build_tables / ibm_core).
 template <int A>
 __device__ void op_ibm_impl(Inst &I, int mode, const double *Eux, const double *Euy,
                             const double *Ro, double *fstar, double *red, int *redi,
-                            IbmShared &sh, int us = 1) {  // us: element stride of Eux, Euy
+                            IbmShared &sh, int us = 1,  // us: element stride of Eux, Euy
+                            double *ef = nullptr) {
     IbmScratch &S = I.s;
     const int t = threadIdx.x;
     const int nbox = I.bw * I.bh;
@@ -161,8 +164,24 @@ __device__ void op_ibm_impl(Inst &I, int mode, const double *Eux, const double *
     const IbmPtrs P = ibm_ptrs<false>(I, sh);  // global working arrays on this path
     MarkerRegs<A> M;
     load_marker_regs<A>(I, M);
+    if (mode == 7) {  // spread as a gather in np.add.at order (df.py:269-283)
+        for (int ti = t; ti < nT; ti += NTI) {
+            double ex, ey;
+            gather2(P, ti, ex, ey);
+            P.fbx[ti] = ex;
+            P.fby[ti] = ey;
+        }
+        __threadfence_block();
+        __syncthreads();
+        for (int sl = t; sl < nbox; sl += NTI) {
+            const int ti = S.tindex[sl];
+            S.fibx[sl] = ti >= 0 ? P.fbx[ti] : 0.0;
+            S.fiby[sl] = ti >= 0 ? P.fby[ti] : 0.0;
+        }
+        return;
+    }
     if (mode >= 2) {
-        if (mode != 4) ibm_dfc_lambda<A>(I, nT, I.Lb);
+        if (mode != 4 && mode != 5) ibm_dfc_lambda<A>(I, nT, I.Lb);
         if (mode == 2) return;
         for (int sl = t; sl < nbox; sl += NTI) {
             const int64_t n = (int64_t)(I.by0 + sl / I.bw) * I.nx + I.bx0 + sl % I.bw;
@@ -174,6 +193,21 @@ __device__ void op_ibm_impl(Inst &I, int mode, const double *Eux, const double *
         __threadfence_block();
         __syncthreads();
         ibm_core<A>(I, nT, P, red, false, M);
+        if (mode >= 5) {  // the Eulerian correction alone: the gather of dfc.py:208-229 from 0
+            for (int ti = t; ti < nT; ti += NTI) {
+                const int sl = S.tslot[ti];
+                const int64_t n = (int64_t)(I.by0 + sl / I.bw) * I.nx + I.bx0 + sl % I.bw;
+                double e[9];
+                for (int q = 0; q < 9; ++q) e[q] = 0.0;
+                for (int x = P.toff[ti], end = P.toff[ti + 1]; x < end; ++x) {
+                    const int k = P.gk[x];
+                    const double wl = P.gw[x];
+                    for (int q = 0; q < 9; ++q) e[q] += S.dfl[9 * k + q] * wl;
+                }
+                for (int q = 0; q < 9; ++q) ef[9 * n + q] = e[q];
+            }
+            return;
+        }
         for (int ti = t; ti < nT; ti += NTI) {
             const int sl = S.tslot[ti];
             const int64_t n = (int64_t)(I.by0 + sl / I.bw) * I.nx + I.bx0 + sl % I.bw;
@@ -200,15 +234,27 @@ __device__ void op_ibm_impl(Inst &I, int mode, const double *Eux, const double *
 
 __global__ __launch_bounds__(NTI) void k_op_ibm(Inst *Ip, int mode, const double *Eux,
                                                const double *Euy, const double *Ro,
-                                               double *fstar) {
+                                               double *fstar, double *ef) {
     __shared__ double red[NTI];
     __shared__ int redi[NTI];
     __shared__ IbmShared sh;
     Inst &I = *Ip;
     if (I.delta == 1)
-        op_ibm_impl<1>(I, mode, Eux, Euy, Ro, fstar, red, redi, sh);
+        op_ibm_impl<1>(I, mode, Eux, Euy, Ro, fstar, red, redi, sh, 1, ef);
     else
-        op_ibm_impl<2>(I, mode, Eux, Euy, Ro, fstar, red, redi, sh);
+        op_ibm_impl<2>(I, mode, Eux, Euy, Ro, fstar, red, redi, sh, 1, ef);
+}
+
+// velocity correction in place (iblbm/ibm/dispatch.py:30-33): u + fib dt / (2 rho), or
+// u + 0.5 fib dt for the incompressible form
+__global__ void k_op_velcorr(double *U, const double *fib, const double *ro, int64_t N,
+                             double dt, int inc) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    for (int c = 0; c < 2; ++c) {
+        const double f = fib[2 * n + c];
+        U[2 * n + c] = inc ? U[2 * n + c] + 0.5 * f * dt : U[2 * n + c] + f * dt / (2.0 * ro[n]);
+    }
 }
 
 namespace {
@@ -523,7 +569,8 @@ int iblbm_op_direct_forcing(const double *Lx, const double *Ly, const double *ud
     const int64_t N = (int64_t)nx * ny;
     const double *dux = op.mem.up(Eux, N), *duy = op.mem.up(Euy, N), *dro = op.mem.up(Ro, N);
     NEED(dux && duy && dro);
-    hipLaunchKernelGGL(k_op_ibm, dim3(1), dim3(NTI), 0, 0, op.d, 0, dux, duy, dro, (double *)nullptr);
+    hipLaunchKernelGGL(k_op_ibm, dim3(1), dim3(NTI), 0, 0, op.d, 0, dux, duy, dro, (double *)nullptr,
+                       (double *)nullptr);
     OPCHK(hipGetLastError());
     OPCHK(hipDeviceSynchronize());
     rc = op.fib_out(fib);
@@ -560,7 +607,8 @@ int iblbm_op_multi_direct_forcing(const double *Lx, const double *Ly, const doub
     const double *dux = op.mem.up(ux.data(), N), *duy = op.mem.up(uy.data(), N);
     const double *dro = op.mem.up(ro, N);
     NEED(dux && duy && dro);
-    hipLaunchKernelGGL(k_op_ibm, dim3(1), dim3(NTI), 0, 0, op.d, 1, dux, duy, dro, (double *)nullptr);
+    hipLaunchKernelGGL(k_op_ibm, dim3(1), dim3(NTI), 0, 0, op.d, 1, dux, duy, dro, (double *)nullptr,
+                       (double *)nullptr);
     OPCHK(hipGetLastError());
     OPCHK(hipDeviceSynchronize());
     rc = op.fib_out(fib_total);
@@ -666,7 +714,8 @@ int iblbm_op_dfc_lambda(const double *Lx, const double *Ly, int Lb, double rho_f
     rc = op.push();
     if (rc) return rc;
     hipLaunchKernelGGL(k_op_ibm, dim3(1), dim3(NTI), 0, 0, op.d, 2, (const double *)nullptr,
-                       (const double *)nullptr, (const double *)nullptr, (double *)nullptr);
+                       (const double *)nullptr, (const double *)nullptr, (double *)nullptr,
+                       (double *)nullptr);
     OPCHK(hipGetLastError());
     OPCHK(hipMemcpy(lambda_k, op.h.s.lam, sizeof(double) * Lb, hipMemcpyDeviceToHost));
     return 0;
@@ -693,11 +742,113 @@ int iblbm_op_dfc_correction(const double *Lx, const double *Ly, const double *ud
         OPCHK(hipMemcpy(op.h.s.lam, lambda_k, sizeof(double) * Lb, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_op_ibm, dim3(1), dim3(NTI), 0, 0, op.d, lambda_k ? 4 : 3,
                        (const double *)nullptr, (const double *)nullptr, (const double *)nullptr,
-                       dfs);
+                       dfs, (double *)nullptr);
     OPCHK(hipGetLastError());
     OPCHK(hipDeviceSynchronize());
     OPCHK(hipMemcpy(fstar, dfs, sizeof(double) * 9 * N, hipMemcpyDeviceToHost));
     OPCHK(hipMemcpy(force, op.h.s.q, sizeof(double) * 2 * Lb, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int iblbm_op_dfc_correction_field(const double *Lx, const double *Ly, const double *ud, int Lb,
+                                  const double *fstar, const double *lambda_k, double dx,
+                                  double dy, double Larea, int ny, int nx, int delta, double *ef,
+                                  double *force) {
+    if (!Lx || !Ly || !ud || !fstar || !ef || !force)
+        return op_fail(IBLBM_EINVAL, "op_dfc_correction_field: null argument");
+    OpIbm op;
+    int rc = op.setup(Lx, Ly, ud, Lb, dx, dy, 1.0, Larea, ny, nx, delta, 0);
+    if (rc) return rc;
+    op.h.ibm = 3;
+    op.h.dfc_scale = 1.0;
+    rc = op.push();
+    if (rc) return rc;
+    const int64_t N = (int64_t)nx * ny;
+    double *dfs = op.mem.up(fstar, 9 * N);
+    double *def = op.mem.get<double>(9 * N);
+    NEED(dfs && def);
+    OPCHK(hipMemset(def, 0, sizeof(double) * 9 * N));
+    if (lambda_k)
+        OPCHK(hipMemcpy(op.h.s.lam, lambda_k, sizeof(double) * Lb, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_op_ibm, dim3(1), dim3(NTI), 0, 0, op.d, lambda_k ? 5 : 6,
+                       (const double *)nullptr, (const double *)nullptr, (const double *)nullptr,
+                       dfs, def);
+    OPCHK(hipGetLastError());
+    OPCHK(hipDeviceSynchronize());
+    OPCHK(hipMemcpy(ef, def, sizeof(double) * 9 * N, hipMemcpyDeviceToHost));
+    OPCHK(hipMemcpy(force, op.h.s.q, sizeof(double) * 2 * Lb, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int iblbm_op_velocity_correction(double *U, const double *fib, const double *ro, int64_t N,
+                                 double dt, int inc) {
+    if (!U || !fib || !ro || N < 0)
+        return op_fail(IBLBM_EINVAL, "op_velocity_correction: bad argument");
+    if (N == 0) return 0;
+    Dev m;
+    double *du = m.up(U, 2 * N);
+    const double *dfib = m.up(fib, 2 * N), *dro = m.up(ro, N);
+    NEED(du && dfib && dro);
+    hipLaunchKernelGGL(k_op_velcorr, dim3(nblk(N, 256)), dim3(256), 0, 0, du, dfib, dro, N, dt,
+                       inc);
+    OPCHK(hipGetLastError());
+    OPCHK(hipMemcpy(U, du, sizeof(double) * 2 * N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int iblbm_op_interp_force(const double *Lx, const double *Ly, const double *ud, int Lb,
+                          const double *Eux, const double *Euy, const double *Ro, double dx,
+                          double dy, double dt, int ny, int nx, int delta, int inc, double *Lfx,
+                          double *Lfy, double *Lux, double *Luy) {
+    if (!Lx || !Ly || !ud || !Eux || !Euy || !Ro || !Lfx || !Lfy || !Lux || !Luy)
+        return op_fail(IBLBM_EINVAL, "op_interp_force: null argument");
+    OpIbm op;
+    int rc = op.setup(Lx, Ly, ud, Lb, dx, dy, dt, 1.0, ny, nx, delta, inc);
+    if (rc) return rc;
+    op.h.ibm = 1;
+    rc = op.push();
+    if (rc) return rc;
+    const int64_t N = (int64_t)nx * ny;
+    const double *dux = op.mem.up(Eux, N), *duy = op.mem.up(Euy, N), *dro = op.mem.up(Ro, N);
+    NEED(dux && duy && dro);
+    hipLaunchKernelGGL(k_op_ibm, dim3(1), dim3(NTI), 0, 0, op.d, 0, dux, duy, dro,
+                       (double *)nullptr, (double *)nullptr);
+    OPCHK(hipGetLastError());
+    OPCHK(hipDeviceSynchronize());
+    OPCHK(hipMemcpy(Lfx, op.h.s.Lfx, sizeof(double) * Lb, hipMemcpyDeviceToHost));
+    OPCHK(hipMemcpy(Lfy, op.h.s.Lfy, sizeof(double) * Lb, hipMemcpyDeviceToHost));
+    OPCHK(hipMemcpy(Lux, op.h.s.Lux, sizeof(double) * Lb, hipMemcpyDeviceToHost));
+    OPCHK(hipMemcpy(Luy, op.h.s.Luy, sizeof(double) * Lb, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int iblbm_op_spread(const double *Lfx, const double *Lfy, const double *Lx, const double *Ly,
+                    int Lb, double dx, double dy, double Larea, int ny, int nx, int delta,
+                    double *Efx, double *Efy) {
+    if (!Lfx || !Lfy || !Lx || !Ly || !Efx || !Efy)
+        return op_fail(IBLBM_EINVAL, "op_spread: null argument");
+    OpIbm op;
+    int rc = op.setup(Lx, Ly, nullptr, Lb, dx, dy, 1.0, Larea, ny, nx, delta, 0);
+    if (rc) return rc;
+    op.h.ibm = 1;
+    rc = op.push();
+    if (rc) return rc;
+    OPCHK(hipMemcpy(op.h.s.Lfx, Lfx, sizeof(double) * Lb, hipMemcpyHostToDevice));
+    OPCHK(hipMemcpy(op.h.s.Lfy, Lfy, sizeof(double) * Lb, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_op_ibm, dim3(1), dim3(NTI), 0, 0, op.d, 7, (const double *)nullptr,
+                       (const double *)nullptr, (const double *)nullptr, (double *)nullptr,
+                       (double *)nullptr);
+    OPCHK(hipGetLastError());
+    OPCHK(hipDeviceSynchronize());
+    const int B = op.h.bw * op.h.bh;
+    std::vector<double> fx(B), fy(B);
+    OPCHK(hipMemcpy(fx.data(), op.h.s.fibx, sizeof(double) * B, hipMemcpyDeviceToHost));
+    OPCHK(hipMemcpy(fy.data(), op.h.s.fiby, sizeof(double) * B, hipMemcpyDeviceToHost));
+    for (int sl = 0; sl < B; ++sl) {  // accumulate into the caller's fields
+        const int64_t n = (int64_t)(op.h.by0 + sl / op.h.bw) * nx + op.h.bx0 + sl % op.h.bw;
+        Efx[n] = Efx[n] + fx[sl];
+        Efy[n] = Efy[n] + fy[sl];
+    }
     return 0;
 }
 

@@ -16,7 +16,14 @@ from . import _abi
 from ._abi import DELTA, dptr, f64, check
 
 _DELTA_TYPE_IDS = dict(DELTA)
-_DELTA_STENCIL_A = {0: 2, 1: 1}
+_DELTA_STENCIL_A = {"peskin4pt": 2, "hat": 1}
+_BLOCK = 256
+
+
+def _grid(n):
+    """Workgroups of _BLOCK covering n items (the reference's launch helper,
+    gpu_kernels.py:748-753); the kernels below choose their own launch shape and ignore it."""
+    return ((int(n) + _BLOCK - 1) // _BLOCK,)
 
 
 def _lib():
@@ -119,12 +126,27 @@ def ibm_direct_forcing_gpu(Lx, Ly, desired_vel, Eux, Euy, Ro, dx, dy, dt, Larea,
 
 
 def ibm_direct_forcing_fields_gpu(Lx, Ly, desired_vel, Eux, Euy, Ro, dx, dy, dt, Larea, ny,
-                                  nx, delta_type="peskin4pt"):
+                                  nx, delta_type="peskin4pt", incompressible=False):
     """``ibm_direct_forcing_fields_gpu`` (gpu_kernels.py:1045): (Efx, Efy, Lux, Luy)."""
     fib, Lux, Luy = ibm_direct_forcing_gpu(Lx, Ly, desired_vel, Eux, Euy, Ro, dx, dy, dt, Larea,
-                                           ny, nx, delta_type, True, False)
+                                           ny, nx, delta_type, True, incompressible)
     return (fib[:, 0].reshape(int(ny), int(nx)).copy(), fib[:, 1].reshape(int(ny), int(nx)).copy(),
             Lux, Luy)
+
+
+def velocity_correction_gpu(U, fib, ro, dt, incompressible=False):
+    """``velocity_correction_gpu`` (gpu_kernels.py:930), in place on U (N, 2): u + fib dt / (2
+    rho), or u + 0.5 fib dt when incompressible (the arithmetic of dispatch.py:30-33)."""
+    if not isinstance(U, np.ndarray) or U.dtype != np.float64:
+        raise ValueError("velocity_correction_gpu updates U in place: U must be a float64 array")
+    Uc = U if U.flags.c_contiguous else np.ascontiguousarray(U)
+    Uc = Uc.reshape(-1, 2)
+    fib, ro = f64(fib).reshape(-1, 2), f64(ro).reshape(-1)
+    check(_lib().iblbm_op_velocity_correction(dptr(Uc), dptr(fib), dptr(ro), Uc.shape[0],
+                                              float(dt), int(bool(incompressible))),
+          "velocity_correction_gpu")
+    if Uc is not U and not np.shares_memory(Uc, U):
+        U[...] = Uc.reshape(U.shape)
 
 
 def ibm_multi_direct_forcing_gpu(Lx, Ly, desired_vel, U, ro, dx, dy, dt, Larea, ny, nx,
@@ -145,7 +167,7 @@ def ibm_multi_direct_forcing_gpu(Lx, Ly, desired_vel, U, ro, dx, dy, dt, Larea, 
                  "diverged": bool(st.diverged), "converged": bool(st.converged)}
 
 
-def compute_lambda_gpu(Lx, Ly, rho_f, Larea, dx, dy, ny, nx, delta_type="peskin4pt"):
+def compute_lambda_gpu(Lx, Ly, rho_f, Larea, dx, dy, ny, nx, delta_type):
     """``compute_lambda_gpu`` (gpu_kernels.py:1091): lambda (Lb,)."""
     Lx, Ly = f64(Lx).reshape(-1), f64(Ly).reshape(-1)
     lam = np.empty(Lx.shape[0])
@@ -155,24 +177,101 @@ def compute_lambda_gpu(Lx, Ly, rho_f, Larea, dx, dy, ny, nx, delta_type="peskin4
     return lam
 
 
-def dfc_correction_gpu(Lx, Ly, desired_vel, fstar, lambda_k, dx, dy, Larea, ny, nx,
-                       delta_type="peskin4pt", lattice=None, correction_scale=1.0):
-    """DFC correction (gpu_kernels.py:1147 / dfc.py:256-342).
+def dfc_correction_gpu(Lx, Ly, desired_vel, fstar, lambda_k, dx, dy, Larea, ny, nx, delta_type,
+                       lattice):
+    """``dfc_correction_gpu`` (gpu_kernels.py:1147-1223): (Ef_corr (N, 9), dfc_force (Lb, 2)).
 
-    Returns (fstar_corrected (N,9), dfc_force (Lb,2)); the reference's GPU wrapper returns the
-    Eulerian correction instead, whose sum with fstar is this first output.
-    """
+    Ef_corr is the Eulerian correction alone (zero outside the marker window), which the
+    caller adds to fstar (``iblbm/ibm/dfc.py:307-316``); ``lattice`` is accepted for the
+    signature (the D2Q9 constants are compiled in).  lambda_k None recomputes it."""
+    Lx, Ly = f64(Lx).reshape(-1), f64(Ly).reshape(-1)
+    ud = f64(desired_vel).reshape(-1, 2)
+    fs = f64(fstar).reshape(-1, 9)
+    lam = f64(lambda_k).reshape(-1) if lambda_k is not None else None
+    ef = np.empty_like(fs)
+    force = np.empty((Lx.shape[0], 2))
+    check(_lib().iblbm_op_dfc_correction_field(
+        dptr(Lx), dptr(Ly), dptr(ud), Lx.shape[0], dptr(fs),
+        dptr(lam) if lam is not None else None, float(dx), float(dy), float(Larea), int(ny),
+        int(nx), _delta_id(delta_type), dptr(ef), dptr(force)), "dfc_correction_gpu")
+    return ef, force
+
+
+def apply_dfc_correction_gpu(Lx, Ly, desired_vel, fstar, dx, dy, Larea, ny, nx, delta_type,
+                             lattice=None, lambda_cache=None, correction_scale=1.0):
+    """``apply_dfc_correction`` (``iblbm/ibm/dfc.py:256-342``) in one device call, with the CPU
+    path's arithmetic (the scale applied to lambda * deviation before spreading):
+    (fstar_corrected (N, 9), dfc_force (Lb, 2), lambda_k (Lb,))."""
     Lx, Ly = f64(Lx).reshape(-1), f64(Ly).reshape(-1)
     ud = f64(desired_vel).reshape(-1, 2)
     fs = f64(fstar).reshape(-1, 9).copy()
-    lam = f64(lambda_k).reshape(-1) if lambda_k is not None else None
+    lam = (f64(lambda_cache).reshape(-1) if lambda_cache is not None else
+           compute_lambda_gpu(Lx, Ly, 1.0, Larea, dx, dy, ny, nx, delta_type))
     force = np.empty((Lx.shape[0], 2))
     check(_lib().iblbm_op_dfc_correction(dptr(Lx), dptr(Ly), dptr(ud), Lx.shape[0], dptr(fs),
-                                         dptr(lam) if lam is not None else None, float(dx),
-                                         float(dy), float(Larea), int(ny), int(nx),
-                                         _delta_id(delta_type), float(correction_scale),
-                                         dptr(force)), "dfc_correction_gpu")
-    return fs, force
+                                         dptr(lam), float(dx), float(dy), float(Larea), int(ny),
+                                         int(nx), _delta_id(delta_type), float(correction_scale),
+                                         dptr(force)), "apply_dfc_correction_gpu")
+    return fs, force, lam
+
+
+class _MarkerKernel:
+    """A reference-private kernel symbol (``_ibm_interp_force_kern``, ``_ibm_spread_kern``,
+    gpu_kernels.py:739-740) as ``ibm/forcing.py:57-63`` and ``:157-164`` call it:
+    ``kern(grid, block, args)`` with the reference's argument tuple, outputs written into the
+    arrays the caller passes.  grid and block are accepted and ignored."""
+
+    def __init__(self, name, fn):
+        self.name, self._fn = name, fn
+
+    def __call__(self, grid, block, args):
+        self._fn(*args)
+
+    def __repr__(self):
+        return f"<marker kernel {self.name}>"
+
+
+def _out(a, n):
+    if not isinstance(a, np.ndarray) or a.dtype != np.float64 or not a.flags.c_contiguous \
+            or a.size != n:
+        raise ValueError("output arrays must be C-contiguous float64 arrays of the right size")
+    return a
+
+
+def _interp_force(Lx, Ly, desired_vel, Eux, Euy, Ro, dx, dy, dt, nx, ny, Lb, delta_type_id,
+                  stencil_a, incompressible, Lfx, Lfy, Lux, Luy):
+    """ibm_interp_force_kernel (gpu_kernels.py:236-286): per marker the interpolated velocity
+    and the force 2 rho~ (ud - U~) / dt (incompressible: 2 (ud - U~) / dt)."""
+    Lb = int(Lb)
+    Lx, Ly = f64(Lx).reshape(-1), f64(Ly).reshape(-1)
+    ud = f64(desired_vel).reshape(-1, 2)
+    Eux, Euy, Ro = (f64(a).reshape(-1) for a in (Eux, Euy, Ro))
+    if _DELTA_STENCIL_A[{0: "peskin4pt", 1: "hat"}[int(delta_type_id)]] != int(stencil_a):
+        raise ValueError("stencil_a does not match delta_type_id")
+    outs = [_out(a, Lb) for a in (Lfx, Lfy, Lux, Luy)]
+    check(_lib().iblbm_op_interp_force(dptr(Lx), dptr(Ly), dptr(ud), Lb, dptr(Eux), dptr(Euy),
+                                       dptr(Ro), float(dx), float(dy), float(dt), int(ny),
+                                       int(nx), _delta_id(int(delta_type_id)),
+                                       int(bool(incompressible)), *map(dptr, outs)),
+          "_ibm_interp_force_kern")
+
+
+def _spread(Lfx, Lfy, Lx, Ly, dx, dy, Larea, nx, ny, Lb, delta_type_id, stencil_a, Efx, Efy):
+    """ibm_spread_kernel (gpu_kernels.py:294-330): adds sum_k Lf_k W ds to Efx, Efy (ny, nx)
+    in place, in np.add.at order (exact for the zeroed fields the reference passes)."""
+    Lb = int(Lb)
+    Lfx, Lfy, Lx, Ly = (f64(a).reshape(-1) for a in (Lfx, Lfy, Lx, Ly))
+    if _DELTA_STENCIL_A[{0: "peskin4pt", 1: "hat"}[int(delta_type_id)]] != int(stencil_a):
+        raise ValueError("stencil_a does not match delta_type_id")
+    Efx, Efy = _out(Efx, int(nx) * int(ny)), _out(Efy, int(nx) * int(ny))
+    check(_lib().iblbm_op_spread(dptr(Lfx), dptr(Lfy), dptr(Lx), dptr(Ly), Lb, float(dx),
+                                 float(dy), float(Larea), int(ny), int(nx),
+                                 _delta_id(int(delta_type_id)), dptr(Efx), dptr(Efy)),
+          "_ibm_spread_kern")
+
+
+_ibm_interp_force_kern = _MarkerKernel("_ibm_interp_force_kern", _interp_force)
+_ibm_spread_kern = _MarkerKernel("_ibm_spread_kern", _spread)
 
 
 def estimate_optimal_omega_from_type(Lx, Ly, delta_type, Larea, dx):

@@ -315,6 +315,32 @@ int iblbm_op_dfc_correction(const double *Lx, const double *Ly, const double *ud
                             double Larea, int ny, int nx, int delta, double correction_scale,
                             double *force);
 
+/* dfc_correction_gpu (gpu_kernels.py:1147-1223): the Eulerian correction field alone,
+ * ef (N,9) = sum_k dF_k W(x - X_k) ds gathered from zero in np.add.at order (dfc.py:189-229),
+ * zero outside the window, and the marker forces q (Lb,2) (:232-253); the caller adds it to
+ * fstar (dfc.py:307-316).  lambda_k NULL: recomputed as compute_lambda does. */
+int iblbm_op_dfc_correction_field(const double *Lx, const double *Ly, const double *ud, int Lb,
+                                  const double *fstar, const double *lambda_k, double dx,
+                                  double dy, double Larea, int ny, int nx, int delta, double *ef,
+                                  double *force);
+/* velocity_correction_gpu (gpu_kernels.py:930-953, dispatch.py:18-33): U (N,2) in place,
+ * u + fib dt / (2 rho), or u + 0.5 fib dt when incompressible. */
+int iblbm_op_velocity_correction(double *U, const double *fib, const double *ro, int64_t N,
+                                 double dt, int incompressible);
+/* ibm_interp_force_kernel (gpu_kernels.py:236-286, bound from ibm/forcing.py:57-63): per
+ * marker the interpolated velocity Lux, Luy and the force Lf = 2 rho~ (ud - U~) / dt
+ * (incompressible: 2 (ud - U~) / dt), in the CPU summation order of df.py:94-128. */
+int iblbm_op_interp_force(const double *Lx, const double *Ly, const double *ud, int Lb,
+                          const double *Eux, const double *Euy, const double *Ro, double dx,
+                          double dy, double dt, int ny, int nx, int delta, int incompressible,
+                          double *Lfx, double *Lfy, double *Lux, double *Luy);
+/* ibm_spread_kernel (gpu_kernels.py:294-330, bound from ibm/forcing.py:157-164): adds the
+ * spread of the marker forces, sum_k Lf_k W ds in np.add.at order (df.py:131-153), to the
+ * (ny, nx) fields Efx, Efy in place (the reference passes zeroed fields). */
+int iblbm_op_spread(const double *Lfx, const double *Lfy, const double *Lx, const double *Ly,
+                    int Lb, double dx, double dy, double Larea, int ny, int nx, int delta,
+                    double *Efx, double *Efy);
+
 /* --- learned MDF force predictor ---
  * No counterpart in the reference: the MDF sub-step surrogate (an 8 -> 64 -> 64 -> 2 tanh MLP
  * per marker from (change of q / sig, direction of q, marker direction, sub-cell offset) to

@@ -117,3 +117,52 @@ def test_gpu_requested_without_device_fails_loudly():
     from iblbm_amd.solver import Context
     with pytest.raises(RuntimeError):
         Context(0)
+
+
+# Every name the reference imports from its gpu_kernels module (iblbm/ibm/dispatch.py:21,
+# mdf.py:131, multi_particle.py:112, forcing.py:57-63 and :157-164, df.py:81, dfc.py:296 and
+# :307, lbm/collision/{bgk,trt,cm_mrt}.py, lbm/streaming.py:83, lbm/equilibrium.py:48,
+# lbm/macroscopic.py:56), with the parameter names and defaults of its definition
+# (iblbm/gpu_kernels.py:751-1147).
+REFERENCE_OPERATORS = {
+    "_grid": (["n"], {}),
+    "collision_gpu": (["fstar", "feq", "U", "fib", "tau", "dt"], {}),
+    "collision_trt_gpu": (["fstar", "feq", "U", "fib", "tau", "tau_minus", "dt"], {}),
+    "collision_cm_mrt_gpu": (["fstar", "feq", "U", "fib", "S_diag", "dt"], {}),
+    "streaming_gpu": (["fstar_old", "f", "nx", "ny"], {}),
+    "macroscopic_gpu": (["fstar", "incompressible"], {"incompressible": False}),
+    "feq_gpu": (["ro", "U", "incompressible"], {"incompressible": False}),
+    "velocity_correction_gpu": (["U", "fib", "ro", "dt", "incompressible"],
+                                {"incompressible": False}),
+    "ibm_direct_forcing_gpu": (["Lx", "Ly", "desired_vel", "Eux", "Euy", "Ro", "dx", "dy", "dt",
+                                "Larea", "ny", "nx", "delta_type", "return_interp_vel",
+                                "incompressible"],
+                               {"delta_type": "peskin4pt", "return_interp_vel": False,
+                                "incompressible": False}),
+    "ibm_direct_forcing_fields_gpu": (["Lx", "Ly", "desired_vel", "Eux", "Euy", "Ro", "dx", "dy",
+                                       "dt", "Larea", "ny", "nx", "delta_type",
+                                       "incompressible"],
+                                      {"delta_type": "peskin4pt", "incompressible": False}),
+    "compute_lambda_gpu": (["Lx", "Ly", "rho_f", "Larea", "dx", "dy", "ny", "nx", "delta_type"],
+                           {}),
+    "dfc_correction_gpu": (["Lx", "Ly", "desired_vel", "fstar", "lambda_k", "dx", "dy", "Larea",
+                            "ny", "nx", "delta_type", "lattice"], {}),
+}
+REFERENCE_CONSTANTS = {"_BLOCK": 256, "_DELTA_TYPE_IDS": {"peskin4pt": 0, "hat": 1},
+                       "_DELTA_STENCIL_A": {"peskin4pt": 2, "hat": 1}}
+
+
+def test_every_reference_import_exists_with_its_signature():
+    import inspect
+    from iblbm_amd import gpu_kernels as G
+    for name, (params, defaults) in REFERENCE_OPERATORS.items():
+        sig = inspect.signature(getattr(G, name))
+        assert list(sig.parameters) == params, (name, list(sig.parameters))
+        got = {k: p.default for k, p in sig.parameters.items()
+               if p.default is not inspect.Parameter.empty}
+        assert got == defaults, (name, got)
+    for name, value in REFERENCE_CONSTANTS.items():
+        assert getattr(G, name) == value, name
+    assert G._grid(257) == (2,) and G._grid(256) == (1,)
+    for name in ("_ibm_interp_force_kern", "_ibm_spread_kern"):
+        assert callable(getattr(G, name))

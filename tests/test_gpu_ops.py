@@ -77,7 +77,9 @@ def test_dfc(dt):
     lam = G.compute_lambda_gpu(U["ibm_Lx"], U["ibm_Ly"], 1.0, La, dx, dx, ny, nx, dt)
     np.testing.assert_array_equal(lam, U[f"dfc_{dt}_lambda"])
     for given in (lam, None):
-        fc, force = G.dfc_correction_gpu(U["ibm_Lx"], U["ibm_Ly"], U["ibm_ud"],
-                                         U[f"dfc_{dt}_fstar"], given, dx, dx, La, ny, nx, dt)
+        fc, force, lam_used = G.apply_dfc_correction_gpu(U["ibm_Lx"], U["ibm_Ly"], U["ibm_ud"],
+                                                         U[f"dfc_{dt}_fstar"], dx, dx, La, ny,
+                                                         nx, dt, lambda_cache=given)
         np.testing.assert_array_equal(fc, U[f"dfc_{dt}_fcorr"])
         np.testing.assert_array_equal(force, U[f"dfc_{dt}_force"])
+        np.testing.assert_array_equal(lam_used, U[f"dfc_{dt}_lambda"])
