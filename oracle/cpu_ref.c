@@ -540,3 +540,156 @@ double ref_a_norm_inf(const double *Lx, const double *Ly, int Lb, int dtype, dou
     }
     return best;
 }
+
+/* ---------------------------------------------------------------------------------------
+ * Zou-He face closures with prescribed or extrapolated velocities.  U is the state's
+ * velocity of the previous step (N, 2), read only (the closures' writes of rho and U are
+ * overwritten by the macroscopic pass that follows, so only fstar is produced here; the
+ * velocities a closure sets are kept in locals).  Face index sets as
+ * iblbm/boundary/indices.py:33-50. */
+
+/* Left face at node f with velocity (u0, u1): rho from the known populations over 1 - u0. */
+static inline void zh_left_rho(double *f, double u0, double u1) {
+    const double ro = (1.0 / (1.0 - u0)) * (f[0] + f[2] + f[4] + 2.0 * (f[3] + f[6] + f[7]));
+    f[1] = f[3] + (2.0 / 3.0) * ro * u0;
+    f[5] = f[7] - 0.5 * (f[2] - f[4]) + (1.0 / 6.0) * ro * u0 + 0.5 * ro * u1;
+    f[8] = f[6] + 0.5 * (f[2] - f[4]) + (1.0 / 6.0) * ro * u0 - 0.5 * ro * u1;
+}
+/* Right face, rho given; u0 from the known populations (pressure form). */
+static inline void zh_right_p(double *f, double ro, double u1) {
+    const double u0 = -1.0 + (f[0] + f[2] + f[4] + 2.0 * (f[1] + f[5] + f[8])) / ro;
+    f[3] = f[1] - (2.0 / 3.0) * ro * u0;
+    f[7] = f[5] + 0.5 * (f[2] - f[4]) - (1.0 / 6.0) * ro * u0 - 0.5 * ro * u1;
+    f[6] = f[8] - 0.5 * (f[2] - f[4]) - (1.0 / 6.0) * ro * u0 + 0.5 * ro * u1;
+}
+
+static inline double extrap(double a1, double a2) { return (4.0 * a1 - a2) / 3.0; }
+
+/* iblbm/boundary/velocity_inlet.py:45-157 (left and right faces with their corners, then
+ * top, bottom, then the corner refills). */
+void ref_bc_velocity_inlet(double *fs, const double *U, int nx, int ny, double in_u,
+                           double dens) {
+    init_w();
+    for (int j = 0; j < ny; ++j) zh_left_rho(fs + 9 * ((long)j * nx), in_u, 0.0);
+    for (int j = 0; j < ny; ++j) {
+        const long n = (long)j * nx + nx - 1;
+        zh_right_p(fs + 9 * n, dens, extrap(U[2 * (n - 1) + 1], U[2 * (n - 2) + 1]));
+    }
+    for (int i = 1; i < nx - 1; ++i) {  /* top: no-slip, tangential u extrapolated */
+        const long n = (long)(ny - 1) * nx + i;
+        double *f = fs + 9 * n;
+        const double u0 = extrap(U[2 * (n - nx)], U[2 * (n - 2L * nx)]), u1 = 0.0;
+        const double ro = (1.0 / (1.0 + u1)) * (f[0] + f[1] + f[3] + 2.0 * (f[2] + f[5] + f[6]));
+        f[4] = f[2] - (2.0 / 3.0) * ro * u1;
+        f[7] = f[5] + 0.5 * (f[1] - f[3]) - 0.5 * ro * u0;
+        f[8] = f[6] - 0.5 * (f[1] - f[3]) + 0.5 * ro * u0;
+    }
+    for (int i = 1; i < nx - 1; ++i) {  /* bottom */
+        const long n = i;
+        double *f = fs + 9 * n;
+        const double u0 = extrap(U[2 * (n + nx)], U[2 * (n + 2L * nx)]), u1 = 0.0;
+        const double ro = (1.0 / (1.0 - u1)) * (f[0] + f[1] + f[3] + 2.0 * (f[4] + f[7] + f[8]));
+        f[2] = f[4] + (2.0 / 3.0) * ro * u1;
+        f[5] = f[7] - 0.5 * (f[1] - f[3]) + 0.5 * ro * u0;
+        f[6] = f[8] + 0.5 * (f[1] - f[3]) - 0.5 * ro * u0;
+    }
+    double fe[9];
+    feq_node(dens, in_u, 0.0, 0, fe); /* inlet corners */
+    memcpy(fs, fe, sizeof fe);
+    memcpy(fs + 9 * ((long)(ny - 1) * nx), fe, sizeof fe);
+    /* outlet corners: u0 extrapolated from the bottom / top face values just set */
+    for (int c = 0; c < 2; ++c) {
+        const int j = c == 0 ? 0 : ny - 1;
+        const int jj = c == 0 ? 1 : ny - 2, jk = c == 0 ? 2 : ny - 3;
+        const double b1 = extrap(U[2 * ((long)jj * nx + nx - 2)], U[2 * ((long)jk * nx + nx - 2)]);
+        const double b2 = extrap(U[2 * ((long)jj * nx + nx - 3)], U[2 * ((long)jk * nx + nx - 3)]);
+        feq_node(dens, extrap(b1, b2), 0.0, 0, fe);
+        memcpy(fs + 9 * ((long)j * nx + nx - 1), fe, sizeof fe);
+    }
+}
+
+/* iblbm/boundary/open_boundary.py:44-162: rho = dens on every face, normal velocity from the
+ * known populations, tangential velocity extrapolated; corners f_eq(dens, U of the diagonal
+ * inner node). */
+void ref_bc_open(double *fs, const double *U, int nx, int ny, double dens) {
+    init_w();
+    const double ro = dens;
+    for (int j = 1; j < ny - 1; ++j) {  /* left */
+        const long n = (long)j * nx;
+        double *f = fs + 9 * n;
+        const double u0 = 1.0 - (f[0] + f[2] + f[4] + 2.0 * (f[3] + f[7] + f[6])) / ro;
+        const double u1 = extrap(U[2 * (n + 1) + 1], U[2 * (n + 2) + 1]);
+        f[1] = f[3] + (2.0 / 3.0) * ro * u0;
+        f[5] = f[7] - 0.5 * (f[2] - f[4]) + (1.0 / 6.0) * ro * u0 + 0.5 * ro * u1;
+        f[8] = f[6] + 0.5 * (f[2] - f[4]) + (1.0 / 6.0) * ro * u0 - 0.5 * ro * u1;
+    }
+    for (int j = 1; j < ny - 1; ++j) {  /* right */
+        const long n = (long)j * nx + nx - 1;
+        zh_right_p(fs + 9 * n, ro, extrap(U[2 * (n - 1) + 1], U[2 * (n - 2) + 1]));
+    }
+    for (int i = 1; i < nx - 1; ++i) {  /* top */
+        const long n = (long)(ny - 1) * nx + i;
+        double *f = fs + 9 * n;
+        const double u1 = -1.0 + (f[0] + f[1] + f[3] + 2.0 * (f[2] + f[5] + f[6])) / ro;
+        const double u0 = extrap(U[2 * (n - nx)], U[2 * (n - 2L * nx)]);
+        f[4] = f[2] - (2.0 / 3.0) * ro * u1;
+        f[7] = f[5] + 0.5 * (f[1] - f[3]) - (1.0 / 6.0) * ro * u1 - 0.5 * ro * u0;
+        f[8] = f[6] - 0.5 * (f[1] - f[3]) - (1.0 / 6.0) * ro * u1 + 0.5 * ro * u0;
+    }
+    for (int i = 1; i < nx - 1; ++i) {  /* bottom */
+        const long n = i;
+        double *f = fs + 9 * n;
+        const double u1 = 1.0 - (f[0] + f[1] + f[3] + 2.0 * (f[4] + f[7] + f[8])) / ro;
+        const double u0 = extrap(U[2 * (n + nx)], U[2 * (n + 2L * nx)]);
+        f[2] = f[4] + (2.0 / 3.0) * ro * u1;
+        f[5] = f[7] - 0.5 * (f[1] - f[3]) + (1.0 / 6.0) * ro * u1 + 0.5 * ro * u0;
+        f[6] = f[8] + 0.5 * (f[1] - f[3]) + (1.0 / 6.0) * ro * u1 - 0.5 * ro * u0;
+    }
+    const long c[4] = {0, nx - 1, (long)nx * (ny - 1), (long)nx * ny - 1};
+    const long d[4] = {nx + 1, 2L * nx - 2, (long)nx * (ny - 2) + 1, (long)nx * (ny - 1) - 2};
+    for (int k = 0; k < 4; ++k) {
+        double fe[9];
+        feq_node(dens, U[2 * d[k]], U[2 * d[k] + 1], 0, fe);
+        memcpy(fs + 9 * c[k], fe, sizeof fe);
+    }
+}
+
+/* iblbm/boundary/tg_analytical.py:29-136: every face (left and right with their corners)
+ * takes the analytic velocity; Ub holds it per boundary node in the order left (ny), right
+ * (ny), top (nx - 2), bottom (nx - 2), corners (0,0), (nx-1,0), (0,ny-1), (nx-1,ny-1). */
+void ref_bc_tg(double *fs, const double *Ub, int nx, int ny) {
+    init_w();
+    const double *UL = Ub, *UR = Ub + 2L * ny, *UT = Ub + 4L * ny, *UB = UT + 2L * (nx - 2);
+    const double *UC = UB + 2L * (nx - 2);
+    for (int j = 0; j < ny; ++j) zh_left_rho(fs + 9 * ((long)j * nx), UL[2 * j], UL[2 * j + 1]);
+    for (int j = 0; j < ny; ++j) {
+        double *f = fs + 9 * ((long)j * nx + nx - 1);
+        const double u0 = UR[2 * j], u1 = UR[2 * j + 1];
+        const double ro = (1.0 / (1.0 + u0)) * (f[0] + f[2] + f[4] + 2.0 * (f[1] + f[5] + f[8]));
+        f[3] = f[1] - (2.0 / 3.0) * ro * u0;
+        f[7] = f[5] + 0.5 * (f[2] - f[4]) - (1.0 / 6.0) * ro * u0 - 0.5 * ro * u1;
+        f[6] = f[8] - 0.5 * (f[2] - f[4]) - (1.0 / 6.0) * ro * u0 + 0.5 * ro * u1;
+    }
+    for (int i = 1; i < nx - 1; ++i) {
+        double *f = fs + 9 * ((long)(ny - 1) * nx + i);
+        const double u0 = UT[2 * (i - 1)], u1 = UT[2 * (i - 1) + 1];
+        const double ro = (1.0 / (1.0 + u1)) * (f[0] + f[1] + f[3] + 2.0 * (f[2] + f[5] + f[6]));
+        f[4] = f[2] - (2.0 / 3.0) * ro * u1;
+        f[7] = f[5] + 0.5 * (f[1] - f[3]) - (1.0 / 6.0) * ro * u1 - 0.5 * ro * u0;
+        f[8] = f[6] - 0.5 * (f[1] - f[3]) - (1.0 / 6.0) * ro * u1 + 0.5 * ro * u0;
+    }
+    for (int i = 1; i < nx - 1; ++i) {
+        double *f = fs + 9 * (long)i;
+        const double u0 = UB[2 * (i - 1)], u1 = UB[2 * (i - 1) + 1];
+        const double ro = (1.0 / (1.0 - u1)) * (f[0] + f[1] + f[3] + 2.0 * (f[4] + f[7] + f[8]));
+        f[2] = f[4] + (2.0 / 3.0) * ro * u1;
+        f[5] = f[7] - 0.5 * (f[1] - f[3]) + (1.0 / 6.0) * ro * u1 + 0.5 * ro * u0;
+        f[6] = f[8] + 0.5 * (f[1] - f[3]) + (1.0 / 6.0) * ro * u1 - 0.5 * ro * u0;
+    }
+    const long c[4] = {0, nx - 1, (long)nx * (ny - 1), (long)nx * ny - 1};
+    for (int k = 0; k < 4; ++k) {
+        double fe[9];
+        feq_node(1.0, UC[2 * k], UC[2 * k + 1], 0, fe);
+        memcpy(fs + 9 * c[k], fe, sizeof fe);
+    }
+}

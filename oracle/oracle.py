@@ -50,6 +50,9 @@ def lib():
             "ref_collide_cm": (None, [p, p, p, p, d, l, p]),
             "ref_bc_dirichlet_neumann": (None, [p, i, i, d, d]),
             "ref_bc_settling": (None, [p, i, i, d, i]),
+            "ref_bc_velocity_inlet": (None, [p, p, i, i, d, d]),
+            "ref_bc_open": (None, [p, p, i, i, d]),
+            "ref_bc_tg": (None, [p, p, i, i]),
             "ref_delta": (d, [i, d]),
             "ref_df": (None, [p, p, p, i, p, p, p, d, d, d, d, i, i, i, i, p, p, p, p, p, p]),
             "ref_mdf": (None, [p, p, p, i, p, p, d, d, d, d, i, i, i, i, i, d, d, p, p]),
@@ -133,6 +136,87 @@ def bc_settling(fstar, nx, ny, dens=1.0, incompressible=False):
     out = _c(fstar).copy()
     lib().ref_bc_settling(_p(out), nx, ny, dens, int(incompressible))
     return out
+
+
+def bc_velocity_inlet(fstar, U, nx, ny, in_u, dens=1.0):
+    """``iblbm/boundary/velocity_inlet.py:45-157``; U is the previous step's velocity."""
+    out, U = _c(fstar).copy(), _c(U)
+    lib().ref_bc_velocity_inlet(_p(out), _p(U), nx, ny, in_u, dens)
+    return out
+
+
+def bc_open(fstar, U, nx, ny, dens=1.0):
+    """``iblbm/boundary/open_boundary.py:44-162``; U is the previous step's velocity."""
+    out, U = _c(fstar).copy(), _c(U)
+    lib().ref_bc_open(_p(out), _p(U), nx, ny, dens)
+    return out
+
+
+def tg_field(X, Y, t, u0, L, nu):
+    """``iblbm/diagnostics/core.py:216-239`` (same NumPy calls)."""
+    decay = np.exp(-2.0 * nu * (np.pi / L) ** 2 * t)
+    ux = -u0 * np.cos(np.pi * X / L) * np.sin(np.pi * Y / L) * decay
+    uy = u0 * np.sin(np.pi * X / L) * np.cos(np.pi * Y / L) * decay
+    return ux, uy
+
+
+def tg_nu(cfg):
+    """The Taylor-Green viscosity of ``tg_analytical.py:59-61`` (and ``init.py:186-187``)."""
+    lattice_D = cfg.cylinder_D_ratio * (cfg.NN - 1)
+    return cfg.lattice_u * lattice_D / cfg.Re
+
+
+def tg_boundary_velocity(cfg, nx, ny, t):
+    """The analytic velocity of every boundary node at time t, computed as
+    ``tg_analytical.py:52-136`` computes it face by face: left (ny), right (ny), top (nx - 2),
+    bottom (nx - 2), corners (0,0), (nx-1,0), (0,ny-1), (nx-1,ny-1); (n, 2)."""
+    L_lat = 0.5 * (cfg.NN - 1)
+    nu = tg_nu(cfg)
+    X = np.arange(nx, dtype=float) - (nx - 1) / 2.0
+    Y = np.arange(ny, dtype=float) - (ny - 1) / 2.0
+    j = np.arange(ny)
+    i = np.arange(1, nx - 1)
+    parts = [tg_field(np.full_like(j, X[0], dtype=float), Y[j], t, cfg.tg_u0, L_lat, nu),
+             tg_field(np.full_like(j, X[nx - 1], dtype=float), Y[j], t, cfg.tg_u0, L_lat, nu),
+             tg_field(X[i], np.full_like(i, Y[ny - 1], dtype=float), t, cfg.tg_u0, L_lat, nu),
+             tg_field(X[i], np.full_like(i, Y[0], dtype=float), t, cfg.tg_u0, L_lat, nu),
+             tg_field(X[np.array([0, nx - 1, 0, nx - 1])], Y[np.array([0, 0, ny - 1, ny - 1])],
+                      t, cfg.tg_u0, L_lat, nu)]
+    return np.concatenate([np.stack([ux, uy], 1) for ux, uy in parts])
+
+
+def bc_tg(fstar, Ub, nx, ny):
+    """``iblbm/boundary/tg_analytical.py:29-136`` with the boundary velocities Ub."""
+    out, Ub = _c(fstar).copy(), _c(Ub)
+    lib().ref_bc_tg(_p(out), _p(Ub), nx, ny)
+    return out
+
+
+def tg_desired_velocity(cfg, st, t):
+    """``iblbm/runtime/scenarios/taylor_green.py:21-32``: the analytic velocity at the
+    markers."""
+    L_lat = 0.5 * (cfg.NN - 1)
+    Xc = st.Lx / st.dx - (st.nx - 1) / 2.0
+    Yc = st.Ly / st.dy - (st.ny - 1) / 2.0
+    ux, uy = tg_field(Xc, Yc, t, cfg.tg_u0, L_lat, tg_nu(cfg))
+    ud = np.empty((len(Xc), 2))
+    ud[:, 0] = ux
+    ud[:, 1] = uy
+    return ud
+
+
+def oscillating_markers(t, Lx_c, Ly_c, lattice_u, KC, r, lattice_r):
+    """``iblbm/physics/motion.py:15-52``: x(t) = A sin(2 pi f0 t); markers shifted by -x(t),
+    desired velocity (x(t), 0)."""
+    f0 = lattice_u / (KC * 2.0 * lattice_r)
+    A = KC * 2.0 * r / (2.0 * np.pi)
+    xt = A * np.sin(f0 * 2.0 * np.pi * t)
+    Lx = Lx_c - xt
+    Ly = Ly_c.copy()
+    ud = np.empty((len(Lx_c), 2))
+    ud[:, 0] = xt
+    ud[:, 1] = 0.0
+    return Lx, Ly, ud
 
 
 def delta(delta_type, r):
@@ -221,7 +305,10 @@ def initialize(cfg, perturb=None):
     ``perturb=(seed, amp)`` applies u <- u*(1 + amp*xi), xi ~ U(-1,1) from
     ``np.random.default_rng(seed)`` over the (N, 2) velocity, then feq = fstar = f_eq.
     """
-    nx = int(cfg.NN * cfg.xmax - (cfg.xmax - 1))
+    if getattr(cfg, "nx_formula", "standard") == "standard":
+        nx = int(cfg.NN * cfg.xmax - (cfg.xmax - 1))
+    else:
+        nx = int(cfg.NN * cfg.xmax)
     dx = 1.0 / (cfg.NN - 1)
     ny = round(cfg.ymax / dx) + 1
     dy = cfg.ymax / (ny - 1)
@@ -233,7 +320,14 @@ def initialize(cfg, perturb=None):
     tau = 3.0 * (cfg.lattice_u * 2.0 * lattice_r / cfg.Re) + 0.5
     N = nx * ny
     U = np.zeros((N, 2))
-    if getattr(cfg, "motion_type", None) is None:
+    if getattr(cfg, "scenario_type", None) == "taylor_green":  # init.py:178-191
+        L_lat = 0.5 * (cfg.NN - 1)
+        XX, YY = np.meshgrid(np.arange(nx, dtype=float) - (nx - 1) / 2.0,
+                             np.arange(ny, dtype=float) - (ny - 1) / 2.0)
+        ux, uy = tg_field(XX.ravel(), YY.ravel(), 0.0, cfg.tg_u0, L_lat, tg_nu(cfg))
+        U[:, 0] = ux
+        U[:, 1] = uy
+    elif getattr(cfg, "motion_type", None) is None:
         U[:, 0] = cfg.inflow_u
     ro = 1.0 * np.ones(N)
     if perturb is not None:
@@ -310,21 +404,35 @@ class FixedCylinder:
         if self.method == "MDF":
             self.omega = omega_from_type(self.st.Lx, self.st.Ly, self.delta, self.st.Larea,
                                          self.st.dx)
+        self.tg = getattr(cfg, "scenario_type", None) == "taylor_green"
+        self.moving = getattr(cfg, "motion_type", None) is not None
+        self.Lx_c, self.Ly_c = self.st.Lx.copy(), self.st.Ly.copy()
         self.f_post = collide(self.st, cfg)
 
     def step(self):
         cfg, st = self.cfg, self.st
         self.ttt += 1
+        t = self.ttt * st.dt
+        if self.tg and cfg.tg_with_ibm:  # taylor_green.py:48-49 (pre_step)
+            st.desired_velocity = tg_desired_velocity(cfg, st, t)
         st.fstar = stream(st.fstar, self.f_post, st.nx, st.ny)
-        if cfg.bc_type == "dirichlet_neumann":
+        if self.tg:  # boundary/__init__.py:14: taylor_green selects tg_analytical
+            st.fstar = bc_tg(st.fstar, tg_boundary_velocity(cfg, st.nx, st.ny, t), st.nx, st.ny)
+        elif cfg.bc_type == "dirichlet_neumann":
             st.fstar = bc_dirichlet_neumann(st.fstar, st.nx, st.ny, st.in_u, st.dens)
         elif cfg.bc_type == "settling_channel":
             st.fstar = bc_settling(st.fstar, st.nx, st.ny, st.dens, st.incompressible)
+        elif cfg.bc_type == "velocity_inlet":
+            st.fstar = bc_velocity_inlet(st.fstar, st.U, st.nx, st.ny, st.in_u, st.dens)
+        elif cfg.bc_type == "open_boundary":
+            st.fstar = bc_open(st.fstar, st.U, st.nx, st.ny, st.dens)
         else:
             raise KeyError(cfg.bc_type)
         st.ro, st.U = macroscopic(st.fstar, st.incompressible)
         Ro = st.ro.reshape(st.ny, st.nx)
-        if self.method == "DF":
+        if self.tg and not cfg.tg_with_ibm:  # dispatch.py:38-39: no IBM at all
+            st.fib = np.zeros((st.N, 2))
+        elif self.method == "DF":
             st.fib = direct_forcing(st.Lx, st.Ly, st.desired_velocity,
                                     st.U[:, 0].reshape(st.ny, st.nx),
                                     st.U[:, 1].reshape(st.ny, st.nx), Ro, st.dx, st.dy, st.dt,
@@ -335,7 +443,7 @@ class FixedCylinder:
                 st.ny, st.nx, cfg.mdf_iterations, cfg.mdf_min_iterations, self.delta,
                 self.omega, cfg.mdf_tolerance)
         elif self.method == "DFC":
-            if self.lambda_cache is None:
+            if self.lambda_cache is None or self.moving:  # strategies/dfc.py:32, :48-49
                 self.lambda_cache = dfc_lambda(st.Lx, st.Ly, 1.0, st.Larea, st.dx, st.dy,
                                                st.ny, st.nx, self.delta)
             st.fstar, self.dfc_force = dfc_correction(
@@ -344,18 +452,26 @@ class FixedCylinder:
             st.ro, st.U = macroscopic(st.fstar, st.incompressible)
         else:
             raise KeyError(self.method)
-        if self.method in ("DF", "MDF"):
+        if self.method in ("DF", "MDF") and not (self.tg and not cfg.tg_with_ibm):
             if st.incompressible:
                 st.U = st.U + 0.5 * st.fib * st.dt
             else:
                 st.U = st.U + st.fib * st.dt / (2.0 * st.ro[:, None])
         st.feq = feq(st.ro, st.U, st.incompressible)
         self.f_post = collide(st, cfg)
+        if getattr(cfg, "motion_type", None) == "oscillating":  # fixed_cylinder.py:55-66
+            st.Lx, st.Ly, st.desired_velocity = oscillating_markers(
+                self.ttt, self.Lx_c, self.Ly_c, cfg.lattice_u, cfg.KC, st.r, st.lattice_r)
         if self.ttt % cfg.check_interval == 0:
-            if self.method == "DFC":
-                cd, cl = cd_cl_dfc(self.dfc_force, float(st.ro.mean()), st.in_u, st.lattice_r)
+            # fixed_cylinder.py:69-71: the oscillating cylinder is scaled by lattice_u
+            u_ref = cfg.lattice_u if getattr(cfg, "motion_type", None) == "oscillating" \
+                else st.in_u
+            if self.tg and not cfg.tg_with_ibm:
+                cd, cl = 0.0, 0.0
+            elif self.method == "DFC":
+                cd, cl = cd_cl_dfc(self.dfc_force, float(st.ro.mean()), u_ref, st.lattice_r)
             else:
-                cd, cl = cd_cl(st.fib, st.ro, st.in_u, st.lattice_r)
+                cd, cl = cd_cl(st.fib, st.ro, u_ref, st.lattice_r)
             self.Cd.append(cd)
             self.Cl.append(cl)
 

@@ -44,6 +44,11 @@ def perturb_of(sc):
 
 RUN_CASES = ["run_df_bgk_hat", "run_mdf_trt_p4", "run_dfc_cm_hat", "run_df_cm_p4",
              "run_dfc_trt_p4", "run_mdf_bgk_hat"]
+# runs with the velocity-inlet, open and Taylor-Green boundaries (oscillating cylinder and
+# Taylor-Green vortex scenarios)
+BC_RUN_CASES = ["run_vi_df_bgk_hat", "run_vi_mdf_cm_p4", "run_osc_df_bgk_hat",
+                "run_osc_mdf_trt_p4", "run_osc_dfc_cm_hat", "run_tg_lbm_d10", "run_tg_df_d10",
+                "run_tg_mdf_d10", "run_tg_dfc_d10"]
 
 
 def rel_linf(a, b):

@@ -346,6 +346,83 @@ def case_run_mdf_bgk_hat():
         use_convergence=False), record_mdf=True)
 
 
+def case_run_vi_df_bgk_hat():
+    # velocity inlet, pressure outlet, no-slip walls (velocity_inlet.py:45-157)
+    _run_case("run_vi_df_bgk_hat", dict(
+        Re=10.0, NN=21, xmax=2.0, ymax=1.0, cylinder_center=(0.5, 0.5), cylinder_D_ratio=0.25,
+        lattice_u=0.05, inflow_u=0.05, bc_type="velocity_inlet", delta_type="hat",
+        ibm_method="DF", collision_model="BGK", marker_spacing_factor=1.0, max_steps=150,
+        check_interval=10, use_convergence=False, mdf_iterations=1),
+        perturb=(31, 1e-3))
+
+
+def case_run_vi_mdf_cm_p4():
+    _run_case("run_vi_mdf_cm_p4", dict(
+        Re=12.0, NN=25, xmax=2.0, ymax=1.0, cylinder_center=(0.5, 0.5), cylinder_D_ratio=0.25,
+        lattice_u=0.04, inflow_u=0.04, bc_type="velocity_inlet", delta_type="peskin4pt",
+        ibm_method="MDF", collision_model="CM_MRT", mdf_iterations=20, mdf_min_iterations=5,
+        mdf_tolerance=1e-4, marker_spacing_factor=1.0, max_steps=120, check_interval=1,
+        use_convergence=False), perturb=(32, 1e-3), record_mdf=True)
+
+
+def _osc(name, method, delta, model, steps=120):
+    # oscillating cylinder in a pressure-open box (open_boundary.py:44-162, motion.py:15-52,
+    # fixed_cylinder.py:55-66), the reference's oscillating builder at a small size
+    _run_case(name, dict(
+        Re=20.0, NN=41, xmax=1.5, ymax=1.0, cylinder_center=(0.75, 0.5), cylinder_D_ratio=0.2,
+        lattice_u=0.1, inflow_u=0.0, bc_type="open_boundary", delta_type=delta,
+        ibm_method=method, collision_model=model, trt_magic_param=0.25,
+        mdf_iterations=20 if method == "MDF" else 1,
+        mdf_min_iterations=5 if method == "MDF" else 1, mdf_tolerance=1e-4,
+        max_steps=steps, check_interval=10, use_convergence=False, motion_type="oscillating",
+        KC=5.0, marker_spacing_factor=2.0 / 3.0, nx_formula="simple"),
+        record_mdf=method == "MDF")
+
+
+def case_run_osc_df_bgk_hat():
+    _osc("run_osc_df_bgk_hat", "DF", "hat", "BGK")
+
+
+def case_run_osc_mdf_trt_p4():
+    _osc("run_osc_mdf_trt_p4", "MDF", "peskin4pt", "TRT")
+
+
+def case_run_osc_dfc_cm_hat():
+    _osc("run_osc_dfc_cm_hat", "DFC", "hat", "CM_MRT")
+
+
+def _tg(name, D, with_ibm, method="DF", delta="peskin4pt", model="BGK", steps=None):
+    # Taylor-Green vortex of scenarios/taylor_green.py:16-85 (D = L lattice units, one
+    # convective time), with the analytic boundary of tg_analytical.py:29-168
+    NN = 2 * D + 1
+    T = int(round(D / 0.04))
+    _run_case(name, dict(
+        Re=10.0, NN=NN, xmax=1.0, ymax=1.0, cylinder_center=(0.5, 0.5), cylinder_D_ratio=0.5,
+        lattice_u=0.04, inflow_u=0.04, bc_type="velocity_inlet", ibm_method=method,
+        delta_type=delta, collision_model=model, max_steps=steps or T,
+        convergence_threshold=1e-10, convergence_start=steps or T,
+        check_interval=max(1, T // 10), use_convergence=False, marker_spacing_factor=1.0,
+        nx_formula="standard", scenario_type="taylor_green", tg_L=1.0, tg_u0=0.04,
+        tg_T_end=float(D) / 0.04, tg_with_ibm=with_ibm, mdf_iterations=10 if method == "MDF"
+        else 1, mdf_min_iterations=5 if method == "MDF" else 1, mdf_tolerance=1e-4))
+
+
+def case_run_tg_lbm_d10():
+    _tg("run_tg_lbm_d10", 10, False)
+
+
+def case_run_tg_df_d10():
+    _tg("run_tg_df_d10", 10, True, "DF", "peskin4pt", "BGK")
+
+
+def case_run_tg_mdf_d10():
+    _tg("run_tg_mdf_d10", 10, True, "MDF", "hat", "TRT")
+
+
+def case_run_tg_dfc_d10():
+    _tg("run_tg_dfc_d10", 10, True, "DFC", "hat", "CM_MRT")
+
+
 def _run_sed(name, cfg_kwargs, extra=None):
     """Sedimentation run: final fields, per-record particle states, per-step forces."""
     import iblbm
