@@ -134,9 +134,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--schedule", default=None,
                     help="pin the launch schedule instead of autotuning: mode,bulk_threads,"
-                         "bulk_rows,bulk_waves,step_rows,step_order[,row_layout[,stores]] "
-                         "(e.g. fused,256,1,0,1,blocked,tiles,normal), for profiling one "
-                         "schedule's kernels")
+                         "bulk_rows,bulk_waves,step_rows,step_order[,row_layout[,stores"
+                         "[,bulk_delay_us]]] (e.g. fused,256,1,0,1,blocked,tiles,normal,0), "
+                         "for profiling one schedule's kernels")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch schedule instead of the autotuner's")
     ap.add_argument("--surrogate-steps", type=int, default=2000,
@@ -197,7 +197,8 @@ def main():
         sim.ctx.set_schedule(mode=f[0], bulk_threads=int(f[1]), bulk_rows=int(f[2]),
                              bulk_waves=int(f[3]), step_rows=int(f[4]), step_order=order,
                              row_layout=f[6] if len(f) > 6 else "tiles",
-                             stores=f[7] if len(f) > 7 else "normal")
+                             stores=f[7] if len(f) > 7 else "normal",
+                             bulk_delay_us=float(f[8]) if len(f) > 8 else 0.0)
         tuned = sim.ctx.schedule()
     elif not args.no_tune and not args.serial:  # launch schedule from the autotuner
         tuned = sim.ctx.autotune(60)

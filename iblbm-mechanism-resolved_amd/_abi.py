@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(_HERE, "libiblbm_hip.so")
 EINVAL, EKEY, EDEVICE, ENOMEM = -1, -2, -3, -4
 
 COLL = {"BGK": 0, "TRT": 1, "CM_MRT": 2}
-BC = {"dirichlet_neumann": 0, "settling_channel": 1}
+BC = {"dirichlet_neumann": 0, "settling_channel": 1, "velocity_inlet": 2, "open_boundary": 3,
+      "tg_analytical": 4}
 IBM = {None: 0, "NONE": 0, "DF": 1, "MDF": 2, "DFC": 3}
 DELTA = {"peskin4pt": 0, "hat": 1}
 DELTA_RADIUS = {"peskin4pt": 2, "hat": 1}
@@ -47,7 +48,8 @@ class Schedule(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("bulk_threads", ctypes.c_int32),
                 ("bulk_rows", ctypes.c_int32), ("bulk_waves", ctypes.c_int32),
                 ("step_rows", ctypes.c_int32), ("step_order", ctypes.c_int32),
-                ("row_layout", ctypes.c_int32), ("store_mode", ctypes.c_int32)]
+                ("row_layout", ctypes.c_int32), ("store_mode", ctypes.c_int32),
+                ("bulk_delay", ctypes.c_int32)]
 
 
 class MdfStats(ctypes.Structure):
@@ -85,6 +87,9 @@ SIGNATURES = {
     "iblbm_set_markers_batch": (_i, [_VP, _i, _I, _I, _D, _D, _D, _D, _D]),
     "iblbm_download_marker_forces_batch": (_i, [_VP, _i, _I, _D, _l]),
     "iblbm_marker_box": (_i, [_D, _D, _i, _d, _d, _i, _i, _i, _I]),
+    "iblbm_set_boundary_velocity": (_i, [_VP, _i, _D]),
+    "iblbm_set_analytic_boundary": (_i, [_VP, _i, _D, _i, _D, _i]),
+    "iblbm_set_desired_velocity": (_i, [_VP, _i, _D]),
     "iblbm_step": (_i, [_VP, _i, _i]),
     "iblbm_sync": (_i, [_VP]),
     "iblbm_graph_run": (_i, [_VP, _i, _i]),

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/iblbm_hip.h"
+#include "edge.h"
 #include "fused_step.h"
 #include "scratch_alloc.h"
 
@@ -311,12 +312,14 @@ __device__ __attribute__((always_inline)) TileNode row_node(const LaunchTable &T
         n.interior = tx * TILE_X >= 1 && tx * TILE_X + TILE_X <= R.nx - 1 && n.j >= 1 &&
                      n.j <= R.ny - 2;
     }
-    // nodes a marker stencil touches belong to the window workgroup
+    // nodes a marker stencil touches belong to the window workgroup; the outermost ring of
+    // an instance whose closure k_edge runs belongs to k_edge
     const bool in_box = R.ibm != 0 & ti >= R.bx0 & ti < R.bx0 + R.bw & tj >= R.by0 &
                         tj < R.by0 + R.bh;
     bool touched = false;
     if (in_box) touched = R.tindex[(tj - R.by0) * R.bw + (ti - R.bx0)] >= 0;
-    n.mine = in_grid && !touched;
+    const bool ring0 = (R.bc >= 2) & (ti == 0 | tj == 0 | ti == R.nx - 1 | tj == R.ny - 1);
+    n.mine = in_grid && !touched && !ring0;
     return n;
 }
 
@@ -458,6 +461,65 @@ __global__ __launch_bounds__(NTI) void k_ibm(LaunchTable T) {
     ibm_role<S, MODEL>(I, T.parity, T.diag, red);
 }
 
+// The three outermost rings of the instances whose closure reads the previous velocity or an
+// analytic velocity (edge.h).  One thread per ring node; workgroup b covers ring slots
+// [edge_blocks[b].y, + 256) of instance edge_blocks[b].x.
+template <typename S>
+__global__ __launch_bounds__(256) void k_edge(LaunchTable T) {
+    __shared__ double red[4];
+    const int2 eb = T.edge_blocks[blockIdx.x];
+    const Inst &I = T.insts[eb.x];
+    const int s = eb.y + (int)threadIdx.x;
+    const long long t = T.stepc[T.parity];  // this step's number (1-based)
+    if (blockIdx.x == 0 && threadIdx.x == 0) T.stepc[T.parity ^ 1] = t + 1;
+    const double *side_prev = I.side + (size_t)2 * I.nring * T.parity;
+    double *side_next = I.side + (size_t)2 * I.nring * (T.parity ^ 1);
+    double rho_b = 0.0;
+    if (s < I.nring) {
+        int i, j;
+        ring_node(s, I.nx, I.ny, i, j);
+        const S *__restrict__ A = reinterpret_cast<const S *>(I.f[T.parity]);
+        double fs[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const int si = i - EXi[q], sj = j - EYi[q];
+            fs[q] = (si >= 0 && si < I.nx && sj >= 0 && sj < I.ny)
+                        ? Store<S>::ld(A, q * I.qs + (int64_t)sj * I.pitch + si, q)
+                        : 0.0;
+        }
+        const bool outer = i == 0 || j == 0 || i == I.nx - 1 || j == I.ny - 1;
+        if (outer) {
+            double tgv[2] = {0.0, 0.0};
+            if (I.bc == IBLBM_BC_TG_ANALYTICAL) {
+                const int k = tg_index(i, j, I.nx, I.ny);
+                const double d = I.tgd[t < I.tg_nt ? t : I.tg_nt - 1];
+                tgv[0] = I.tgc[2 * k] * d;
+                tgv[1] = I.tgc[2 * k + 1] * d;
+            }
+            edge_closure(I, side_prev, tgv, i, j, fs);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) I.edge_fs[9 * s + q] = fs[q];
+        }
+        double r, u0, u1;
+        macro9(fs, I.inc != 0, r, u0, u1);
+        side_next[2 * s] = u0;
+        side_next[2 * s + 1] = u1;
+        if (outer) {
+            double fe[9], out[9];
+            if (I.cp.model != 2) feq9(r, u0, u1, I.inc != 0, fe);
+            collide<false>(I.cp, fs, fe, u0, u1, 0.0, 0.0, out);
+            store9(reinterpret_cast<S *>(I.f[T.parity ^ 1]), I, i, j, out);
+            rho_b = r;
+        }
+    }
+    if (T.diag) {  // the outer ring's rho, a fixed shuffle tree, then the 4 wavefronts
+        const double w = wave_sum(rho_b);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
+        __syncthreads();
+        if (threadIdx.x == 0) T.edge_rho[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+    }
+}
+
 // One whole time step in one launch, for contexts whose windows all take the fused path:
 // workgroups [0, n_fast) run the windows (fused_step.h), the others run four bulk tiles each
 // (one node per thread).  One launch per step leaves no cross-stream wait between steps.
@@ -477,7 +539,14 @@ __global__ __launch_bounds__(NTF) void k_step(LaunchTable T) {
         return;
     }
     // bulk: every wavefront walks the rows with a grid stride (no workgroup barrier: a
-    // wavefront moves on as soon as its rows are done)
+    // wavefront moves on as soon as its rows are done).  With a start delay the windows'
+    // first loads (their critical path) meet an idle memory system; the wait is bounded by
+    // the real-time clock and by an iteration cap.
+    if (T.bulk_delay > 0) {
+        const long long t0 = (long long)wall_clock64();
+        for (int k = 0; k < 100000 && (long long)wall_clock64() - t0 < T.bulk_delay; ++k)
+            __builtin_amdgcn_s_sleep(1);
+    }
     constexpr bool EB = sizeof(S) == 4 || MODEL >= 0;
     const int first = ((int)blockIdx.x - T.n_fast) * (NTF / 64) + (int)(threadIdx.x >> 6);
     const int nw = ((int)gridDim.x - T.n_fast) * (NTF / 64);
@@ -592,11 +661,16 @@ __global__ __launch_bounds__(256) void k_state(const Inst *Ip, int parity, doubl
     const bool in_box = I.ibm != 0 && i >= I.bx0 && i < I.bx0 + I.bw && j >= I.by0 &&
                         j < I.by0 + I.bh;
     const int sl = in_box ? (j - I.by0) * I.bw + (i - I.bx0) : 0;
+    const bool outer = i == 0 || j == 0 || i == I.nx - 1 || j == I.ny - 1;
     if (in_box && I.s.tindex[sl] >= 0) {
         for (int q = 0; q < 9; ++q) fs[q] = I.s.fs[q * I.box_cap + sl];
         r = I.s.ro[sl];
         u0 = I.s.ux0[sl];
         u1 = I.s.uy0[sl];
+    } else if (I.bc >= 2 && outer) {  // the boundary kernel's post-closure populations
+        const int rs = ring_slot(i, j, I.nx, I.ny);
+        for (int q = 0; q < 9; ++q) fs[q] = I.edge_fs[9 * rs + q];
+        macro9(fs, I.inc != 0, r, u0, u1);
     } else {
         fstar_at(buf<S>(I, parity), I, i, j, fs);
         macro9(fs, I.inc != 0, r, u0, u1);
@@ -683,7 +757,7 @@ struct Ctx {
     int last_steps = 0;
     int64_t diag_step = -1;
     std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
-    iblbm_schedule sched{IBLBM_MODE_AUTO, 256, 1, 0, 2, 0, 0, 0};
+    iblbm_schedule sched{IBLBM_MODE_AUTO, 256, 1, 0, 2, 0, 0, 0, 0};
     // asynchronous diagnostics: two pinned host slots [row partials | per-instance box sums]
     // filled by copies queued behind the steps, each with its completion event
     double *h_diag[2] = {nullptr, nullptr};
@@ -696,6 +770,14 @@ struct Ctx {
     double *h_bd = nullptr, *d_bd = nullptr;
     int *h_bi = nullptr, *d_bi = nullptr;
     size_t bd_cap = 0, bi_cap = 0;
+    // boundary kernel of the closures that read the previous velocity (edge.h): its
+    // workgroups (instance, first ring slot), their rho partials, the first workgroup of each
+    // instance, and the device step counter it reads
+    int2 *d_edge_blocks = nullptr;
+    double *d_edge_rho = nullptr;
+    long long *d_stepc = nullptr;
+    int n_edge_blocks = 0, cap_edge_blocks = 0;
+    std::vector<int> h_edge_prefix;
 };
 
 void clear_graphs(Ctx &C) {
@@ -789,6 +871,34 @@ int ensure_tables(Ctx &C) {
         C.d_rho_partial = (double *)p;
         C.cap_blocks = blocks;
     }
+    // the boundary kernel's workgroups: 256 ring slots each, instance by instance
+    std::vector<int2> eb;
+    C.h_edge_prefix.assign(n + 1, 0);
+    for (int k = 0; k < n; ++k) {
+        const Inst &d = C.inst[k].d;
+        if (d.bc >= 2)
+            for (int s0 = 0; s0 < d.nring; s0 += 256) eb.push_back(make_int2(k, s0));
+        C.h_edge_prefix[k + 1] = (int)eb.size();
+    }
+    C.n_edge_blocks = (int)eb.size();
+    if (C.n_edge_blocks > C.cap_edge_blocks) {
+        void *p;
+        HIPCHK(C.mem.alloc(&p, sizeof(int2) * C.n_edge_blocks));
+        C.d_edge_blocks = (int2 *)p;
+        HIPCHK(C.mem.alloc(&p, sizeof(double) * C.n_edge_blocks));
+        C.d_edge_rho = (double *)p;
+        HIPCHK(hipMemsetAsync(C.d_edge_rho, 0, sizeof(double) * C.n_edge_blocks, C.stream));
+        C.cap_edge_blocks = C.n_edge_blocks;
+    }
+    if (C.n_edge_blocks && !C.d_stepc) {
+        void *p;
+        HIPCHK(C.mem.alloc(&p, sizeof(long long) * 2));
+        C.d_stepc = (long long *)p;
+        HIPCHK(hipMemsetAsync(C.d_stepc, 0, sizeof(long long) * 2, C.stream));
+    }
+    if (C.n_edge_blocks)
+        HIPCHK(hipMemcpyAsync(C.d_edge_blocks, eb.data(), sizeof(int2) * eb.size(),
+                              hipMemcpyHostToDevice, C.stream));
     std::vector<Inst> hd(n);
     for (int k = 0; k < n; ++k) hd[k] = C.inst[k].d;
     HIPCHK(hipMemcpyAsync(C.d_insts, hd.data(), sizeof(Inst) * n, hipMemcpyHostToDevice,
@@ -825,6 +935,11 @@ LaunchTable make_table(Ctx &C, int parity, int diag) {
     T.diag = diag;
     T.rows2 = 0;
     T.blocked = 0;
+    T.bulk_delay = 0;
+    T.edge_blocks = C.d_edge_blocks;
+    T.edge_rho = C.d_edge_rho;
+    T.stepc = C.d_stepc;
+    T.n_edge_blocks = C.n_edge_blocks;
     return T;
 }
 
@@ -882,9 +997,30 @@ void pick_kernels(const Ctx &C, StepKernel &bulk, StepKernel &ibm, StepKernel &s
 //    both depend on both kernels of the previous step through events, so the window
 //    workgroups overlap the bulk tiles of the same step.
 //  serial: window kernel then bulk kernel on the context stream.
+// The device step counter the boundary kernel reads: slot `parity` holds the number (from 1)
+// of the next step.  Launched outside any graph capture, before a batch of steps.
+__global__ void k_set_step(long long *stepc, int parity, long long t) { stepc[parity] = t; }
+int set_step_counter(Ctx &C) {
+    if (!C.n_edge_blocks || !C.d_stepc) return 0;
+    hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, C.stream, C.d_stepc, C.parity,
+                       (long long)(C.step + 1));
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 int launch_steps(Ctx &C, int nsteps, int diag_every, bool concurrent) {
+    for (const HostInst &h : C.inst)
+        if (h.d.bc == IBLBM_BC_TG_ANALYTICAL && (!h.d.tgc || !h.d.tgd || h.d.tg_nt < 1))
+            return fail(IBLBM_EINVAL, "step: an analytic-boundary instance has no velocity "
+                                      "table (iblbm_set_analytic_boundary)");
     StepKernel kb, ki, ks;
     pick_kernels(C, kb, ki, ks);
+    const StepKernel ke = C.inst[0].d.prec == 1 ? (StepKernel)k_edge<float>
+                                                : (StepKernel)k_edge<double>;
+    auto edge = [&](const LaunchTable &T) {  // the closures that read the previous velocity
+        if (C.n_edge_blocks)
+            hipLaunchKernelGGL(ke, dim3(C.n_edge_blocks), dim3(256), 0, C.stream, T);
+    };
     const int rows = C.n_tiles * TILE_Y;
     const int mode = concurrent ? effective_mode(C) : IBLBM_MODE_SERIAL;
     if (mode == IBLBM_MODE_FUSED) {
@@ -896,7 +1032,9 @@ int launch_steps(Ctx &C, int nsteps, int diag_every, bool concurrent) {
             LaunchTable T = make_table(C, (C.parity + s) & 1, diag);
             T.rows2 = C.sched.step_rows == 2 ? 1 : 0;
             T.blocked = C.sched.step_order;
+            T.bulk_delay = C.sched.bulk_delay;
             hipLaunchKernelGGL(ks, dim3(blocks), dim3(NTF), 0, C.stream, T);
+            edge(T);
         }
         HIPCHK(hipGetLastError());
         return 0;
@@ -923,11 +1061,13 @@ int launch_steps(Ctx &C, int nsteps, int diag_every, bool concurrent) {
             }
             hipLaunchKernelGGL(ki, dim3(C.n_ibm), dim3(NTI), 0, C.side, T);
             if (rows) hipLaunchKernelGGL(kb, dim3(bblocks), dim3(bt), 0, C.stream, T);
+            edge(T);
             HIPCHK(hipEventRecord(C.ev_main, C.stream));
             HIPCHK(hipEventRecord(C.ev_side, C.side));
         } else {
             if (C.n_ibm) hipLaunchKernelGGL(ki, dim3(C.n_ibm), dim3(NTI), 0, C.stream, T);
             if (rows) hipLaunchKernelGGL(kb, dim3(bblocks), dim3(bt), 0, C.stream, T);
+            edge(T);
         }
     }
     if (two) HIPCHK(hipStreamWaitEvent(C.stream, C.ev_side, 0));  // join back
@@ -1016,7 +1156,10 @@ int iblbm_instance_add(void *p, const iblbm_case *c, int *id) {
     if (c->nx < 3 || c->ny < 3) return fail(IBLBM_EINVAL, "instance_add: grid must be >= 3x3");
     if (c->collision < 0 || c->collision > 2)
         return fail(IBLBM_EKEY, "instance_add: unknown collision model");
-    if (c->bc < 0 || c->bc > 1) return fail(IBLBM_EKEY, "instance_add: unknown bc_type");
+    if (c->bc < 0 || c->bc > IBLBM_BC_TG_ANALYTICAL)
+        return fail(IBLBM_EKEY, "instance_add: unknown bc_type");
+    if (c->bc >= 2 && (c->nx < 7 || c->ny < 7))
+        return fail(IBLBM_EINVAL, "instance_add: this boundary closure needs a grid >= 7x7");
     if (c->ibm < 0 || c->ibm > 3) return fail(IBLBM_EKEY, "instance_add: unknown ibm_method");
     if (c->delta < 0 || c->delta > 1) return fail(IBLBM_EKEY, "instance_add: unknown delta_type");
     if (c->precision != IBLBM_FP64 && c->precision != IBLBM_FP32_DEV)
@@ -1090,6 +1233,17 @@ int iblbm_instance_add(void *p, const iblbm_case *c, int *id) {
     HIPCHK(hipMemsetAsync(d.s.fiby, 0, sizeof(double) * B, C->stream));
     HIPCHK(hipMemsetAsync(d.stats, 0, sizeof(double) * 4, C->stream));
     HIPCHK(hipMemsetAsync(d.diag, 0, sizeof(double) * 8, C->stream));
+    if (c->bc >= 2) {  // ring velocities (two steps) and the outer ring's populations
+        d.nring = ring_count(d.nx, d.ny);
+        if (C->mem.alloc(&ptr, sizeof(double) * 4 * (size_t)d.nring) != hipSuccess)
+            return fail(IBLBM_ENOMEM, "instance_add: boundary buffers do not fit");
+        d.side = (double *)ptr;
+        HIPCHK(hipMemsetAsync(ptr, 0, sizeof(double) * 4 * (size_t)d.nring, C->stream));
+        if (C->mem.alloc(&ptr, sizeof(double) * 9 * (size_t)d.nring) != hipSuccess)
+            return fail(IBLBM_ENOMEM, "instance_add: boundary buffers do not fit");
+        d.edge_fs = (double *)ptr;
+        HIPCHK(hipMemsetAsync(ptr, 0, sizeof(double) * 9 * (size_t)d.nring, C->stream));
+    }
     d.Lb = 0;
     if (c->ibm != 0) d.ibm = 0;  // inactive until markers arrive
     h.c.ibm = c->ibm;
@@ -1146,6 +1300,9 @@ int iblbm_set_markers(void *p, int id, const double *Lx, const double *Ly, const
         return fail(IBLBM_EINVAL, "set_markers: window box exceeds box_capacity");
     if (bx0 < 0 || by0 < 0 || bx0 + bw > d.nx || by0 + bh > d.ny)
         return fail(IBLBM_EINVAL, "set_markers: window box outside the grid");
+    if (d.bc >= 2 && (bx0 < 3 || by0 < 3 || bx0 + bw > d.nx - 3 || by0 + bh > d.ny - 3))
+        return fail(IBLBM_EINVAL, "set_markers: with this boundary closure the window must stay "
+                                  "three nodes away from every wall");
     if (!(Larea > 0.0)) return fail(IBLBM_EINVAL, "set_markers: Larea must be positive");
     if (d.np_part > 1 && d.poff[d.np_part] != Lb)
         return fail(IBLBM_EINVAL, "set_markers: Lb differs from the particles' marker total");
@@ -1594,6 +1751,8 @@ int iblbm_step(void *p, int nsteps, int diag_every) {
     HIPCHK(hipSetDevice(C->device));
     int rc = ensure_tables(*C);
     if (rc) return rc;
+    rc = set_step_counter(*C);
+    if (rc) return rc;
     HIPCHK(hipEventRecord(C->ev0, C->stream));
     rc = launch_steps(*C, nsteps, diag_every, C->concurrent);
     if (rc) return rc;
@@ -1633,6 +1792,8 @@ int iblbm_graph_run(void *p, int nsteps, int diag_every) {
     } else {
         exec = it->second;
     }
+    rc = set_step_counter(*C);
+    if (rc) return rc;
     HIPCHK(hipEventRecord(C->ev0, C->stream));
     HIPCHK(hipGraphLaunch(exec, C->stream));
     HIPCHK(hipEventRecord(C->ev1, C->stream));
@@ -1700,6 +1861,8 @@ int iblbm_set_schedule(void *p, const iblbm_schedule *in) {
         return fail(IBLBM_EINVAL, "set_schedule: row_layout must be 0 (tiles) or 1 (flat)");
     if (in->store_mode != 0 && in->store_mode != 1)
         return fail(IBLBM_EINVAL, "set_schedule: store_mode must be 0 or 1 (non-temporal)");
+    if (in->bulk_delay < 0 || in->bulk_delay > 1000)
+        return fail(IBLBM_EINVAL, "set_schedule: bulk_delay must be in [0, 1000] (10 ns ticks)");
     HIPCHK(hipSetDevice(C->device));
     HIPCHK(hipStreamSynchronize(C->stream));
     C->sched = *in;
@@ -1737,7 +1900,10 @@ int iblbm_autotune(void *p, int steps, double *best_ms, int *n_tried) {
             if (C->n_fast == C->n_ibm && st == 0)  // the fused step stores normally
                 for (int order : {0, 1, 6, 8})
                     for (int r = 1; r <= 2; ++r)
-                        cand.push_back({IBLBM_MODE_FUSED, 256, 1, 0, r, order, lay, st});
+                        for (int dl : {0, 100, 200})  // bulk start delay: 0, 1, 2 us
+                            if (dl == 0 || C->n_fast > 0)
+                                cand.push_back({IBLBM_MODE_FUSED, 256, 1, 0, r, order, lay, st,
+                                                dl});
             // the two-kernel launch with the windows beside the bulk (two streams) or before
             // it (serial: no contention for the CUs, no overlap either)
             std::vector<int> modes2 = {IBLBM_MODE_SERIAL};
@@ -1746,7 +1912,7 @@ int iblbm_autotune(void *p, int steps, double *best_ms, int *n_tried) {
                 for (int bt : {256, 512, 1024})
                     for (int r = 1; r <= 2; ++r)
                         for (int wv : {0, 16 * C->n_cu, 32 * C->n_cu})
-                            cand.push_back({mode2, bt, r, wv, 2, 0, lay, st});
+                            cand.push_back({mode2, bt, r, wv, 2, 0, lay, st, 0});
         }
     // the trial steps start from a copy of every instance's source populations
     const int n = (int)C->inst.size();
@@ -1826,9 +1992,76 @@ int iblbm_autotune(void *p, int steps, double *best_ms, int *n_tried) {
     return 0;
 }
 
+int iblbm_set_boundary_velocity(void *p, int id, const double *U) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || !U) return fail(IBLBM_EINVAL, "set_boundary_velocity: bad argument");
+    Inst &d = C->inst[id].d;
+    if (d.bc < 2)
+        return fail(IBLBM_EINVAL, "set_boundary_velocity: the instance's closure reads no "
+                                  "previous velocity");
+    std::vector<double> v(2 * (size_t)d.nring);
+    for (int s = 0; s < d.nring; ++s) {
+        int i, j;
+        ring_node(s, d.nx, d.ny, i, j);
+        const int64_t n = (int64_t)j * d.nx + i;
+        v[2 * s] = U[2 * n];
+        v[2 * s + 1] = U[2 * n + 1];
+    }
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipMemcpyAsync(d.side + 2 * (size_t)d.nring * C->parity, v.data(),
+                          sizeof(double) * v.size(), hipMemcpyHostToDevice, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return 0;
+}
+
+int iblbm_set_analytic_boundary(void *p, int id, const double *coef, int ncoef,
+                                const double *decay, int nt) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || !coef || !decay || nt < 1)
+        return fail(IBLBM_EINVAL, "set_analytic_boundary: bad argument");
+    Inst &d = C->inst[id].d;
+    if (d.bc != IBLBM_BC_TG_ANALYTICAL)
+        return fail(IBLBM_EINVAL, "set_analytic_boundary: the instance's closure is not analytic");
+    const int want = 2 * d.ny + 2 * (d.nx - 2) + 4;
+    if (ncoef != want)
+        return fail(IBLBM_EINVAL, "set_analytic_boundary: ncoef must be 2 ny + 2 (nx - 2) + 4 = " +
+                                      std::to_string(want));
+    HIPCHK(hipSetDevice(C->device));
+    void *a, *b;
+    if (C->mem.alloc(&a, sizeof(double) * 2 * (size_t)ncoef) != hipSuccess ||
+        C->mem.alloc(&b, sizeof(double) * (size_t)nt) != hipSuccess)
+        return fail(IBLBM_ENOMEM, "set_analytic_boundary: tables do not fit");
+    HIPCHK(hipMemcpyAsync(a, coef, sizeof(double) * 2 * (size_t)ncoef, hipMemcpyHostToDevice,
+                          C->stream));
+    HIPCHK(hipMemcpyAsync(b, decay, sizeof(double) * (size_t)nt, hipMemcpyHostToDevice,
+                          C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    d.tgc = (const double *)a;
+    d.tgd = (const double *)b;
+    d.tg_nt = nt;
+    C->tables_dirty = true;
+    return ensure_tables(*C);
+}
+
+int iblbm_set_desired_velocity(void *p, int id, const double *ud) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || !ud) return fail(IBLBM_EINVAL, "set_desired_velocity: bad argument");
+    const Inst &d = C->inst[id].d;
+    if (d.ibm == 0 || d.Lb < 1)
+        return fail(IBLBM_EINVAL, "set_desired_velocity: the instance has no markers yet");
+    HIPCHK(hipSetDevice(C->device));
+    HIPCHK(hipMemcpyAsync((void *)d.ud, ud, sizeof(double) * 2 * (size_t)d.Lb,
+                          hipMemcpyHostToDevice, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return 0;
+}
+
 int iblbm_init_flow(void *p, int id, double rho, double ux, double uy, double amp,
                     uint64_t seed) {
     Ctx *C = as_ctx(p);
+    if (valid_id(C, id) && C->inst[id].d.bc >= 2)
+        return fail(IBLBM_EINVAL, "init_flow: this boundary closure needs the host's initial "
+                                  "velocity (iblbm_set_boundary_velocity)");
     if (!valid_id(C, id)) return fail(IBLBM_EINVAL, "init_flow: bad instance id");
     HIPCHK(hipSetDevice(C->device));
     const Inst &d = C->inst[id].d;
@@ -1868,7 +2101,7 @@ int iblbm_diag_fetch(void *p, int slot) {
     int rc = ensure_tables(*C);
     if (rc) return rc;
     const size_t rows = (size_t)C->n_tiles * TILE_Y, n = C->inst.size();
-    const size_t need = rows + 8 * n;
+    const size_t need = rows + 8 * n + (size_t)C->n_edge_blocks;
     if (need > C->h_diag_cap) {  // (re)size both slots once no copy into them is pending
         HIPCHK(hipStreamSynchronize(C->stream));
         for (int k = 0; k < 2; ++k) {
@@ -1892,6 +2125,10 @@ int iblbm_diag_fetch(void *p, int slot) {
         else
             for (int q = 0; q < 5; ++q) h[rows + 8 * k + q] = 0.0;
     }
+    if (C->n_edge_blocks) {  // the boundary kernel's outer-ring partials
+        HIPCHK(hipMemcpyAsync(h + rows + 8 * n, C->d_edge_rho, sizeof(double) * C->n_edge_blocks,
+                              hipMemcpyDeviceToHost, C->stream));
+    }
     HIPCHK(hipEventRecord(C->ev_diag[slot], C->stream));
     C->diag_step_slot[slot] = C->diag_step;
     return 0;
@@ -1910,6 +2147,9 @@ int iblbm_get_diag_slot(void *p, int id, int slot, iblbm_diag *out) {
     for (int b = b0; b < b1; ++b) rs += h[b];  // the order of iblbm_get_diag
     const double *box = h + rows + 8 * (size_t)id;
     rs += box[0];
+    if (!C->h_edge_prefix.empty())
+        for (int e = C->h_edge_prefix[id]; e < C->h_edge_prefix[id + 1]; ++e)
+            rs += h[rows + 8 * C->inst.size() + e];
     out->rho_sum = rs;
     out->force[0] = box[1];
     out->force[1] = box[2];
@@ -1972,6 +2212,14 @@ int iblbm_get_diag(void *p, int id, iblbm_diag *out) {
     double rs = 0.0;
     for (double v : part) rs += v;
     rs += box[0];
+    const int e0 = C->h_edge_prefix.empty() ? 0 : C->h_edge_prefix[id];
+    const int e1 = C->h_edge_prefix.empty() ? 0 : C->h_edge_prefix[id + 1];
+    if (e1 > e0) {  // the boundary kernel's outer-ring partials
+        std::vector<double> ep((size_t)(e1 - e0));
+        HIPCHK(hipMemcpy(ep.data(), C->d_edge_rho + e0, sizeof(double) * ep.size(),
+                         hipMemcpyDeviceToHost));
+        for (double v : ep) rs += v;
+    }
     out->rho_sum = rs;
     out->force[0] = box[1];
     out->force[1] = box[2];

@@ -94,6 +94,13 @@ struct Inst {
     double *diag;   // [rho_box, force_x, force_y, torque_fib, torque_q]
     double cxc, cyc;  // torque centre in lattice units (cx/dx, cy/dy)
     double cxd, cyd;  // torque centre in domain units
+    // closures run by k_edge (bc >= 2, edge.h): the three outermost rings' velocities of the
+    // previous and of this step (2 x nring x 2), the outer ring's post-closure populations
+    // of the last step (nring x 9, for downloads), the analytic-velocity coefficients
+    // (A, B per boundary node: u = (A d(t), B d(t))) and decay table d(t), t < tg_nt
+    double *side, *edge_fs;
+    const double *tgc, *tgd;
+    int nring, tg_nt;
 };
 
 struct LaunchTable {
@@ -106,6 +113,14 @@ struct LaunchTable {
     int diag;                // take diagnostic reductions this step
     int rows2;               // bulk wavefronts keep two rows in flight (else one)
     int blocked;             // fused launch: each bulk wavefront takes one block of rows
+    int bulk_delay;          // fused launch: bulk start delay in 10 ns ticks
+    // boundary kernel (k_edge): per workgroup (instance, first ring slot), and its rho
+    // partial sums; the step number t of this step is stepc[parity] (the kernel writes t + 1
+    // to the other slot for the next step)
+    const int2 *edge_blocks;
+    double *edge_rho;
+    long long *stepc;
+    int n_edge_blocks;
 };
 
 // ---------------------------------------------------------------- storage traits

@@ -94,8 +94,6 @@ def initialize(cfg, perturb=None, fields: bool = True) -> HostState:
     ``fields=False`` leaves out the full-grid U0 / feq0 arrays (set to None), for instances
     whose initial state is formed on the device.
     """
-    if cfg.scenario_type == "taylor_green":
-        raise ValueError("taylor_green is outside the GPU hot path of this package")
     nx, ny, dx, dy = grid_of(cfg)
     N = nx * ny
     phy_char = cfg.phy_l * cfg.cylinder_D_ratio
@@ -109,7 +107,10 @@ def initialize(cfg, perturb=None, fields: bool = True) -> HostState:
     U = feq0 = None
     if fields:
         U = np.zeros((N, 2))
-        if cfg.motion_type is None:
+        if cfg.scenario_type == "taylor_green":  # init.py:178-191
+            from .taylor_green import initial_velocity
+            U = initial_velocity(cfg, nx, ny)
+        elif cfg.motion_type is None:
             U[:, 0] = cfg.inflow_u
         ro = 1.0 * np.ones(N)
         if perturb is not None:

@@ -57,6 +57,22 @@ class Context:
         check(self.lib.iblbm_set_markers(self.h, k, dptr(Lx), dptr(Ly), dptr(ud), Lx.shape[0],
                                          float(Larea)), "set_markers")
 
+    def set_boundary_velocity(self, k, U):
+        """The previous velocity the first step's boundary closure reads (N, 2)."""
+        U = f64(U).reshape(-1, 2)
+        check(self.lib.iblbm_set_boundary_velocity(self.h, k, dptr(U)), "set_boundary_velocity")
+
+    def set_analytic_boundary(self, k, coef, decay):
+        """Analytic boundary tables: per boundary node (A, B), and d(t) for t = 0, 1, ..."""
+        coef, decay = f64(coef).reshape(-1, 2), f64(decay).reshape(-1)
+        check(self.lib.iblbm_set_analytic_boundary(self.h, k, dptr(coef), coef.shape[0],
+                                                   dptr(decay), decay.shape[0]),
+              "set_analytic_boundary")
+
+    def set_desired_velocity(self, k, ud):
+        ud = f64(ud).reshape(-1, 2)
+        check(self.lib.iblbm_set_desired_velocity(self.h, k, dptr(ud)), "set_desired_velocity")
+
     def set_torque_center(self, k, cx, cy):
         check(self.lib.iblbm_set_torque_center(self.h, k, float(cx), float(cy)), "torque_center")
 
@@ -125,17 +141,19 @@ class Context:
                 "bulk_waves": sc.bulk_waves, "step_rows": sc.step_rows,
                 "step_order": {0: "stride", 1: "blocked"}.get(sc.step_order, sc.step_order),
                 "row_layout": ROW_LAYOUTS[sc.row_layout],
-                "stores": STORE_MODES[sc.store_mode]}
+                "stores": STORE_MODES[sc.store_mode], "bulk_delay_us": sc.bulk_delay / 100.0}
 
     def set_schedule(self, mode="auto", bulk_threads=256, bulk_rows=1, bulk_waves=0,
-                     step_rows=2, step_order="stride", row_layout="tiles", stores="normal"):
+                     step_rows=2, step_order="stride", row_layout="tiles", stores="normal",
+                     bulk_delay_us=0.0):
         """step_order: "stride", "blocked" (one block per wavefront) or a block size k >= 2;
         row_layout: "tiles" (64-column tiles of lattice rows) or "flat" (64 consecutive
-        row-major nodes per bulk row); stores: "normal" or "nontemporal"."""
+        row-major nodes per bulk row); stores: "normal" or "nontemporal"; bulk_delay_us: the
+        fused launch's bulk start delay (0 to 10 us, in steps of 10 ns)."""
         order = {"stride": 0, "blocked": 1}.get(step_order, step_order)
         sc = _abi.Schedule(_abi.MODES[mode], bulk_threads, bulk_rows, bulk_waves, step_rows,
                            int(order), ROW_LAYOUTS.index(row_layout),
-                           STORE_MODES.index(stores))
+                           STORE_MODES.index(stores), int(round(bulk_delay_us * 100)))
         check(self.lib.iblbm_set_schedule(self.h, ctypes.byref(sc)), "set_schedule")
 
     def row_layout_in_use(self) -> str:
@@ -255,9 +273,23 @@ class Context:
         return ms.value, n.value
 
 
+def bc_name(cfg) -> str:
+    """The boundary closure a configuration runs: ``tg_analytical`` for the Taylor-Green
+    scenario (``iblbm/boundary/__init__.py:62-75``), else ``bc_type``."""
+    return "tg_analytical" if cfg.scenario_type == "taylor_green" else cfg.bc_type
+
+
+def ibm_name(cfg):
+    """The immersed-boundary method a configuration runs (none for a Taylor-Green vortex
+    without its cylinder, ``iblbm/ibm/dispatch.py:38-39``)."""
+    if cfg.scenario_type == "taylor_green" and not cfg.tg_with_ibm:
+        return None
+    return cfg.ibm_method
+
+
 def make_case(cfg, st, precision=_abi.FP64) -> _abi.Case:
     """Translate a ``SimConfig`` + initial state into the library's instance description."""
-    if cfg.bc_type not in _abi.BC:
+    if bc_name(cfg) not in _abi.BC:
         raise KeyError(f"bc_type={cfg.bc_type!r} is outside the GPU hot path "
                        f"(supported: {sorted(_abi.BC)})")
     if cfg.ibm_method not in ("DF", "MDF", "DFC"):
@@ -267,8 +299,8 @@ def make_case(cfg, st, precision=_abi.FP64) -> _abi.Case:
     c = _abi.Case()
     c.nx, c.ny = st.nx, st.ny
     c.collision = _abi.COLL[cfg.collision_model]
-    c.bc = _abi.BC[cfg.bc_type]
-    c.ibm = _abi.IBM[cfg.ibm_method]
+    c.bc = _abi.BC[bc_name(cfg)]
+    c.ibm = _abi.IBM[ibm_name(cfg)]
     c.delta = _abi.DELTA[cfg.delta_type]
     c.incompressible = int(bool(cfg.incompressible_lbgk))
     c.precision = precision
@@ -301,6 +333,21 @@ def pre_collide(cfg, st):
     return gpu_kernels.collision_cm_mrt_gpu(st.feq0, st.feq0, st.U0, fib, st.S_diag, st.dt)
 
 
+def oscillating_markers(t, Lx_c, Ly_c, lattice_u, KC, r, lattice_r):
+    """``update_oscillating`` (``iblbm/physics/motion.py:15-52``): x(t) = A sin(2 pi f0 t)
+    with A = KC D / (2 pi), f0 = U / (KC D); the markers move by -x(t) and their desired
+    velocity is (x(t), 0)."""
+    f0 = lattice_u / (KC * 2.0 * lattice_r)
+    A = KC * 2.0 * r / (2.0 * np.pi)
+    xt = A * np.sin(f0 * 2.0 * np.pi * t)
+    Lx = Lx_c - xt
+    Ly = Ly_c.copy()
+    ud = np.empty((len(Lx_c), 2))
+    ud[:, 0] = xt
+    ud[:, 1] = 0.0
+    return Lx, Ly, ud
+
+
 PRECISIONS = {"fp64": _abi.FP64, "fp32dev": _abi.FP32_DEV}
 ROW_LAYOUTS = ["tiles", "flat"]
 STORE_MODES = ["normal", "nontemporal"]
@@ -315,8 +362,9 @@ class FixedCylinder:
     """
 
     def __init__(self, cfg, perturb=None, device=0, ctx=None, precision="fp64", init="host"):
-        if cfg.motion_type is not None:
-            raise ValueError("FixedCylinder runs motion_type=None; use the sedimentation driver")
+        if cfg.motion_type not in (None, "oscillating"):
+            raise ValueError("FixedCylinder runs motion_type None or 'oscillating'; use the "
+                             "sedimentation driver")
         if precision not in PRECISIONS:
             raise KeyError(f"precision={precision!r} (expected one of {sorted(PRECISIONS)})")
         if init not in ("host", "device"):
@@ -330,26 +378,64 @@ class FixedCylinder:
         self.ctx = ctx or Context(device)
         case = make_case(cfg, self.st, PRECISIONS[precision])
         self.omega = None
-        if cfg.ibm_method == "MDF":
+        self.has_ibm = ibm_name(cfg) is not None
+        if self.has_ibm and cfg.ibm_method == "MDF":
             self.omega = gpu_kernels.estimate_optimal_omega_from_type(
                 self.st.Lx, self.st.Ly, cfg.delta_type, self.st.Larea, self.st.dx)
             case.mdf_omega = self.omega
         self.k = self.ctx.add(case)
-        self.ctx.set_markers(self.k, self.st.Lx, self.st.Ly, self.st.desired_velocity,
-                             self.st.Larea)
-        cx, cy = cfg.cylinder_center
-        self.ctx.set_torque_center(self.k, cx, cy)
+        self.Lx_c, self.Ly_c = self.st.Lx.copy(), self.st.Ly.copy()
+        if self.has_ibm:
+            self.ctx.set_markers(self.k, self.st.Lx, self.st.Ly, self.st.desired_velocity,
+                                 self.st.Larea)
+            cx, cy = cfg.cylinder_center
+            self.ctx.set_torque_center(self.k, cx, cy)
         if init == "host":
             self.ctx.upload_f(self.k, pre_collide(cfg, self.st))
         else:
             seed, amp = perturb if perturb is not None else (0, 0.0)
             self.ctx.init_flow(self.k, 1.0, cfg.inflow_u, 0.0, amp, seed)
+        name = bc_name(cfg)
+        if name in ("velocity_inlet", "open_boundary"):  # the first step's previous velocity
+            self.ctx.set_boundary_velocity(self.k, self.st.U0)
+        elif name == "tg_analytical":
+            from . import taylor_green as TG
+            self.ctx.set_analytic_boundary(self.k, TG.boundary_coefficients(cfg, self.st.nx,
+                                                                            self.st.ny),
+                                           TG.decay_table(cfg, max(int(cfg.max_steps), 1)))
+        # host work between steps: the Taylor-Green markers' desired velocity
+        # (taylor_green.py:48-49, before the step) and the oscillating markers
+        # (fixed_cylinder.py:55-66, after it)
+        self.tg_markers = name == "tg_analytical" and self.has_ibm
+        self.oscillating = cfg.motion_type == "oscillating"
+        self._markers_due = None
         self.ttt = 0
 
     def advance(self, n, diag_every=0, graph=False):
-        if n > 0:
+        if n <= 0:
+            return
+        if not (self.tg_markers or self.oscillating):
             self.ctx.step(n, diag_every, graph)
             self.ttt += n
+            return
+        cfg, st = self.cfg, self.st
+        for _ in range(n):  # one step at a time, with the host's marker updates around it
+            if self.tg_markers:
+                from . import taylor_green as TG
+                ud = TG.desired_velocity(cfg, st.Lx, st.Ly, st.dx, st.dy, st.nx, st.ny,
+                                         (self.ttt + 1) * st.dt)
+                self.ctx.set_desired_velocity(self.k, ud)
+            if self._markers_due is not None:  # the positions computed after the last step
+                self.ctx.set_markers(self.k, st.Lx, st.Ly, self._markers_due, st.Larea)
+                self._markers_due = None
+            self.ctx.step(1, diag_every, False)
+            self.ttt += 1
+            if self.oscillating:
+                # the host's positions move now, as the reference's do; the device's marker
+                # window keeps the step's until the next step, so that the state and fib read
+                # between steps are those of the step just taken
+                st.Lx, st.Ly, self._markers_due = oscillating_markers(
+                    self.ttt, self.Lx_c, self.Ly_c, cfg.lattice_u, cfg.KC, st.r, st.lattice_r)
 
     @property
     def predictor(self):
@@ -362,7 +448,9 @@ class FixedCylinder:
         the one fetched into ``slot``."""
         d = self.ctx.diag(self.k, slot)
         ro_mean = d.rho_sum / self.st.nodenums
-        denom = ro_mean * self.st.in_u ** 2 * self.st.lattice_r
+        # fixed_cylinder.py:69-71: an oscillating cylinder is scaled by lattice_u
+        u_ref = self.cfg.lattice_u if self.cfg.motion_type == "oscillating" else self.st.in_u
+        denom = ro_mean * u_ref ** 2 * self.st.lattice_r
         return float(abs(d.force[0]) / denom), float(d.force[1] / denom)
 
 
@@ -424,7 +512,7 @@ def run(cfg, verbose: bool = True, callback=None, save_dir=None, perturb=None,
     termination_reason, plus ``mdf_stats`` (last step of each diagnostic interval) and
     ``sim`` (the device-resident simulation, whose state can be downloaded).
     """
-    if cfg.motion_type is not None:
+    if cfg.motion_type == "sedimentation":
         raise ValueError(f"motion_type={cfg.motion_type!r}: sedimentation runs through "
                          "iblbm_amd.sedimentation.run")
     if cfg.stream_first:

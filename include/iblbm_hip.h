@@ -47,6 +47,12 @@ extern "C" {
 /* Boundary closures: iblbm/boundary/{dirichlet_neumann,settling_channel}.py */
 #define IBLBM_BC_DIRICHLET_NEUMANN 0
 #define IBLBM_BC_SETTLING_CHANNEL 1
+/* Closures that read the previous step's velocity or an analytic velocity (run by a small
+ * boundary kernel beside the bulk): iblbm/boundary/{velocity_inlet,open_boundary,
+ * tg_analytical}.py.  Marker windows must stay three nodes away from every wall. */
+#define IBLBM_BC_VELOCITY_INLET 2
+#define IBLBM_BC_OPEN 3
+#define IBLBM_BC_TG_ANALYTICAL 4
 /* IBM methods: iblbm/ibm/strategies/{df,mdf,dfc}.py */
 #define IBLBM_IBM_NONE 0
 #define IBLBM_IBM_DF 1
@@ -161,6 +167,22 @@ int iblbm_download_marker_forces_batch(void *ctx, int n, const int *ids, double 
 int iblbm_marker_box(const double *Lx, const double *Ly, int Lb, double dx, double dy, int nx,
                      int ny, int delta, int box[4]);
 
+/* Closures that read the previous step's velocity (IBLBM_BC_VELOCITY_INLET, IBLBM_BC_OPEN):
+ * the velocity U (N,2) the first step's closure reads, i.e. the initial state's velocity
+ * (iblbm/init.py:175-195); the library keeps the three outermost rings of it. */
+int iblbm_set_boundary_velocity(void *ctx, int id, const double *U);
+/* IBLBM_BC_TG_ANALYTICAL (iblbm/boundary/tg_analytical.py:29-168): the boundary velocity at
+ * step t is (A_k d(t), B_k d(t)) for boundary node k, coef = (A_k, B_k) in the order left (ny
+ * nodes, j = 0..ny-1), right (ny), top (i = 1..nx-2), bottom (i = 1..nx-2), corners (0,0),
+ * (nx-1,0), (0,ny-1), (nx-1,ny-1); ncoef = 2 ny + 2 (nx - 2) + 4.  decay[t] for t < nt (the
+ * last entry holds beyond).  The host forms both with the reference's NumPy expressions
+ * (iblbm/diagnostics/core.py:216-239), so the products reproduce its velocities. */
+int iblbm_set_analytic_boundary(void *ctx, int id, const double *coef, int ncoef,
+                                const double *decay, int nt);
+/* Replaces the markers' desired velocity (Lb,2) only, keeping their positions and window
+ * tables (iblbm/runtime/scenarios/taylor_green.py:21-32 updates it every step). */
+int iblbm_set_desired_velocity(void *ctx, int id, const double *ud);
+
 /* Runs nsteps steps of every instance (asynchronous).  diag_every > 0 takes the diagnostic
  * reductions on steps whose counter is a multiple of it. */
 int iblbm_step(void *ctx, int nsteps, int diag_every);
@@ -206,6 +228,9 @@ typedef struct {
                            * normally; 1: with non-temporal stores, which leave the last-level
                            * cache to the loads (pays once the populations outgrow it: about
                            * 256 MB on MI355X).  The fused step always stores normally */
+    int32_t bulk_delay;   /* fused launch: the bulk wavefronts start this many 10 ns ticks
+                           * (0..1000) after their workgroup, so that the windows' first
+                           * loads meet an idle memory system; 0 starts them at once */
 } iblbm_schedule;
 int iblbm_get_schedule(void *ctx, iblbm_schedule *out);
 int iblbm_set_schedule(void *ctx, const iblbm_schedule *in);
