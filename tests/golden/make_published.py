@@ -1,0 +1,72 @@
+"""Copy the reference's published benchmark results into tests/golden/published.npz.
+
+The reference ships the outputs of its long runs as data files: the Taylor-Green spatial
+convergence tables (``data/taylor_green/{df,mdf,dfc}/convergence.npz`` with Peskin-4 markers,
+``data/taylor_green/{df,mdf,dfc}/hat/convergence.npz`` with the hat kernel) and the
+oscillating-cylinder runs at Re = 100, KC = 5 on the full 2401 x 1601 lattice for 40000 steps
+(``data/oscillating_cylinder/re100_kc5/<ibm>_<collision>_<delta>/status.json``: Cd and Cl every
+50 steps, Cd_peak from step 8000; ``oscillating_re100_kc5_summary_50tail.json``: the
+statistics of the last half).  This script reads them as data (``numpy.load`` without
+pickles, ``json``) and writes the numbers into one fixture, since the reference tree is not
+present where the GPU tests run.  Nothing of the reference is executed.
+
+Usage:  python tests/golden/make_published.py
+Output: tests/golden/published.npz
+"""
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+
+REF = "/root/reference/data"
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "published.npz")
+
+
+def main():
+    out = {}
+    for m in ("df", "mdf", "dfc"):
+        for sub, tag in (("", "p4"), ("hat", "hat")):
+            z = np.load(os.path.join(REF, "taylor_green", m, sub, "convergence.npz"))
+            key = f"tg_{m}_{tag}"
+            out[key + "_resolutions"] = z["resolutions"].astype(np.int64)
+            if "errors_ibm" in z.files:  # df/convergence.npz also holds the pure-LBM column
+                out[key + "_errors"] = z["errors_ibm"]
+                out[key + "_slope"] = z["slope_ibm"]
+                out["tg_lbm_p4_resolutions"] = z["resolutions"].astype(np.int64)
+                out["tg_lbm_p4_errors"] = z["errors_lbm"]
+                out["tg_lbm_p4_slope"] = z["slope_lbm"]
+            else:
+                out[key + "_errors"] = z["errors"]
+                out[key + "_slope"] = z["slope"]
+            for extra in ("mdf_iterations", "mdf_min_iterations", "mdf_tolerance"):
+                if extra in z.files:
+                    out[f"{key}_{extra}"] = z[extra]
+    base = os.path.join(REF, "oscillating_cylinder", "re100_kc5")
+    with open(os.path.join(base, "oscillating_re100_kc5_summary_50tail.json")) as fh:
+        summary = {c["name"]: c for c in json.load(fh)["cases"]}
+    names = sorted(d for d in os.listdir(base) if os.path.isdir(os.path.join(base, d)))
+    out["osc_names"] = np.array(names)
+    for n in names:
+        with open(os.path.join(base, n, "status.json")) as fh:
+            s = json.load(fh)
+        assert s["step"] == s["final_step"] == s["config"]["max_steps"]
+        out[f"osc_{n}_steps"] = np.array(s["steps"], dtype=np.int64)
+        out[f"osc_{n}_Cd"] = np.array(s["Cd_history"], dtype=np.float64)
+        out[f"osc_{n}_Cl"] = np.array(s["Cl_history"], dtype=np.float64)
+        out[f"osc_{n}_Cd_peak"] = np.float64(s["Cd_peak"])
+        out[f"osc_{n}_validation_start_step"] = np.int64(s["validation_start_step"])
+        out[f"osc_{n}_NN"] = np.int64(s["config"]["NN"])
+        out[f"osc_{n}_max_steps"] = np.int64(s["config"]["max_steps"])
+        c = summary[n]
+        out[f"osc_{n}_tail"] = np.array([c["tail_start_step"], c["tail_n_samples"],
+                                         c["Cd_peak_50tail"], c["Cd_min_50tail"],
+                                         c["Cd_amp_50tail"], c["Cd_mean_50tail"],
+                                         c["Cl_amp_50tail"]], dtype=np.float64)
+    np.savez_compressed(OUT, **out)
+    print(f"wrote {OUT}: {len(out)} arrays, {os.path.getsize(OUT)} bytes")
+
+
+if __name__ == "__main__":
+    main()

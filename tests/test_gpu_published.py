@@ -1,0 +1,183 @@
+"""The reference's published long-run results, reproduced on the device at their full sizes.
+
+* Taylor-Green spatial convergence (the reference's ``scenarios/taylor_green.py`` set-up:
+  [-L, L]^2 with D = L = 10, 20, 40, 80 lattice units, u0 = 0.04, Re = 10, one convective time
+  L / u0, every face held at the analytic velocity, optionally a cylinder whose markers follow
+  the analytic field): the L2 error of the velocity against the analytic field
+  (``iblbm/diagnostics/core.py:241-263``) at each resolution and the fitted order, against the
+  reference's tables (``data/taylor_green/{df,mdf,dfc}[/hat]/convergence.npz``).  The
+  pure-lattice column and the hat-kernel columns reproduce the table to 1e-11 and 2e-6
+  relative; the reference's Peskin-4 IBM columns come from a code state that differs from the
+  shipped one (the oracle of the shipped code lands 3e-4..4e-3 from them, the device on the
+  oracle's bits), so those are held to 5e-3 and their orders to 0.02.  At D = 10 and 20 the
+  device's errors equal the oracle's bit for bit.
+* Oscillating cylinder, Re = 100, KC = 5, on the full 2401 x 1601 lattice for 40000 steps
+  (``scenarios/oscillating.py``): Cd and Cl every 50 steps, Cd_peak from step 8000 and the
+  statistics of the last half, against the reference's runs
+  (``data/oscillating_cylinder/re100_kc5/*/status.json`` and its 50%-tail summary).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from iblbm_amd import SimConfig
+from iblbm_amd import taylor_green as TG
+from iblbm_amd.solver import FixedCylinder
+
+pytestmark = pytest.mark.gpu
+PUB = np.load(os.path.join(os.path.dirname(__file__), "golden", "published.npz"))
+
+
+def tg_config(D, method=None, delta="peskin4pt", mdf_it=1, mdf_min=1, model="BGK"):
+    """``make_tg_config`` (``scenarios/taylor_green.py:16-85``)."""
+    u0, Re = 0.04, 10.0
+    NN = 2 * D + 1
+    T = 0.5 * (NN - 1) / u0
+    steps = int(round(T))
+    return SimConfig(Re=Re, NN=NN, xmax=1.0, ymax=1.0, cylinder_center=(0.5, 0.5),
+                     cylinder_D_ratio=0.5, lattice_u=u0, inflow_u=u0, bc_type="velocity_inlet",
+                     ibm_method=method or "DF", delta_type=delta, collision_model=model,
+                     max_steps=steps, convergence_threshold=1e-10, convergence_start=steps,
+                     check_interval=max(1, int(T) // 10), use_convergence=False,
+                     marker_spacing_factor=1.0, nx_formula="standard",
+                     scenario_type="taylor_green", tg_L=1.0, tg_u0=u0, tg_T_end=T,
+                     tg_with_ibm=method is not None, mdf_iterations=mdf_it,
+                     mdf_min_iterations=mdf_min)
+
+
+def tg_error(cfg, precision="fp64"):
+    sim = FixedCylinder(cfg, precision=precision)
+    sim.advance(cfg.max_steps)
+    _, _, U = sim.ctx.download_state(sim.k)
+    e = TG.l2_error(U, TG.analytic_velocity(cfg, sim.st.nx, sim.st.ny, cfg.max_steps * sim.st.dt))
+    sim.ctx.close()
+    return e
+
+
+def order(dxs, errs):
+    return float(np.polyfit(np.log(dxs), np.log(errs), 1)[0])
+
+
+# (column, IBM method, delta, MDF iterations, MDF minimum, error tolerance)
+TG_COLUMNS = [("lbm_p4", None, "peskin4pt", 1, 1, 1e-11),
+              ("df_p4", "DF", "peskin4pt", 1, 1, 5e-3),
+              ("mdf_p4", "MDF", "peskin4pt", 20, 5, 5e-3),
+              ("dfc_p4", "DFC", "peskin4pt", 1, 1, 5e-3),
+              ("df_hat", "DF", "hat", 1, 1, 2e-6),
+              ("mdf_hat", "MDF", "hat", 20, 5, 2e-6),
+              ("dfc_hat", "DFC", "hat", 1, 1, 2e-6)]
+
+
+@pytest.mark.parametrize("col", TG_COLUMNS, ids=[c[0] for c in TG_COLUMNS])
+def test_taylor_green_convergence_table(col):
+    name, method, delta, mi, mn, tol = col
+    res = PUB[f"tg_{name}_resolutions"]
+    ref = PUB[f"tg_{name}_errors"]
+    errs = np.array([tg_error(tg_config(int(D), method, delta, mi, mn)) for D in res])
+    rel = np.abs(errs / ref - 1.0)
+    dxs = 1.0 / (2 * res)
+    print(f"TG {name}: errors {errs.tolist()} rel to the reference's table {rel.tolist()}; "
+          f"order {order(dxs, errs):.4f} (table {float(PUB[f'tg_{name}_slope']):.4f})")
+    assert np.all(rel <= tol)
+    slope_tol = 1e-9 if tol < 1e-10 else (1e-5 if tol < 1e-5 else 0.02)
+    assert abs(order(dxs, errs) - float(PUB[f"tg_{name}_slope"])) <= slope_tol
+    assert order(dxs, errs) > 1.75  # second order within the reference's own scatter
+
+
+@pytest.mark.parametrize("col", [TG_COLUMNS[0], TG_COLUMNS[1], TG_COLUMNS[5]],
+                         ids=["lbm_p4", "df_p4", "mdf_hat"])
+def test_taylor_green_matches_oracle_bitwise(col):
+    import oracle.oracle as O
+    name, method, delta, mi, mn, _ = col
+    for D in (10, 20):
+        cfg = tg_config(D, method, delta, mi, mn)
+        ref = O.FixedCylinder(cfg)
+        for _ in range(cfg.max_steps):
+            ref.step()
+        e_ref = TG.l2_error(ref.st.U, TG.analytic_velocity(cfg, ref.st.nx, ref.st.ny,
+                                                           cfg.max_steps * ref.st.dt))
+        assert tg_error(cfg) == e_ref
+
+
+@pytest.mark.parametrize("col", [TG_COLUMNS[0], TG_COLUMNS[1], TG_COLUMNS[6]],
+                         ids=["lbm_p4", "df_p4", "dfc_hat"])
+def test_taylor_green_fp32dev_keeps_the_order(col):
+    """FP32-deviation storage: the errors within 1% of FP64's up to D = 40, the order kept."""
+    name, method, delta, mi, mn, _ = col
+    res = PUB[f"tg_{name}_resolutions"][:3]
+    e64 = np.array([tg_error(tg_config(int(D), method, delta, mi, mn)) for D in res])
+    e32 = np.array([tg_error(tg_config(int(D), method, delta, mi, mn), "fp32dev") for D in res])
+    print(f"TG {name} fp32dev: {e32.tolist()} vs fp64 {e64.tolist()}")
+    assert np.all(np.abs(e32 / e64 - 1.0) <= 1e-2)
+    dxs = 1.0 / (2 * res)
+    assert abs(order(dxs, e32) - order(dxs, e64)) <= 0.02
+
+
+def osc_config(name):
+    """``make_oscillating_cylinder_config`` (``scenarios/oscillating.py:12-49``)."""
+    ibm, rest = name.split("_", 1)
+    delta = "hat" if rest.endswith("hat") else "peskin4pt"
+    model = {"bgk": "BGK", "trt": "TRT", "cm_mrt": "CM_MRT"}[rest.rsplit("_", 1)[0]]
+    return SimConfig(Re=100.0, NN=int(PUB[f"osc_{name}_NN"]), xmax=1.5, ymax=1.0,
+                     cylinder_center=(0.75, 0.5), cylinder_D_ratio=1.0 / 20.0, lattice_u=0.1,
+                     inflow_u=0.0, bc_type="open_boundary", delta_type=delta,
+                     ibm_method=ibm.upper(), collision_model=model,
+                     mdf_iterations=20 if ibm == "mdf" else 1,
+                     mdf_min_iterations=5 if ibm == "mdf" else 1,
+                     max_steps=int(PUB[f"osc_{name}_max_steps"]), check_interval=50,
+                     use_convergence=False, motion_type="oscillating", KC=5.0,
+                     marker_spacing_factor=2.0 / 3.0, nx_formula="simple")
+
+
+def tail_stats(steps, cd, cl, start):
+    m = steps >= start
+    c, l = cd[m], cl[m]
+    return np.array([m.sum(), c.max(), c.min(), 0.5 * (c.max() - c.min()), c.mean(),
+                     0.5 * (l.max() - l.min())])
+
+
+# one run per (IBM method, collision model, kernel) pairing that covers each once; every
+# one of the reference's 18 runs with IBLBM_ALL_PUBLISHED=1 (about 12 s each)
+OSC_DEFAULT = ["df_bgk_hat", "df_cm_mrt_p4", "mdf_trt_p4", "mdf_bgk_hat", "dfc_cm_mrt_hat",
+               "dfc_trt_p4"]
+OSC_CASES = ([str(n) for n in PUB["osc_names"]] if os.environ.get("IBLBM_ALL_PUBLISHED")
+             else OSC_DEFAULT)
+
+
+@pytest.mark.parametrize("name", OSC_CASES)
+def test_oscillating_cylinder_full_size(name):
+    cfg = osc_config(name)
+    sim = FixedCylinder(cfg)
+    print(f"oscillating {name}: {sim.st.nx} x {sim.st.ny}, {sim.st.Lx.size} markers, "
+          f"{cfg.max_steps} steps", flush=True)
+    steps, cd, cl = [], [], []
+    while sim.ttt < cfg.max_steps:
+        sim.advance(cfg.check_interval, diag_every=cfg.check_interval)
+        c, l = sim.cd_cl()
+        steps.append(sim.ttt)
+        cd.append(c)
+        cl.append(l)
+        if sim.ttt % 5000 == 0:
+            print(f"  step {sim.ttt}: Cd {c:.6f} (reference "
+                  f"{PUB[f'osc_{name}_Cd'][len(cd) - 1]:.6f})", flush=True)
+    steps, cd, cl = np.array(steps), np.array(cd), np.array(cl)
+    sim.ctx.close()
+    np.testing.assert_array_equal(steps, PUB[f"osc_{name}_steps"])
+    ref_cd, ref_cl = PUB[f"osc_{name}_Cd"], PUB[f"osc_{name}_Cl"]
+    dcd = np.abs(cd - ref_cd)
+    peak = cd[steps >= PUB[f"osc_{name}_validation_start_step"]].max()
+    tail = tail_stats(steps, cd, cl, PUB[f"osc_{name}_tail"][0])
+    print(f"  max |dCd| {dcd.max():.3e} (at step {steps[dcd.argmax()]}), max |dCl| "
+          f"{np.abs(cl - ref_cl).max():.3e}; Cd_peak {peak:.6f} (reference "
+          f"{float(PUB[f'osc_{name}_Cd_peak']):.6f}); tail {tail.tolist()} (reference "
+          f"{PUB[f'osc_{name}_tail'][1:].tolist()})")
+    assert np.all(np.isfinite(cd))
+    # the reference's histories are reproduced to round-off (the device sums rho and the
+    # marker forces in a fixed tree order, not NumPy's); Cd_peak is published to 6 decimals
+    assert dcd.max() <= 1e-9 * np.abs(ref_cd).max()
+    assert np.abs(cl - ref_cl).max() <= 1e-9 * np.abs(ref_cd).max()
+    assert abs(peak - float(PUB[f"osc_{name}_Cd_peak"])) <= 5.01e-7
+    ref_tail = PUB[f"osc_{name}_tail"]
+    assert tail[0] == ref_tail[1]
+    np.testing.assert_allclose(tail[1:], ref_tail[2:], atol=1.5e-4)  # the summary's 4 decimals
