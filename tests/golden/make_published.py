@@ -6,9 +6,12 @@ convergence tables (``data/taylor_green/{df,mdf,dfc}/convergence.npz`` with Pesk
 oscillating-cylinder runs at Re = 100, KC = 5 on the full 2401 x 1601 lattice for 40000 steps
 (``data/oscillating_cylinder/re100_kc5/<ibm>_<collision>_<delta>/status.json``: Cd and Cl every
 50 steps, Cd_peak from step 8000; ``oscillating_re100_kc5_summary_50tail.json``: the
-statistics of the last half).  This script reads them as data (``numpy.load`` without
-pickles, ``json``) and writes the numbers into one fixture, since the reference tree is not
-present where the GPU tests run.  Nothing of the reference is executed.
+statistics of the last half) and the fixed-cylinder runs at Re = 20, 40, 100, 200
+(``data/fixed_cylinder/re*/<ibm>_<collision>_<delta>/status.json``: Cd and Cl every 200
+steps, the final step, the stop reason and the averages over the last 30 %).  This script
+reads them as data (``numpy.load`` without pickles, ``json``) and writes the numbers into one
+fixture, since the reference tree is not present where the GPU tests run.  Nothing of the
+reference is executed.
 
 Usage:  python tests/golden/make_published.py
 Output: tests/golden/published.npz
@@ -64,6 +67,35 @@ def main():
                                          c["Cd_peak_50tail"], c["Cd_min_50tail"],
                                          c["Cd_amp_50tail"], c["Cd_mean_50tail"],
                                          c["Cl_amp_50tail"]], dtype=np.float64)
+    # fixed cylinder, Re 20 / 40 / 100 / 200 (data/fixed_cylinder/re*/<case>/status.json)
+    fc = []
+    for re_dir in sorted(os.listdir(os.path.join(REF, "fixed_cylinder"))):
+        base = os.path.join(REF, "fixed_cylinder", re_dir)
+        for n in sorted(os.listdir(base)):
+            path = os.path.join(base, n, "status.json")
+            if not os.path.isfile(path):
+                continue
+            with open(path) as fh:
+                s = json.load(fh)
+            key = f"fc_{re_dir}_{n}"
+            fc.append(f"{re_dir}_{n}")
+            c = s["config"]
+            out[key + "_steps"] = np.array(s["steps"], dtype=np.int64)
+            out[key + "_Cd"] = np.array(s["Cd_history"], dtype=np.float64)
+            out[key + "_Cl"] = np.array(s["Cl_history"], dtype=np.float64)
+            out[key + "_avg"] = np.array([s["Cd_avg"], s["Cl_avg"], s["avg_start_step"]],
+                                         dtype=np.float64)
+            out[key + "_run"] = np.array([c["Re"], c["NN"], c["xmax"], c["ymax"],
+                                          c["max_steps"], c["check_interval"],
+                                          c["mdf_iterations"], c["mdf_min_iterations"],
+                                          s["final_step"], s["elapsed_sec"]], dtype=np.float64)
+            out[key + "_termination"] = np.array(str(s.get("termination_reason")))
+            out[key + "_delta"] = np.array(c["delta_type"])
+            out[key + "_collision"] = np.array(c["collision_model"])
+            out[key + "_ibm"] = np.array(c["ibm_method"])
+            assert c["bc_type"] == "dirichlet_neumann"
+            assert abs(c["marker_spacing_factor"] - 2.0 / 3.0) < 1e-15
+    out["fc_names"] = np.array(fc)
     np.savez_compressed(OUT, **out)
     print(f"wrote {OUT}: {len(out)} arrays, {os.path.getsize(OUT)} bytes")
 

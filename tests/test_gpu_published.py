@@ -181,3 +181,65 @@ def test_oscillating_cylinder_full_size(name):
     ref_tail = PUB[f"osc_{name}_tail"]
     assert tail[0] == ref_tail[1]
     np.testing.assert_allclose(tail[1:], ref_tail[2:], atol=1.5e-4)  # the summary's 4 decimals
+
+
+def fc_config(name):
+    """``make_dirichlet_neumann_fixed_cylinder_config`` (``scenarios/steady.py:14-78``) with
+    the published run's own settings.  Runs whose file names a stop reason were made with the
+    Cd-window stop (range of the last 5 samples below 1e-4 after step 20000); the others
+    (no stop reason in the file) without it, and ran their full length."""
+    key = f"fc_{name}"
+    Re, NN, xmax, ymax, max_steps, ci, mi, mn, _, _ = PUB[key + "_run"]
+    window_stop = str(PUB[key + "_termination"]) != "None"
+    return SimConfig(Re=float(Re), NN=int(NN), xmax=float(xmax), ymax=float(ymax),
+                     cylinder_center=(0.5, 0.5), cylinder_D_ratio=1.0 / 40.0, lattice_u=0.1,
+                     inflow_u=0.1, bc_type="dirichlet_neumann",
+                     delta_type=str(PUB[key + "_delta"]), ibm_method=str(PUB[key + "_ibm"]),
+                     collision_model=str(PUB[key + "_collision"]), mdf_iterations=int(mi),
+                     mdf_min_iterations=int(mn), max_steps=int(max_steps),
+                     check_interval=int(ci), use_convergence=False,
+                     cd_convergence_tol=1e-4 if window_stop else None,
+                     cd_convergence_window=5, cd_convergence_start_step=20_000,
+                     marker_spacing_factor=2.0 / 3.0, nx_formula="standard")
+
+
+# the Cd-window stop (Re 20 DF BGK hat, 21000 steps), an MDF run, a DFC run and the largest
+# lattice (Re 200, 3001 x 2401, 60000 steps); all 72 with IBLBM_ALL_PUBLISHED=1
+FC_DEFAULT = ["re20_df_bgk_hat", "re40_mdf_cm_mrt_hat", "re100_dfc_trt_p4", "re200_df_bgk_p4"]
+FC_CASES = ([str(n) for n in PUB["fc_names"]] if os.environ.get("IBLBM_ALL_PUBLISHED")
+            else FC_DEFAULT)
+
+
+@pytest.mark.parametrize("name", FC_CASES)
+def test_fixed_cylinder_full_size(name, tmp_path):
+    """``run()`` with its status writer against the reference's result file of the same run:
+    the Cd / Cl histories, the final step, the stop reason and the 30 %-tail averages."""
+    import json
+    from iblbm_amd.solver import run
+    key = f"fc_{name}"
+    cfg = fc_config(name)
+    print(f"fixed cylinder {name}: NN {cfg.NN}, xmax {cfg.xmax}, {cfg.ibm_method} "
+          f"{cfg.collision_model} {cfg.delta_type}, up to {cfg.max_steps} steps", flush=True)
+    res = run(cfg, verbose=False, save_dir=str(tmp_path))
+    st = json.load(open(tmp_path / "status.json"))
+    steps = np.array(st["steps"])
+    cd, cl = np.array(st["Cd_history"]), np.array(st["Cl_history"])
+    ref_cd, ref_cl = PUB[key + "_Cd"], PUB[key + "_Cl"]
+    run_pub = PUB[key + "_run"]
+    n = min(len(cd), len(ref_cd))
+    dcd = np.abs(cd[:n] - ref_cd[:n])
+    print(f"  {res['sim'].st.nx} x {res['sim'].st.ny}: final step {st['final_step']} "
+          f"(reference {int(run_pub[8])}), stop {st['termination_reason']} (reference "
+          f"{PUB[key + '_termination']}); max |dCd| {dcd.max():.3e}, max |dCl| "
+          f"{np.abs(cl[:n] - ref_cl[:n]).max():.3e}; Cd_avg {st['Cd_avg']} (reference "
+          f"{PUB[key + '_avg'][0]})", flush=True)
+    res["sim"].ctx.close()
+    np.testing.assert_array_equal(steps, PUB[key + "_steps"])
+    assert st["final_step"] == int(run_pub[8])
+    if str(PUB[key + "_termination"]) != "None":
+        assert st["termination_reason"] == str(PUB[key + "_termination"])
+    assert dcd.max() <= 1e-9 * np.abs(ref_cd).max()
+    assert np.abs(cl - ref_cl).max() <= 1e-9 * np.abs(ref_cd).max()
+    assert st["avg_start_step"] == int(PUB[key + "_avg"][2])
+    assert abs(st["Cd_avg"] - PUB[key + "_avg"][0]) <= 1.01e-6  # 6 decimals published
+    assert abs(st["Cl_avg"] - PUB[key + "_avg"][1]) <= 1.01e-6

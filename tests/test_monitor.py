@@ -78,7 +78,7 @@ def test_fixed_cylinder_status(tmp_path):
     mon.finalize(120, "max_steps")
     st = json.load(open(tmp_path / "status.json"))
     assert st["completed"] and st["Cd_history"] == cds and st["steps"][-1] == 120
-    assert st["Cd_avg"] == round(np.mean(cds[3:]), 6) and st["avg_start_step"] == 80
+    assert st["Cd_avg"] == round(np.mean(cds[4:]), 6) and st["avg_start_step"] == 100
     # scripts/make_fig2_cd_comparison.py:50-51
     assert st.get("config", {}).get("check_interval", 200) == cfg.check_interval
 
