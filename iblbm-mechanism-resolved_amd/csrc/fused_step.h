@@ -29,8 +29,8 @@ struct FusedShared {
     float mlpy[FAST_LB * MLP_OUT];                   // predictor outputs
     MlpLds mlpw;                                     // predictor weights (learned mode)
     // predictor inputs, copied from global memory straight into LDS during P1: per marker
-    // the previous converged force and forcing slip (fprev), the position, and the
-    // normalisation
+    // the previous step's force before its last sub-step and its forcing slip (fprev), the
+    // position, and the normalisation
     alignas(16) double pf[2 * FAST_LB], pq[2 * FAST_LB], plx[FAST_LB], ply[FAST_LB];
     alignas(16) float pnrm[MLP_NORM + 2];
 };
@@ -303,9 +303,13 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
         // Predictor (MDF with a mode set), computed ahead of the loop while few registers are
         // taken.  The forcing slip q = R (ud - U) (U interpolated before forcing, R the marker
         // density, 1 for the incompressible form) is what the first MDF sub-step turns into a
-        // force, and a fixed number of sub-steps maps q to the converged force linearly: the
-        // change of force is a geometry-set operator on the change of q.  fprev holds per
-        // marker the previous converged force (2 Lb) and q (2 Lb after it); only the marker's
+        // force, and a fixed number of sub-steps maps q to the force linearly: the change of
+        // force is a geometry-set operator on the change of q.  The force learned is plain
+        // MDF's force before its last sub-step (N - 1 sub-steps from zero map q to it
+        // linearly), so that the prediction plus the one true correction that follows
+        // reproduces plain MDF's N-sub-step result, truncation included.  fprev holds per
+        // marker the previous step's force before its last sub-step (2 Lb) and q (2 Lb after
+        // it); only the marker's
         // own thread touches its entries.  Features: change of q since the previous step /
         // sig, direction of q, marker direction from the body centre, sub-cell offset.  The
         // network predicts (change of force) / sig with sig = max over the wavefront's markers
@@ -448,6 +452,10 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
                 facy = pfy;
             }
         }
+        // the last applied sub-step's force increment: plain MDF's force before it is what the
+        // network learns (collect mode), so that the one true correction a learned step always
+        // applies lands on plain MDF's own answer, truncation included
+        double lastx = 0.0, lasty = 0.0;
         for (int it = 0; it < n_iter; ++seq) {
             const bool predict = PRED && warm && seq == 0;
             double rv = 0.0, Fx = 0.0, Fy = 0.0;
@@ -479,11 +487,15 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
             const int act = ctl.at_a(res, it);
             if (act == 1) break;
             if (act == 0) {
-                facx += I.omega * Fx;
-                facy += I.omega * Fy;
+                lastx = I.omega * Fx;
+                lasty = I.omega * Fy;
+                facx += lastx;
+                facy += lasty;
             } else {
                 facx = 0.0;
                 facy = 0.0;
+                lastx = 0.0;
+                lasty = 0.0;
             }
             __syncthreads();  // B: velocities updated
             if (act == 2) {
@@ -501,13 +513,16 @@ __device__ __attribute__((always_inline)) void ibm_fused_role(Inst &Ig, int pari
         }
         if (pmode != 0) {
             if (t < Lb) {
-                if (record) {  // training target: scaled change of the converged force
+                // the force before the last applied sub-step (the prediction itself after a
+                // learned step that needed one correction)
+                const double px = facx - lastx, py = facy - lasty;
+                if (record) {  // training target: its scaled change since the previous step
                     const double sig = (double)rowp[MLP_IN];
-                    rowp[MLP_IN] = (float)((facx - S.fprev[2 * t]) / sig);
-                    rowp[MLP_IN + 1] = (float)((facy - S.fprev[2 * t + 1]) / sig);
+                    rowp[MLP_IN] = (float)((px - S.fprev[2 * t]) / sig);
+                    rowp[MLP_IN + 1] = (float)((py - S.fprev[2 * t + 1]) / sig);
                 }
-                S.fprev[2 * t] = facx;
-                S.fprev[2 * t + 1] = facy;
+                S.fprev[2 * t] = px;
+                S.fprev[2 * t + 1] = py;
             }
             if (t == 0) {
                 if (record) S.pctr[0] = pctr0 + Lb;

@@ -1,9 +1,11 @@
 """The learned MDF force predictor of a device-resident MDF instance.
 
 Modes: "off" (plain MDF), "collect" (plain MDF that records, per marker and step, the
-features and the change of the converged force, both scaled), "warm" (the previous converged
-force is the starting guess, no network), "learned" (previous force plus the network's
-predicted change).  The features are the change of the forcing slip q = R (ud - U) before
+features and the target, both scaled: the change since the previous step of plain MDF's force
+before its last sub-step, so that a learned step's one true correction lands on plain MDF's
+own result, truncation included), "warm" (the previous step's force before its last sub-step
+is the starting guess, no network), "learned" (that force plus the network's predicted
+change).  The features are the change of the forcing slip q = R (ud - U) before
 forcing since the previous step, the direction of q, the marker direction from the body
 centre and the sub-cell offset; changes are divided by sig, the largest change of q over the
 64 markers of the marker's wavefront, so the prediction keeps its accuracy in scale while the
@@ -74,7 +76,8 @@ class Predictor:
               "predictor_set_params")
 
     def previous_force(self, Lb: int) -> np.ndarray:
-        """The previous step's converged marker forces, then its forcing slips q, (2 Lb, 2)."""
+        """The previous step's marker forces before its last MDF sub-step, then its forcing
+        slips q, (2 Lb, 2)."""
         F = np.empty((2 * int(Lb), 2))
         check(self.ctx.lib.iblbm_predictor_previous_force(self.ctx.h, self.k, _abi.dptr(F), 0),
               "predictor_previous_force")

@@ -369,14 +369,15 @@ int iblbm_op_spread(const double *Lfx, const double *Lfy, const double *Lx, cons
 /* --- learned MDF force predictor ---
  * No counterpart in the reference: the MDF sub-step surrogate (an 8 -> 64 -> 64 -> 2 tanh MLP
  * per marker from (change of q / sig, direction of q, marker direction, sub-cell offset) to
- * the change of the converged force / sig, where q = R (ud - U) is the forcing slip before
+ * the change of plain MDF's force before its last sub-step (since the previous converged
+ * force) / sig, where q = R (ud - U) is the forcing slip before
  * forcing and sig the largest change of q over the markers of the marker's wavefront (64
  * markers); trained on the device on the plain MDF loop's own converged forces, BF16 MFMA
  * inference in the window workgroup).  After a prediction the step always applies at least
  * one true MDF correction and restarts plain MDF once if the divergence guard trips. */
 #define IBLBM_PRED_OFF 0     /* plain MDF */
 #define IBLBM_PRED_COLLECT 1 /* plain MDF, recording (features, scaled force change) */
-#define IBLBM_PRED_WARM 2    /* previous step's converged force as the start, no network */
+#define IBLBM_PRED_WARM 2    /* previous step's force before its last sub-step, no network */
 #define IBLBM_PRED_LEARNED 3 /* previous force + the network's predicted change */
 #define IBLBM_MLP_IN 8
 #define IBLBM_MLP_OUT 2
@@ -390,7 +391,7 @@ int iblbm_predictor_samples(void *ctx, int id, int64_t *n);
 /* All samples written since the predictor was enabled (sample r sits in row r % capacity). */
 int iblbm_predictor_samples_written(void *ctx, int id, int64_t *n);
 /* Copies n samples (features[8], target[2] as float) starting at sample number `first`;
- * the target is (change of converged force) / sig. */
+ * the target is (change of plain MDF's force before its last sub-step) / sig. */
 int iblbm_predictor_data(void *ctx, int id, int64_t first, int64_t n, float *rows);
 /* Normalisation from the samples, then `steps` Adam steps (minibatch 128, seed) on the device;
  * losses (may be NULL) receives the minibatch loss of every step. */
@@ -398,8 +399,9 @@ int iblbm_predictor_train(void *ctx, int id, int steps, double lr, uint32_t seed
                           float *losses);
 int iblbm_predictor_params(void *ctx, int id, float *params, float *norm);
 int iblbm_predictor_set_params(void *ctx, int id, const float *params, const float *norm);
-/* The previous step's converged marker forces (Lb, 2) followed by its forcing slips q before
- * forcing (Lb, 2), 4 Lb values: read (set = 0), or write and mark valid (set = 1). */
+/* The previous step's marker forces before its last MDF sub-step (Lb, 2) followed by its
+ * forcing slips q before forcing (Lb, 2), 4 Lb values: read (set = 0), or write and mark valid
+ * (set = 1). */
 int iblbm_predictor_previous_force(void *ctx, int id, double *F, int set);
 /* Sums since the last reset: [true MDF sub-steps applied, steps, first-sub-step residual,
  * plain-MDF restarts]. */

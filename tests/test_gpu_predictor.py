@@ -65,7 +65,8 @@ def _cfg():
 def _held_out_mse(pred, advance, steps):
     """Network and warm-start errors on `steps` further plain-MDF steps (not trained on).
 
-    Rows hold the scaled features and the scaled change of the converged force; the warm
+    Rows hold the scaled features and the scaled change of the force plain MDF holds before
+    its last sub-step; the warm
     start predicts no change, so its error is the change itself.  Both are normalised by the
     training set's target scale."""
     n0 = pred.samples_written()
@@ -109,9 +110,13 @@ def test_learned_predictor_gain():
         res[mode] = (sim.cd_cl()[0], pred.stats())
         print(mode, res[mode])
     cd_plain = res["collect"][0]
-    for mode in ("warm", "learned"):
-        assert abs(res[mode][0] - cd_plain) <= 1e-3 * abs(cd_plain)
+    # the learned form lands on plain MDF's own result (its prediction stands in for plain
+    # MDF's force before the last sub-step); the network-free warm start, a step behind,
+    # is the ablation it is measured against
+    for mode, tol in (("warm", 2e-3), ("learned", 1e-4)):
+        assert abs(res[mode][0] - cd_plain) <= tol * abs(cd_plain)
         assert res[mode][1]["restarts"] == 0
+    assert abs(res["learned"][0] - cd_plain) < 0.1 * abs(res["warm"][0] - cd_plain)
     # fewer true MDF sub-steps than plain MDF (which always takes at least 5)
     assert res["learned"][1]["substeps_per_step"] < 0.5 * res["collect"][1]["substeps_per_step"]
 
@@ -161,8 +166,8 @@ def test_learned_predictor_on_a_settling_disc():
         res[mode] = (float(p0.vel[1]), float(p0.pos[1]), pred.stats())
         print(mode, res[mode])
     v_plain, y_plain = res["collect"][0], res["collect"][1]
-    for mode in ("warm", "learned"):
-        assert abs(res[mode][0] - v_plain) <= 1e-3 * abs(v_plain)
+    for mode, tol in (("warm", 1e-3), ("learned", 5e-5)):
+        assert abs(res[mode][0] - v_plain) <= tol * abs(v_plain)
         assert res[mode][2]["restarts"] == 0
     assert res["learned"][2]["substeps_per_step"] < 0.5 * res["collect"][2]["substeps_per_step"]
 
