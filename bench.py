@@ -145,15 +145,24 @@ def main():
                     help="steps of the FP32-deviation comparison of an fp64 run (0: skip)")
     ap.add_argument("--shedding-steps", type=int, default=200_000,
                     help="steps of each form in --workload shedding")
+    ap.add_argument("--mdf-iterations", type=int, default=0,
+                    help="cylinder workload: fix the MDF sub-steps per step to this many "
+                         "(minimum = maximum; 0 keeps the workload's [5, 20]), to bound what "
+                         "the sub-steps cost")
+    ap.add_argument("--copy-mb", default=None,
+                    help="--workload copy: also plain FP32 copies of these sizes (MB of "
+                         "source, comma-separated)")
     ap.add_argument("--workload", choices=["cylinder", "matrix72", "sed1", "sed64", "shedding",
-                                           "copy"],
+                                           "copy", "large"],
                     default="cylinder",
                     help="cylinder: Re=100 1601x641 (default); matrix72: the batched 72-case "
                          "Re x method x kernel x collision matrix (about 300 M nodes); sed1: "
                          "one settling disc (257x1921, CM-MRT + MDF + Peskin-4), plain MDF "
                          "against the learned predictor; sed64: "
                          "64 two-disc settling cases (193x1921, DFC + hat + CM-MRT) stepped "
-                         "as one ensemble; copy: "
+                         "as one ensemble; large: the FP64 Re=200 cylinder on 3001x2401 "
+                         "(1.04 GB of populations per step, beyond the last-level cache); "
+                         "copy: "
                          "a 1 GiB FP64 device copy, five times (the reference of the HBM "
                          "traffic counters and of a plain copy's bandwidth)")
     args = ap.parse_args()
@@ -167,7 +176,8 @@ def main():
         print(json.dumps(matrix_bench(args)), flush=True)
         return
     if args.workload == "copy":
-        print(json.dumps(copy_bench()), flush=True)
+        sizes = [float(v) for v in args.copy_mb.split(",")] if args.copy_mb else None
+        print(json.dumps(copy_bench(sizes_mb=sizes)), flush=True)
         return
     if args.workload == "sed64":
         print(json.dumps(sed_bench(args)), flush=True)
@@ -178,8 +188,24 @@ def main():
     if args.workload == "shedding":
         print(json.dumps(shedding_bench(args)), flush=True)
         return
+    if args.workload == "large":
+        r = large_lattice_leg(steps=max(1, args.steps // 10), warm=max(0, args.warmup // 2),
+                              schedule=args.schedule)
+        print(json.dumps({
+            "metric": "MLUPS (fp64 DF+BGK+P4 step, Re=200 cylinder 3001x2401)",
+            "value": r["MLUPS"], "unit": "MLUPS", "n_gpus": 1, "steps": r["steps"],
+            "warmup": r["warmup"], "ms_per_step": round(r["device_us_per_step"] / 1e3, 5),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (uniform inflow)", "config": {"workload": r["workload"]},
+            "roofline": {"bound": "hbm", "achieved": r["achieved_GBs"], "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": r["hbm_frac"], "traffic": None,
+                         "kernel": KERNEL_OF_MODE[r["launch"]], "bytes_per_node": 144},
+            "leg": r}), flush=True)
+        return
     from iblbm_amd.solver import FixedCylinder
     cfg = workload_cfg()
+    if args.mdf_iterations > 0:
+        cfg.mdf_iterations = cfg.mdf_min_iterations = args.mdf_iterations
     ci = cfg.check_interval
     steps, warm = max(1, args.steps), max(0, args.warmup)
     sim = FixedCylinder(cfg, perturb=PERTURB, precision=args.precision)
@@ -192,13 +218,7 @@ def main():
     # graph so that the chunk's graph is captured outside the timed region
     tuned = None
     if args.schedule:
-        f = args.schedule.split(",")
-        order = f[5] if f[5] in ("stride", "blocked") else int(f[5])
-        sim.ctx.set_schedule(mode=f[0], bulk_threads=int(f[1]), bulk_rows=int(f[2]),
-                             bulk_waves=int(f[3]), step_rows=int(f[4]), step_order=order,
-                             row_layout=f[6] if len(f) > 6 else "tiles",
-                             stores=f[7] if len(f) > 7 else "normal",
-                             bulk_delay_us=float(f[8]) if len(f) > 8 else 0.0)
+        sim.ctx.set_schedule(**parse_schedule(args.schedule))
         tuned = sim.ctx.schedule()
     elif not args.no_tune and not args.serial:  # launch schedule from the autotuner
         tuned = sim.ctx.autotune(60)
@@ -253,7 +273,9 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32 (deviation storage; FP64 IBM)",
         "data": "synthetic (seeded 1e-3 inflow perturbation, seed 1234)",
         "config": {"workload": "fixed cylinder Re=100, 1601x641, D=40, 126 markers, TRT "
-                               "Lambda=1/4, MDF [5,20] tol 1e-4, Peskin-4, U=0.04",
+                               "Lambda=1/4, MDF [5,20] tol 1e-4, Peskin-4, U=0.04" +
+                               (f" (MDF fixed at {args.mdf_iterations} sub-steps)"
+                                if args.mdf_iterations > 0 else ""),
                    "precision": args.precision, "launch": mode, "graph": graph,
                    "schedule": tuned, "setup_steps": setup_steps,
                    "nodes": N, "check_interval": ci, "Cd_last": cd, "Cl_last": cl,
@@ -283,9 +305,12 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def copy_bench(reps=5):
+def copy_bench(reps=5, sizes_mb=None):
     """A 1 GiB FP64 device-to-device copy, `reps` times: the bytes the HBM counters must
-    report for it are known exactly, which calibrates them, and its rate is a plain copy's."""
+    report for it are known exactly, which calibrates them, and its rate is a plain copy's.
+    With sizes_mb, also a plain FP32 copy of each size (MB of source data; read + write is
+    twice that), timed with device events over 500 copies: the rate a streaming kernel of
+    the same traffic can reach, inside and beyond the 256 MB last-level cache."""
     import torch
     n = 1 << 27
     a = torch.ones(n, dtype=torch.float64, device="cuda")
@@ -297,8 +322,32 @@ def copy_bench(reps=5):
         b.copy_(a)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / reps
-    return {"metric": "GB/s (1 GiB FP64 device copy, read + write)", "value": round(2 * 8 * n / dt / 1e9, 1),
-            "unit": "GB/s", "reps": reps, "ms_per_copy": round(dt * 1e3, 4)}
+    out = {"metric": "GB/s (1 GiB FP64 device copy, read + write)",
+           "value": round(2 * 8 * n / dt / 1e9, 1), "unit": "GB/s", "reps": reps,
+           "ms_per_copy": round(dt * 1e3, 4)}
+    del a, b
+    if sizes_mb:
+        by_size = {}
+        for mb in sizes_mb:
+            m = int(mb * 1e6) // 4
+            x = torch.ones(m, dtype=torch.float32, device="cuda")
+            y = torch.empty_like(x)
+            for _ in range(20):
+                y.copy_(x)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(500):
+                y.copy_(x)
+            e1.record()
+            torch.cuda.synchronize()
+            us = e0.elapsed_time(e1) * 1e3 / 500
+            by_size[str(mb)] = {"us": round(us, 3), "traffic_MB": 2 * mb,
+                                "GBs": round(2 * mb * 1e3 / us, 1),
+                                "frac_of_8TBs": round(2 * mb * 1e3 / us / HBM_PEAK_GBS, 4)}
+            del x, y
+        out["by_size"] = by_size
+    return out
 
 
 def matrix_legs(steps, warm=20, plain=True):
@@ -371,7 +420,18 @@ MATRIX_WORKLOAD = ("72 fixed cylinders: Re {20,40,100,200} x {DF,MDF,DFC} x {hat
                    "D = (Ny - 1) / 16, U = 0.04")
 
 
-def large_lattice_leg(steps=400, warm=200):
+def parse_schedule(text):
+    """--schedule text -> Context.set_schedule keywords."""
+    f = text.split(",")
+    order = f[5] if f[5] in ("stride", "blocked") else int(f[5])
+    return dict(mode=f[0], bulk_threads=int(f[1]), bulk_rows=int(f[2]), bulk_waves=int(f[3]),
+                step_rows=int(f[4]), step_order=order,
+                row_layout=f[6] if len(f) > 6 else "tiles",
+                stores=f[7] if len(f) > 7 else "normal",
+                bulk_delay_us=float(f[8]) if len(f) > 8 else 0.0)
+
+
+def large_lattice_leg(steps=400, warm=200, schedule=None):
     """FP64 on one lattice far beyond the 256 MB last-level cache: the reference's Re = 200
     fixed cylinder (``scenarios/steady.py``: NN = 2401, xmax = 1.25 -> 3001 x 2401, D = 60,
     DF + BGK + Peskin-4), 1.04 GB of populations per step.  Time per step on the device
@@ -385,7 +445,11 @@ def large_lattice_leg(steps=400, warm=200):
                     use_convergence=False, marker_spacing_factor=2.0 / 3.0,
                     nx_formula="standard")
     sim = FixedCylinder(cfg)
-    tuned = sim.ctx.autotune(20)
+    if schedule:
+        sim.ctx.set_schedule(**parse_schedule(schedule))
+        tuned = sim.ctx.schedule()
+    else:
+        tuned = sim.ctx.autotune(20)
     ci = cfg.check_interval
     advance_aligned(sim, warm, ci, True)
     sim.ctx.sync()

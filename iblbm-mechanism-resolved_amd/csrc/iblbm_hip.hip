@@ -338,103 +338,82 @@ __device__ __attribute__((always_inline)) void lattice_ij(const RowRun &R, const
 }
 
 // Bulk rows of one time step (every node that no marker stencil touches): a wavefront takes
-// rows first, first + nw, ... below stop (all rows when stop < 0); with TWO it keeps two rows'
-// loads in flight at a time.  Each row's FP64 partial rho sum is a fixed shuffle tree over the
-// wavefront.
+// rows first, first + nw, ... below stop (all rows when stop < 0), NR rows at a time with all
+// NR rows' loads issued before any of them is finished (NR = 1, 2 or 4: four FP32 rows hold
+// the bytes of two FP64 rows in flight).  Each row's FP64 partial rho sum is a fixed shuffle
+// tree over the wavefront.
 // EB: edge rows take the batched loads of edge_loads / edge_fix (else the per-lane general
 // pull); chosen per kernel where it does not raise the register count.
-template <typename S, int MODEL, bool TWO, bool EB, int FL, bool NT = false>
+template <typename S, int MODEL, int NR, bool EB, int FL, bool NT = false>
 __device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, int first, int nw,
                                                         int stop = -1) {
+    static_assert(NR == 1 || NR == 2 || NR == 4, "rows in flight: 1, 2 or 4");
     const int lane = (int)threadIdx.x & 63;
     const int n_rows = stop >= 0 ? stop : T.n_tiles * TILE_Y;
-    RowRun R;  // empty tile range: the first row loads it
-    R.tb_lo = 1;
-    R.tb_hi = 0;
-    if constexpr (TWO) {
-        RowRun R1;
-        R1.tb_lo = 1;
-        R1.tb_hi = 0;
-        for (int r0 = first; r0 < n_rows; r0 += 2 * nw) {
-            const bool v1 = r0 + nw < n_rows;
-            const int r1 = v1 ? r0 + nw : r0;  // a missing second row repeats the first, unused
-            const TileNode n0 = row_node<FL>(T, R, r0, lane);
-            TileNode n1 = row_node<FL>(T, R1, r1, lane);
-            n1.mine = n1.mine && v1;
-            S a0[9], a1[9];
-            double rho0 = 0.0, rho1 = 0.0;
-            if (n0.interior && n1.interior) {  // uniform: both rows' 18 loads in one batch
-                bulk_pull_interior(R, n0.i, n0.j, a0);
-                bulk_pull_interior(R1, n1.i, n1.j, a1);
-                __builtin_amdgcn_sched_barrier(0);
-                const double r0v =
-                    bulk_finish<MODEL, NT>(*R.I, R, n0.i, n0.j, a0, n0.mine);
-                const double r1v =
-                    bulk_finish<MODEL, NT>(*R1.I, R1, n1.i, n1.j, a1, n1.mine);
-                rho0 = n0.mine ? r0v : 0.0;
-                rho1 = n1.mine ? r1v : 0.0;
-            } else if constexpr (EB) {  // edge rows: the same batched loads, then the closure
-                int i0, j0, i1, j1;
-                lattice_ij<FL>(R, n0, i0, j0);
-                lattice_ij<FL>(R1, n1, i1, j1);
-                bulk_pull_edge(R, i0, j0, a0);
-                bulk_pull_edge(R1, i1, j1, a1);
-                __builtin_amdgcn_sched_barrier(0);
-                edge_fix(R, i0, j0, a0);
-                edge_fix(R1, i1, j1, a1);
-                const double r0v =
-                    bulk_finish<MODEL, NT>(*R.I, R, n0.i, n0.j, a0, n0.mine);
-                const double r1v =
-                    bulk_finish<MODEL, NT>(*R1.I, R1, n1.i, n1.j, a1, n1.mine);
-                rho0 = n0.mine ? r0v : 0.0;
-                rho1 = n1.mine ? r1v : 0.0;
-            } else {
-                if (n0.mine) {
-                    int i0, j0;
-                    lattice_ij<FL>(R, n0, i0, j0);
-                    bulk_pull(*R.I, T.parity, i0, j0, a0);
-                    rho0 = bulk_finish<MODEL, NT>(*R.I, R, n0.i, n0.j, a0, true);
-                }
-                if (n1.mine) {
-                    int i1, j1;
-                    lattice_ij<FL>(R1, n1, i1, j1);
-                    bulk_pull(*R1.I, T.parity, i1, j1, a1);
-                    rho1 = bulk_finish<MODEL, NT>(*R1.I, R1, n1.i, n1.j, a1, true);
-                }
+    RowRun R[NR];  // empty tile ranges: the first row of each slot loads it
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        R[k].tb_lo = 1;
+        R[k].tb_hi = 0;
+    }
+    for (int r0 = first; r0 < n_rows; r0 += NR * nw) {
+        int rr[NR];
+        bool vv[NR];
+        TileNode nn[NR];
+        bool all_interior = true;
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            vv[k] = r0 + k * nw < n_rows;
+            rr[k] = vv[k] ? r0 + k * nw : r0;  // a missing row repeats the first, unused
+            nn[k] = row_node<FL>(T, R[k], rr[k], lane);
+            nn[k].mine = nn[k].mine && vv[k];
+            all_interior = all_interior && nn[k].interior;
+        }
+        S a[NR][9];
+        double rho[NR];
+#pragma unroll
+        for (int k = 0; k < NR; ++k) rho[k] = 0.0;
+        if (all_interior) {  // uniform: every row's loads in one batch
+#pragma unroll
+            for (int k = 0; k < NR; ++k) bulk_pull_interior(R[k], nn[k].i, nn[k].j, a[k]);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                const double v = bulk_finish<MODEL, NT>(*R[k].I, R[k], nn[k].i, nn[k].j, a[k],
+                                                        nn[k].mine);
+                rho[k] = nn[k].mine ? v : 0.0;
             }
-            if (T.diag) {
-                const double w0 = wave_sum(rho0), w1 = wave_sum(rho1);
-                if (lane == 0) {
-                    T.rho_partial[r0] = w0;
-                    if (v1) T.rho_partial[r1] = w1;
+        } else if constexpr (EB) {  // edge rows: the same batched loads, then the closure
+            int ii[NR], jj[NR];
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                lattice_ij<FL>(R[k], nn[k], ii[k], jj[k]);
+                bulk_pull_edge(R[k], ii[k], jj[k], a[k]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                edge_fix(R[k], ii[k], jj[k], a[k]);
+                const double v = bulk_finish<MODEL, NT>(*R[k].I, R[k], nn[k].i, nn[k].j, a[k],
+                                                        nn[k].mine);
+                rho[k] = nn[k].mine ? v : 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                if (nn[k].mine) {
+                    int i0, j0;
+                    lattice_ij<FL>(R[k], nn[k], i0, j0);
+                    bulk_pull(*R[k].I, T.parity, i0, j0, a[k]);
+                    rho[k] = bulk_finish<MODEL, NT>(*R[k].I, R[k], nn[k].i, nn[k].j, a[k], true);
                 }
             }
         }
-    } else {
-        for (int r0 = first; r0 < n_rows; r0 += nw) {
-            const TileNode n0 = row_node<FL>(T, R, r0, lane);
-            S a0[9];
-            double rho0 = 0.0;
-            if (n0.interior) {
-                bulk_pull_interior(R, n0.i, n0.j, a0);
-                const double r = bulk_finish<MODEL, NT>(*R.I, R, n0.i, n0.j, a0, n0.mine);
-                rho0 = n0.mine ? r : 0.0;
-            } else if constexpr (EB) {  // edge row: batched loads, then the closure per lane
-                int i0, j0;
-                lattice_ij<FL>(R, n0, i0, j0);
-                bulk_pull_edge(R, i0, j0, a0);
-                edge_fix(R, i0, j0, a0);
-                const double r = bulk_finish<MODEL, NT>(*R.I, R, n0.i, n0.j, a0, n0.mine);
-                rho0 = n0.mine ? r : 0.0;
-            } else if (n0.mine) {
-                int i0, j0;
-                lattice_ij<FL>(R, n0, i0, j0);
-                bulk_pull(*R.I, T.parity, i0, j0, a0);
-                rho0 = bulk_finish<MODEL, NT>(*R.I, R, n0.i, n0.j, a0, true);
-            }
-            if (T.diag) {
-                const double w0 = wave_sum(rho0);
-                if (lane == 0) T.rho_partial[r0] = w0;
+        if (T.diag) {
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                const double w = wave_sum(rho[k]);
+                if (lane == 0 && vv[k]) T.rho_partial[rr[k]] = w;
             }
         }
     }
@@ -445,12 +424,12 @@ __device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, i
 // with a grid stride (the schedule picks both).
 // NT: non-temporal stores (a compile-time choice: as a run-time flag the compiler merges the
 // two store forms and drops the hint)
-template <typename S, int MODEL, bool TWO, int FL, bool NT>
+template <typename S, int MODEL, int NR, int FL, bool NT>
 __global__ __launch_bounds__(1024) void k_bulk(LaunchTable T) {
     const int wpb = (int)blockDim.x >> 6;
     constexpr bool EB = sizeof(S) == 4;
-    bulk_rows<S, MODEL, TWO, EB, FL, NT>(T, (int)blockIdx.x * wpb + (int)(threadIdx.x >> 6),
-                                 (int)gridDim.x * wpb);
+    bulk_rows<S, MODEL, NR, EB, FL, NT>(T, (int)blockIdx.x * wpb + (int)(threadIdx.x >> 6),
+                                        (int)gridDim.x * wpb);
 }
 
 // Immersed-boundary workgroups of one time step (one per instance with markers).
@@ -560,14 +539,18 @@ __global__ __launch_bounds__(NTF) void k_step(LaunchTable T) {
             r0 = (int)min((int64_t)n_rows, (int64_t)first * T.blocked);
             r1 = min(n_rows, r0 + T.blocked);
         }
-        if (T.rows2)
-            bulk_rows<S, MODEL, true, EB, FL>(T, r0, 1, r1);
+        if (sizeof(S) == 4 && T.nrows == 4)
+            bulk_rows<S, MODEL, (sizeof(S) == 4 ? 4 : 2), EB, FL>(T, r0, 1, r1);
+        else if (T.nrows >= 2)
+            bulk_rows<S, MODEL, 2, EB, FL>(T, r0, 1, r1);
         else
-            bulk_rows<S, MODEL, false, EB, FL>(T, r0, 1, r1);
-    } else if (T.rows2) {
-        bulk_rows<S, MODEL, true, EB, FL>(T, first, nw);
+            bulk_rows<S, MODEL, 1, EB, FL>(T, r0, 1, r1);
+    } else if (sizeof(S) == 4 && T.nrows == 4) {
+        bulk_rows<S, MODEL, (sizeof(S) == 4 ? 4 : 2), EB, FL>(T, first, nw);
+    } else if (T.nrows >= 2) {
+        bulk_rows<S, MODEL, 2, EB, FL>(T, first, nw);
     } else {
-        bulk_rows<S, MODEL, false, EB, FL>(T, first, nw);
+        bulk_rows<S, MODEL, 1, EB, FL>(T, first, nw);
     }
 }
 
@@ -933,7 +916,7 @@ LaunchTable make_table(Ctx &C, int parity, int diag) {
     T.n_tiles = C.n_tiles;
     T.parity = parity;
     T.diag = diag;
-    T.rows2 = 0;
+    T.nrows = 1;
     T.blocked = 0;
     T.bulk_delay = 0;
     T.edge_blocks = C.d_edge_blocks;
@@ -945,27 +928,31 @@ LaunchTable make_table(Ctx &C, int parity, int diag) {
 
 typedef void (*StepKernel)(LaunchTable);
 
+template <typename S, int MODEL, int FL, bool NT>
+StepKernel pick_bulk(int nr) {
+    if constexpr (sizeof(S) == 4) {  // four rows in flight: FP32 storage only
+        if (nr == 4) return k_bulk<S, MODEL, 4, FL, NT>;
+    }
+    return nr >= 2 ? k_bulk<S, MODEL, 2, FL, NT> : k_bulk<S, MODEL, 1, FL, NT>;
+}
 template <typename S, int MODEL, int FL>
-void pick_layout(bool pred, bool two, bool nt, StepKernel &bulk, StepKernel &ibm,
+void pick_layout(bool pred, int nr, bool nt, StepKernel &bulk, StepKernel &ibm,
                  StepKernel &step) {
-    if (nt)
-        bulk = two ? k_bulk<S, MODEL, true, FL, true> : k_bulk<S, MODEL, false, FL, true>;
-    else
-        bulk = two ? k_bulk<S, MODEL, true, FL, false> : k_bulk<S, MODEL, false, FL, false>;
+    bulk = nt ? pick_bulk<S, MODEL, FL, true>(nr) : pick_bulk<S, MODEL, FL, false>(nr);
     ibm = k_ibm<S, MODEL>;
     step = pred ? k_step<S, MODEL, true, FL> : k_step<S, MODEL, false, FL>;
 }
 template <typename S, int MODEL>
-void pick_pair(bool pred, bool two, bool flat, bool nt, StepKernel &bulk, StepKernel &ibm,
+void pick_pair(bool pred, int nr, bool flat, bool nt, StepKernel &bulk, StepKernel &ibm,
                StepKernel &step) {
     if constexpr (sizeof(S) == 4) {  // FP64 contexts keep tiles (see apply_row_layout)
-        if (flat) return pick_layout<S, MODEL, 1>(pred, two, nt, bulk, ibm, step);
+        if (flat) return pick_layout<S, MODEL, 1>(pred, nr, nt, bulk, ibm, step);
     }
-    pick_layout<S, MODEL, 0>(pred, two, nt, bulk, ibm, step);
+    pick_layout<S, MODEL, 0>(pred, nr, nt, bulk, ibm, step);
 }
 
 void pick_kernels(const Ctx &C, StepKernel &bulk, StepKernel &ibm, StepKernel &step) {
-    const bool two = C.sched.bulk_rows == 2;
+    const int two = C.sched.bulk_rows;  // rows in flight of the bulk kernel
     int model = C.inst[0].d.cp.model;
     bool pred = false;  // the step kernel with the predictor modes only when one is set
     for (const HostInst &h : C.inst) {
@@ -1030,7 +1017,7 @@ int launch_steps(Ctx &C, int nsteps, int diag_every, bool concurrent) {
             const int64_t stepno = C.step + s + 1;
             const int diag = (diag_every > 0 && stepno % diag_every == 0) ? 1 : 0;
             LaunchTable T = make_table(C, (C.parity + s) & 1, diag);
-            T.rows2 = C.sched.step_rows == 2 ? 1 : 0;
+            T.nrows = C.sched.step_rows;
             T.blocked = C.sched.step_order;
             T.bulk_delay = C.sched.bulk_delay;
             hipLaunchKernelGGL(ks, dim3(blocks), dim3(NTF), 0, C.stream, T);
@@ -1053,7 +1040,7 @@ int launch_steps(Ctx &C, int nsteps, int diag_every, bool concurrent) {
         const int64_t stepno = C.step + s + 1;
         const int diag = (diag_every > 0 && stepno % diag_every == 0) ? 1 : 0;
         LaunchTable T = make_table(C, (C.parity + s) & 1, diag);
-        T.rows2 = C.sched.bulk_rows == 2 ? 1 : 0;
+        T.nrows = C.sched.bulk_rows;
         if (two) {
             if (s > 0) {
                 HIPCHK(hipStreamWaitEvent(C.stream, C.ev_side, 0));
@@ -1852,8 +1839,9 @@ int iblbm_set_schedule(void *p, const iblbm_schedule *in) {
         return fail(IBLBM_EINVAL, "set_schedule: unknown mode");
     if (in->bulk_threads < 64 || in->bulk_threads > 1024 || in->bulk_threads % 64)
         return fail(IBLBM_EINVAL, "set_schedule: bulk_threads must be a multiple of 64 in [64, 1024]");
-    if ((in->bulk_rows != 1 && in->bulk_rows != 2) || (in->step_rows != 1 && in->step_rows != 2))
-        return fail(IBLBM_EINVAL, "set_schedule: rows in flight must be 1 or 2");
+    auto rows_ok = [](int r) { return r == 1 || r == 2 || r == 4; };
+    if (!rows_ok(in->bulk_rows) || !rows_ok(in->step_rows))
+        return fail(IBLBM_EINVAL, "set_schedule: rows in flight must be 1, 2 or 4");
     if (in->bulk_waves < 0) return fail(IBLBM_EINVAL, "set_schedule: bulk_waves < 0");
     if (in->step_order < 0 || in->step_order > 64)
         return fail(IBLBM_EINVAL, "set_schedule: step_order must be in [0, 64]");
@@ -1895,11 +1883,13 @@ int iblbm_autotune(void *p, int steps, double *best_ms, int *n_tried) {
         pop_bytes += 2 * 9 * (size_t)h.d.qs * (h.d.prec == 1 ? sizeof(float) : sizeof(double));
     std::vector<int> stores = {0};
     if (pop_bytes > ((size_t)256 << 20)) stores.push_back(1);
+    const bool f32_ctx = C->inst[0].d.prec == 1;  // four rows in flight: FP32 storage only
     for (int st : stores)
         for (int lay : layouts) {
             if (C->n_fast == C->n_ibm && st == 0)  // the fused step stores normally
                 for (int order : {0, 1, 6, 8})
-                    for (int r = 1; r <= 2; ++r)
+                    for (int r : {1, 2, 4})
+                        if (r < 4 || f32_ctx)
                         for (int dl : {0, 100, 200})  // bulk start delay: 0, 1, 2 us
                             if (dl == 0 || C->n_fast > 0)
                                 cand.push_back({IBLBM_MODE_FUSED, 256, 1, 0, r, order, lay, st,
@@ -1910,7 +1900,8 @@ int iblbm_autotune(void *p, int steps, double *best_ms, int *n_tried) {
             if (C->n_ibm > 0) modes2.push_back(IBLBM_MODE_TWO_STREAMS);
             for (int mode2 : modes2)
                 for (int bt : {256, 512, 1024})
-                    for (int r = 1; r <= 2; ++r)
+                    for (int r : {1, 2, 4})
+                        if (r < 4 || f32_ctx)
                         for (int wv : {0, 16 * C->n_cu, 32 * C->n_cu})
                             cand.push_back({mode2, bt, r, wv, 2, 0, lay, st, 0});
         }

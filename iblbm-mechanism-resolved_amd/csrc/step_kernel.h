@@ -111,7 +111,7 @@ struct LaunchTable {
     int n_inst, n_ibm, n_fast, n_tiles;
     int parity;              // source buffer index of this step
     int diag;                // take diagnostic reductions this step
-    int rows2;               // bulk wavefronts keep two rows in flight (else one)
+    int nrows;               // rows a bulk wavefront keeps in flight: 1, 2 or 4 (FP32)
     int blocked;             // fused launch: each bulk wavefront takes one block of rows
     int bulk_delay;          // fused launch: bulk start delay in 10 ns ticks
     // boundary kernel (k_edge): per workgroup (instance, first ring slot), and its rho

@@ -30,3 +30,26 @@ def test_fp32dev_tracks_fp64_reference(case):
     print(f"{case}: fp32dev rel Linf f={err_f:.3e} Cd={err_cd:.3e}")
     assert err_f <= F_TOL
     assert err_cd <= CD_TOL
+
+
+@pytest.mark.parametrize("schedule", [
+    dict(step_rows=4), dict(step_rows=4, step_order="blocked"), dict(step_rows=4, step_order=8),
+    dict(step_rows=4, row_layout="flat"),
+    dict(mode="two_streams", bulk_threads=1024, bulk_rows=4, bulk_waves=0),
+    dict(mode="serial", bulk_threads=256, bulk_rows=4, bulk_waves=7, row_layout="flat"),
+    dict(mode="two_streams", bulk_threads=512, bulk_rows=4, bulk_waves=0, stores="nontemporal")])
+def test_fp32dev_four_rows_in_flight_match_default_schedule(schedule):
+    """Four bulk rows in flight (FP32 storage only) change how the rows are spread over the
+    wavefronts, not the arithmetic: populations equal the default schedule's bit for bit."""
+    outs = []
+    for sc in (None, schedule):
+        g, s = golden("run_mdf_trt_p4")
+        cfg = golden_cfg(s)
+        sim = FixedCylinder(cfg, perturb=perturb_of(s), precision="fp32dev")
+        if sc is not None:
+            sim.ctx.set_schedule(**sc)
+        sim.advance(cfg.max_steps, diag_every=cfg.check_interval, graph=True)
+        outs.append((sim.ctx.download_f(sim.k), sim.cd_cl()))
+        sim.ctx.close()
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    assert outs[0][1] == outs[1][1]
