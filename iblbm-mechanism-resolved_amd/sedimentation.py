@@ -40,6 +40,58 @@ def np_sum_sparse(vals: np.ndarray, start: int, n_total: int) -> float:
     return res
 
 
+class WindowSums:
+    """``float(np.sum(v))`` of full-grid (N,) vectors that are zero outside one rectangular
+    window, for the window's values only.  NumPy reduces a contiguous float64 vector in
+    buffers of _NP_BLOCK elements aligned to element 0 (pairwise inside a buffer, buffers
+    added in order); the window's values are scattered into zeroed copies of the buffers
+    they fall in, every buffer of every vector is reduced by one NumPy call along its last
+    axis (the same pairwise reduction per buffer), and the buffer sums are added in order.
+    The scatter indices are kept per window geometry."""
+
+    def __init__(self, nx: int, n_total: int):
+        self.nx, self.N = int(nx), int(n_total)
+        self._key = None
+
+    def _layout(self, box):
+        if box == self._key:
+            return
+        x0, y0, bw, bh = box
+        ii = np.arange(bw)
+        flat = ((y0 + np.arange(bh))[:, None] * self.nx + (x0 + ii)[None, :]).ravel()
+        b0 = flat[0] // _NP_BLOCK
+        b1 = flat[-1] // _NP_BLOCK
+        self.nb = int(b1 - b0 + 1)
+        self.rel = flat - b0 * _NP_BLOCK  # offsets into the (nb * _NP_BLOCK) buffers
+        # a buffer cut short by the end of the vector is reduced on its own, at its length
+        self.last_len = min(_NP_BLOCK, self.N - int(b1) * _NP_BLOCK)
+        # zeroed once per geometry: every call writes the same window positions only
+        self.buf = np.zeros((3, self.nb * _NP_BLOCK))
+        self._key = box
+
+    def sums(self, box, *vals):
+        """One NumPy-order sum per window-shaped value array in vals (at most three)."""
+        self._layout(box)
+        k = len(vals)
+        for q, v in enumerate(vals):
+            self.buf[q, self.rel] = v
+        buf = self.buf[:k].reshape(k, self.nb, _NP_BLOCK)
+        if self.last_len == _NP_BLOCK:
+            part = np.sum(buf, axis=2)
+        else:
+            part = np.empty((k, self.nb))
+            part[:, :-1] = np.sum(buf[:, :-1, :], axis=2)
+            for q in range(k):
+                part[q, -1] = np.sum(buf[q, -1, :self.last_len])
+        out = []
+        for q in range(k):
+            res = 0.0
+            for x in part[q]:
+                res += float(x)
+            out.append(res)
+        return out
+
+
 def compute_net_gravity(rho_ratio, r_lattice, g_lattice, gravity_direction="down"):
     """``iblbm/physics/gravity.py:16-34``."""
     mag = (rho_ratio - 1.0) * (np.pi * r_lattice**2) * g_lattice
@@ -292,6 +344,7 @@ class Sedimentation:
         self._mdf_read = None
         self.d_lattice = 2.0 * st.lattice_r
         self.termination_reason = "max_steps"
+        self._wsums = None
 
     @property
     def predictor(self):
@@ -347,27 +400,26 @@ class Sedimentation:
 
     def _window_force(self, fx, fy, box, p):
         """-(grid sums) of a window fib and its torque about particle p, in NumPy's order
-        over the full (N,) / (ny, nx) arrays (strategies/mdf.py:86-101)."""
+        over the full (N,) / (ny, nx) arrays (strategies/mdf.py:86-101).  Outside the
+        window both force components are zero, and so is the torque term there (a signed
+        zero, which changes no sum that is not itself zero)."""
         cfg, st = self.cfg, self.st
         x0, y0, bw, bh = box
-        nx, N = st.nx, st.nodenums
-        start = y0 * nx
-        n_rows = bh * nx
-        seg_x = np.zeros(n_rows)
-        seg_y = np.zeros(n_rows)
-        for r in range(bh):
-            seg_x[r * nx + x0:r * nx + x0 + bw] = fx[r * bw:(r + 1) * bw]
-            seg_y[r * nx + x0:r * nx + x0 + bw] = fy[r * bw:(r + 1) * bw]
-        F = -np.array([np_sum_sparse(seg_x, start, N), np_sum_sparse(seg_y, start, N)])
-        T = 0.0
-        if cfg.enable_rotation:  # strategies/mdf.py:94-101
-            cx_lat = float(p.pos[0]) / st.dx
-            cy_lat = float(p.pos[1]) / st.dy
-            II = np.tile(np.arange(nx, dtype=float), bh)
-            JJ = np.repeat(np.arange(y0, y0 + bh, dtype=float), nx)
-            term = (II - cx_lat) * seg_y - (JJ - cy_lat) * seg_x
-            T = -np_sum_sparse(term, start, N)
-        return F, T
+        if self._wsums is None:
+            self._wsums = WindowSums(st.nx, st.nodenums)
+        fx = np.asarray(fx, dtype=np.float64)
+        fy = np.asarray(fy, dtype=np.float64)
+        if not cfg.enable_rotation:
+            sx, sy = self._wsums.sums(box, fx, fy)
+            return -np.array([sx, sy]), 0.0
+        # strategies/mdf.py:94-101: (i - cx) fy - (j - cy) fx over the grid
+        cx_lat = float(p.pos[0]) / st.dx
+        cy_lat = float(p.pos[1]) / st.dy
+        II = np.tile(np.arange(x0, x0 + bw, dtype=float), bh)
+        JJ = np.repeat(np.arange(y0, y0 + bh, dtype=float), bw)
+        term = (II - cx_lat) * fy - (JJ - cy_lat) * fx
+        sx, sy, tq = self._wsums.sums(box, fx, fy, term)
+        return -np.array([sx, sy]), -tq
 
     def _full_step(self, forces):
         full_step_particles(self.particles, forces, self.cfg, self.st, self.g)
@@ -503,3 +555,84 @@ def stop_final_step(ttt: int, reason) -> int:
     (``iblbm/runtime/loop.py:106-109``, ``:142-148``), so a run stopped at check step ``ttt``
     reports ``ttt - 1``; a run that reaches ``max_steps`` reports ``max_steps``."""
     return int(ttt) - 1 if reason not in (None, "max_steps") else int(ttt)
+
+
+# Channel presets of the single-particle settling benchmark (``scenarios/sedimentation.py:20-
+# 33``): height in channel widths and the initial centre, in units of the width.
+DOMAIN_PRESETS = {"glowinski2001": {"ymax": 3.0, "cylinder_center": (0.5, 2.0)},
+                  "tall60D": {"ymax": 7.5, "cylinder_center": (0.5, 6.5)},
+                  "tall80D": {"ymax": 10.0, "cylinder_center": (0.5, 9.0)}}
+
+
+def make_sedimentation_config(rho_ratio=1.01, ibm_method="DF", delta_type="peskin4pt",
+                              collision_model="BGK", NN=None, Ma_max=0.08, tau_min=0.52,
+                              case_name="glowinski2001", center_x_offset_dx=0.0,
+                              diagnostics_interval=0, retraction_dx=0.0, nu_phys_override=None,
+                              enable_rotation=False, rotation_coupling="indirect",
+                              mdf_iterations=20, time_integrator="verlet",
+                              incompressible_lbgk=False, settling_inertia_model="none",
+                              sedimentation_stop_at_contact=False,
+                              sedimentation_stop_offset_d=0.0,
+                              sedimentation_reference_basis="particle_basis"):
+    """The single-disc settling configuration of ``make_sedimentation_config``
+    (``scenarios/sedimentation.py:36-249``): a disc of diameter d = W / 8 in a channel of
+    width W = 2 cm, g = 980 cm/s^2, nu = 0.1 cm^2/s at a density ratio of 1.25 and 0.01
+    otherwise (or ``nu_phys_override``).  The lattice follows from a Mach bound:
+
+    * Archimedes number Ar = |rho_ratio - 1| g d^3 / nu^2;
+    * the smallest lattice diameter D_min = 4 sqrt(3 Ar) nu_min / Ma_max with
+      nu_min = (tau_min - 0.5) / 3, hence NN_min = ceil(D_min / (d / W)) + 1; without NN the
+      first of 81, 161, 321, 641, 1281 at least NN_min;
+    * nu_lat = max(0.8 Ma_max D / sqrt(3 Ar), nu_min), g_lat = Ar nu_lat^2 / (|rho - 1| D^3),
+      the terminal-speed estimate u_t = sqrt(|rho - 1| g_lat D);
+    * max_steps = int(2 (ymax / dx) / u_t): twice the channel height at u_t.
+
+    Every expression keeps the reference's operand order, so the floats (g_lat above all) are
+    the ones its result files record."""
+    from .config import SimConfig
+    W_phys, d_phys, g_phys = 2.0, 0.25, 980.0
+    nu_phys = 0.1 if abs(rho_ratio - 1.25) < 1e-6 else 0.01
+    if nu_phys_override is not None:
+        nu_phys = float(nu_phys_override)
+    if case_name not in DOMAIN_PRESETS:
+        raise ValueError(f"case_name={case_name!r} (expected one of {sorted(DOMAIN_PRESETS)})")
+    preset = DOMAIN_PRESETS[case_name]
+    xmax, ymax = 1.0, preset["ymax"]
+    D_ratio = d_phys / W_phys
+    delta_rho = abs(rho_ratio - 1.0)
+    Ar = delta_rho * g_phys * d_phys**3 / nu_phys**2
+    nu_lat_min = (tau_min - 0.5) / 3.0
+    if delta_rho > 1e-15 and Ar > 0:
+        D_lat_min = 4.0 * np.sqrt(3.0 * Ar) * nu_lat_min / Ma_max
+        NN_min = int(np.ceil(D_lat_min / D_ratio)) + 1
+    else:
+        NN_min = 81
+    if NN is None:
+        NN = next((c for c in (81, 161, 321, 641, 1281) if c >= NN_min), NN_min)
+    dx = 1.0 / (NN - 1)
+    lattice_D = D_ratio * (NN - 1)
+    if delta_rho > 1e-15 and Ar > 0:
+        nu_lat = max(Ma_max * lattice_D / np.sqrt(3.0 * Ar) * 0.8, nu_lat_min)
+    else:
+        nu_lat = 0.05
+    g_lattice = Ar * nu_lat**2 / (delta_rho * lattice_D**3) if delta_rho > 1e-15 else 1e-6
+    u_t_est = np.sqrt(delta_rho * g_lattice * lattice_D) if delta_rho > 0 else 0.0
+    lattice_u = max(u_t_est, 0.01)
+    Re_input = lattice_u * lattice_D / nu_lat
+    max_steps = int(2.0 * (ymax / dx) / u_t_est) if u_t_est > 1e-10 else 500_000
+    cx_base, cy = preset["cylinder_center"]
+    return SimConfig(
+        Re=Re_input, NN=NN, xmax=xmax, ymax=ymax, cylinder_center=(cx_base + center_x_offset_dx
+                                                                   * dx, cy),
+        cylinder_D_ratio=D_ratio, lattice_u=lattice_u, inflow_u=0.0,
+        bc_type="settling_channel", delta_type=delta_type, ibm_method=ibm_method,
+        collision_model=collision_model, mdf_iterations=mdf_iterations, max_steps=max_steps,
+        use_convergence=False, check_interval=100, motion_type="sedimentation",
+        rho_ratio=rho_ratio, gravity=g_lattice, enable_rotation=enable_rotation,
+        retraction_dx=retraction_dx, rotation_coupling=rotation_coupling,
+        marker_spacing_factor=1.0, nx_formula="standard",
+        diagnostics_interval=diagnostics_interval, time_integrator=time_integrator,
+        incompressible_lbgk=incompressible_lbgk, settling_inertia_model=settling_inertia_model,
+        sedimentation_stop_at_contact=sedimentation_stop_at_contact,
+        sedimentation_stop_offset_d=sedimentation_stop_offset_d,
+        sedimentation_reference_basis=sedimentation_reference_basis)

@@ -8,7 +8,9 @@ oscillating-cylinder runs at Re = 100, KC = 5 on the full 2401 x 1601 lattice fo
 50 steps, Cd_peak from step 8000; ``oscillating_re100_kc5_summary_50tail.json``: the
 statistics of the last half) and the fixed-cylinder runs at Re = 20, 40, 100, 200
 (``data/fixed_cylinder/re*/<ibm>_<collision>_<delta>/status.json``: Cd and Cl every 200
-steps, the final step, the stop reason and the averages over the last 30 %).  This script
+steps, the final step, the stop reason and the averages over the last 30 %), and the
+single-disc settling runs (``data/single_particle_sedimentation*/**``: the trajectory every
+100 steps, the stop and the Reynolds-number peaks).  This script
 reads them as data (``numpy.load`` without pickles, ``json``) and writes the numbers into one
 fixture, since the reference tree is not present where the GPU tests run.  Nothing of the
 reference is executed.
@@ -96,6 +98,51 @@ def main():
             assert c["bc_type"] == "dirichlet_neumann"
             assert abs(c["marker_spacing_factor"] - 2.0 / 3.0) < 1e-15
     out["fc_names"] = np.array(fc)
+    # single-disc settling runs (data/single_particle_sedimentation*/**/status.json and
+    # sedimentation_history.json): the ones with the Velocity Verlet integrator, no or the
+    # explicit-history settling inertia and indirect rotation coupling
+    cols = ["step", "t_star", "y_star", "vy_star", "vx_star", "x", "y", "vx", "vy",
+            "f_hydro_x", "f_hydro_y", "angle", "omega", "torque", "mdf_iter", "mdf_residual"]
+    sed = []
+    for top in ("single_particle_sedimentation", "single_particle_sedimentation_tall60D",
+                "single_particle_sedimentation_tall80D"):
+        for root, _, files in sorted(os.walk(os.path.join(REF, top))):
+            if "status.json" not in files or "sedimentation_history.json" not in files:
+                continue
+            with open(os.path.join(root, "status.json")) as fh:
+                s = json.load(fh)
+            c = s.get("config", {})
+            if (c.get("time_integrator") != "verlet" or c.get("rotation_coupling") != "indirect"
+                    or c.get("settling_inertia_model") not in ("none", "explicit_history")):
+                continue
+            rel = os.path.relpath(root, REF)
+            name = rel.replace("/", "__")
+            with open(os.path.join(root, "sedimentation_history.json")) as fh:
+                h = json.load(fh)
+            key = f"sed_{name}"
+            sed.append(name)
+            present = [k for k in cols if k in h[0]]
+            out[key + "_columns"] = np.array(present)
+            out[key + "_history"] = np.array([[float(r[k]) for k in present] for r in h])
+            preset = ("tall60D" if "tall60D" in top else "tall80D" if "tall80D" in top
+                      else "glowinski2001")
+            out[key + "_strings"] = np.array([c["ibm_method"], c["delta_type"],
+                                              c["collision_model"], c["settling_inertia_model"],
+                                              c["sedimentation_reference_basis"], preset,
+                                              str(s.get("termination_reason"))])
+            out[key + "_numbers"] = np.array(
+                [c["rho_ratio"], c["NN"], c["retraction_dx"], c["gravity"], c["max_steps"],
+                 c["check_interval"], c["mdf_iterations"], s["final_step"],
+                 0.01 if "nu001" in rel else np.nan], dtype=np.float64)
+            for pk in ("Re_standard_peaks", "Re_particle_basis_peaks", "peak_steps"):
+                out[f"{key}_{pk}"] = np.array(s.get(pk, []), dtype=np.float64)
+            m = s.get("mdf_iter_stats")
+            if m:
+                out[key + "_mdf_iter_stats"] = np.array(
+                    [m["n_steps"], m["iter_avg"], m["iter_max"], m["iter_min"],
+                     m["diverge_count"], m["converge_count"], m["residual_avg"],
+                     m["residual_max"]], dtype=np.float64)
+    out["sed_names"] = np.array(sed)
     np.savez_compressed(OUT, **out)
     print(f"wrote {OUT}: {len(out)} arrays, {os.path.getsize(OUT)} bytes")
 

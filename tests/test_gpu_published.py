@@ -247,3 +247,72 @@ def test_fixed_cylinder_full_size(name, tmp_path):
     assert st["avg_start_step"] == int(PUB[key + "_avg"][2])
     assert abs(st["Cd_avg"] - PUB[key + "_avg"][0]) <= 1.01e-6  # 6 decimals published
     assert abs(st["Cl_avg"] - PUB[key + "_avg"][1]) <= 1.01e-6
+
+
+def sed_config(name):
+    """The single-disc settling configuration of the reference's runner
+    (``scripts/experiments/run_single_particle_sedimentation_experiments.py:287-305`` over
+    ``scenarios/sedimentation.py:36-249``) with the published run's settings."""
+    from iblbm_amd.sedimentation import make_sedimentation_config
+    key = f"sed_{name}"
+    ibm, delta, coll, inertia, basis, preset, _ = [str(v) for v in PUB[key + "_strings"]]
+    rho, NN, ret, _, _, _, mi, _, nu = PUB[key + "_numbers"]
+    return make_sedimentation_config(
+        rho_ratio=float(rho), ibm_method=ibm, delta_type=delta, collision_model=coll,
+        NN=int(NN), case_name=preset, retraction_dx=float(ret),
+        nu_phys_override=None if np.isnan(nu) else float(nu), enable_rotation=True,
+        rotation_coupling="indirect", mdf_iterations=int(mi), time_integrator="verlet",
+        settling_inertia_model=inertia, sedimentation_stop_at_contact=True,
+        sedimentation_reference_basis=basis)
+
+
+# the MDF + Peskin-4 run (about 70 s on the device: the particle update runs on the host
+# between steps); every published run within the device path with IBLBM_ALL_PUBLISHED=1
+SED_DEFAULT = [
+    "single_particle_sedimentation__method_matrix__rho150__mdf_bgk_verlet_explicit_history_p4",
+]
+SED_CASES = ([str(n) for n in PUB["sed_names"]] if os.environ.get("IBLBM_ALL_PUBLISHED")
+             else SED_DEFAULT)
+
+
+@pytest.mark.parametrize("name", SED_CASES)
+def test_single_disc_settling_full_size(name, tmp_path):
+    """``sedimentation.run`` with its status writer against the reference's result files of
+    the same run: the trajectory every 100 steps (position, velocity, angle, rotation rate,
+    hydrodynamic force and torque, MDF sub-steps and residual), the stop at the wall and
+    the Reynolds-number peaks."""
+    import json
+    from iblbm_amd.sedimentation import run
+    key = f"sed_{name}"
+    cfg = sed_config(name)
+    nums = PUB[key + "_numbers"]
+    assert cfg.gravity == nums[3] and cfg.max_steps == int(nums[4])
+    print(f"settling {name}: NN {cfg.NN}, ymax {cfg.ymax}, {cfg.ibm_method} "
+          f"{cfg.collision_model} {cfg.delta_type} {cfg.settling_inertia_model}", flush=True)
+    res = run(cfg, verbose=False, save_dir=str(tmp_path))
+    h = json.load(open(tmp_path / "sedimentation_history.json"))
+    st = json.load(open(tmp_path / "status.json"))
+    res["sim"].ctx.close()
+    cols = [str(c) for c in PUB[key + "_columns"]]
+    ref = PUB[key + "_history"]
+    got = np.array([[float(r[c]) for c in cols] for r in h])
+    n = min(len(got), len(ref))
+    worst = {}
+    for k, c in enumerate(cols):
+        scale = max(np.abs(ref[:n, k]).max(), 1e-300)
+        worst[c] = float(np.abs(got[:n, k] - ref[:n, k]).max() / scale)
+    print(f"  {len(got)} records (reference {len(ref)}), final step {st['final_step']} "
+          f"(reference {int(nums[7])}), stop {st['termination_reason']}; worst column "
+          f"difference / column scale: " + ", ".join(f"{c} {v:.1e}" for c, v in worst.items()),
+          flush=True)
+    assert len(got) == len(ref)
+    np.testing.assert_array_equal(got[:, 0], ref[:, 0])  # steps
+    assert st["termination_reason"] == str(PUB[key + "_strings"][6])
+    # the current loop reports the step counter minus one for a run stopped by its check
+    # (iblbm/runtime/loop.py:106-109, :142-148); some of the result files come from a loop
+    # that reported the counter itself, on the same trajectory and check step
+    ref_final = int(nums[7])
+    assert st["final_step"] == ref_final or (st["final_step"] == ref_final - 1 and
+                                             ref_final == int(ref[-1, 0]))
+    for c, v in worst.items():
+        assert v <= 1e-6, c
