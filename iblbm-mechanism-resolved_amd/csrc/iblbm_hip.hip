@@ -11,6 +11,7 @@
 #include <map>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/iblbm_hip.h"
@@ -419,6 +420,69 @@ __device__ __attribute__((always_inline)) void bulk_rows(const LaunchTable &T, i
     }
 }
 
+// One row at a time, software-pipelined: a wavefront issues its next row's loads before the
+// current row's collision and stores, so a row's loads are in flight while it computes (the
+// register footprint of two rows in flight, without their lock-step).  Only interior rows are
+// loaded ahead; an edge row takes its batched (EB) or per-lane pull at its turn.
+template <typename S, int MODEL, bool EB, int FL, bool NT = false>
+__device__ __attribute__((always_inline)) void bulk_rows_pipe(const LaunchTable &T, int first,
+                                                             int nw, int stop = -1) {
+    const int lane = (int)threadIdx.x & 63;
+    const int n_rows = stop >= 0 ? stop : T.n_tiles * TILE_Y;
+    if (first >= n_rows) return;
+    RowRun R[2];
+    R[0].tb_lo = R[1].tb_lo = 1;
+    R[0].tb_hi = R[1].tb_hi = 0;
+    S a[2][9];
+    TileNode nn[2];
+    int rr[2];
+    bool pre[2];
+    rr[0] = first;
+    nn[0] = row_node<FL>(T, R[0], first, lane);
+    pre[0] = nn[0].interior;
+    if (pre[0]) bulk_pull_interior(R[0], nn[0].i, nn[0].j, a[0]);
+    // slot C holds the current row, slot 1 - C receives the next; both indices are constants
+    // in each instantiation, so the arrays stay in registers
+    auto body = [&](auto cc) -> bool {
+        constexpr int C = decltype(cc)::value, D = 1 - C;
+        const int rn = rr[C] + nw;
+        const bool more = rn < n_rows;
+        if (more) {
+            rr[D] = rn;
+            nn[D] = row_node<FL>(T, R[D], rn, lane);
+            pre[D] = nn[D].interior;
+            if (pre[D]) bulk_pull_interior(R[D], nn[D].i, nn[D].j, a[D]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        double rho = 0.0;
+        if (pre[C]) {
+            const double v = bulk_finish<MODEL, NT>(*R[C].I, R[C], nn[C].i, nn[C].j, a[C],
+                                                    nn[C].mine);
+            rho = nn[C].mine ? v : 0.0;
+        } else if constexpr (EB) {
+            int i0, j0;
+            lattice_ij<FL>(R[C], nn[C], i0, j0);
+            bulk_pull_edge(R[C], i0, j0, a[C]);
+            edge_fix(R[C], i0, j0, a[C]);
+            const double v = bulk_finish<MODEL, NT>(*R[C].I, R[C], nn[C].i, nn[C].j, a[C],
+                                                    nn[C].mine);
+            rho = nn[C].mine ? v : 0.0;
+        } else if (nn[C].mine) {
+            int i0, j0;
+            lattice_ij<FL>(R[C], nn[C], i0, j0);
+            bulk_pull(*R[C].I, T.parity, i0, j0, a[C]);
+            rho = bulk_finish<MODEL, NT>(*R[C].I, R[C], nn[C].i, nn[C].j, a[C], true);
+        }
+        if (T.diag) {
+            const double w = wave_sum(rho);
+            if (lane == 0) T.rho_partial[rr[C]] = w;
+        }
+        return more;
+    };
+    while (body(std::integral_constant<int, 0>{}) && body(std::integral_constant<int, 1>{})) {
+    }
+}
+
 // Bulk kernel of the two-kernel step.  Any workgroup size up to 1024 (a tile of 64 x
 // blockDim / 64 rows per workgroup); the grid either covers every row once or walks them
 // with a grid stride (the schedule picks both).
@@ -428,8 +492,11 @@ template <typename S, int MODEL, int NR, int FL, bool NT>
 __global__ __launch_bounds__(1024) void k_bulk(LaunchTable T) {
     const int wpb = (int)blockDim.x >> 6;
     constexpr bool EB = sizeof(S) == 4;
-    bulk_rows<S, MODEL, NR, EB, FL, NT>(T, (int)blockIdx.x * wpb + (int)(threadIdx.x >> 6),
-                                        (int)gridDim.x * wpb);
+    const int first = (int)blockIdx.x * wpb + (int)(threadIdx.x >> 6);
+    if constexpr (NR == 3 && sizeof(S) == 4)
+        bulk_rows_pipe<S, MODEL, EB, FL, NT>(T, first, (int)gridDim.x * wpb);
+    else
+        bulk_rows<S, MODEL, NR, EB, FL, NT>(T, first, (int)gridDim.x * wpb);
 }
 
 // Immersed-boundary workgroups of one time step (one per instance with markers).
@@ -527,6 +594,9 @@ __global__ __launch_bounds__(NTF) void k_step(LaunchTable T) {
             __builtin_amdgcn_s_sleep(1);
     }
     constexpr bool EB = sizeof(S) == 4 || MODEL >= 0;
+    // the pipelined rows fit the step kernel's registers with FP32 storage and the BGK / TRT
+    // collision only (FP64 and the central-moment collision would spill)
+    constexpr bool PIPE = sizeof(S) == 4 && (MODEL == 0 || MODEL == 1);
     const int first = ((int)blockIdx.x - T.n_fast) * (NTF / 64) + (int)(threadIdx.x >> 6);
     const int nw = ((int)gridDim.x - T.n_fast) * (NTF / 64);
     if (T.blocked) {  // one block of adjacent rows per wavefront
@@ -541,12 +611,16 @@ __global__ __launch_bounds__(NTF) void k_step(LaunchTable T) {
         }
         if (sizeof(S) == 4 && T.nrows == 4)
             bulk_rows<S, MODEL, (sizeof(S) == 4 ? 4 : 2), EB, FL>(T, r0, 1, r1);
+        else if (PIPE && T.nrows == 3)
+            bulk_rows_pipe<S, MODEL, EB, FL>(T, r0, 1, r1);
         else if (T.nrows >= 2)
             bulk_rows<S, MODEL, 2, EB, FL>(T, r0, 1, r1);
         else
             bulk_rows<S, MODEL, 1, EB, FL>(T, r0, 1, r1);
     } else if (sizeof(S) == 4 && T.nrows == 4) {
         bulk_rows<S, MODEL, (sizeof(S) == 4 ? 4 : 2), EB, FL>(T, first, nw);
+    } else if (PIPE && T.nrows == 3) {
+        bulk_rows_pipe<S, MODEL, EB, FL>(T, first, nw);
     } else if (T.nrows >= 2) {
         bulk_rows<S, MODEL, 2, EB, FL>(T, first, nw);
     } else {
@@ -930,8 +1004,9 @@ typedef void (*StepKernel)(LaunchTable);
 
 template <typename S, int MODEL, int FL, bool NT>
 StepKernel pick_bulk(int nr) {
-    if constexpr (sizeof(S) == 4) {  // four rows in flight: FP32 storage only
+    if constexpr (sizeof(S) == 4) {  // four rows in flight, or one pipelined: FP32 storage
         if (nr == 4) return k_bulk<S, MODEL, 4, FL, NT>;
+        if (nr == 3) return k_bulk<S, MODEL, 3, FL, NT>;
     }
     return nr >= 2 ? k_bulk<S, MODEL, 2, FL, NT> : k_bulk<S, MODEL, 1, FL, NT>;
 }
@@ -1839,9 +1914,10 @@ int iblbm_set_schedule(void *p, const iblbm_schedule *in) {
         return fail(IBLBM_EINVAL, "set_schedule: unknown mode");
     if (in->bulk_threads < 64 || in->bulk_threads > 1024 || in->bulk_threads % 64)
         return fail(IBLBM_EINVAL, "set_schedule: bulk_threads must be a multiple of 64 in [64, 1024]");
-    auto rows_ok = [](int r) { return r == 1 || r == 2 || r == 4; };
+    auto rows_ok = [](int r) { return r >= 1 && r <= 4; };
     if (!rows_ok(in->bulk_rows) || !rows_ok(in->step_rows))
-        return fail(IBLBM_EINVAL, "set_schedule: rows in flight must be 1, 2 or 4");
+        return fail(IBLBM_EINVAL, "set_schedule: rows in flight must be 1, 2, 3 (one row, "
+                                  "pipelined) or 4");
     if (in->bulk_waves < 0) return fail(IBLBM_EINVAL, "set_schedule: bulk_waves < 0");
     if (in->step_order < 0 || in->step_order > 64)
         return fail(IBLBM_EINVAL, "set_schedule: step_order must be in [0, 64]");
@@ -1883,7 +1959,7 @@ int iblbm_autotune(void *p, int steps, double *best_ms, int *n_tried) {
         pop_bytes += 2 * 9 * (size_t)h.d.qs * (h.d.prec == 1 ? sizeof(float) : sizeof(double));
     std::vector<int> stores = {0};
     if (pop_bytes > ((size_t)256 << 20)) stores.push_back(1);
-    const bool f32_ctx = C->inst[0].d.prec == 1;  // four rows in flight: FP32 storage only
+    const bool f32_ctx = C->inst[0].d.prec == 1;  // rows 3 and 4: FP32 storage only
     for (int st : stores)
         for (int lay : layouts) {
             if (C->n_fast == C->n_ibm && st == 0)  // the fused step stores normally
@@ -1900,8 +1976,8 @@ int iblbm_autotune(void *p, int steps, double *best_ms, int *n_tried) {
             if (C->n_ibm > 0) modes2.push_back(IBLBM_MODE_TWO_STREAMS);
             for (int mode2 : modes2)
                 for (int bt : {256, 512, 1024})
-                    for (int r : {1, 2, 4})
-                        if (r < 4 || f32_ctx)
+                    for (int r : {1, 2, 3, 4})
+                        if (r < 3 || f32_ctx)
                         for (int wv : {0, 16 * C->n_cu, 32 * C->n_cu})
                             cand.push_back({mode2, bt, r, wv, 2, 0, lay, st, 0});
         }

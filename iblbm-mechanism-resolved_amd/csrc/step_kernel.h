@@ -111,7 +111,8 @@ struct LaunchTable {
     int n_inst, n_ibm, n_fast, n_tiles;
     int parity;              // source buffer index of this step
     int diag;                // take diagnostic reductions this step
-    int nrows;               // rows a bulk wavefront keeps in flight: 1, 2 or 4 (FP32)
+    int nrows;               // rows a bulk wavefront keeps in flight: 1, 2, 4 (FP32), or
+                             // 3: one row with the next row's loads issued ahead
     int blocked;             // fused launch: each bulk wavefront takes one block of rows
     int bulk_delay;          // fused launch: bulk start delay in 10 ns ticks
     // boundary kernel (k_edge): per workgroup (instance, first ring slot), and its rho

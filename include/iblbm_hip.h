@@ -211,12 +211,15 @@ typedef struct {
     int32_t mode;         /* IBLBM_MODE_AUTO (fused where eligible) or an explicit mode */
     int32_t bulk_threads; /* bulk workgroup size of the two-kernel modes: 64..1024, x64;
                            * a workgroup covers a 64 x (bulk_threads / 64) tile of rows */
-    int32_t bulk_rows;    /* rows a bulk wavefront keeps in flight: 1, 2 or 4 (4 with
-                           * FP32-deviation storage; FP64 contexts take 2 for it) */
+    int32_t bulk_rows;    /* rows a bulk wavefront keeps in flight: 1, 2 or 4; 3: one row
+                           * at a time, software-pipelined (the next row's loads issued
+                           * before the current row's collision).  3 and 4 apply to
+                           * FP32-deviation storage (FP64 contexts take 2 for them; in the
+                           * fused step 3 also needs the BGK or TRT collision) */
     int32_t bulk_waves;   /* 0: one row per wavefront (grid covers every row); > 0: that
                            * many wavefronts walk the rows with a grid stride */
-    int32_t step_rows;    /* rows in flight of a bulk wavefront in the fused launch: 1, 2
-                           * or 4 (as bulk_rows) */
+    int32_t step_rows;    /* rows in flight of a bulk wavefront in the fused launch: 1 to
+                           * 4, as bulk_rows */
     int32_t step_order;   /* fused launch: 0 rows by grid stride; 1 one block of adjacent rows
                            * per bulk wavefront, all wavefronts; k >= 2 blocks of k rows (the
                            * wavefronts past the last block idle, leaving memory bandwidth to

@@ -99,3 +99,13 @@ def test_device_initial_state():
     assert np.max(np.abs(xi)) <= 1e-3 * (1 + 1e-9)
     assert abs(np.mean(xi)) < 1e-4 and abs(np.std(xi) - 1e-3 / np.sqrt(3)) < 1e-4
     ctx.close()
+
+
+@pytest.mark.parametrize("schedule", [dict(step_rows=3), dict(step_rows=4),
+                                      dict(mode="serial", bulk_rows=3),
+                                      dict(mode="two_streams", bulk_rows=4, bulk_threads=512)])
+def test_fp64_takes_two_rows_for_the_fp32_only_schedules(schedule):
+    """Rows-in-flight values 3 and 4 apply to FP32 storage; an FP64 context runs them as two
+    rows in flight, with unchanged results."""
+    fast = [c for c in RUN_CASES if _fused_alone(c)][:2]
+    _run_ensemble(fast, schedule)

@@ -35,12 +35,17 @@ def test_fp32dev_tracks_fp64_reference(case):
 @pytest.mark.parametrize("schedule", [
     dict(step_rows=4), dict(step_rows=4, step_order="blocked"), dict(step_rows=4, step_order=8),
     dict(step_rows=4, row_layout="flat"),
+    dict(step_rows=3), dict(step_rows=3, step_order="blocked"), dict(step_rows=3, step_order=8,
+                                                                     row_layout="flat"),
+    dict(mode="two_streams", bulk_threads=512, bulk_rows=3, bulk_waves=0),
+    dict(mode="serial", bulk_threads=1024, bulk_rows=3, bulk_waves=9, row_layout="flat"),
     dict(mode="two_streams", bulk_threads=1024, bulk_rows=4, bulk_waves=0),
     dict(mode="serial", bulk_threads=256, bulk_rows=4, bulk_waves=7, row_layout="flat"),
     dict(mode="two_streams", bulk_threads=512, bulk_rows=4, bulk_waves=0, stores="nontemporal")])
-def test_fp32dev_four_rows_in_flight_match_default_schedule(schedule):
-    """Four bulk rows in flight (FP32 storage only) change how the rows are spread over the
-    wavefronts, not the arithmetic: populations equal the default schedule's bit for bit."""
+def test_fp32dev_rows_in_flight_match_default_schedule(schedule):
+    """Four bulk rows in flight, or one row with the next row's loads issued ahead (FP32
+    storage only), change how the rows are spread over the wavefronts and when their loads
+    are issued, not the arithmetic: populations equal the default schedule's bit for bit."""
     outs = []
     for sc in (None, schedule):
         g, s = golden("run_mdf_trt_p4")
