@@ -153,7 +153,7 @@ def main():
                     help="--workload copy: also plain FP32 copies of these sizes (MB of "
                          "source, comma-separated)")
     ap.add_argument("--workload", choices=["cylinder", "matrix72", "sed1", "sed64", "shedding",
-                                           "copy", "large"],
+                                           "copy", "large", "mdf_large"],
                     default="cylinder",
                     help="cylinder: Re=100 1601x641 (default); matrix72: the batched 72-case "
                          "Re x method x kernel x collision matrix (about 300 M nodes); sed1: "
@@ -162,6 +162,9 @@ def main():
                          "64 two-disc settling cases (193x1921, DFC + hat + CM-MRT) stepped "
                          "as one ensemble; large: the FP64 Re=200 cylinder on 3001x2401 "
                          "(1.04 GB of populations per step, beyond the last-level cache); "
+                         "mdf_large: plain MDF against the learned predictor on the matrix's "
+                         "Re=200 MDF cylinder (4801x1921, 377 markers, generic window), FP32 "
+                         "and FP64; "
                          "copy: "
                          "a 1 GiB FP64 device copy, five times (the reference of the HBM "
                          "traffic counters and of a plain copy's bandwidth)")
@@ -187,6 +190,20 @@ def main():
         return
     if args.workload == "shedding":
         print(json.dumps(shedding_bench(args)), flush=True)
+        return
+    if args.workload == "mdf_large":
+        legs = {p: large_window_surrogate_leg(p, not args.no_graph) for p in ("fp32dev", "fp64")}
+        r = legs["fp32dev"]
+        print(json.dumps({
+            "metric": "MLUPS (fp32dev TRT+MDF step, Re=200 cylinder 4801x1921, learned MDF "
+                      "predictor on the generic window)",
+            "value": r["MLUPS_surrogate"], "unit": "MLUPS", "n_gpus": 1,
+            "steps": r["window"]["steps"], "warmup": r["window"]["start"],
+            "ms_per_step": round(r["surrogate_us"] / 1e3, 5), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32 (deviation storage; FP64 IBM)",
+            "data": "synthetic (seeded 1e-3 inflow perturbation, seed 1234)",
+            "config": {"workload": r["workload"]}, "legs": legs}), flush=True)
         return
     if args.workload == "large":
         r = large_lattice_leg(steps=max(1, args.steps // 10), warm=max(0, args.warmup // 2),
@@ -295,6 +312,8 @@ def main():
         out["mdf_step"] = surrogate_leg(cfg, args.precision, graph, win)
         if args.precision == "fp64" and args.variant_steps > 0:
             out["mdf_step_fp32dev"] = surrogate_leg(cfg, "fp32dev", graph, win)
+    if args.precision == "fp64" and args.surrogate_steps > 0:
+        out["mdf_step_large_window"] = large_window_surrogate_leg("fp32dev", graph)
     if args.precision == "fp64" and args.variant_steps > 0:
         out["fp32dev"] = fp32dev_leg(cfg, args.variant_steps, warm, graph)
         out["fp64_beyond_cache"] = large_lattice_leg()
@@ -866,6 +885,27 @@ def surrogate_leg(cfg, precision, graph, window=SURROGATE_WINDOW):
                       "adam_steps": len(losses),
                       "loss_first100": float(losses[:100].mean()),
                       "loss_last100": float(losses[-100:].mean())}}
+
+
+# The largest MDF instance of the 72-case matrix run alone: its 377-marker window exceeds the
+# fused path and takes the generic window kernel beside the bulk kernel, and in FP32 storage
+# the window is the longer of the two.  Training from step 2000 to 3000, then both forms for
+# 1000 steps from the step-3000 state.
+LARGE_WINDOW = {"train_from": 2000, "start": 3000, "steps": 1000, "sample_every": 100}
+
+
+def large_window_surrogate_leg(precision, graph):
+    """``surrogate_leg`` on the Re = 200 cylinder of the matrix (4801 x 1921, D = 120, MDF +
+    Peskin-4 + TRT, 377 markers), with the window and bulk times of each form."""
+    from iblbm_amd.ensemble import cylinder_cfg
+    cfg = cylinder_cfg(200.0, 1921, "MDF", "peskin4pt", "TRT")
+    r = surrogate_leg(cfg, precision, graph, LARGE_WINDOW)
+    r["workload"] = ("fixed cylinder Re=200, 4801x1921, D=120, 377 markers, TRT, MDF [5,20] "
+                     "tol 1e-4, Peskin-4 (the largest MDF instance of the 72-case matrix)")
+    r["nodes"] = 4801 * 1921
+    r["MLUPS_plain"] = round(r["nodes"] / r["plain_us"], 2)
+    r["MLUPS_surrogate"] = round(r["nodes"] / r["surrogate_us"], 2)
+    return r
 
 
 def fp32dev_leg(cfg, n, warm, graph):

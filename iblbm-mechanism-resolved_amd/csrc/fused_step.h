@@ -78,55 +78,6 @@ __device__ inline void light_copy(const Inst &G, Inst &L) {
     L.data_cap = G.data_cap;
 }
 
-// The MDF loop decisions (mdf.py:175-229), made identically by every wavefront of the window
-// from the block-wide residual.  In the warm-start modes the loop needs only one true
-// correction.  It restarts plain MDF from the uncorrected velocity once when the predicted
-// start is no better than no forcing at all (the first residual after the prediction is not
-// below plain MDF's first residual, or is not finite: a wrong or corrupted network), or when
-// the divergence guard trips.
-struct MdfCtl {
-    bool mdf, warm;
-    int n_iter, min_iter, min_plain;
-    double tol, prev = INFINITY, last = INFINITY;
-    double plain0 = INFINITY;  // warm modes: plain MDF's first residual (slip before forcing)
-    int iter_count = 0, diverged = 0, converged = 0, restarted = 0, applied = 0;
-    // after barrier A: 0 apply this sub-step, 1 stop (divergence guard), 2 restart plain MDF
-    __device__ int at_a(double res, int it) {
-        if (!mdf) return 0;
-        last = res;
-        if (warm && !restarted && it == 0 && !(res < plain0)) {  // NaN compares false
-            restarted = 1;
-            prev = INFINITY;
-            min_iter = min_plain;
-            return 2;
-        }
-        if (it > 0 && res > prev * 1.5) {  // divergence guard, mdf.py:192-195
-            if (warm && !restarted) {
-                restarted = 1;
-                prev = INFINITY;
-                min_iter = min_plain;
-                return 2;
-            }
-            diverged = 1;
-            iter_count = it + 1;
-            return 1;
-        }
-        return 0;
-    }
-    // after barrier B of an applied sub-step: true when the loop has converged
-    __device__ bool at_b(double res, int it) {
-        ++applied;
-        if (!mdf) return true;
-        prev = res;
-        iter_count = it + 1;
-        if (it + 1 >= min_iter && res < tol) {
-            converged = 1;
-            return true;
-        }
-        return false;
-    }
-};
-
 // Sum over the NTF threads: fixed shuffle tree per wavefront, then the wavefronts in order
 // (the result is valid in thread 0).
 __device__ inline double fused_block_sum(double v, double *red) {
@@ -141,13 +92,6 @@ __device__ inline double fused_block_sum(double v, double *red) {
     }
     __syncthreads();
     return r;
-}
-
-// Largest value over the wavefront (every lane gets it).
-__device__ inline float wave_maxf(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
 }
 
 // 16 bytes per lane from global memory straight into LDS (global_load_lds_dwordx4): lane l of

@@ -499,12 +499,13 @@ __global__ __launch_bounds__(1024) void k_bulk(LaunchTable T) {
         bulk_rows<S, MODEL, NR, EB, FL, NT>(T, first, (int)gridDim.x * wpb);
 }
 
-// Immersed-boundary workgroups of one time step (one per instance with markers).
-template <typename S, int MODEL>
+// Immersed-boundary workgroups of one time step (one per instance with markers).  PRED:
+// with the learned predictor's modes (picked when an instance has one set).
+template <typename S, int MODEL, bool PRED>
 __global__ __launch_bounds__(NTI) void k_ibm(LaunchTable T) {
     __shared__ double red[NTI];
     Inst &I = T.insts[T.ibm_inst[blockIdx.x]];
-    ibm_role<S, MODEL>(I, T.parity, T.diag, red);
+    ibm_role<S, MODEL, PRED>(I, T.parity, T.diag, red);
 }
 
 // The three outermost rings of the instances whose closure reads the previous velocity or an
@@ -1014,7 +1015,7 @@ template <typename S, int MODEL, int FL>
 void pick_layout(bool pred, int nr, bool nt, StepKernel &bulk, StepKernel &ibm,
                  StepKernel &step) {
     bulk = nt ? pick_bulk<S, MODEL, FL, true>(nr) : pick_bulk<S, MODEL, FL, false>(nr);
-    ibm = k_ibm<S, MODEL>;
+    ibm = pred ? k_ibm<S, MODEL, true> : k_ibm<S, MODEL, false>;
     step = pred ? k_step<S, MODEL, true, FL> : k_step<S, MODEL, false, FL>;
 }
 template <typename S, int MODEL>
@@ -2306,8 +2307,10 @@ int iblbm_predictor_mode(void *p, int id, int mode, int64_t capacity) {
     Inst &d = h.d;
     if (mode != 0 && h.c.ibm != IBLBM_IBM_MDF)
         return fail(IBLBM_EINVAL, "predictor_mode: the predictor drives MDF instances only");
-    if (mode != 0 && d.ibm != 0 && !d.fast)
-        return fail(IBLBM_EINVAL, "predictor_mode: the window does not take the fused path");
+    // the fused window or the generic one (one particle, markers in the threads' registers)
+    if (mode != 0 && (d.np_part > 1 || d.lb_cap > NTI))
+        return fail(IBLBM_EINVAL, "predictor_mode: the predictor drives one-particle windows of "
+                                  "at most " + std::to_string(NTI) + " markers");
     HIPCHK(hipSetDevice(C->device));
     IbmScratch &S = d.s;
     if (mode != 0 && !S.mlp_p) {  // first use: allocate and initialise
@@ -2323,7 +2326,11 @@ int iblbm_predictor_mode(void *p, int id, int mode, int64_t capacity) {
         S.fprev = (double *)A(sizeof(double) * 4 * d.lb_cap);  // forces, then slips
         S.pstat = (double *)A(sizeof(double) * 4);
         S.pctr = (long long *)A(sizeof(long long) * 2);
-        if (!S.mlp_p || !S.mlp_norm || !S.mlp_wb || !S.data || !S.fprev || !S.pstat || !S.pctr)
+        const size_t tiles = (size_t)(d.lb_cap + 15) / 16;  // the generic window's tiles
+        S.mlpx = (unsigned short *)A(sizeof(unsigned short) * 256 * tiles);
+        S.mlpy = (float *)A(sizeof(float) * 16 * MLP_OUT * tiles);
+        if (!S.mlp_p || !S.mlp_norm || !S.mlp_wb || !S.data || !S.fprev || !S.pstat || !S.pctr ||
+            !S.mlpx || !S.mlpy)
             return fail(IBLBM_ENOMEM, "predictor_mode: allocation failed");
         d.data_cap = capacity > INT32_MAX ? INT32_MAX : (int)capacity;
         // Glorot-uniform weights from a fixed seed, zero biases and Adam moments

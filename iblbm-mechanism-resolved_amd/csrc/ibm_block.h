@@ -20,6 +20,7 @@
 #pragma once
 #include <type_traits>
 
+#include "mlp.h"
 #include "step_kernel.h"
 
 namespace iblbm {
@@ -68,6 +69,13 @@ __device__ inline unsigned long long dpp_max_u64_lane63(unsigned long long v) {
     v = step(v, integral_constant<int, 0x118>(), integral_constant<int, 0xf>());  // row_shr:8
     v = step(v, integral_constant<int, 0x142>(), integral_constant<int, 0xa>());  // row_bcast:15
     v = step(v, integral_constant<int, 0x143>(), integral_constant<int, 0xc>());  // row_bcast:31
+    return v;
+}
+
+// Largest value over the wavefront (every lane gets it).
+__device__ inline float wave_maxf(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
     return v;
 }
 
@@ -260,6 +268,56 @@ __device__ void ibm_dfc_lambda(Inst &I, int nT, int Lb) {
 }
 
 // ------------------------------------------------------------------ the IBM core
+
+// The MDF loop decisions (mdf.py:175-229), made identically by every wavefront of the window
+// from the block-wide residual.  In the warm-start modes the loop needs only one true
+// correction.  It restarts plain MDF from the uncorrected velocity once when the predicted
+// start is no better than no forcing at all (the first residual after the prediction is not
+// below plain MDF's first residual, or is not finite: a wrong or corrupted network), or when
+// the divergence guard trips.
+struct MdfCtl {
+    bool mdf, warm;
+    int n_iter, min_iter, min_plain;
+    double tol, prev = INFINITY, last = INFINITY;
+    double plain0 = INFINITY;  // warm modes: plain MDF's first residual (slip before forcing)
+    int iter_count = 0, diverged = 0, converged = 0, restarted = 0, applied = 0;
+    // after barrier A: 0 apply this sub-step, 1 stop (divergence guard), 2 restart plain MDF
+    __device__ int at_a(double res, int it) {
+        if (!mdf) return 0;
+        last = res;
+        if (warm && !restarted && it == 0 && !(res < plain0)) {  // NaN compares false
+            restarted = 1;
+            prev = INFINITY;
+            min_iter = min_plain;
+            return 2;
+        }
+        if (it > 0 && res > prev * 1.5) {  // divergence guard, mdf.py:192-195
+            if (warm && !restarted) {
+                restarted = 1;
+                prev = INFINITY;
+                min_iter = min_plain;
+                return 2;
+            }
+            diverged = 1;
+            iter_count = it + 1;
+            return 1;
+        }
+        return 0;
+    }
+    // after barrier B of an applied sub-step: true when the loop has converged
+    __device__ bool at_b(double res, int it) {
+        ++applied;
+        if (!mdf) return true;
+        prev = res;
+        iter_count = it + 1;
+        if (it + 1 >= min_iter && res < tol) {
+            converged = 1;
+            return true;
+        }
+        return false;
+    }
+};
+
 
 // Touched-node working arrays (MDF velocity, dt/(2 rho), fib, rho), marker forces and gather
 // lists.  LDS or global scratch is a compile-time choice, so that LDS accesses compile to
@@ -726,12 +784,252 @@ __device__ void ibm_core(Inst &I, int nT, const IbmPtrs &P, double *red, bool st
     __syncthreads();
 }
 
+// MDF of a one-particle window with the learned force predictor (predictor modes 1-3), on
+// the generic window: the same arithmetic, order and decisions as the fused window's
+// predictor (fused_step.h), with the inference tiles' feature rows and outputs in global
+// scratch (S.mlpx, S.mlpy) and the BF16 weight blocks and biases read from global memory.
+// Each marker's stencil is in its thread's registers (Lb <= NTI).
+//  pre-pass: the forcing slip q = R (ud - U) before forcing, its change since the previous
+//    step, the FP32 features and sig (largest change over the marker's wavefront), the
+//    training row (collect) or the network's predicted change of force (learned);
+//  warm modes: the prediction F_prev (+ sig x net) is spread with weight 1 first, then at
+//    least one true MDF correction follows and plain MDF restarts once from the uncorrected
+//    velocity when the predicted start is no better than no forcing;
+//  afterwards: the force before the last applied sub-step is kept as the next step's F_prev
+//    (and its scaled change is the training target).
+template <int A>
+__device__ void ibm_core_learned(Inst &I, int nT, const IbmPtrs &P, double *red,
+                                 const MarkerRegs<A> &M) {
+    constexpr int K = (2 * A + 1) * (2 * A + 1);
+    const int t = threadIdx.x, Lb = I.Lb;
+    IbmScratch &S = I.s;
+    const double dt = I.cp.dt, omega = I.omega;
+    const int pmode = I.pmode;
+    const bool warm = pmode >= 2, mk = t < Lb;
+    const int n_iter = I.n_iter;
+    const int min_iter = max(1, min(I.min_iter, n_iter));
+    const long long pctr0 = S.pctr[0], pctr1 = S.pctr[1];
+    const bool fprev_ok = pctr1 != 0;
+    const bool record = pmode == 1 && fprev_ok;
+    for (int ti = t; ti < nT; ti += NTI) {
+        P.fbx[ti] = 0.0;
+        P.fby[ti] = 0.0;
+    }
+    double R = 0.0;  // rho at the marker
+    if (mk) {
+#pragma unroll
+        for (int s = 0; s < K; ++s) R += P.rot[M.ti[s]] * M.w[s];
+    }
+    // interpolation + force (df.py:239-263) of this marker from the working velocity
+    auto interp_force = [&](double &lx, double &ly, double &Fx, double &Fy) {
+        lx = 0.0;
+        ly = 0.0;
+#pragma unroll
+        for (int s = 0; s < K; ++s) {
+            lx += P.uwx[M.ti[s]] * M.w[s];
+            ly += P.uwy[M.ti[s]] * M.w[s];
+        }
+        Fx = 2.0 * R * (M.udx - lx) / dt;
+        Fy = 2.0 * R * (M.udy - ly) / dt;
+    };
+    // pre-pass: this marker's previous q and force, its q now and the features
+    double pfx = 0.0, pfy = 0.0, ex = 0.0, ey = 0.0, ax = 0.0, ay = 0.0, rv0 = -INFINITY;
+    if (mk) {
+        double spx = 0.0, spy = 0.0;
+        if (fprev_ok) {
+            spx = S.fprev[2 * I.lb_cap + 2 * t];
+            spy = S.fprev[2 * I.lb_cap + 2 * t + 1];
+            if (warm) {
+                pfx = S.fprev[2 * t];
+                pfy = S.fprev[2 * t + 1];
+            }
+        }
+        double lx, ly, Fx, Fy;
+        interp_force(lx, ly, Fx, Fy);
+        const double vx = M.udx - lx, vy = M.udy - ly;
+        rv0 = sqrt(vx * vx + vy * vy);
+        ax = R * (M.udx - lx);
+        ay = R * (M.udy - ly);
+        S.fprev[2 * I.lb_cap + 2 * t] = ax;  // the next step's previous q
+        S.fprev[2 * I.lb_cap + 2 * t + 1] = ay;
+        ex = ax - spx;
+        ey = ay - spy;
+    }
+    __threadfence_block();
+    // plain MDF's first residual, the bar a prediction must clear (warm modes)
+    const double plain0 = warm ? block_reduce(rv0, red, true) : INFINITY;
+    // the features feed a BF16 network: FP32 arithmetic from here
+    const float fex = (float)ex, fey = (float)ey, fax = (float)ax, fay = (float)ay;
+    const float a = sqrtf(fax * fax + fay * fay);
+    const float sig =
+        fmaxf(wave_maxf(sqrtf(fex * fex + fey * fey)) + 1e-9f * wave_maxf(a), 1e-30f);
+    float feat[MLP_IN];
+#pragma unroll
+    for (int q = 0; q < MLP_IN; ++q) feat[q] = 0.f;
+    if (mk) {
+        feat[0] = fex / sig;
+        feat[1] = fey / sig;
+        feat[2] = a > 0.f ? fax / a : 0.f;
+        feat[3] = a > 0.f ? fay / a : 0.f;
+        const double mx = I.Lx[t], my = I.Ly[t];
+        const float rx = (float)(mx - I.cxd), ry = (float)(my - I.cyd);
+        const float rr = sqrtf(rx * rx + ry * ry);
+        const double gx = mx / I.dx + 0.5, gy = my / I.dy + 0.5;
+        feat[4] = rr > 0.f ? rx / rr : 0.f;
+        feat[5] = rr > 0.f ? ry / rr : 0.f;
+        feat[6] = (float)(gx - floor(gx) - 0.5);
+        feat[7] = (float)(gy - floor(gy) - 0.5);
+    }
+    float *rowp = nullptr;  // this marker's training sample (collect mode)
+    if (record && mk) {      // features now, sig in the target's place until the end
+        rowp = S.data + ((pctr0 + t) % I.data_cap) * MLP_ROW;
+#pragma unroll
+        for (int q = 0; q < MLP_IN; ++q) rowp[q] = feat[q];
+        rowp[MLP_IN] = sig;
+    }
+    if (pmode == 3) {  // inference: 16-marker tiles over the wavefronts
+        const float *nrm = S.mlp_norm;
+        const int ntile = (Lb + 15) >> 4;
+        float xn[MLP_IN];
+#pragma unroll
+        for (int q = 0; q < MLP_IN; ++q) xn[q] = (feat[q] - nrm[q]) / nrm[MLP_IN + q];
+        if (t < 16 * ntile) mlp_put_row(S.mlpx + 16 * t, xn);  // rows from Lb on: unused
+        __threadfence_block();
+        __syncthreads();  // feature rows published
+        for (int tile = t >> 6; tile < ntile; tile += NTI / 64)
+            mlp_tile16(reinterpret_cast<const mlp_short4 *>(S.mlp_wb), S.mlp_p + MLP_B1,
+                       S.mlp_p + MLP_B2, S.mlp_p + MLP_B3, S.mlpx + 256 * tile,
+                       S.mlpy + 16 * MLP_OUT * tile);
+        __threadfence_block();
+        __syncthreads();  // outputs published
+        if (mk) {
+            pfx += (double)sig * (double)(S.mlpy[MLP_OUT * t] * nrm[2 * MLP_IN]);
+            pfy += (double)sig * (double)(S.mlpy[MLP_OUT * t + 1] * nrm[2 * MLP_IN + 1]);
+        }
+    }
+    double facx = 0.0, facy = 0.0;  // this marker's force accumulated over the step
+    if (warm) {  // the prediction, spread with weight 1 (not counted as a sub-step)
+        if (mk) {
+            P.Lfx[t] = pfx;
+            P.Lfy[t] = pfy;
+        }
+        facx = pfx;
+        facy = pfy;
+        __threadfence_block();
+        __syncthreads();
+        for (int ti = t; ti < nT; ti += NTI) {
+            double sx, sy;
+            gather2(P, ti, sx, sy);
+            const double vs = P.vs[ti];
+            P.fbx[ti] = sx;
+            P.fby[ti] = sy;
+            P.uwx[ti] = P.uwx[ti] + sx * vs;
+            P.uwy[ti] = P.uwy[ti] + sy * vs;
+        }
+        __threadfence_block();
+        __syncthreads();
+    }
+    MdfCtl ctl;
+    ctl.mdf = true;
+    ctl.warm = warm;
+    ctl.n_iter = n_iter;
+    ctl.min_plain = min_iter;
+    ctl.min_iter = warm ? 1 : min_iter;  // at least one true correction after a prediction
+    ctl.tol = I.tol;
+    ctl.plain0 = plain0;
+    double lastx = 0.0, lasty = 0.0;  // the last applied sub-step's force increment
+    bool first = true;
+    for (int it = 0; it < n_iter;) {
+        double Fx = 0.0, Fy = 0.0, rmax = -INFINITY;
+        if (mk) {
+            double lx, ly;
+            interp_force(lx, ly, Fx, Fy);
+            P.Lfx[t] = Fx;
+            P.Lfy[t] = Fy;
+            const double vx = M.udx - lx, vy = M.udy - ly;
+            rmax = sqrt(vx * vx + vy * vy);
+        }
+        __threadfence_block();
+        const double res = block_reduce(rmax, red, true);  // its barrier publishes Lf
+        if (first && t == 0) S.pstat[2] += res;  // first residual after the prediction
+        first = false;
+        const int act = ctl.at_a(res, it);
+        if (act == 1) break;
+        if (act == 0) {
+            lastx = omega * Fx;
+            lasty = omega * Fy;
+            facx += lastx;
+            facy += lasty;
+        } else {
+            facx = 0.0;
+            facy = 0.0;
+            lastx = 0.0;
+            lasty = 0.0;
+        }
+        for (int ti = t; ti < nT; ti += NTI) {
+            if (act == 0) {  // mdf.py:197-200
+                double ex2, ey2;
+                gather2(P, ti, ex2, ey2);
+                const double fx = ex2 * omega, fy = ey2 * omega;
+                const double vs = P.vs[ti];
+                P.fbx[ti] = P.fbx[ti] + fx;
+                P.fby[ti] = P.fby[ti] + fy;
+                P.uwx[ti] = P.uwx[ti] + fx * vs;
+                P.uwy[ti] = P.uwy[ti] + fy * vs;
+            } else {  // restart from the uncorrected velocity
+                const int sl = S.tslot[ti];
+                P.fbx[ti] = 0.0;
+                P.fby[ti] = 0.0;
+                P.uwx[ti] = S.ux0[sl];
+                P.uwy[ti] = S.uy0[sl];
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+        if (act == 2) {
+            it = 0;
+            continue;
+        }
+        if (ctl.at_b(res, it)) break;
+        ++it;
+    }
+    if (t == 0) {
+        I.stats[0] = ctl.iter_count;
+        I.stats[1] = ctl.last;
+        I.stats[2] = ctl.diverged;
+        I.stats[3] = ctl.converged;
+    }
+    if (mk) {
+        // the force before the last applied sub-step (the prediction itself after a learned
+        // step that needed one correction)
+        const double px = facx - lastx, py = facy - lasty;
+        if (record) {  // training target: its scaled change since the previous step
+            const double sg = (double)rowp[MLP_IN];
+            rowp[MLP_IN] = (float)((px - S.fprev[2 * t]) / sg);
+            rowp[MLP_IN + 1] = (float)((py - S.fprev[2 * t + 1]) / sg);
+        }
+        S.fprev[2 * t] = px;
+        S.fprev[2 * t + 1] = py;
+    }
+    if (t == 0) {
+        if (record) S.pctr[0] = pctr0 + Lb;
+        S.pctr[1] = 1;
+        S.pstat[0] += ctl.applied;
+        S.pstat[1] += 1.0;
+        S.pstat[3] += ctl.restarted;
+    }
+    __threadfence_block();
+    __syncthreads();
+}
+
 // ------------------------------------------------------------------ the workgroup
 
 // Window workgroup of one step.  It owns the touched nodes only (the bulk tiles skip exactly
 // those): stream/close them, moments, the IBM stage, velocity correction, forced collision.
 // The tables it reads were built by k_tables when the markers were set.
-template <typename ST, int MODEL, int A, bool LDS>
+// PRED: compiled with the learned-predictor modes (one-particle MDF windows of at most NTI
+// markers whose instance has a mode set; every other window runs ibm_core in either build).
+template <typename ST, int MODEL, int A, bool LDS, bool PRED>
 __device__ void ibm_role_impl(Inst &I, int parity, int diag, double *red, IbmShared &sh) {
     const int t = threadIdx.x;
     const ST *__restrict__ Asrc = buf<ST>(I, parity);
@@ -764,7 +1062,10 @@ __device__ void ibm_role_impl(Inst &I, int parity, int diag, double *red, IbmSha
     }
     __threadfence_block();
     __syncthreads();
-    ibm_core<A>(I, nT, P, red, false, M);
+    if (PRED && I.pmode != 0 && I.ibm == 2 && I.np_part <= 1 && Lb <= NTI)
+        ibm_core_learned<A>(I, nT, P, red, M);
+    else
+        ibm_core<A>(I, nT, P, red, false, M);
 
     // P3: velocity correction (DF/MDF), equilibrium, collision, store
     const bool correct = I.ibm == 1 || I.ibm == 2;
@@ -851,20 +1152,20 @@ __device__ void ibm_role_impl(Inst &I, int parity, int diag, double *red, IbmSha
     }
 }
 
-template <typename ST, int MODEL>
+template <typename ST, int MODEL, bool PRED>
 __device__ void ibm_role(Inst &I, int parity, int diag, double *red) {
     __shared__ IbmShared sh;
     const bool fits = ibm_fits_lds(I, I.nT);
     if (I.delta == 1) {
         if (fits)
-            ibm_role_impl<ST, MODEL, 1, true>(I, parity, diag, red, sh);
+            ibm_role_impl<ST, MODEL, 1, true, PRED>(I, parity, diag, red, sh);
         else
-            ibm_role_impl<ST, MODEL, 1, false>(I, parity, diag, red, sh);
+            ibm_role_impl<ST, MODEL, 1, false, PRED>(I, parity, diag, red, sh);
     } else {
         if (fits)
-            ibm_role_impl<ST, MODEL, 2, true>(I, parity, diag, red, sh);
+            ibm_role_impl<ST, MODEL, 2, true, PRED>(I, parity, diag, red, sh);
         else
-            ibm_role_impl<ST, MODEL, 2, false>(I, parity, diag, red, sh);
+            ibm_role_impl<ST, MODEL, 2, false, PRED>(I, parity, diag, red, sh);
     }
 }
 

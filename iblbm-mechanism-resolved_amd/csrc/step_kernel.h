@@ -61,6 +61,10 @@ struct IbmScratch {
     unsigned short *mlp_wb;
     double *fprev, *pstat;
     long long *pctr;
+    // the generic window's inference tiles: BF16 feature rows (16 per marker) and outputs
+    // (MLP_OUT per marker), lb_cap rounded up to whole 16-marker tiles
+    unsigned short *mlpx;
+    float *mlpy;
 };
 
 struct Inst {

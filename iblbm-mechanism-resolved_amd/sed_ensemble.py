@@ -96,6 +96,10 @@ class ParticleBank:
         self.hcoef = None
         if self.model == "explicit_history":
             self.hcoef = np.array([(np.pi * p.lattice_r**2) / p.mass for p in parts])
+        # feng_b2's post-correction coefficient rho_f / rho_s (inertia/feng_b2.py:33-40)
+        self.rinv = None
+        if self.model == "feng_b2":
+            self.rinv = np.array([1.0 / p.rho_ratio for p in parts])
         self.pos = np.array([p.pos for p in parts], dtype=float)
         self.vel = np.array([p.vel for p in parts], dtype=float)
         self.vel_prev = np.array([p.vel_prev for p in parts], dtype=float)
@@ -229,7 +233,8 @@ class ParticleBank:
         return F, T
 
     def full_step(self, F, T):
-        """Verlet full step of every particle (``verlet_full``, array form)."""
+        """Verlet full step of every particle (``verlet_full`` and ``post_correction``, array
+        form)."""
         F_total = F + self.grav
         vel_new = self.vel_half + (self.half_dt[:, None] * F_total) / self.mass[:, None]
         if self.hcoef is not None:
@@ -239,6 +244,8 @@ class ParticleBank:
             self.omega = self._keep_frozen(self.omega_half + (self.half_dt * T) / self.I,
                                            self.omega)
             self.torque = self._keep_frozen(T, self.torque)
+        if self.rinv is not None:  # post_correction
+            vel_new = vel_new + self.rinv[:, None] * (self.vel - self.vel_prev)
         self.vel_prev = self._keep_frozen(self.vel.copy(), self.vel_prev)
         self.vel = self._keep_frozen(vel_new, self.vel)
         self.force = self._keep_frozen(F, self.force)

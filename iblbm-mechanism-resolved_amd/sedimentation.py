@@ -1,8 +1,8 @@
 """Settling-particle runs on the GPU hot path (single disc and multi-particle DFC).
 
 The lattice, the marker window (MDF/DF/DFC) and the collision run on the device; the rigid-
-body update (Velocity Verlet with the ``none`` or ``explicit_history`` inertia model,
-``iblbm/physics/inertia``) stays on the host in FP64 with the reference's expressions, as in
+body update (Velocity Verlet with the ``none``, ``explicit_history`` or ``feng_b2`` inertia
+model, ``iblbm/physics/inertia``) stays on the host in FP64 with the reference's expressions, as in
 ``iblbm/runtime/scenarios/sedimentation_single.py:91-293`` and ``sedimentation_multi.py:49-124``.
 Per step the host uploads the moved markers and reads back the window fib (DF/MDF) or the
 marker forces (DFC).  The force and torque sums are formed in NumPy's own summation order
@@ -108,7 +108,8 @@ def _history_dv(model, vel, vel_prev, mass, r_lattice):
 
 
 def verlet_half(model, pos, vel, vel_prev, force, mass, rho_ratio, r_lat, g, dt, dx, gdir):
-    """``velocity_verlet_half`` of the ``none`` / ``explicit_history`` inertia models."""
+    """``velocity_verlet_half`` of the ``none`` / ``explicit_history`` inertia models
+    (``feng_b2`` integrates as ``none``, ``inertia/feng_b2.py:43-48``)."""
     F_total = force + compute_net_gravity(rho_ratio, r_lat, g, gdir)
     hdv = _history_dv(model, vel, vel_prev, mass, r_lat)
     if hdv is None:
@@ -124,6 +125,15 @@ def verlet_full(model, vel_half, vel, vel_prev, force, mass, rho_ratio, r_lat, g
     if hdv is None:
         return vel_half + 0.5 * dt * F_total / mass
     return vel_half + 0.5 * dt * F_total / mass + 0.5 * hdv
+
+
+def post_correction(model, vel_new, vel, vel_prev, rho_ratio):
+    """The inertia model's correction after the full step: ``feng_b2`` adds
+    (rho_f / rho_s) (U^n - U^{n-1}) (``inertia/feng_b2.py:33-48``); the others return the
+    velocity unchanged (``inertia/none.py:61-62``)."""
+    if model == "feng_b2":
+        return vel_new + (1.0 / rho_ratio) * (vel - vel_prev)
+    return vel_new
 
 
 def update_markers(Lx_c, Ly_c, cx_init, cy_init, pos_new, angle=0.0):
@@ -242,6 +252,8 @@ def full_step_particles(particles, forces, cfg, st, g):
         if cfg.enable_rotation:  # rigid_body.py:77-82
             p.omega = p.omega_half + 0.5 * st.dt * T / p.I
             p.torque = T
+        vel_new = post_correction(cfg.settling_inertia_model, vel_new, p.vel, p.vel_prev,
+                                  p.rho_ratio)
         p.vel_prev = p.vel.copy()
         p.vel = vel_new
         p.force = F
@@ -287,9 +299,10 @@ class Sedimentation:
         ``state._mdf_omega``, which a caller may preset."""
         if cfg.motion_type != "sedimentation":
             raise ValueError("Sedimentation needs motion_type='sedimentation'")
-        if cfg.settling_inertia_model not in ("none", "explicit_history"):
+        if cfg.settling_inertia_model not in ("none", "explicit_history", "feng_b2"):
             raise ValueError(f"settling_inertia_model={cfg.settling_inertia_model!r} is outside "
-                             "the GPU hot path (supported: 'none', 'explicit_history')")
+                             "the GPU hot path (supported: 'none', 'explicit_history', "
+                             "'feng_b2')")
         if cfg.time_integrator != "verlet":
             raise ValueError("only the Velocity Verlet integrator is on the GPU hot path")
         if cfg.rotation_coupling == "iterative" and cfg.enable_rotation:
@@ -531,7 +544,8 @@ def run(cfg, verbose: bool = True, callback=None, save_dir=None, device: int = 0
                 mon(sim.ttt, sim.particles, sim.mdf_stats, elapsed=time.time() - t0)
             if verbose:
                 r0 = recs[0]
-                print(f"step {sim.ttt:>7d} | y*={r0['y_star']:.4f} vy*={r0['vy_star']:.6f}")
+                print(f"step {sim.ttt:>7d} | y*={r0['y_star']:.4f} vy*={r0['vy_star']:.6f}",
+                      flush=True)
             if callback is not None:
                 callback(step=sim.ttt, Cd=0.0, Cl=0.0, error=1.0, state=sim, elapsed=0.0,
                          converged=False)
@@ -632,6 +646,78 @@ def make_sedimentation_config(rho_ratio=1.01, ibm_method="DF", delta_type="peski
         retraction_dx=retraction_dx, rotation_coupling=rotation_coupling,
         marker_spacing_factor=1.0, nx_formula="standard",
         diagnostics_interval=diagnostics_interval, time_integrator=time_integrator,
+        incompressible_lbgk=incompressible_lbgk, settling_inertia_model=settling_inertia_model,
+        sedimentation_stop_at_contact=sedimentation_stop_at_contact,
+        sedimentation_stop_offset_d=sedimentation_stop_offset_d,
+        sedimentation_reference_basis=sedimentation_reference_basis)
+
+
+def make_two_particle_config(ibm_method="DF", collision_model="BGK", delta_type="peskin4pt",
+                             rotation_coupling="semi_implicit", enable_rotation=True,
+                             mdf_iterations=20, time_integrator="euler_explicit",
+                             marker_spacing_factor=0.83, incompressible_lbgk=True,
+                             settling_inertia_model="explicit_history",
+                             sedimentation_stop_at_contact=False,
+                             sedimentation_stop_offset_d=2.0,
+                             sedimentation_reference_basis="standard",
+                             streamwise_extent_factor=1.0, particle_selection="both",
+                             grid_refinement_factor=1.0):
+    """The two-disc settling benchmark of ``make_two_particle_config_reference_benchmark``
+    (``scenarios/sedimentation.py:489-652``): discs of d = 0.2 m at density ratios 1.5 (heavy)
+    and 1.25 (light) in a 2 m wide channel, g = 9.81 m/s^2, released at (0.8, -0.13) and
+    (1.2, 0.13) m and settling along +x (the channel is turned on its side).
+
+    * dx = 0.0025 / kappa m and dt = 2.344e-4 / kappa^2 s (kappa = ``grid_refinement_factor``;
+      tau = 0.59 at every kappa), so NN = round(2 / dx) + 1 (801 at kappa = 1) and the
+      lattice diameter is 0.1 (NN - 1);
+    * the channel is 5 widths long times ``streamwise_extent_factor``;
+    * g_lat = g dt^2 / dx, the heavy disc's terminal-speed estimate
+      u_t = sqrt(0.5 g_lat D) sets ``lattice_u`` and ``Re``;
+    * max_steps = round(60000 f kappa^2), check_interval = round(500 kappa^2).
+
+    ``particle_selection="light_only"`` keeps only the light disc (one-element particle list).
+    The expressions keep the reference's operand order, so g_lat is the float its result
+    files record."""
+    from .config import SimConfig
+    W_phys, d_phys, g_phys = 2.0, 0.2, 9.81
+    rho_heavy, rho_light = 1.5, 1.25
+    delta_rho_heavy = rho_heavy - 1.0
+    if grid_refinement_factor <= 0:
+        raise ValueError(f"grid_refinement_factor must be > 0 (got {grid_refinement_factor})")
+    dx_phys = 0.0025 / grid_refinement_factor
+    dt_phys = 2.344e-4 / grid_refinement_factor**2
+    if abs(2.0 / dx_phys - round(2.0 / dx_phys)) > 1e-9:
+        raise ValueError(f"grid_refinement_factor={grid_refinement_factor} does not give an "
+                         "integer lattice")
+    nu_lat_target = (0.59 - 0.5) / 3.0
+    NN = int(round(W_phys / dx_phys)) + 1
+    D_ratio = d_phys / W_phys
+    lattice_D = D_ratio * (NN - 1)
+    xmax = (10.0 / W_phys) * streamwise_extent_factor
+    g_lattice = g_phys * dt_phys**2 / dx_phys
+    u_t_est = float(np.sqrt(delta_rho_heavy * g_lattice * lattice_D))
+    Re_est = u_t_est * lattice_D / nu_lat_target
+    heavy = {"center": (0.8 / W_phys, (-0.13 + 1.0) / 2.0), "rho_ratio": rho_heavy}
+    light = {"center": (1.2 / W_phys, (0.13 + 1.0) / 2.0), "rho_ratio": rho_light}
+    if particle_selection == "both":
+        particles = [heavy, light]
+    elif particle_selection == "light_only":
+        particles = [light]
+    else:
+        raise ValueError(f"particle_selection must be 'both' or 'light_only' "
+                         f"(got {particle_selection!r})")
+    max_steps = int(round(60000 * streamwise_extent_factor * grid_refinement_factor**2))
+    check_interval = int(round(500 * grid_refinement_factor**2))
+    return SimConfig(
+        Re=Re_est, NN=NN, xmax=xmax, ymax=1.0, cylinder_center=particles[0]["center"],
+        cylinder_D_ratio=D_ratio, lattice_u=max(u_t_est, 0.01), inflow_u=0.0,
+        bc_type="settling_channel", delta_type=delta_type, ibm_method=ibm_method,
+        collision_model=collision_model, mdf_iterations=mdf_iterations, max_steps=max_steps,
+        use_convergence=False, check_interval=check_interval, motion_type="sedimentation",
+        rho_ratio=particles[0]["rho_ratio"], gravity=g_lattice, enable_rotation=enable_rotation,
+        rotation_coupling=rotation_coupling, gravity_direction="right",
+        particles_config=particles, marker_spacing_factor=marker_spacing_factor,
+        nx_formula="standard", time_integrator=time_integrator,
         incompressible_lbgk=incompressible_lbgk, settling_inertia_model=settling_inertia_model,
         sedimentation_stop_at_contact=sedimentation_stop_at_contact,
         sedimentation_stop_offset_d=sedimentation_stop_offset_d,

@@ -10,7 +10,9 @@ statistics of the last half) and the fixed-cylinder runs at Re = 20, 40, 100, 20
 (``data/fixed_cylinder/re*/<ibm>_<collision>_<delta>/status.json``: Cd and Cl every 200
 steps, the final step, the stop reason and the averages over the last 30 %), and the
 single-disc settling runs (``data/single_particle_sedimentation*/**``: the trajectory every
-100 steps, the stop and the Reynolds-number peaks).  This script
+100 steps, the stop and the Reynolds-number peaks) and the two-disc settling runs
+(``data/two_particle_sedimentation/method_matrix/*``: both discs' trajectories every 500
+steps, the stop and the Reynolds-number peaks per disc).  This script
 reads them as data (``numpy.load`` without pickles, ``json``) and writes the numbers into one
 fixture, since the reference tree is not present where the GPU tests run.  Nothing of the
 reference is executed.
@@ -99,8 +101,8 @@ def main():
             assert abs(c["marker_spacing_factor"] - 2.0 / 3.0) < 1e-15
     out["fc_names"] = np.array(fc)
     # single-disc settling runs (data/single_particle_sedimentation*/**/status.json and
-    # sedimentation_history.json): the ones with the Velocity Verlet integrator, no or the
-    # explicit-history settling inertia and indirect rotation coupling
+    # sedimentation_history.json): the ones with the Velocity Verlet integrator, the none,
+    # explicit-history or feng_b2 settling inertia and indirect rotation coupling
     cols = ["step", "t_star", "y_star", "vy_star", "vx_star", "x", "y", "vx", "vy",
             "f_hydro_x", "f_hydro_y", "angle", "omega", "torque", "mdf_iter", "mdf_residual"]
     sed = []
@@ -113,7 +115,8 @@ def main():
                 s = json.load(fh)
             c = s.get("config", {})
             if (c.get("time_integrator") != "verlet" or c.get("rotation_coupling") != "indirect"
-                    or c.get("settling_inertia_model") not in ("none", "explicit_history")):
+                    or c.get("settling_inertia_model") not in ("none", "explicit_history",
+                                                               "feng_b2")):
                 continue
             rel = os.path.relpath(root, REF)
             name = rel.replace("/", "__")
@@ -143,6 +146,40 @@ def main():
                      m["diverge_count"], m["converge_count"], m["residual_avg"],
                      m["residual_max"]], dtype=np.float64)
     out["sed_names"] = np.array(sed)
+    # two-disc settling runs (data/two_particle_sedimentation/method_matrix/*): per record the
+    # step and, per disc, the columns below
+    pcols = ["x", "y", "vx", "vy", "t_star", "vx_star", "vy_star"]
+    two = []
+    base = os.path.join(REF, "two_particle_sedimentation", "method_matrix")
+    for name in sorted(os.listdir(base)):
+        d = os.path.join(base, name)
+        with open(os.path.join(d, "status.json")) as fh:
+            s = json.load(fh)
+        c = s["config"]
+        if c.get("time_integrator") != "verlet" or c.get("settling_inertia_model") not in (
+                "none", "explicit_history", "feng_b2"):
+            continue
+        with open(os.path.join(d, "sedimentation_history.json")) as fh:
+            h = json.load(fh)
+        key = f"two_{name}"
+        two.append(name)
+        out[key + "_history"] = np.array(
+            [[float(r["step"])] + [float(p[k]) for p in r["particles"] for k in pcols]
+             for r in h])
+        out[key + "_rho"] = np.array([float(p["rho_ratio"]) for p in h[0]["particles"]])
+        out[key + "_strings"] = np.array([c["ibm_method"], c["delta_type"],
+                                          c["collision_model"], c["settling_inertia_model"],
+                                          c["rotation_coupling"], c["gravity_direction"],
+                                          str(s.get("termination_reason")),
+                                          c["sedimentation_reference_basis"]])
+        out[key + "_numbers"] = np.array(
+            [c["rho_ratio"], c["NN"], c["gravity"], c["max_steps"], c["check_interval"],
+             c["mdf_iterations"], c["marker_spacing_factor"], s["final_step"],
+             c["sedimentation_stop_offset_d"]], dtype=np.float64)
+        for pk in ("Re_standard_peaks", "Re_particle_basis_peaks", "peak_steps"):
+            out[f"{key}_{pk}"] = np.array(s.get(pk, []), dtype=np.float64)
+    out["two_names"] = np.array(two)
+    out["two_columns"] = np.array(pcols)
     np.savez_compressed(OUT, **out)
     print(f"wrote {OUT}: {len(out)} arrays, {os.path.getsize(OUT)} bytes")
 

@@ -316,3 +316,68 @@ def test_single_disc_settling_full_size(name, tmp_path):
                                              ref_final == int(ref[-1, 0]))
     for c, v in worst.items():
         assert v <= 1e-6, c
+
+
+def two_config(name):
+    """The two-disc settling configuration of the reference's runner
+    (``scripts/experiments/run_two_particle_sedimentation_experiments.py:287-304`` over
+    ``scenarios/sedimentation.py:489-652``): semi-implicit rotation coupling, marker spacing
+    0.83, the offset stop at 2 diameters, 20 MDF sub-steps for MDF."""
+    from iblbm_amd.sedimentation import make_two_particle_config
+    key = f"two_{name}"
+    ibm, delta, coll, inertia = [str(v) for v in PUB[key + "_strings"][:4]]
+    return make_two_particle_config(
+        ibm_method=ibm, collision_model=coll, delta_type=delta,
+        rotation_coupling="semi_implicit", enable_rotation=True,
+        mdf_iterations=20 if ibm == "MDF" else 1, time_integrator="verlet",
+        marker_spacing_factor=0.83, incompressible_lbgk=False,
+        settling_inertia_model=inertia, sedimentation_stop_offset_d=2.0)
+
+
+# the DF run with the feng_b2 correction (about a minute on the device); every published run
+# with IBLBM_ALL_PUBLISHED=1 (the coupled-MDF runs take several minutes each)
+TWO_DEFAULT = ["df_bgk_verlet_feng_b2"]
+TWO_CASES = ([str(n) for n in PUB["two_names"]] if os.environ.get("IBLBM_ALL_PUBLISHED")
+             else TWO_DEFAULT)
+
+
+@pytest.mark.parametrize("name", TWO_CASES)
+def test_two_disc_settling_full_size(name, tmp_path):
+    """Two discs (density ratios 1.5 and 1.25, D = 80 lattice units, 303 markers each)
+    settling along a 4001 x 801 channel (``sedimentation.run`` with its status writer)
+    against the reference's result files of the same run: both discs' positions, velocities
+    and scaled times every 500 steps, the offset stop and the Reynolds-number peaks."""
+    import json
+    from iblbm_amd.sedimentation import run
+    key = f"two_{name}"
+    cfg = two_config(name)
+    nums = PUB[key + "_numbers"]
+    assert cfg.gravity == nums[2] and cfg.max_steps == int(nums[3])
+    assert [p["rho_ratio"] for p in cfg.particles_config] == list(PUB[key + "_rho"])
+    print(f"two discs {name}: {cfg.ibm_method} {cfg.collision_model} {cfg.delta_type} "
+          f"{cfg.settling_inertia_model}", flush=True)
+    res = run(cfg, verbose=True, save_dir=str(tmp_path))  # a line per record: long runs
+    h = json.load(open(tmp_path / "sedimentation_history.json"))
+    st = json.load(open(tmp_path / "status.json"))
+    res["sim"].ctx.close()
+    cols = [str(c) for c in PUB["two_columns"]]
+    ref = PUB[key + "_history"]
+    got = np.array([[float(r["step"])] + [float(p[c]) for p in r["particles"] for c in cols]
+                    for r in h])
+    n = min(len(got), len(ref))
+    worst = np.abs(got[:n, 1:] - ref[:n, 1:]).max(axis=0) / np.maximum(
+        np.abs(ref[:n, 1:]).max(axis=0), 1e-300)
+    labels = [f"{c}{i}" for i in range(len(cfg.particles_config)) for c in cols]
+    print(f"  {len(got)} records (reference {len(ref)}), final step {st['final_step']} "
+          f"(reference {int(nums[7])}), stop {st['termination_reason']}; worst column "
+          f"difference / column scale: " + ", ".join(f"{c} {v:.1e}" for c, v in
+                                                      zip(labels, worst)), flush=True)
+    assert len(got) == len(ref)
+    np.testing.assert_array_equal(got[:, 0], ref[:, 0])  # steps
+    assert st["termination_reason"] == str(PUB[key + "_strings"][6])
+    assert st["final_step"] == int(nums[7])
+    assert worst.max() <= 1e-6, labels[int(np.argmax(worst))]
+    # peak Reynolds numbers per disc, published to two decimals
+    for pk in ("Re_standard_peaks", "Re_particle_basis_peaks"):
+        np.testing.assert_allclose(st[pk], PUB[f"{key}_{pk}"], atol=0.0051, rtol=0)
+    np.testing.assert_array_equal(st["peak_steps"], PUB[key + "_peak_steps"])
