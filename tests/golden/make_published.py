@@ -11,8 +11,8 @@ statistics of the last half) and the fixed-cylinder runs at Re = 20, 40, 100, 20
 steps, the final step, the stop reason and the averages over the last 30 %), and the
 single-disc settling runs (``data/single_particle_sedimentation*/**``: the trajectory every
 100 steps, the stop and the Reynolds-number peaks) and the two-disc settling runs
-(``data/two_particle_sedimentation/method_matrix/*``: both discs' trajectories every 500
-steps, the stop and the Reynolds-number peaks per disc).  This script
+(``data/two_particle_sedimentation/**`` with the Velocity Verlet integrator: both discs'
+trajectories every check interval, the stop and the Reynolds-number peaks per disc).  This script
 reads them as data (``numpy.load`` without pickles, ``json``) and writes the numbers into one
 fixture, since the reference tree is not present where the GPU tests run.  Nothing of the
 reference is executed.
@@ -146,36 +146,47 @@ def main():
                      m["diverge_count"], m["converge_count"], m["residual_avg"],
                      m["residual_max"]], dtype=np.float64)
     out["sed_names"] = np.array(sed)
-    # two-disc settling runs (data/two_particle_sedimentation/method_matrix/*): per record the
-    # step and, per disc, the columns below
+    # two-disc settling runs (data/two_particle_sedimentation/**, Velocity Verlet): per record
+    # the step and, per disc, the columns below
     pcols = ["x", "y", "vx", "vy", "t_star", "vx_star", "vy_star"]
+    # (the method matrix, the 60D and 80D channels, the grid-refinement runs and the light
+    # disc alone: the builder arguments follow from the run's folder and grid)
     two = []
-    base = os.path.join(REF, "two_particle_sedimentation", "method_matrix")
-    for name in sorted(os.listdir(base)):
-        d = os.path.join(base, name)
+    base = os.path.join(REF, "two_particle_sedimentation")
+    runs = sorted((os.path.relpath(r, base), r) for r, _, f in os.walk(base)
+                  if "status.json" in f and "sedimentation_history.json" in f)
+    for rel, d in runs:
+        name = os.path.basename(d)
         with open(os.path.join(d, "status.json")) as fh:
             s = json.load(fh)
         c = s["config"]
         if c.get("time_integrator") != "verlet" or c.get("settling_inertia_model") not in (
                 "none", "explicit_history", "feng_b2"):
             continue
+        extent = 1.2 if rel.startswith("extended_60D") else (
+            1.6 if rel.startswith("extended_80D") else 1.0)
+        kappa = (c["NN"] - 1) / 800.0
+        selection = "light_only" if rel.startswith("isolated_light") else "both"
         with open(os.path.join(d, "sedimentation_history.json")) as fh:
             h = json.load(fh)
         key = f"two_{name}"
+        assert name not in two
         two.append(name)
+        # one disc alone is recorded in the single-particle format
+        parts = (lambda r: r["particles"]) if "particles" in h[0] else (lambda r: [r])
         out[key + "_history"] = np.array(
-            [[float(r["step"])] + [float(p[k]) for p in r["particles"] for k in pcols]
-             for r in h])
-        out[key + "_rho"] = np.array([float(p["rho_ratio"]) for p in h[0]["particles"]])
+            [[float(r["step"])] + [float(p[k]) for p in parts(r) for k in pcols] for r in h])
+        out[key + "_rho"] = np.array([float(p["rho_ratio"]) for p in c.get(
+            "particles_config", [])] or ([1.5, 1.25] if "particles" in h[0] else [1.25]))
         out[key + "_strings"] = np.array([c["ibm_method"], c["delta_type"],
                                           c["collision_model"], c["settling_inertia_model"],
                                           c["rotation_coupling"], c["gravity_direction"],
                                           str(s.get("termination_reason")),
-                                          c["sedimentation_reference_basis"]])
+                                          c["sedimentation_reference_basis"], selection, rel])
         out[key + "_numbers"] = np.array(
             [c["rho_ratio"], c["NN"], c["gravity"], c["max_steps"], c["check_interval"],
              c["mdf_iterations"], c["marker_spacing_factor"], s["final_step"],
-             c["sedimentation_stop_offset_d"]], dtype=np.float64)
+             c["sedimentation_stop_offset_d"], extent, kappa], dtype=np.float64)
         for pk in ("Re_standard_peaks", "Re_particle_basis_peaks", "peak_steps"):
             out[f"{key}_{pk}"] = np.array(s.get(pk, []), dtype=np.float64)
     out["two_names"] = np.array(two)
