@@ -315,6 +315,8 @@ def main():
     if args.precision == "fp64" and args.surrogate_steps > 0:
         out["mdf_step_large_window"] = large_window_surrogate_leg("fp32dev", graph)
     if args.precision == "fp64" and args.variant_steps > 0:
+        out["persistent"] = {p: persistent_leg(cfg, p) for p in ("fp64", "fp32dev")}
+    if args.precision == "fp64" and args.variant_steps > 0:
         out["fp32dev"] = fp32dev_leg(cfg, args.variant_steps, warm, graph)
         out["fp64_beyond_cache"] = large_lattice_leg()
         out["matrix72"] = dict(matrix_legs(max(10, args.steps // 200)),
@@ -906,6 +908,32 @@ def large_window_surrogate_leg(precision, graph):
     r["MLUPS_plain"] = round(r["nodes"] / r["plain_us"], 2)
     r["MLUPS_surrogate"] = round(r["nodes"] / r["surrogate_us"], 2)
     return r
+
+
+def persistent_leg(cfg, precision, n=2000, chunk=200):
+    """Time per step of the workload's steps in two launch forms from one state, with the
+    autotuned fused schedule: per-step launches replayed as graphs of `chunk` steps (the
+    headline's form) and persistent launches of `chunk` steps (one cooperative launch, a
+    grid-wide barrier between steps).  Device-timed with HIP events on the context stream;
+    the two forms compute the same populations bit for bit."""
+    from iblbm_amd.solver import FixedCylinder
+    sim = FixedCylinder(cfg, perturb=PERTURB, precision=precision)
+    tuned = sim.ctx.autotune(20)
+    sim.advance(chunk, graph=True)
+    sim.advance(chunk, persistent=True)
+    sim.ctx.sync()
+    out = {"precision": precision, "steps": n, "chunk": chunk, "schedule": tuned}
+    for form in ("graph", "persistent"):
+        sim.ctx.mark(0)
+        for _ in range(n // chunk):
+            sim.advance(chunk, graph=form == "graph", persistent=form == "persistent")
+        sim.ctx.mark(1)
+        sim.ctx.sync()
+        out[f"{form}_us_per_step"] = round(sim.ctx.elapsed_ms(0, 1) * 1e3 / n, 3)
+    out["persistent_vs_graph"] = round(out["graph_us_per_step"] / out["persistent_us_per_step"],
+                                       4)
+    sim.ctx.close()
+    return out
 
 
 def fp32dev_leg(cfg, n, warm, graph):

@@ -93,6 +93,7 @@ SIGNATURES = {
     "iblbm_step": (_i, [_VP, _i, _i]),
     "iblbm_sync": (_i, [_VP]),
     "iblbm_graph_run": (_i, [_VP, _i, _i]),
+    "iblbm_step_persistent": (_i, [_VP, _i, _i]),
     "iblbm_set_concurrent": (_i, [_VP, _i]),
     "iblbm_step_mode": (_i, [_VP, _I]),
     "iblbm_get_schedule": (_i, [_VP, ctypes.POINTER(Schedule)]),

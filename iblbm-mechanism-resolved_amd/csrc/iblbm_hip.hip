@@ -571,9 +571,7 @@ __global__ __launch_bounds__(256) void k_edge(LaunchTable T) {
 // workgroups [0, n_fast) run the windows (fused_step.h), the others run four bulk tiles each
 // (one node per thread).  One launch per step leaves no cross-stream wait between steps.
 template <typename S, int MODEL, bool PRED, int FL>
-__global__ __launch_bounds__(NTF) void k_step(LaunchTable T) {
-    __shared__ double red[NTF / 64 + 3];  // wavefront partials, then 3 residual slots
-    __shared__ FusedShared sh;
+__device__ __forceinline__ void step_body(const LaunchTable &T, double *red, FusedShared &sh) {
     if ((int)blockIdx.x < T.n_fast) {
         __builtin_amdgcn_s_setprio(3);  // issue ahead of bulk wavefronts sharing the CU
         // The instance table and descriptors are read-only during a step: reading them through
@@ -626,6 +624,49 @@ __global__ __launch_bounds__(NTF) void k_step(LaunchTable T) {
         bulk_rows<S, MODEL, 2, EB, FL>(T, first, nw);
     } else {
         bulk_rows<S, MODEL, 1, EB, FL>(T, first, nw);
+    }
+}
+template <typename S, int MODEL, bool PRED, int FL>
+__global__ __launch_bounds__(NTF) void k_step(LaunchTable T) {
+    __shared__ double red[NTF / 64 + 3];  // wavefront partials, then 3 residual slots
+    __shared__ FusedShared sh;
+    step_body<S, MODEL, PRED, FL>(T, red, sh);
+}
+
+// Grid-wide barrier of a cooperative launch (every workgroup resident): after a workgroup
+// barrier (each wavefront's stores complete), thread 0 arrives with an agent-scope release
+// (the step's writes written back for every XCD) and waits with agent-scope acquires until
+// all workgroups of this step have arrived.  The wait is bounded: past the spin limit it
+// records an error and goes on, so the launch always ends.
+__device__ inline void grid_barrier(unsigned *bar, unsigned target, int *err) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        int k = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (++k > (1 << 22)) {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+    }
+    __syncthreads();
+}
+
+// Several time steps in one persistent launch: the fused step's body, then a grid-wide
+// barrier, step after step (parity and diagnostic steps as the per-step launches have them).
+template <typename S, int MODEL, bool PRED, int FL>
+__global__ __launch_bounds__(NTF) void k_step_persist(LaunchTable T, PersistArgs A) {
+    __shared__ double red[NTF / 64 + 3];
+    __shared__ FusedShared sh;
+    for (int s = 0; s < A.nsteps; ++s) {
+        LaunchTable Ts = T;
+        Ts.parity = (A.parity0 + s) & 1;
+        const long long stepno = A.step0 + s + 1;
+        Ts.diag = (A.diag_every > 0 && stepno % A.diag_every == 0) ? 1 : 0;
+        step_body<S, MODEL, PRED, FL>(Ts, red, sh);
+        if (s + 1 < A.nsteps) grid_barrier(A.bar, gridDim.x * (unsigned)(s + 1), A.err);
     }
 }
 
@@ -797,6 +838,7 @@ struct HostInst {
 struct Ctx {
     int device = 0;
     int n_cu = 256;
+    unsigned *d_bar = nullptr;  // persistent launch: grid-barrier counter, then its error flag
     hipStream_t stream = nullptr, side = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_main = nullptr, ev_side = nullptr;
     bool concurrent = true;
@@ -1025,6 +1067,31 @@ void pick_pair(bool pred, int nr, bool flat, bool nt, StepKernel &bulk, StepKern
         if (flat) return pick_layout<S, MODEL, 1>(pred, nr, nt, bulk, ibm, step);
     }
     pick_layout<S, MODEL, 0>(pred, nr, nt, bulk, ibm, step);
+}
+
+typedef void (*PersistKernel)(LaunchTable, PersistArgs);
+
+// The persistent step kernel of a context whose instances share one collision model (the
+// tile row layout; nullptr where no such kernel is built).
+template <typename S>
+PersistKernel pick_persist_model(int model, bool pred) {
+    switch (model) {
+        case 0: return pred ? k_step_persist<S, 0, true, 0> : k_step_persist<S, 0, false, 0>;
+        case 1: return pred ? k_step_persist<S, 1, true, 0> : k_step_persist<S, 1, false, 0>;
+        case 2: return pred ? k_step_persist<S, 2, true, 0> : k_step_persist<S, 2, false, 0>;
+        default: return nullptr;
+    }
+}
+PersistKernel pick_persist(const Ctx &C) {
+    int model = C.inst[0].d.cp.model;
+    bool pred = false;
+    for (const HostInst &h : C.inst) {
+        if (h.d.cp.model != model) model = -1;
+        pred = pred || h.d.pmode != 0;
+    }
+    if (C.inst[0].d.flat != 0) return nullptr;
+    return C.inst[0].d.prec == 1 ? pick_persist_model<float>(model, pred)
+                                 : pick_persist_model<double>(model, pred);
 }
 
 void pick_kernels(const Ctx &C, StepKernel &bulk, StepKernel &ibm, StepKernel &step) {
@@ -1566,7 +1633,7 @@ int iblbm_download_marker_forces_batch(void *p, int n, const int *ids, double *F
             return fail(IBLBM_EINVAL, "download_marker_forces_batch: bad instance id");
         const Inst &d = C->inst[ids[b]].d;
         const bool dfc = d.ibm == IBLBM_IBM_DFC;
-        const bool df_multi = d.ibm == IBLBM_IBM_DF && d.np_part >= 2;
+        const bool df_multi = d.df_multi != 0;
         if (!dfc && !df_multi)
             return fail(IBLBM_EINVAL, "download_marker_forces_batch: needs DFC instances or DF "
                                       "windows of several particles");
@@ -1620,6 +1687,7 @@ int iblbm_set_particles(void *p, int id, int np, const int *lb, const double *la
         h.pfb_np = np;
     }
     d.np_part = np;
+    d.df_multi = h.c.ibm == IBLBM_IBM_DF ? 1 : 0;
     d.poff[0] = 0;
     for (int q = 0; q < np; ++q) {
         d.poff[q + 1] = d.poff[q] + lb[q];
@@ -1657,8 +1725,9 @@ int iblbm_download_marker_force(void *p, int id, double *F) {
     Ctx *C = as_ctx(p);
     if (!valid_id(C, id) || !F) return fail(IBLBM_EINVAL, "download_marker_force: bad argument");
     const Inst &d = C->inst[id].d;
-    if (d.ibm != IBLBM_IBM_DF || d.np_part < 2)
-        return fail(IBLBM_EINVAL, "download_marker_force: needs a DF window of several particles");
+    if (!d.df_multi)
+        return fail(IBLBM_EINVAL, "download_marker_force: needs a DF window of a particle list "
+                                  "(iblbm_set_particles)");
     HIPCHK(hipSetDevice(C->device));
     HIPCHK(hipStreamSynchronize(C->stream));
     HIPCHK(hipMemcpy(F, d.s.mf, sizeof(double) * 2 * d.Lb, hipMemcpyDeviceToHost));
@@ -1827,6 +1896,61 @@ int iblbm_step(void *p, int nsteps, int diag_every) {
     C->step += nsteps;
     C->parity = (C->parity + nsteps) & 1;
     C->last_steps = nsteps;
+    return 0;
+}
+
+int iblbm_step_persistent(void *p, int nsteps, int diag_every) {
+    Ctx *C = as_ctx(p);
+    if (!C || nsteps < 1) return fail(IBLBM_EINVAL, "step_persistent: bad argument");
+    if (C->inst.empty()) return fail(IBLBM_EINVAL, "step_persistent: no instance");
+    HIPCHK(hipSetDevice(C->device));
+    int rc = ensure_tables(*C);
+    if (rc) return rc;
+    if (effective_mode(*C) != IBLBM_MODE_FUSED || C->n_edge_blocks)
+        return fail(IBLBM_EINVAL, "step_persistent: needs the fused launch (every window on the "
+                                  "fused path) and no boundary kernel");
+    const PersistKernel kern = pick_persist(*C);
+    if (!kern)
+        return fail(IBLBM_EINVAL, "step_persistent: needs one collision model and the tile "
+                                  "row layout");
+    if (!C->d_bar) {
+        void *q = nullptr;
+        if (C->mem.alloc(&q, 2 * sizeof(unsigned)) != hipSuccess)
+            return fail(IBLBM_ENOMEM, "step_persistent: barrier counter");
+        C->d_bar = (unsigned *)q;
+    }
+    HIPCHK(hipMemsetAsync(C->d_bar, 0, 2 * sizeof(unsigned), C->stream));
+    const int rows = C->n_tiles * TILE_Y;
+    const int bulk = std::max(1, std::min(C->n_cu - C->n_fast, (rows + NTF / 64 - 1) / (NTF / 64)));
+    const int blocks = C->n_fast + bulk;  // one workgroup per CU (the window's LDS fills one)
+    LaunchTable T = make_table(*C, C->parity, 0);
+    T.nrows = C->sched.step_rows;
+    T.blocked = C->sched.step_order;
+    T.bulk_delay = C->sched.bulk_delay;
+    PersistArgs A;
+    A.bar = C->d_bar;
+    A.err = (int *)(C->d_bar + 1);
+    A.nsteps = nsteps;
+    A.diag_every = diag_every;
+    A.step0 = C->step;
+    A.parity0 = C->parity;
+    A.pad = 0;
+    void *args[] = {&T, &A};
+    HIPCHK(hipEventRecord(C->ev0, C->stream));
+    HIPCHK(hipLaunchCooperativeKernel((const void *)kern, dim3(blocks), dim3(NTF), args, 0,
+                                      C->stream));
+    HIPCHK(hipEventRecord(C->ev1, C->stream));
+    if (diag_every > 0) {
+        const int64_t last = ((C->step + nsteps) / diag_every) * diag_every;
+        if (last > C->step) C->diag_step = last;
+    }
+    C->step += nsteps;
+    C->parity = (C->parity + nsteps) & 1;
+    C->last_steps = nsteps;
+    int err = 0;
+    HIPCHK(hipMemcpyAsync(&err, C->d_bar + 1, sizeof err, hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    if (err) return fail(IBLBM_EDEVICE, "step_persistent: a grid barrier timed out");
     return 0;
 }
 

@@ -89,7 +89,9 @@ struct Inst {
     const double *Lx, *Ly, *ud;
     // particles of a DF / MDF window (markers concatenated in particle order); with more
     // than one, contributions are spread per particle and the MDF iteration is coupled
-    int np_part, pad_np;
+    int np_part;
+    int df_multi;  // a DF window of a particle list (any length): per-particle closures, marker
+                   // forces kept (multi_particle.py:347-421)
     int poff[IBM_MAXP + 1];     // first marker of each particle
     double plarea[IBM_MAXP];    // arc length per marker of each particle
     IbmScratch s;
@@ -105,6 +107,16 @@ struct Inst {
     double *side, *edge_fs;
     const double *tgc, *tgd;
     int nring, tg_nt;
+};
+
+// Arguments of the persistent step launch: the grid barrier's arrival counter and error flag,
+// the number of steps, the diagnostic interval, the step counter and parity it starts from.
+struct PersistArgs {
+    unsigned *bar;
+    int *err;
+    int nsteps, diag_every;
+    long long step0;
+    int parity0, pad;
 };
 
 struct LaunchTable {

@@ -342,8 +342,9 @@ class Sedimentation:
                 om = min(float(om), 1.0)
             self.omega = om
             # several particles iterate coupled from min(0.8 omega, 0.8)
-            # (multi_particle.py:32-36, :85-86)
-            case.mdf_omega = min(om * 0.8, 0.8) if self.multi else om
+            # (multi_particle.py:32-36, :85-86); a one-element particle list runs the
+            # single-particle MDF with the estimate itself (multi_particle.py:432-448)
+            case.mdf_omega = min(om * 0.8, 0.8) if len(self.particles) > 1 else om
             if mdf_omega is not None:
                 case.mdf_omega = float(mdf_omega)
         self.k = self.ctx.add(case)
@@ -404,7 +405,10 @@ class Sedimentation:
             Lbt = sum(p.Lb for p in self.particles)
             return marker_forces_to_particles(self.particles,
                                               self.ctx.download_marker_force(self.k, Lbt), cfg, st)
-        if self.multi:  # coupled MDF: each particle's own field (multi_particle.py:302-316)
+        if self.multi and len(self.particles) > 1:
+            # coupled MDF: each particle's own field (multi_particle.py:302-316); one particle
+            # runs the single-particle MDF, whose force and torque are the window fib's sums
+            # (multi_particle.py:432-448; no published run holds this case)
             for i, p in enumerate(self.particles):
                 out.append(self._window_force(*self.ctx.download_particle_fib(self.k, i), p))
             return out

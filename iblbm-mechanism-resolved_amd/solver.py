@@ -119,6 +119,12 @@ class Context:
         fn = self.lib.iblbm_graph_run if graph else self.lib.iblbm_step
         check(fn(self.h, int(n), int(diag_every)), "step")
 
+    def step_persistent(self, n, diag_every=0):
+        """n steps in one persistent launch of the fused step kernel (grid barrier between
+        steps); returns when they are done.  Needs the fused launch, one collision model, the
+        tile row layout and no boundary kernel (ValueError otherwise)."""
+        check(self.lib.iblbm_step_persistent(self.h, int(n), int(diag_every)), "step_persistent")
+
     def sync(self):
         check(self.lib.iblbm_sync(self.h), "sync")
 
@@ -411,13 +417,20 @@ class FixedCylinder:
         self._markers_due = None
         self.ttt = 0
 
-    def advance(self, n, diag_every=0, graph=False):
+    def advance(self, n, diag_every=0, graph=False, persistent=False):
+        """n steps: launched per step (replayed as a graph with graph=True) or, with
+        persistent=True, as one persistent launch (fixed bodies on the fused path)."""
         if n <= 0:
             return
         if not (self.tg_markers or self.oscillating):
-            self.ctx.step(n, diag_every, graph)
+            if persistent:
+                self.ctx.step_persistent(n, diag_every)
+            else:
+                self.ctx.step(n, diag_every, graph)
             self.ttt += n
             return
+        if persistent:
+            raise ValueError("moving markers are set between steps: no persistent launch")
         cfg, st = self.cfg, self.st
         for _ in range(n):  # one step at a time, with the host's marker updates around it
             if self.tg_markers:

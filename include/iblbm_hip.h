@@ -190,6 +190,13 @@ int iblbm_sync(void *ctx);
 
 /* Captures nsteps steps into a hipGraph and replays it (the step counter advances). */
 int iblbm_graph_run(void *ctx, int nsteps, int diag_every);
+/* Runs nsteps steps as ONE persistent cooperative launch of the fused step kernel, with a
+ * grid-wide barrier between steps (replaces the per-step loop body of
+ * iblbm/runtime/loop.py:59-87 like iblbm_step; no reference counterpart for the form).  Needs
+ * every window on the fused path, one collision model, the tile row layout and no boundary
+ * kernel (IBLBM_EINVAL otherwise); returns after the launch has finished (IBLBM_EDEVICE if a
+ * grid barrier timed out).  Results equal iblbm_step's bit for bit. */
+int iblbm_step_persistent(void *ctx, int nsteps, int diag_every);
 /* on = 1 (default): a step runs as one fused launch when every marker window of the context
  * qualifies for it, otherwise the marker-window workgroups run on a second stream concurrently
  * with the bulk tiles of the same step; on = 0: window and bulk kernels run in order on one
@@ -388,8 +395,10 @@ int iblbm_op_spread(const double *Lfx, const double *Lfy, const double *Lx, cons
 #define IBLBM_MLP_OUT 2
 #define IBLBM_MLP_NP 4866   /* W1[8][64] b1[64] W2[64][64] b2[64] W3[64][2] b3[2] */
 #define IBLBM_MLP_NORM 18   /* feature mean[8], feature scale[8], target scale[2] */
-/* Sets the mode of an MDF instance whose window takes the fused path; the first non-zero
- * mode allocates the predictor with room for `capacity` training samples. */
+/* Sets the mode of a one-particle MDF instance of at most 512 markers (lb_capacity), whose
+ * window runs in the fused step or in the window kernel of the two-kernel step (the same
+ * arithmetic in both); the first non-zero mode allocates the predictor with room for
+ * `capacity` training samples.  IBLBM_EINVAL for several particles or a larger capacity. */
 int iblbm_predictor_mode(void *ctx, int id, int mode, int64_t capacity);
 /* Training samples recorded so far (capped at the capacity). */
 int iblbm_predictor_samples(void *ctx, int id, int64_t *n);
