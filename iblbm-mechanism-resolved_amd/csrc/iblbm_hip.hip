@@ -635,21 +635,24 @@ __global__ __launch_bounds__(NTF) void k_step(LaunchTable T) {
 
 // Grid-wide barrier of a cooperative launch (every workgroup resident): after a workgroup
 // barrier (each wavefront's stores complete), thread 0 arrives with an agent-scope release
-// (the step's writes written back for every XCD) and waits with agent-scope acquires until
-// all workgroups of this step have arrived.  The wait is bounded: past the spin limit it
-// records an error and goes on, so the launch always ends.
+// (the step's writes written back for every XCD), polls the counter with relaxed agent-scope
+// loads until all workgroups of this step have arrived, then takes one agent-scope acquire
+// fence (an acquire on every poll would invalidate the caches under the workgroups still
+// working).  The wait is bounded: past the spin limit it records an error and goes on, so the
+// launch always ends.
 __device__ inline void grid_barrier(unsigned *bar, unsigned target, int *err) {
     __syncthreads();
     if (threadIdx.x == 0) {
         __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         int k = 0;
-        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             if (++k > (1 << 22)) {
                 __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
             }
-            __builtin_amdgcn_s_sleep(2);
+            __builtin_amdgcn_s_sleep(1);
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
 }

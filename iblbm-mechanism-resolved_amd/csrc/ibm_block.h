@@ -417,97 +417,6 @@ __device__ inline void gather2(const IbmPtrs &P, int ti, double &ex, double &ey)
     ey = sy;
 }
 
-// Spreads of up to GB touched nodes per thread (ti = base + t + u NTI) as gathers in np.add.at
-// order, the nodes' lists walked in lockstep: each round issues the next four entries (marker,
-// weight) of every node, then their marker forces, so a round costs two memory latencies for
-// all the thread's nodes.  Each node's sum runs over its own list in order, exactly as
-// gather2 forms it.
-constexpr int GB = 4;   // touched nodes per thread and batch
-constexpr int GE = 4;   // list entries per node and round
-__device__ inline void gather_nodes(const IbmPtrs &P, int base, int nT, double (&ex)[GB],
-                                    double (&ey)[GB]) {
-    int b[GB], e[GB], len = 0;
-#pragma unroll
-    for (int u = 0; u < GB; ++u) {
-        const int ti = base + (int)threadIdx.x + u * NTI;
-        b[u] = ti < nT ? P.toff[ti] : 0;
-        e[u] = ti < nT ? P.toff[ti + 1] : 0;
-        ex[u] = 0.0;
-        ey[u] = 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < GB; ++u) len = max(len, e[u] - b[u]);
-    for (int x = 0; x < len; x += GE) {
-        int k[GB][GE];
-        double w[GB][GE];
-#pragma unroll
-        for (int u = 0; u < GB; ++u)
-#pragma unroll
-            for (int j = 0; j < GE; ++j) {
-                const int idx = b[u] + x + j;
-                const bool v = idx < e[u];
-                k[u][j] = v ? P.gk[idx] : 0;
-                w[u][j] = v ? P.gw[idx] : 0.0;
-            }
-        double fa[GB][GE], fb[GB][GE];
-#pragma unroll
-        for (int u = 0; u < GB; ++u)
-#pragma unroll
-            for (int j = 0; j < GE; ++j) {
-                const bool v = b[u] + x + j < e[u];
-                fa[u][j] = v ? P.Lfx[k[u][j]] : 0.0;
-                fb[u][j] = v ? P.Lfy[k[u][j]] : 0.0;
-            }
-#pragma unroll
-        for (int u = 0; u < GB; ++u)
-#pragma unroll
-            for (int j = 0; j < GE; ++j)
-                if (b[u] + x + j < e[u]) {
-                    ex[u] += fa[u][j] * w[u][j];
-                    ey[u] += fb[u][j] * w[u][j];
-                }
-    }
-}
-
-// The node side of one DF pass or MDF sub-step (mdf.py:197-200) for every touched node: the
-// spread (gather_nodes), then fib (DF) or the accumulated fib and working velocity (MDF,
-// relaxation omega).  Each batch reads all its nodes' values before it stores any.
-__device__ inline void spread_update(const IbmPtrs &P, int nT, bool mdf, double omega) {
-    for (int base = 0; base < nT; base += GB * NTI) {
-        double fbx[GB], fby[GB], uwx[GB], uwy[GB], vs[GB];
-        if (mdf) {
-#pragma unroll
-            for (int u = 0; u < GB; ++u) {
-                const int ti = base + (int)threadIdx.x + u * NTI;
-                if (ti < nT) {
-                    fbx[u] = P.fbx[ti];
-                    fby[u] = P.fby[ti];
-                    uwx[u] = P.uwx[ti];
-                    uwy[u] = P.uwy[ti];
-                    vs[u] = P.vs[ti];
-                }
-            }
-        }
-        double ex[GB], ey[GB];
-        gather_nodes(P, base, nT, ex, ey);
-#pragma unroll
-        for (int u = 0; u < GB; ++u) {
-            const int ti = base + (int)threadIdx.x + u * NTI;
-            if (ti >= nT) continue;
-            if (mdf) {
-                const double fx = ex[u] * omega, fy = ey[u] * omega;
-                P.fbx[ti] = fbx[u] + fx;
-                P.fby[ti] = fby[u] + fy;
-                P.uwx[ti] = uwx[u] + fx * vs[u];
-                P.uwy[ti] = uwy[u] + fy * vs[u];
-            } else {
-                P.fbx[ti] = ex[u];
-                P.fby[ti] = ey[u];
-            }
-        }
-    }
-}
-
 // Touched-node inputs of the core from box-slot arrays S.ro, S.ux0, S.uy0 (operator path).
 __device__ inline void ibm_touched_init(const Inst &I, int nT, const IbmPtrs &P) {
     const IbmScratch &S = I.s;
@@ -786,7 +695,21 @@ __device__ void ibm_core(Inst &I, int nT, const IbmPtrs &P, double *red, bool st
                 }
             }
             // spread as a gather in np.add.at order (df.py:269-283)
-            spread_update(P, nT, mdf, omega);
+            for (int ti = t; ti < nT; ti += NTI) {
+                double ex, ey;
+                gather2(P, ti, ex, ey);
+                if (mdf) {  // mdf.py:197-200
+                    const double fx = ex * omega, fy = ey * omega;
+                    const double vs = P.vs[ti];
+                    P.fbx[ti] = P.fbx[ti] + fx;
+                    P.fby[ti] = P.fby[ti] + fy;
+                    P.uwx[ti] = P.uwx[ti] + fx * vs;
+                    P.uwy[ti] = P.uwy[ti] + fy * vs;
+                } else {
+                    P.fbx[ti] = ex;
+                    P.fby[ti] = ey;
+                }
+            }
             __threadfence_block();
             __syncthreads();
             if (mdf) {
@@ -994,9 +917,15 @@ __device__ void ibm_core_learned(Inst &I, int nT, const IbmPtrs &P, double *red,
         facy = pfy;
         __threadfence_block();
         __syncthreads();
-        // weight 1: fib = the spread, U = U + spread dt / (2 rho) (fib starts at +0.0, and a
-        // spread is a sum started at +0.0, never -0.0, so 0 + x * 1 = x exactly)
-        spread_update(P, nT, true, 1.0);
+        for (int ti = t; ti < nT; ti += NTI) {
+            double sx, sy;
+            gather2(P, ti, sx, sy);
+            const double vs = P.vs[ti];
+            P.fbx[ti] = sx;
+            P.fby[ti] = sy;
+            P.uwx[ti] = P.uwx[ti] + sx * vs;
+            P.uwy[ti] = P.uwy[ti] + sy * vs;
+        }
         __threadfence_block();
         __syncthreads();
     }
@@ -1037,10 +966,17 @@ __device__ void ibm_core_learned(Inst &I, int nT, const IbmPtrs &P, double *red,
             lastx = 0.0;
             lasty = 0.0;
         }
-        if (act == 0) {
-            spread_update(P, nT, true, omega);  // mdf.py:197-200
-        } else {
-            for (int ti = t; ti < nT; ti += NTI) {  // restart from the uncorrected velocity
+        for (int ti = t; ti < nT; ti += NTI) {
+            if (act == 0) {  // mdf.py:197-200
+                double ex2, ey2;
+                gather2(P, ti, ex2, ey2);
+                const double fx = ex2 * omega, fy = ey2 * omega;
+                const double vs = P.vs[ti];
+                P.fbx[ti] = P.fbx[ti] + fx;
+                P.fby[ti] = P.fby[ti] + fy;
+                P.uwx[ti] = P.uwx[ti] + fx * vs;
+                P.uwy[ti] = P.uwy[ti] + fy * vs;
+            } else {  // restart from the uncorrected velocity
                 const int sl = S.tslot[ti];
                 P.fbx[ti] = 0.0;
                 P.fby[ti] = 0.0;
@@ -1107,36 +1043,22 @@ __device__ void ibm_role_impl(Inst &I, int parity, int diag, double *red, IbmSha
     load_marker_regs<A>(I, M);  // in flight during P1
     if constexpr (LDS) ibm_stage_lists(I, nT, sh);
 
-    // P1: stream + close the touched nodes, macroscopic moments, core inputs; a thread's nodes
-    // in batches of GB, every pull of a batch issued before its first store
-    for (int base = 0; base < nT; base += GB * NTI) {
-        int sl[GB];
+    // P1: stream + close the touched nodes, macroscopic moments, core inputs
+    for (int ti = t; ti < nT; ti += NTI) {
+        const int sl = S.tslot[ti];
+        double fs[9];
+        fstar_at(Asrc, I, I.bx0 + sl % I.bw, I.by0 + sl / I.bw, fs);
 #pragma unroll
-        for (int u = 0; u < GB; ++u) {
-            const int ti = base + t + u * NTI;
-            sl[u] = ti < nT ? S.tslot[ti] : 0;
-        }
-        double fs[GB][9];
-#pragma unroll
-        for (int u = 0; u < GB; ++u)
-            if (base + t + u * NTI < nT)
-                fstar_at(Asrc, I, I.bx0 + sl[u] % I.bw, I.by0 + sl[u] / I.bw, fs[u]);
-#pragma unroll
-        for (int u = 0; u < GB; ++u) {
-            const int ti = base + t + u * NTI;
-            if (ti >= nT) continue;
-#pragma unroll
-            for (int q = 0; q < 9; ++q) S.fs[q * I.box_cap + sl[u]] = fs[u][q];
-            double r, u0, u1;
-            macro9(fs[u], inc, r, u0, u1);
-            S.ro[sl[u]] = r;
-            S.ux0[sl[u]] = u0;
-            S.uy0[sl[u]] = u1;
-            P.uwx[ti] = u0;
-            P.uwy[ti] = u1;
-            P.rot[ti] = r;
-            P.vs[ti] = dt / (2.0 * r);  // mdf.py:117
-        }
+        for (int q = 0; q < 9; ++q) S.fs[q * I.box_cap + sl] = fs[q];
+        double r, u0, u1;
+        macro9(fs, inc, r, u0, u1);
+        S.ro[sl] = r;
+        S.ux0[sl] = u0;
+        S.uy0[sl] = u1;
+        P.uwx[ti] = u0;
+        P.uwy[ti] = u1;
+        P.rot[ti] = r;
+        P.vs[ti] = dt / (2.0 * r);  // mdf.py:117
     }
     __threadfence_block();
     __syncthreads();
@@ -1148,30 +1070,27 @@ __device__ void ibm_role_impl(Inst &I, int parity, int diag, double *red, IbmSha
     // P3: velocity correction (DF/MDF), equilibrium, collision, store
     const bool correct = I.ibm == 1 || I.ibm == 2;
     double rho_acc = 0.0, fxs = 0.0, fys = 0.0, trq = 0.0;
-    // a thread's nodes in batches of GB, every load of a batch issued before its first store
-    for (int base = 0; base < nT; base += GB * NTI) {
-        int sl[GB];
+    for (int t0 = t; t0 < nT; t0 += 2 * NTI) {
+        const int t1 = t0 + NTI;
+        const bool h1 = t1 < nT;
+        const int sl0 = S.tslot[t0], sl1 = h1 ? S.tslot[t1] : sl0;
+        double fa[9], fb[9], ra, rb = 1.0, ua0, ua1, ub0 = 0.0, ub1 = 0.0;
 #pragma unroll
-        for (int u = 0; u < GB; ++u) {
-            const int ti = base + t + u * NTI;
-            sl[u] = ti < nT ? S.tslot[ti] : 0;
+        for (int q = 0; q < 9; ++q) fa[q] = S.fs[q * I.box_cap + sl0];
+        ra = S.ro[sl0];
+        ua0 = S.ux0[sl0];
+        ua1 = S.uy0[sl0];
+        if (h1) {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) fb[q] = S.fs[q * I.box_cap + sl1];
+            rb = S.ro[sl1];
+            ub0 = S.ux0[sl1];
+            ub1 = S.uy0[sl1];
         }
-        double fsb[GB][9], rb[GB], ub0[GB], ub1[GB], fbx[GB], fby[GB];
-#pragma unroll
-        for (int u = 0; u < GB; ++u) {
-            const int ti = base + t + u * NTI;
-            if (ti >= nT) continue;
-#pragma unroll
-            for (int q = 0; q < 9; ++q) fsb[u][q] = S.fs[q * I.box_cap + sl[u]];
-            rb[u] = S.ro[sl[u]];
-            ub0[u] = S.ux0[sl[u]];
-            ub1[u] = S.uy0[sl[u]];
-            fbx[u] = correct ? P.fbx[ti] : 0.0;
-            fby[u] = correct ? P.fby[ti] : 0.0;
-        }
-        auto finish = [&](int sl, const double *fs, double r, double u0, double u1, double fx,
-                          double fy) {
+        auto finish = [&](int ti, int sl, const double *fs, double r, double u0, double u1) {
             const int i = I.bx0 + sl % I.bw, j = I.by0 + sl / I.bw;
+            const double fx = correct ? P.fbx[ti] : 0.0;
+            const double fy = correct ? P.fby[ti] : 0.0;
             if (correct) {
                 if (inc) {
                     u0 = u0 + 0.5 * fx * dt;
@@ -1203,10 +1122,8 @@ __device__ void ibm_role_impl(Inst &I, int parity, int diag, double *red, IbmSha
                 trq += ((double)i - I.cxc) * fy - ((double)j - I.cyc) * fx;
             }
         };
-#pragma unroll
-        for (int u = 0; u < GB; ++u)
-            if (base + t + u * NTI < nT)
-                finish(sl[u], fsb[u], rb[u], ub0[u], ub1[u], fbx[u], fby[u]);
+        finish(t0, sl0, fa, ra, ua0, ua1);
+        if (h1) finish(t1, sl1, fb, rb, ub0, ub1);
     }
     if (diag) {
         const double rs = block_reduce(rho_acc, red, false);
