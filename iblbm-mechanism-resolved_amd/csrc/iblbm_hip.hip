@@ -598,7 +598,30 @@ __device__ __forceinline__ void step_body(const LaunchTable &T, double *red, Fus
     constexpr bool PIPE = sizeof(S) == 4 && (MODEL == 0 || MODEL == 1);
     const int first = ((int)blockIdx.x - T.n_fast) * (NTF / 64) + (int)(threadIdx.x >> 6);
     const int nw = ((int)gridDim.x - T.n_fast) * (NTF / 64);
-    if (T.blocked) {  // one block of adjacent rows per wavefront
+    if (T.blocked < 0) {
+        // one block of adjacent rows per workgroup, its wavefronts taking the next rows from a
+        // counter in LDS: the SIMDs issue their oldest wavefronts first, so equal shares per
+        // wavefront leave the last step of a workgroup to one or two wavefronts per SIMD
+        // (fewer loads in flight); a shared counter keeps every wavefront busy to the end
+        const int n_rows = T.n_tiles * TILE_Y;
+        const int b = (int)blockIdx.x - T.n_fast, nb = (int)gridDim.x - T.n_fast;
+        const int r0 = (int)((int64_t)b * n_rows / nb);
+        const int r1 = (int)((int64_t)(b + 1) * n_rows / nb);
+        int *queue = reinterpret_cast<int *>(red);  // bulk workgroups use no reductions
+        if (threadIdx.x == 0) *queue = r0;
+        __syncthreads();
+        const int grab = T.nrows >= 2 ? 2 : 1;
+        for (;;) {
+            int r = 0;
+            if ((threadIdx.x & 63) == 0) r = atomicAdd(queue, grab);
+            r = __builtin_amdgcn_readfirstlane(__shfl(r, 0, 64));
+            if (r >= r1) break;
+            if (grab == 2)
+                bulk_rows<S, MODEL, 2, EB, FL>(T, r, 1, min(r + 2, r1));
+            else
+                bulk_rows<S, MODEL, 1, EB, FL>(T, r, 1, min(r + 1, r1));
+        }
+    } else if (T.blocked) {  // one block of adjacent rows per wavefront
         const int n_rows = T.n_tiles * TILE_Y;
         int r0, r1;
         if (T.blocked == 1) {  // every wavefront, block sizes differing by one
@@ -2047,8 +2070,9 @@ int iblbm_set_schedule(void *p, const iblbm_schedule *in) {
         return fail(IBLBM_EINVAL, "set_schedule: rows in flight must be 1, 2, 3 (one row, "
                                   "pipelined) or 4");
     if (in->bulk_waves < 0) return fail(IBLBM_EINVAL, "set_schedule: bulk_waves < 0");
-    if (in->step_order < 0 || in->step_order > 64)
-        return fail(IBLBM_EINVAL, "set_schedule: step_order must be in [0, 64]");
+    if (in->step_order < -1 || in->step_order > 64)
+        return fail(IBLBM_EINVAL, "set_schedule: step_order must be -1 (workgroup queue) or in "
+                                  "[0, 64]");
     if (in->row_layout != 0 && in->row_layout != 1)
         return fail(IBLBM_EINVAL, "set_schedule: row_layout must be 0 (tiles) or 1 (flat)");
     if (in->store_mode != 0 && in->store_mode != 1)
@@ -2091,9 +2115,9 @@ int iblbm_autotune(void *p, int steps, double *best_ms, int *n_tried) {
     for (int st : stores)
         for (int lay : layouts) {
             if (C->n_fast == C->n_ibm && st == 0)  // the fused step stores normally
-                for (int order : {0, 1, 6, 8})
+                for (int order : {-1, 0, 1, 6, 8})  // -1: the workgroup row queue
                     for (int r : {1, 2, 4})
-                        if (r < 4 || f32_ctx)
+                        if ((r < 4 || f32_ctx) && (order >= 0 || r <= 2))
                         for (int dl : {0, 100, 200})  // bulk start delay: 0, 1, 2 us
                             if (dl == 0 || C->n_fast > 0)
                                 cand.push_back({IBLBM_MODE_FUSED, 256, 1, 0, r, order, lay, st,

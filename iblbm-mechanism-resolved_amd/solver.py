@@ -145,18 +145,20 @@ class Context:
         mode = {v: k for k, v in _abi.MODES.items()}[sc.mode]
         return {"mode": mode, "bulk_threads": sc.bulk_threads, "bulk_rows": sc.bulk_rows,
                 "bulk_waves": sc.bulk_waves, "step_rows": sc.step_rows,
-                "step_order": {0: "stride", 1: "blocked"}.get(sc.step_order, sc.step_order),
+                "step_order": {-1: "queue", 0: "stride", 1: "blocked"}.get(sc.step_order,
+                                                                          sc.step_order),
                 "row_layout": ROW_LAYOUTS[sc.row_layout],
                 "stores": STORE_MODES[sc.store_mode], "bulk_delay_us": sc.bulk_delay / 100.0}
 
     def set_schedule(self, mode="auto", bulk_threads=256, bulk_rows=1, bulk_waves=0,
                      step_rows=2, step_order="stride", row_layout="tiles", stores="normal",
                      bulk_delay_us=0.0):
-        """step_order: "stride", "blocked" (one block per wavefront) or a block size k >= 2;
+        """step_order: "stride", "blocked" (one block per wavefront), a block size k >= 2, or
+        "queue" (one block per workgroup, its wavefronts taking rows from a counter in LDS);
         row_layout: "tiles" (64-column tiles of lattice rows) or "flat" (64 consecutive
         row-major nodes per bulk row); stores: "normal" or "nontemporal"; bulk_delay_us: the
         fused launch's bulk start delay (0 to 10 us, in steps of 10 ns)."""
-        order = {"stride": 0, "blocked": 1}.get(step_order, step_order)
+        order = {"queue": -1, "stride": 0, "blocked": 1}.get(step_order, step_order)
         sc = _abi.Schedule(_abi.MODES[mode], bulk_threads, bulk_rows, bulk_waves, step_rows,
                            int(order), ROW_LAYOUTS.index(row_layout),
                            STORE_MODES.index(stores), int(round(bulk_delay_us * 100)))

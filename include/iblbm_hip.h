@@ -230,7 +230,9 @@ typedef struct {
     int32_t step_order;   /* fused launch: 0 rows by grid stride; 1 one block of adjacent rows
                            * per bulk wavefront, all wavefronts; k >= 2 blocks of k rows (the
                            * wavefronts past the last block idle, leaving memory bandwidth to
-                           * the windows); at most 64 */
+                           * the windows); at most 64; -1 one block per bulk workgroup, its
+                           * wavefronts taking the next 1 or 2 rows (step_rows) from a counter
+                           * in LDS */
     int32_t row_layout;   /* 0: a bulk row is a 64-column tile of one lattice row; 1: a bulk
                            * row is 64 consecutive nodes in row-major node order (no part-
                            * empty tile at the end of a lattice row).  1 applies to contexts
