@@ -444,7 +444,7 @@ MATRIX_WORKLOAD = ("72 fixed cylinders: Re {20,40,100,200} x {DF,MDF,DFC} x {hat
 def parse_schedule(text):
     """--schedule text -> Context.set_schedule keywords."""
     f = text.split(",")
-    order = f[5] if f[5] in ("stride", "blocked") else int(f[5])
+    order = f[5] if f[5] in ("stride", "blocked", "queue") else int(f[5])
     return dict(mode=f[0], bulk_threads=int(f[1]), bulk_rows=int(f[2]), bulk_waves=int(f[3]),
                 step_rows=int(f[4]), step_order=order,
                 row_layout=f[6] if len(f) > 6 else "tiles",
