@@ -483,6 +483,109 @@ __device__ __attribute__((always_inline)) void bulk_rows_pipe(const LaunchTable 
     }
 }
 
+// 4 bytes per lane from global memory straight into LDS (global_load_lds_dword): lane l of the
+// calling wavefront writes lds + 4 l.  The copy counts in vmcnt; it is in LDS once the wave's
+// vmcnt has reached it.
+__device__ __attribute__((always_inline)) void lds_dma4(const void *src, void *lds) {
+    typedef __attribute__((address_space(1))) void GVoid;
+    typedef __attribute__((address_space(3))) void LVoid;
+    __builtin_amdgcn_global_load_lds((GVoid *)src, (LVoid *)lds, 4, 0, 0);
+}
+// An interior row's 9 pull planes into LDS, plane q at buf + 64 q (value of lane l at
+// buf[64 q + l]).  FP64: a tile row's 64 values of a plane are contiguous (512 bytes), copied
+// as two dwords per lane; FP32: one value per lane from its own address (any row layout).
+template <typename S>
+__device__ __attribute__((always_inline)) void row_to_lds(const RowRun &R, const TileNode &n,
+                                                         int lane, S *buf) {
+    const S *__restrict__ A = (const S *)R.A;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        if constexpr (sizeof(S) == 8) {
+            const char *src = (const char *)(A + q * R.qs + (int64_t)(n.j - EYi[q]) * R.pitch +
+                                             (n.i - lane - EXi[q]));
+            char *dst = (char *)(buf + 64 * q);
+            lds_dma4(src + 4 * lane, dst);
+            lds_dma4(src + 256 + 4 * lane, dst + 256);
+        } else {
+            lds_dma4(&A[q * R.qs + (int64_t)(n.j - EYi[q]) * R.pitch + (n.i - EXi[q])],
+                     buf + 64 * q);
+        }
+    }
+}
+
+// Rows taken one at a time from the workgroup's counter (below r1), with the next row's
+// populations copied into LDS (global_load_lds) while the current row computes: the copy
+// holds no registers, so FP64 gets the look-ahead that its registers cannot hold.  Interior
+// rows only are copied ahead; an edge row takes its batched (EB) or per-lane pull at its turn.
+// lds: this wavefront's two row buffers of 9 x 64 values.
+template <typename S, int MODEL, bool EB, int FL>
+__device__ __attribute__((always_inline)) void bulk_rows_lds(const LaunchTable &T, int *queue,
+                                                            int r1, S *lds) {
+    const int lane = (int)threadIdx.x & 63;
+    auto take = [&]() {
+        int r = 0;
+        if (lane == 0) r = atomicAdd(queue, 1);
+        return __builtin_amdgcn_readfirstlane(__shfl(r, 0, 64));
+    };
+    RowRun R[2];
+    R[0].tb_lo = R[1].tb_lo = 1;
+    R[0].tb_hi = R[1].tb_hi = 0;
+    TileNode nn[2];
+    int rr[2];
+    bool pre[2];
+    rr[0] = take();
+    if (rr[0] >= r1) return;
+    nn[0] = row_node<FL>(T, R[0], rr[0], lane);
+    pre[0] = nn[0].interior;
+    if (pre[0]) row_to_lds<S>(R[0], nn[0], lane, lds);
+    auto body = [&](auto cc) -> bool {
+        constexpr int C = decltype(cc)::value, D = 1 - C;
+        S a[9];
+        if (pre[C]) {
+            // every copy this wavefront issued has landed (the only ones outstanding are this
+            // row's, and the previous row's stores)
+            __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) a[q] = lds[C * 9 * 64 + 64 * q + lane];
+        }
+        const int rn = take();
+        const bool more = rn < r1;
+        if (more) {
+            rr[D] = rn;
+            nn[D] = row_node<FL>(T, R[D], rn, lane);
+            pre[D] = nn[D].interior;
+            if (pre[D]) row_to_lds<S>(R[D], nn[D], lane, lds + D * 9 * 64);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        double rho = 0.0;
+        if (pre[C]) {
+            const double v = bulk_finish<MODEL>(*R[C].I, R[C], nn[C].i, nn[C].j, a, nn[C].mine);
+            rho = nn[C].mine ? v : 0.0;
+        } else if constexpr (EB) {
+            int i0, j0;
+            lattice_ij<FL>(R[C], nn[C], i0, j0);
+            bulk_pull_edge(R[C], i0, j0, a);
+            edge_fix(R[C], i0, j0, a);
+            const double v = bulk_finish<MODEL>(*R[C].I, R[C], nn[C].i, nn[C].j, a, nn[C].mine);
+            rho = nn[C].mine ? v : 0.0;
+        } else if (nn[C].mine) {
+            int i0, j0;
+            lattice_ij<FL>(R[C], nn[C], i0, j0);
+            bulk_pull(*R[C].I, T.parity, i0, j0, a);
+            rho = bulk_finish<MODEL>(*R[C].I, R[C], nn[C].i, nn[C].j, a, true);
+        }
+        if (T.diag) {
+            const double w = wave_sum(rho);
+            if (lane == 0) T.rho_partial[rr[C]] = w;
+        }
+        return more;
+    };
+    while (body(std::integral_constant<int, 0>{}) && body(std::integral_constant<int, 1>{})) {
+    }
+}
+
 // Bulk kernel of the two-kernel step.  Any workgroup size up to 1024 (a tile of 64 x
 // blockDim / 64 rows per workgroup); the grid either covers every row once or walks them
 // with a grid stride (the schedule picks both).
@@ -570,7 +673,11 @@ __global__ __launch_bounds__(256) void k_edge(LaunchTable T) {
 // One whole time step in one launch, for contexts whose windows all take the fused path:
 // workgroups [0, n_fast) run the windows (fused_step.h), the others run four bulk tiles each
 // (one node per thread).  One launch per step leaves no cross-stream wait between steps.
-template <typename S, int MODEL, bool PRED, int FL>
+// LA: the row queue's LDS look-ahead form (rows = 3) is compiled in; it is left out where its
+// registers would add spills to the kernel (the central-moment and mixed collisions, and the
+// persistent launch), which then takes two rows at a time.
+template <typename S, int MODEL, bool PRED, int FL,
+          bool LA = !PRED && (MODEL == 0 || MODEL == 1)>
 __device__ __forceinline__ void step_body(const LaunchTable &T, double *red, FusedShared &sh) {
     if ((int)blockIdx.x < T.n_fast) {
         __builtin_amdgcn_s_setprio(3);  // issue ahead of bulk wavefronts sharing the CU
@@ -610,6 +717,18 @@ __device__ __forceinline__ void step_body(const LaunchTable &T, double *red, Fus
         int *queue = reinterpret_cast<int *>(red);  // bulk workgroups use no reductions
         if (threadIdx.x == 0) *queue = r0;
         __syncthreads();
+        if constexpr (LA) {
+            if (T.nrows == 3) {
+                // one row at a time, the next row's populations copied into LDS ahead: the
+                // window's shared memory is unused in a bulk workgroup, and holds two row
+                // buffers per wavefront
+                static_assert(sizeof(FusedShared) >= NTF / 64 * 2 * 9 * 64 * sizeof(double),
+                              "row buffers");
+                S *buf = reinterpret_cast<S *>(&sh) + (threadIdx.x >> 6) * 2 * 9 * 64;
+                bulk_rows_lds<S, MODEL, EB, FL>(T, queue, r1, buf);
+                return;
+            }
+        }
         const int grab = T.nrows >= 2 ? 2 : 1;
         for (;;) {
             int r = 0;
@@ -691,7 +810,7 @@ __global__ __launch_bounds__(NTF) void k_step_persist(LaunchTable T, PersistArgs
         Ts.parity = (A.parity0 + s) & 1;
         const long long stepno = A.step0 + s + 1;
         Ts.diag = (A.diag_every > 0 && stepno % A.diag_every == 0) ? 1 : 0;
-        step_body<S, MODEL, PRED, FL>(Ts, red, sh);
+        step_body<S, MODEL, PRED, FL, false>(Ts, red, sh);
         if (s + 1 < A.nsteps) grid_barrier(A.bar, gridDim.x * (unsigned)(s + 1), A.err);
     }
 }
@@ -2115,9 +2234,12 @@ int iblbm_autotune(void *p, int steps, double *best_ms, int *n_tried) {
     for (int st : stores)
         for (int lay : layouts) {
             if (C->n_fast == C->n_ibm && st == 0)  // the fused step stores normally
-                for (int order : {-1, 0, 1, 6, 8})  // -1: the workgroup row queue
-                    for (int r : {1, 2, 4})
-                        if ((r < 4 || f32_ctx) && (order >= 0 || r <= 2))
+                // -1: the workgroup row queue, whose rows = 3 is one row with the next one
+                // copied into LDS ahead
+                for (int order : {-1, 0, 1, 6, 8})
+                    for (int r : {1, 2, 3, 4})
+                        if ((r < 4 || f32_ctx) && (order >= 0 || r <= 3) &&
+                            (r != 3 || order < 0))
                         for (int dl : {0, 100, 200})  // bulk start delay: 0, 1, 2 us
                             if (dl == 0 || C->n_fast > 0)
                                 cand.push_back({IBLBM_MODE_FUSED, 256, 1, 0, r, order, lay, st,

@@ -232,7 +232,10 @@ typedef struct {
                            * wavefronts past the last block idle, leaving memory bandwidth to
                            * the windows); at most 64; -1 one block per bulk workgroup, its
                            * wavefronts taking the next 1 or 2 rows (step_rows) from a counter
-                           * in LDS */
+                           * in LDS; with step_rows = 3 one row at a time, the next row's
+                           * populations copied into the workgroup's LDS while the current
+                           * row computes (BGK and TRT contexts without the predictor; others
+                           * take 2 rows) */
     int32_t row_layout;   /* 0: a bulk row is a 64-column tile of one lattice row; 1: a bulk
                            * row is 64 consecutive nodes in row-major node order (no part-
                            * empty tile at the end of a lattice row).  1 applies to contexts

@@ -66,6 +66,8 @@ def test_mixed_ensemble_matches_reference_runs(schedule):
                                       dict(step_order=7), dict(step_order=8, bulk_delay_us=2.0),
                                       dict(step_rows=1, step_order="queue"),
                                       dict(step_rows=2, step_order="queue", bulk_delay_us=1.0),
+                                      dict(step_rows=3, step_order="queue"),
+                                      dict(step_rows=3, step_order="queue", bulk_delay_us=1.0),
                                       "autotune"])
 def test_fused_ensemble_matches_reference_runs(schedule):
     # the cases whose window qualifies for the fused launch: one kernel per step for all

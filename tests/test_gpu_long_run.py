@@ -91,3 +91,32 @@ def test_benchmark_configs_match_oracle(which, steps, perturb):
           f"{np.array_equal(f, ref.f_post)}), Cd {cd[-1]:.10f}", flush=True)
     assert err <= 1e-12
     assert rel_linf(cd, ref.Cd) <= 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32dev"])
+def test_benchmark_workload_schedules_bit_identical(precision):
+    """The benchmark workload at full size (1601 x 641, TRT + MDF + Peskin-4), 400 steps under
+    the fused launch's row orders: the workgroup row queue taking one or two rows, and one row
+    with the next row's populations copied into LDS ahead (with and without the bulk start
+    delay) give the default schedule's populations and Cd bit for bit."""
+    cfg = _c2_cfg(400)
+    scheds = [None, dict(step_rows=1, step_order="queue"), dict(step_rows=2, step_order="queue"),
+              dict(step_rows=3, step_order="queue"),
+              dict(step_rows=3, step_order="queue", bulk_delay_us=1.0)]
+    outs = []
+    for sc in scheds:
+        print(f"{precision} schedule {sc}", flush=True)
+        sim = FixedCylinder(cfg, perturb=(1234, 1e-3), precision=precision)
+        if sc is not None:
+            sim.ctx.set_schedule(**sc)
+            assert sim.ctx.step_mode() == "fused"
+        cd = []
+        for _ in range(4):
+            sim.advance(100, diag_every=100, graph=True)
+            cd.append(sim.cd_cl())
+        outs.append((sim.ctx.download_f(sim.k), np.array(cd)))
+        sim.ctx.close()
+    for f, cd in outs[1:]:
+        np.testing.assert_array_equal(f, outs[0][0])
+        np.testing.assert_array_equal(cd, outs[0][1])
+

@@ -36,6 +36,7 @@ def test_fp32dev_tracks_fp64_reference(case):
     dict(step_rows=4), dict(step_rows=4, step_order="blocked"), dict(step_rows=4, step_order=8),
     dict(step_rows=4, row_layout="flat"), dict(step_rows=2, step_order="queue"),
     dict(step_rows=1, step_order="queue", row_layout="flat"),
+    dict(step_rows=3, step_order="queue"), dict(step_rows=3, step_order="queue", row_layout="flat"),
     dict(step_rows=3), dict(step_rows=3, step_order="blocked"), dict(step_rows=3, step_order=8,
                                                                      row_layout="flat"),
     dict(mode="two_streams", bulk_threads=512, bulk_rows=3, bulk_waves=0),
