@@ -177,23 +177,35 @@ __device__ __attribute__((always_inline)) void bulk_pull_interior(const RowRun &
 // with no branch around them; lanes on the boundary then take the closure's values in
 // registers, exactly as fstar_at / fstar_dev32 form them.  Lanes past the grid load valid
 // addresses and are masked by the caller.
-template <typename S, typename V>
-__device__ __attribute__((always_inline)) void edge_loads(const RowRun &R, int i, int j, V *v) {
-    const S *__restrict__ A = (const S *)R.A;
+// The node whose pull a lane at (i, j) takes: itself clamped into the grid, or for the
+// Dirichlet-Neumann outlet column its inner neighbour (dirichlet_neumann.py:39-118: pull of
+// (nx-2, jj)).
+__device__ __attribute__((always_inline)) void edge_base(const RowRun &R, int i, int j, int &ci,
+                                                        int &cj) {
     const int nx = R.nx, ny = R.ny;
-    int ci = i < nx - 1 ? i : nx - 1;
-    int cj = j < ny - 1 ? j : ny - 1;
-    if (R.bc == 0 && ci == nx - 1) {  // dirichlet_neumann.py:39-118: pull of (nx-2, jj)
+    ci = i < nx - 1 ? i : nx - 1;
+    cj = j < ny - 1 ? j : ny - 1;
+    if (R.bc == 0 && ci == nx - 1) {
         ci = nx - 2;
         cj = cj == 0 ? 1 : (cj == ny - 1 ? ny - 2 : cj);
     }
+}
+// Offset of population q's source for that node, clamped into the grid.
+__device__ __attribute__((always_inline)) int64_t edge_off(const RowRun &R, int ci, int cj,
+                                                          int q) {
+    const int nx = R.nx, ny = R.ny;
+    int si = ci - EXi[q], sj = cj - EYi[q];
+    si = si < 0 ? 0 : (si > nx - 1 ? nx - 1 : si);
+    sj = sj < 0 ? 0 : (sj > ny - 1 ? ny - 1 : sj);
+    return q * R.qs + (int64_t)sj * R.pitch + si;
+}
+template <typename S, typename V>
+__device__ __attribute__((always_inline)) void edge_loads(const RowRun &R, int i, int j, V *v) {
+    const S *__restrict__ A = (const S *)R.A;
+    int ci, cj;
+    edge_base(R, i, j, ci, cj);
 #pragma unroll
-    for (int q = 0; q < 9; ++q) {
-        int si = ci - EXi[q], sj = cj - EYi[q];
-        si = si < 0 ? 0 : (si > nx - 1 ? nx - 1 : si);
-        sj = sj < 0 ? 0 : (sj > ny - 1 ? ny - 1 : sj);
-        v[q] = A[q * R.qs + (int64_t)sj * R.pitch + si];
-    }
+    for (int q = 0; q < 9; ++q) v[q] = A[edge_off(R, ci, cj, q)];
 }
 __device__ __attribute__((always_inline)) void edge_fix(const RowRun &R, int i, int j,
                                                        double *fs) {
@@ -491,32 +503,55 @@ __device__ __attribute__((always_inline)) void lds_dma4(const void *src, void *l
     typedef __attribute__((address_space(3))) void LVoid;
     __builtin_amdgcn_global_load_lds((GVoid *)src, (LVoid *)lds, 4, 0, 0);
 }
-// An interior row's 9 pull planes into LDS, plane q at buf + 64 q (value of lane l at
-// buf[64 q + l]).  FP64: a tile row's 64 values of a plane are contiguous (512 bytes), copied
-// as two dwords per lane; FP32: one value per lane from its own address (any row layout).
-template <typename S>
+// A row's 9 pull planes into LDS, plane q at buf + 64 q (value of lane l at buf[64 q + l]):
+// an interior row's plain pulls, an edge row's clamped ones (edge_base / edge_off, the
+// addresses edge_loads reads).  FP64 copies a dword per lane, so lane l of the copy k = 0, 1
+// moves dword l & 1 of the row's value 32 k + l / 2: an interior tile row's plane is 512
+// contiguous bytes, an edge row's values are taken one by one.  FP32 copies one value per lane
+// from its own address (any row layout).
+template <typename S, int FL>
 __device__ __attribute__((always_inline)) void row_to_lds(const RowRun &R, const TileNode &n,
                                                          int lane, S *buf) {
     const S *__restrict__ A = (const S *)R.A;
+    if (n.interior) {
 #pragma unroll
-    for (int q = 0; q < 9; ++q) {
-        if constexpr (sizeof(S) == 8) {
-            const char *src = (const char *)(A + q * R.qs + (int64_t)(n.j - EYi[q]) * R.pitch +
-                                             (n.i - lane - EXi[q]));
-            char *dst = (char *)(buf + 64 * q);
-            lds_dma4(src + 4 * lane, dst);
-            lds_dma4(src + 256 + 4 * lane, dst + 256);
-        } else {
-            lds_dma4(&A[q * R.qs + (int64_t)(n.j - EYi[q]) * R.pitch + (n.i - EXi[q])],
-                     buf + 64 * q);
+        for (int q = 0; q < 9; ++q) {
+            if constexpr (sizeof(S) == 8) {
+                const char *src = (const char *)(A + q * R.qs +
+                                                 (int64_t)(n.j - EYi[q]) * R.pitch +
+                                                 (n.i - lane - EXi[q]));
+                char *dst = (char *)(buf + 64 * q);
+                lds_dma4(src + 4 * lane, dst);
+                lds_dma4(src + 256 + 4 * lane, dst + 256);
+            } else {
+                lds_dma4(&A[q * R.qs + (int64_t)(n.j - EYi[q]) * R.pitch + (n.i - EXi[q])],
+                         buf + 64 * q);
+            }
         }
+    } else if constexpr (sizeof(S) == 8) {  // tiles: the row's values are columns i0 + e
+        int ci[2], cj[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) edge_base(R, n.i - lane + 32 * k + (lane >> 1), n.j, ci[k], cj[k]);
+#pragma unroll
+        for (int q = 0; q < 9; ++q)
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+                lds_dma4((const char *)(A + edge_off(R, ci[k], cj[k], q)) + 4 * (lane & 1),
+                         (char *)(buf + 64 * q) + 256 * k);
+    } else {
+        int i0, j0, ci, cj;
+        lattice_ij<FL>(R, n, i0, j0);
+        edge_base(R, i0, j0, ci, cj);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) lds_dma4(&A[edge_off(R, ci, cj, q)], buf + 64 * q);
     }
 }
 
 // Rows taken one at a time from the workgroup's counter (below r1), with the next row's
 // populations copied into LDS (global_load_lds) while the current row computes: the copy
 // holds no registers, so FP64 gets the look-ahead that its registers cannot hold.  Interior
-// rows only are copied ahead; an edge row takes its batched (EB) or per-lane pull at its turn.
+// rows and, with batched edge rows (EB), edge rows are copied ahead (an edge row's closure
+// then runs on the copied values); otherwise an edge row takes its per-lane pull at its turn.
 // lds: this wavefront's two row buffers of 9 x 64 values.
 template <typename S, int MODEL, bool EB, int FL>
 __device__ __attribute__((always_inline)) void bulk_rows_lds(const LaunchTable &T, int *queue,
@@ -536,8 +571,8 @@ __device__ __attribute__((always_inline)) void bulk_rows_lds(const LaunchTable &
     rr[0] = take();
     if (rr[0] >= r1) return;
     nn[0] = row_node<FL>(T, R[0], rr[0], lane);
-    pre[0] = nn[0].interior;
-    if (pre[0]) row_to_lds<S>(R[0], nn[0], lane, lds);
+    pre[0] = EB || nn[0].interior;
+    if (pre[0]) row_to_lds<S, FL>(R[0], nn[0], lane, lds);
     auto body = [&](auto cc) -> bool {
         constexpr int C = decltype(cc)::value, D = 1 - C;
         S a[9];
@@ -555,19 +590,17 @@ __device__ __attribute__((always_inline)) void bulk_rows_lds(const LaunchTable &
         if (more) {
             rr[D] = rn;
             nn[D] = row_node<FL>(T, R[D], rn, lane);
-            pre[D] = nn[D].interior;
-            if (pre[D]) row_to_lds<S>(R[D], nn[D], lane, lds + D * 9 * 64);
+            pre[D] = EB || nn[D].interior;
+            if (pre[D]) row_to_lds<S, FL>(R[D], nn[D], lane, lds + D * 9 * 64);
         }
         __builtin_amdgcn_sched_barrier(0);
         double rho = 0.0;
         if (pre[C]) {
-            const double v = bulk_finish<MODEL>(*R[C].I, R[C], nn[C].i, nn[C].j, a, nn[C].mine);
-            rho = nn[C].mine ? v : 0.0;
-        } else if constexpr (EB) {
-            int i0, j0;
-            lattice_ij<FL>(R[C], nn[C], i0, j0);
-            bulk_pull_edge(R[C], i0, j0, a);
-            edge_fix(R[C], i0, j0, a);
+            if (!nn[C].interior) {  // the closure on the copied (clamped) pulls
+                int i0, j0;
+                lattice_ij<FL>(R[C], nn[C], i0, j0);
+                edge_fix(R[C], i0, j0, a);
+            }
             const double v = bulk_finish<MODEL>(*R[C].I, R[C], nn[C].i, nn[C].j, a, nn[C].mine);
             rho = nn[C].mine ? v : 0.0;
         } else if (nn[C].mine) {
