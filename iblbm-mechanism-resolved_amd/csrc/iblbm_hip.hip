@@ -580,7 +580,9 @@ __device__ __attribute__((always_inline)) void bulk_rows_lds(const LaunchTable &
             // every copy this wavefront issued has landed (the only ones outstanding are this
             // row's, and the previous row's stores)
             __builtin_amdgcn_sched_barrier(0);
-            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+            // s_waitcnt vmcnt(0) (gfx9 encoding: vmcnt in bits 3:0 and 15:14 = 0, expcnt in
+            // 6:4 and lgkmcnt in 11:8 at their maxima, i.e. not waited on)
+            __builtin_amdgcn_s_waitcnt(0x0F70);
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int q = 0; q < 9; ++q) a[q] = lds[C * 9 * 64 + 64 * q + lane];
