@@ -47,7 +47,7 @@ PERTURB = (1234, 1e-3)
 # FETCH_SIZE / WRITE_SIZE (kB per dispatch), each scaled by what the counter reads for a
 # 1 GiB device copy (FETCH_SIZE reads 0.5x, WRITE_SIZE 1.0x of the bytes moved); the summary
 # is profiles/k_step_fp64_trt_mdf_p4_1601x641_traffic.md.
-TRAFFIC_FP64_C2 = {"nodes": 1026241, "bytes": (2 * 37268.5 + 72435.2) * 1024,
+TRAFFIC_FP64_C2 = {"nodes": 1026241, "bytes": (2 * 36977.6 + 72435.2) * 1024,
                    "source": "profiles/k_step_fp64_trt_mdf_p4_1601x641_traffic.md"}
 
 

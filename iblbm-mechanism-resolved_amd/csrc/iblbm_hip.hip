@@ -2325,35 +2325,57 @@ int iblbm_autotune(void *p, int steps, double *best_ms, int *n_tried) {
     const iblbm_schedule keep = C->sched;
     const bool conc = C->concurrent;
     C->concurrent = true;
-    double best = 1e300;
-    iblbm_schedule pick = keep;
     const int warm = 2, timed = steps + (steps & 1);  // even: the trial ends on the same parity
-    for (const iblbm_schedule &sc : cand) {
+    // one trial: the candidate's time per step over `len` steps from the saved state
+    auto trial = [&](const iblbm_schedule &sc, int len, double &ms_step) -> int {
         C->sched = sc;
         apply_row_layout(*C);
-        rc = ensure_tables(*C);
-        if (!rc) rc = restore();
-        if (!rc) rc = launch_steps(*C, warm, 0, true);
+        int r = ensure_tables(*C);
+        if (!r) r = restore();
+        if (!r) r = launch_steps(*C, warm, 0, true);
         C->parity = parity0;
-        if (!rc) HIPCHK(hipEventRecord(C->ev0, C->stream));
-        if (!rc) rc = launch_steps(*C, timed, 0, true);
-        if (!rc) HIPCHK(hipEventRecord(C->ev1, C->stream));
-        if (!rc) HIPCHK(hipEventSynchronize(C->ev1));
-        if (rc) {
-            release();
-            C->sched = keep;
-            C->concurrent = conc;
-            apply_row_layout(*C);
-            (void)ensure_tables(*C);
-            return rc;
-        }
+        if (!r) HIPCHK(hipEventRecord(C->ev0, C->stream));
+        if (!r) r = launch_steps(*C, len, 0, true);
+        if (!r) HIPCHK(hipEventRecord(C->ev1, C->stream));
+        if (!r) HIPCHK(hipEventSynchronize(C->ev1));
+        if (r) return r;
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, C->ev0, C->ev1));
-        if (ms / timed < best) {
-            best = ms / timed;
-            pick = sc;
-        }
+        ms_step = (double)ms / len;
+        return 0;
+    };
+    auto bail = [&](int r) {
+        release();
+        C->sched = keep;
+        C->concurrent = conc;
+        apply_row_layout(*C);
+        (void)ensure_tables(*C);
+        return r;
+    };
+    // every candidate once, then the four fastest again over four times the steps, twice in
+    // turn, so that a few percent between the leaders is not decided by one short trial
+    std::vector<std::pair<double, int>> first(cand.size());
+    for (size_t k = 0; k < cand.size(); ++k) {
+        double t = 0.0;
+        rc = trial(cand[k], timed, t);
+        if (rc) return bail(rc);
+        first[k] = {t, (int)k};
     }
+    std::sort(first.begin(), first.end());
+    const int lead = (int)std::min<size_t>(4, first.size());
+    std::vector<double> total(lead, 0.0);
+    for (int pass = 0; pass < 2; ++pass)
+        for (int k = 0; k < lead; ++k) {
+            double t = 0.0;
+            rc = trial(cand[first[k].second], 4 * timed, t);
+            if (rc) return bail(rc);
+            total[k] += t;
+        }
+    int win = 0;
+    for (int k = 1; k < lead; ++k)
+        if (total[k] < total[win]) win = k;
+    const iblbm_schedule pick = cand[first[win].second];
+    const double best = total[win] / 2;
     rc = restore();
     HIPCHK(hipStreamSynchronize(C->stream));
     release();
