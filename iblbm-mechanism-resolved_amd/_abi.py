@@ -109,6 +109,7 @@ SIGNATURES = {
     "iblbm_event_elapsed": (_i, [_VP, _i, _i, _D]),
     "iblbm_download_fib": (_i, [_VP, _i, _D]),
     "iblbm_download_state": (_i, [_VP, _i, _D, _D, _D]),
+    "iblbm_download_state_box": (_i, [_VP, _i, _i, _i, _i, _i, _D, _D]),
     "iblbm_download_dfc_force": (_i, [_VP, _i, _D]),
     "iblbm_download_window": (_i, [_VP, _i, _D, _D, _I]),
     "iblbm_step_counter": (_l, [_VP]),

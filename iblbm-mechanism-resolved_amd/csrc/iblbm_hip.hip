@@ -927,15 +927,16 @@ __global__ __launch_bounds__(256) void k_gather_forces(const Inst *insts, const 
     for (int k = threadIdx.x; k < n2; k += blockDim.x) out[2 * o + k] = src[k];
 }
 
-// state.fstar / ro / U of the last step (source buffer A = that step's input); touched nodes
-// come from the IBM scratch (DFC-corrected fstar, velocity-corrected U).
+// state.fstar / ro / U of the last step (source buffer A = that step's input) on the box
+// [x0, x0 + w) x [y0, y0 + h) (box-major outputs; fstar may be null); touched nodes come from
+// the IBM scratch (DFC-corrected fstar, velocity-corrected U).
 template <typename S>
-__global__ __launch_bounds__(256) void k_state(const Inst *Ip, int parity, double *fstar,
-                                               double *ro, double *U) {
+__global__ __launch_bounds__(256) void k_state(const Inst *Ip, int parity, int x0, int y0, int w,
+                                               int h, double *fstar, double *ro, double *U) {
     const Inst &I = *Ip;
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= (int64_t)I.nx * I.ny) return;
-    const int i = (int)(n % I.nx), j = (int)(n / I.nx);
+    if (n >= (int64_t)w * h) return;
+    const int i = x0 + (int)(n % w), j = y0 + (int)(n / w);
     double fs[9], r, u0, u1;
     const bool in_box = I.ibm != 0 && i >= I.bx0 && i < I.bx0 + I.bw && j >= I.by0 &&
                         j < I.by0 + I.bh;
@@ -954,7 +955,8 @@ __global__ __launch_bounds__(256) void k_state(const Inst *Ip, int parity, doubl
         fstar_at(buf<S>(I, parity), I, i, j, fs);
         macro9(fs, I.inc != 0, r, u0, u1);
     }
-    for (int q = 0; q < 9; ++q) fstar[9 * n + q] = fs[q];
+    if (fstar)
+        for (int q = 0; q < 9; ++q) fstar[9 * n + q] = fs[q];
     ro[n] = r;
     U[2 * n] = u0;
     U[2 * n + 1] = u1;
@@ -1019,6 +1021,8 @@ struct Ctx {
     int device = 0;
     int n_cu = 256;
     unsigned *d_bar = nullptr;  // persistent launch: grid-barrier counter, then its error flag
+    double *d_box = nullptr;    // download_state_box staging (ro, U of a box), box_cap doubles
+    size_t box_cap = 0;
     hipStream_t stream = nullptr, side = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_main = nullptr, ev_side = nullptr;
     bool concurrent = true;
@@ -1992,16 +1996,52 @@ int iblbm_download_state(void *p, int id, double *fstar, double *ro, double *U) 
     double *tf = (double *)tmp, *tr = tf + 9 * N, *tu = tr + N;
     if (d.prec == 1)
         hipLaunchKernelGGL(k_state<float>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
-                           C->stream, (const Inst *)(C->d_insts + id), C->parity ^ 1, tf, tr, tu);
+                           C->stream, (const Inst *)(C->d_insts + id), C->parity ^ 1, 0, 0, d.nx,
+                           d.ny, tf, tr, tu);
     else
         hipLaunchKernelGGL(k_state<double>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
-                           C->stream, (const Inst *)(C->d_insts + id), C->parity ^ 1, tf, tr, tu);
+                           C->stream, (const Inst *)(C->d_insts + id), C->parity ^ 1, 0, 0, d.nx,
+                           d.ny, tf, tr, tu);
     hipError_t e = hipStreamSynchronize(C->stream);
     if (e == hipSuccess) e = hipMemcpy(fstar, tf, sizeof(double) * 9 * N, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(ro, tr, sizeof(double) * N, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(U, tu, sizeof(double) * 2 * N, hipMemcpyDeviceToHost);
     (void)hipFree(tmp);
     if (e != hipSuccess) return fail(IBLBM_EDEVICE, std::string("download_state: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int iblbm_download_state_box(void *p, int id, int x0, int y0, int w, int h, double *ro,
+                             double *U) {
+    Ctx *C = as_ctx(p);
+    if (!valid_id(C, id) || !ro || !U || w < 1 || h < 1)
+        return fail(IBLBM_EINVAL, "download_state_box: bad argument");
+    const Inst &d = C->inst[id].d;
+    if (x0 < 0 || y0 < 0 || x0 + w > d.nx || y0 + h > d.ny)
+        return fail(IBLBM_EINVAL, "download_state_box: the box leaves the grid");
+    if (C->step == 0) return fail(IBLBM_EINVAL, "download_state_box: no step has run yet");
+    HIPCHK(hipSetDevice(C->device));
+    const int64_t N = (int64_t)w * h;
+    if ((size_t)(3 * N) > C->box_cap) {  // kept for the next calls (a larger box adds one)
+        void *q = nullptr;
+        if (C->mem.alloc(&q, sizeof(double) * 3 * N) != hipSuccess)
+            return fail(IBLBM_ENOMEM, "download_state_box: staging");
+        C->d_box = (double *)q;
+        C->box_cap = (size_t)(3 * N);
+    }
+    double *tr = C->d_box, *tu = tr + N;
+    if (d.prec == 1)
+        hipLaunchKernelGGL(k_state<float>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
+                           C->stream, (const Inst *)(C->d_insts + id), C->parity ^ 1, x0, y0, w,
+                           h, nullptr, tr, tu);
+    else
+        hipLaunchKernelGGL(k_state<double>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
+                           C->stream, (const Inst *)(C->d_insts + id), C->parity ^ 1, x0, y0, w,
+                           h, nullptr, tr, tu);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(C->stream));
+    HIPCHK(hipMemcpy(ro, tr, sizeof(double) * N, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(U, tu, sizeof(double) * 2 * N, hipMemcpyDeviceToHost));
     return 0;
 }
 

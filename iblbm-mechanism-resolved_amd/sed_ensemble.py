@@ -283,6 +283,9 @@ class SedimentationEnsemble:
         index) at every check interval; ``finalize_outputs`` completes them."""
         if not cfgs:
             raise ValueError("an ensemble needs at least one member")
+        if any(c.time_integrator != "verlet" for c in cfgs):
+            raise ValueError("an ensemble steps its particles with Velocity Verlet; run "
+                             "explicit-Euler members one at a time (sedimentation.run)")
         self.ctx = Context(device)
         cache = {}
         self.members = [Sedimentation(c, device=device, precision=precision, ctx=self.ctx,

@@ -127,6 +127,108 @@ def verlet_full(model, vel_half, vel, vel_prev, force, mass, rho_ratio, r_lat, g
     return vel_half + 0.5 * dt * F_total / mass + 0.5 * hdv
 
 
+def euler_explicit(model, pos, vel, vel_prev, force, mass, rho_ratio, r_lat, g, dt, dx, scheme,
+                   gdir):
+    """``euler_explicit`` of the ``none`` / ``explicit_history`` inertia models
+    (``inertia/none.py:48-59``, ``inertia/explicit_history.py:95-114``; ``feng_b2`` integrates
+    as ``none``) with the position update ``euler_explicit_position_update``
+    (``rigid_body.py:85-98``): x += dt u dx with u the new velocity or the mean of the old and
+    the new (``trapezoidal``)."""
+    F_total = force + compute_net_gravity(rho_ratio, r_lat, g, gdir)
+    hdv = _history_dv(model, vel, vel_prev, mass, r_lat)
+    if hdv is None:
+        vel_new = vel + dt * F_total / mass
+    else:
+        vel_new = vel + dt * F_total / mass + hdv
+    if scheme == "new_velocity":
+        pos_new = pos + dt * vel_new * dx
+    elif scheme == "trapezoidal":
+        pos_new = pos + 0.5 * dt * (vel + vel_new) * dx
+    else:
+        raise ValueError(f"sedimentation_euler_update_scheme={scheme!r}: 'new_velocity' or "
+                         "'trapezoidal'")
+    return pos_new, vel_new
+
+
+def euler_rotation(omega, torque, I, dt, dl_int, scheme):
+    """``euler_explicit_rotation`` (``rigid_body.py:101-121``): omega += (dL_int + dt T) / I,
+    the angle advanced by dt times the new rate or the mean of the old and the new."""
+    omega_new = omega + (dl_int + dt * torque) / I
+    if scheme == "new_velocity":
+        return omega_new, dt * omega_new
+    if scheme == "trapezoidal":
+        return omega_new, 0.5 * dt * (omega + omega_new)
+    raise ValueError(f"sedimentation_euler_update_scheme={scheme!r}: 'new_velocity' or "
+                     "'trapezoidal'")
+
+
+def l_int_box(nx, ny, dx, cx, cy, r_domain):
+    """The node box [i_lo, i_hi) x [j_lo, j_hi) around a disc that ``compute_l_int_z``
+    reads (``diagnostics/internal_flow.py:237-249``: the rounded centre +- int(r) + 2,
+    clipped to the grid)."""
+    ci, cj = cx / dx, cy / dx
+    r_idx = int(r_domain / dx) + 2
+    ci_int, cj_int = int(round(ci)), int(round(cj))
+    return (max(0, ci_int - r_idx), min(nx, ci_int + r_idx + 1),
+            max(0, cj_int - r_idx), min(ny, cj_int + r_idx + 1))
+
+
+def l_int_z(sub_ro, sub_U, i_lo, j_lo, dx, cx, cy, r_domain):
+    """Internal angular momentum of the fluid inside a disc, sum over the nodes within r of
+    the centre of (r_x rho u_y - r_y rho u_x) (``compute_l_int_z``,
+    ``diagnostics/internal_flow.py:220-258``), from rho and U on the box ``l_int_box`` gives
+    ((h, w) and (h, w, 2), box-major) in the same NumPy operations and order."""
+    ci, cj = cx / dx, cy / dx
+    r_lat = r_domain / dx
+    h, w = sub_ro.shape
+    x_lat = np.arange(i_lo, i_lo + w, dtype=float) - ci
+    y_lat = np.arange(j_lo, j_lo + h, dtype=float) - cj
+    RX, RY = np.meshgrid(x_lat, y_lat)
+    mask = RX**2 + RY**2 <= r_lat**2
+    sub_Eux, sub_Euy = sub_U[:, :, 0], sub_U[:, :, 1]
+    return float(np.sum(RX[mask] * sub_ro[mask] * sub_Euy[mask] -
+                        RY[mask] * sub_ro[mask] * sub_Eux[mask]))
+
+
+def euler_step_particles(particles, forces, cfg, st, g, state_box):
+    """The explicit Euler step of every particle from its (force, torque), then its moved
+    markers and desired velocities: the single-particle ``post_ibm``
+    (``sedimentation_single.py:187-262``) and ``_multi_particle_euler_step``
+    (``sedimentation_multi.py:127-202``).  state_box(i_lo, j_lo, w, h) returns rho and U of the
+    step just run on that box."""
+    scheme = cfg.sedimentation_euler_update_scheme
+    for p, (F, T) in zip(particles, forces):
+        pos_new, vel_new = euler_explicit(cfg.settling_inertia_model, p.pos, p.vel, p.vel_prev,
+                                          F, p.mass, p.rho_ratio, p.lattice_r, g, st.dt, st.dx,
+                                          scheme, cfg.gravity_direction)
+        omega_new = 0.0
+        if cfg.enable_rotation:
+            cx, cy = float(p.pos[0]), float(p.pos[1])  # the position before this update
+            i_lo, i_hi, j_lo, j_hi = l_int_box(st.nx, st.ny, st.dx, cx, cy, p.r_domain)
+            ro, U = state_box(i_lo, j_lo, i_hi - i_lo, j_hi - j_lo)
+            l_now = l_int_z(ro, U, i_lo, j_lo, st.dx, cx, cy, p.r_domain)
+            dl_int = l_now - p.l_int_prev
+            p.l_int_prev = l_now
+            omega_new, angle_inc = euler_rotation(p.omega, T, p.I, st.dt, dl_int, scheme)
+            p.angle += angle_inc
+            p.torque = T
+        vel_new = post_correction(cfg.settling_inertia_model, vel_new, p.vel, p.vel_prev,
+                                  p.rho_ratio)
+        if cfg.enable_rotation:
+            p.omega = omega_new
+        p.vel_prev = p.vel.copy()
+        p.vel = vel_new
+        p.pos = pos_new
+        p.force = F
+        p.Lx, p.Ly = update_markers(p.Lx_c, p.Ly_c, p.cx_init, p.cy_init, pos_new,
+                                    angle=p.angle if cfg.enable_rotation else 0.0)
+        if cfg.enable_rotation:
+            p.desired = desired_velocity(vel_new, p.Lb, omega=omega_new, Lx=p.Lx, Ly=p.Ly,
+                                         cx=float(pos_new[0]), cy=float(pos_new[1]), dx=st.dx)
+        else:
+            p.desired = desired_velocity(vel_new, p.Lb)
+
+
 def post_correction(model, vel_new, vel, vel_prev, rho_ratio):
     """The inertia model's correction after the full step: ``feng_b2`` adds
     (rho_f / rho_s) (U^n - U^{n-1}) (``inertia/feng_b2.py:33-48``); the others return the
@@ -194,6 +296,7 @@ class Particle:
         self.cx_init, self.cy_init = cx, cy
         self.vel_half = None
         self.omega_half = 0.0
+        self.l_int_prev = 0.0  # explicit Euler: the fluid's angular momentum inside the disc
 
 
 def make_particles(cfg, st):
@@ -303,8 +406,9 @@ class Sedimentation:
             raise ValueError(f"settling_inertia_model={cfg.settling_inertia_model!r} is outside "
                              "the GPU hot path (supported: 'none', 'explicit_history', "
                              "'feng_b2')")
-        if cfg.time_integrator != "verlet":
-            raise ValueError("only the Velocity Verlet integrator is on the GPU hot path")
+        if cfg.time_integrator not in ("verlet", "euler_explicit"):
+            raise ValueError(f"time_integrator={cfg.time_integrator!r}: 'verlet' or "
+                             "'euler_explicit'")
         if cfg.rotation_coupling == "iterative" and cfg.enable_rotation:
             raise ValueError("rotation_coupling='iterative' is outside the GPU hot path")
         if cfg.prescribed_velocity is not None:
@@ -443,11 +547,19 @@ class Sedimentation:
 
     def step(self):
         self.ttt += 1
-        self._half_step()
-        self._push_markers()
+        euler = self.cfg.time_integrator == "euler_explicit"
+        if not euler:  # Verlet: half step before the IBM (explicit Euler moves after it)
+            self._half_step()
+            self._push_markers()
         ci = int(self.cfg.check_interval)
         self.ctx.step(1, diag_every=1 if self.ttt % ci == 0 else 0)
-        self._full_step(self._forces())
+        if euler:
+            euler_step_particles(self.particles, self._forces(), self.cfg, self.st, self.g,
+                                 lambda x0, y0, w, h: self.ctx.download_state_box(
+                                     self.k, x0, y0, w, h))
+            self._push_markers()  # the markers of the next step
+        else:
+            self._full_step(self._forces())
         self._mdf_read = None  # the last step's MDF statistics are read when asked for
 
     @property

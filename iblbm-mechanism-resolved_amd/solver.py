@@ -94,6 +94,14 @@ class Context:
               "download_state")
         return fs, ro, U
 
+    def download_state_box(self, k, x0, y0, w, h):
+        """rho (h, w) and U (h, w, 2) of the last step on the box [x0, x0 + w) x
+        [y0, y0 + h): the same values as ``download_state``'s ro and U there."""
+        ro, U = np.empty((h, w)), np.empty((h, w, 2))
+        check(self.lib.iblbm_download_state_box(self.h, k, int(x0), int(y0), int(w), int(h),
+                                                dptr(ro), dptr(U)), "download_state_box")
+        return ro, U
+
     def download_fib(self, k):
         nx, ny = self.n[k]
         out = np.empty((nx * ny, 2))

@@ -286,6 +286,14 @@ int iblbm_download_fib(void *ctx, int id, double *fib_aos);
 /* Post-stream/boundary/IBM populations fstar, rho and U (velocity-corrected for DF/MDF) of
  * the last step, i.e. state.fstar, state.ro, state.U of the reference after that step. */
 int iblbm_download_state(void *ctx, int id, double *fstar_aos, double *ro, double *U);
+/* rho and U (velocity-corrected for DF/MDF) of the last step on the box [x0, x0 + w) x
+ * [y0, y0 + h) inside the grid, box-major (ro: w*h, U: (w*h, 2)): the sub-arrays of state.ro
+ * and state.U that the explicit Euler integrator's internal angular momentum reads around a
+ * disc (iblbm/diagnostics/internal_flow.py:220-258, called from
+ * iblbm/runtime/scenarios/sedimentation_single.py:198-206 and sedimentation_multi.py:148-156)
+ * without downloading the whole grid. */
+int iblbm_download_state_box(void *ctx, int id, int x0, int y0, int w, int h, double *ro,
+                             double *U);
 /* DFC marker forces q_k (Lb,2) of the last step (iblbm/ibm/dfc.py:232-253). */
 int iblbm_download_dfc_force(void *ctx, int id, double *q);
 /* fib of the last step on the window box only (box-major, bw*bh each) and the box

@@ -579,6 +579,40 @@ def case_sed_multi_mdf_guard_stop():
     _run_sed_guard("sed_multi_mdf_guard_stop", 40.0, 2)
 
 
+def case_sed_single_df_euler():
+    # the explicit Euler integrator: rotation fed by the fluid's angular momentum in the disc
+    _run_sed("sed_single_df_euler", dict(
+        Re=2.0, NN=81, xmax=1.0, ymax=3.0, cylinder_center=(0.48, 2.2), cylinder_D_ratio=0.25,
+        lattice_u=0.02, inflow_u=0.0, bc_type="settling_channel", delta_type="peskin4pt",
+        ibm_method="DF", collision_model="BGK", marker_spacing_factor=1.0, max_steps=150,
+        check_interval=1, use_convergence=False, motion_type="sedimentation", rho_ratio=1.5,
+        gravity=2e-5, enable_rotation=True, rotation_coupling="indirect",
+        settling_inertia_model="explicit_history", time_integrator="euler_explicit"))
+
+
+def _multi_df_inc(time_integrator):
+    return dict(
+        Re=2.0, NN=81, xmax=1.0, ymax=4.0, cylinder_center=(0.5, 3.0), cylinder_D_ratio=0.25,
+        lattice_u=0.02, inflow_u=0.0, bc_type="settling_channel", delta_type="peskin4pt",
+        ibm_method="DF", collision_model="BGK", marker_spacing_factor=1.0, max_steps=150,
+        check_interval=1, use_convergence=False, motion_type="sedimentation", rho_ratio=1.5,
+        gravity=2e-5, enable_rotation=True, rotation_coupling="semi_implicit",
+        settling_inertia_model="explicit_history", incompressible_lbgk=True,
+        time_integrator=time_integrator,
+        particles_config=[{"center": (0.5, 3.0), "rho_ratio": 1.5},
+                          {"center": (0.53, 2.6), "rho_ratio": 1.3}])
+
+
+def case_sed_multi_df_inc_verlet():
+    # two discs, incompressible LBGK (He & Luo), semi-implicit rotation coupling
+    _run_sed("sed_multi_df_inc_verlet", _multi_df_inc("verlet"))
+
+
+def case_sed_multi_df_inc_euler():
+    # the same with the explicit Euler integrator (the published two-disc Euler runs' options)
+    _run_sed("sed_multi_df_inc_euler", _multi_df_inc("euler_explicit"))
+
+
 CASES = {k[5:]: v for k, v in dict(globals()).items() if k.startswith("case_")}
 
 if __name__ == "__main__":

@@ -11,8 +11,9 @@ statistics of the last half) and the fixed-cylinder runs at Re = 20, 40, 100, 20
 steps, the final step, the stop reason and the averages over the last 30 %), and the
 single-disc settling runs (``data/single_particle_sedimentation*/**``: the trajectory every
 100 steps, the stop and the Reynolds-number peaks) and the two-disc settling runs
-(``data/two_particle_sedimentation/**`` with the Velocity Verlet integrator: both discs'
-trajectories every check interval, the stop and the Reynolds-number peaks per disc).  This script
+(``data/two_particle_sedimentation/**``: both discs' trajectories every check interval, the
+stop and the Reynolds-number peaks per disc), with the Velocity Verlet or the explicit Euler
+integrator.  This script
 reads them as data (``numpy.load`` without pickles, ``json``) and writes the numbers into one
 fixture, since the reference tree is not present where the GPU tests run.  Nothing of the
 reference is executed.
@@ -114,7 +115,8 @@ def main():
             with open(os.path.join(root, "status.json")) as fh:
                 s = json.load(fh)
             c = s.get("config", {})
-            if (c.get("time_integrator") != "verlet" or c.get("rotation_coupling") != "indirect"
+            if (c.get("time_integrator") not in ("verlet", "euler_explicit")
+                    or c.get("rotation_coupling") != "indirect"
                     or c.get("settling_inertia_model") not in ("none", "explicit_history",
                                                                "feng_b2")):
                 continue
@@ -137,6 +139,9 @@ def main():
                 [c["rho_ratio"], c["NN"], c["retraction_dx"], c["gravity"], c["max_steps"],
                  c["check_interval"], c["mdf_iterations"], s["final_step"],
                  0.01 if "nu001" in rel else np.nan], dtype=np.float64)
+            out[key + "_integrator"] = np.array([c["time_integrator"],
+                                                 c.get("sedimentation_euler_update_scheme",
+                                                       "new_velocity")])
             for pk in ("Re_standard_peaks", "Re_particle_basis_peaks", "peak_steps"):
                 out[f"{key}_{pk}"] = np.array(s.get(pk, []), dtype=np.float64)
             m = s.get("mdf_iter_stats")
@@ -160,8 +165,8 @@ def main():
         with open(os.path.join(d, "status.json")) as fh:
             s = json.load(fh)
         c = s["config"]
-        if c.get("time_integrator") != "verlet" or c.get("settling_inertia_model") not in (
-                "none", "explicit_history", "feng_b2"):
+        if c.get("time_integrator") not in ("verlet", "euler_explicit") or c.get(
+                "settling_inertia_model") not in ("none", "explicit_history", "feng_b2"):
             continue
         extent = 1.2 if rel.startswith("extended_60D") else (
             1.6 if rel.startswith("extended_80D") else 1.0)
@@ -189,6 +194,9 @@ def main():
              c["sedimentation_stop_offset_d"], extent, kappa], dtype=np.float64)
         for pk in ("Re_standard_peaks", "Re_particle_basis_peaks", "peak_steps"):
             out[f"{key}_{pk}"] = np.array(s.get(pk, []), dtype=np.float64)
+        out[key + "_integrator"] = np.array([
+            c["time_integrator"], c.get("sedimentation_euler_update_scheme", "new_velocity"),
+            "incompressible" if c.get("incompressible_lbgk") else "compressible"])
     out["two_names"] = np.array(two)
     out["two_columns"] = np.array(pcols)
     np.savez_compressed(OUT, **out)

@@ -1,6 +1,8 @@
 """Settling particles on the GPU path against the reference's own runs (tests/golden/sed_*):
 one particle (DF, MDF) and two (DFC spread as one ring; DF and coupled MDF spread per particle,
-with stencils that share nodes).
+with stencils that share nodes); Velocity Verlet and the explicit Euler integrator (whose
+rotation takes the change of the fluid's angular momentum inside the disc), compressible and
+incompressible LBGK.
 
 Trajectories (position, velocity, angular velocity, angle, force, torque per step), the MDF
 statistics logged with them (iteration count, last residual, and for coupled MDF the guard trips
@@ -16,7 +18,8 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("case", ["sed_single_df_bgk_hat", "sed_single_mdf_cm_p4",
                                   "sed_multi_dfc_cm_hat", "sed_multi_df_bgk_hat",
-                                  "sed_multi_mdf_cm_p4"])
+                                  "sed_multi_mdf_cm_p4", "sed_single_df_euler",
+                                  "sed_multi_df_inc_verlet", "sed_multi_df_inc_euler"])
 def test_sedimentation_matches_reference(case):
     g, sc = golden(case)
     cfg = golden_cfg(sc)
@@ -74,3 +77,32 @@ def test_coupled_mdf_guard_matches_reference(case):
     bad = np.flatnonzero(np.any(got != ref, axis=0))
     assert bad.size == 0, f"columns {bad.tolist()} differ"
     np.testing.assert_array_equal(sim.ctx.download_f(sim.k), g["f_post"])
+
+
+@pytest.mark.parametrize("case", ["sed_single_df_bgk_hat", "sed_multi_df_bgk_hat",
+                                  "sed_single_mdf_cm_p4", "sed_multi_dfc_cm_hat"])
+def test_state_box_matches_full_state(case):
+    """``download_state_box`` (the explicit Euler integrator's read of rho and U around each
+    disc) returns the sub-arrays of ``download_state``'s rho and U, the touched nodes'
+    velocity-corrected U included: for the box around each disc, boxes at two corners and the
+    whole grid; a box leaving the grid is refused."""
+    from iblbm_amd.sedimentation import l_int_box
+    g, sc = golden(case)
+    sim = Sedimentation(golden_cfg(sc))
+    for _ in range(5):
+        sim.step()
+    _, ro, U = sim.ctx.download_state(sim.k)
+    nx, ny = sim.st.nx, sim.st.ny
+    ro2, U2 = ro.reshape(ny, nx), U.reshape(ny, nx, 2)
+    boxes = [(0, 0, nx, ny), (0, 0, 3, 2), (nx - 4, ny - 5, 4, 5)]
+    for p in sim.particles:
+        i_lo, i_hi, j_lo, j_hi = l_int_box(nx, ny, sim.st.dx, float(p.pos[0]),
+                                           float(p.pos[1]), p.r_domain)
+        boxes.append((i_lo, j_lo, i_hi - i_lo, j_hi - j_lo))
+    for x0, y0, w, h in boxes:
+        r, u = sim.ctx.download_state_box(sim.k, x0, y0, w, h)
+        np.testing.assert_array_equal(r, ro2[y0:y0 + h, x0:x0 + w])
+        np.testing.assert_array_equal(u, U2[y0:y0 + h, x0:x0 + w])
+    with pytest.raises(ValueError):
+        sim.ctx.download_state_box(sim.k, nx - 2, 0, 3, 1)
+    sim.ctx.close()
