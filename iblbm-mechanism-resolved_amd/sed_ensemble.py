@@ -14,6 +14,11 @@ time step is:
    are read one by one;
 5. one vectorised Verlet full step.
 
+With the explicit Euler integrator (``sedimentation_multi.py:127-202``) step 1 is empty (the
+rings were placed by the previous step's update) and step 5 is one vectorised Euler update,
+which reads each live disc's rho and U box (``iblbm_download_state_box``) for the fluid's
+angular momentum inside it, then places every ring for the next step.
+
 ``ParticleBank`` evaluates the reference's per-particle expressions on arrays over all
 particles, operation for operation in the same order, so every member's trajectory carries
 the bits of its own single run (tested against ``Sedimentation`` and the reference fixtures).
@@ -22,7 +27,8 @@ from __future__ import annotations
 
 import numpy as np
 
-from .sedimentation import Sedimentation, compute_net_gravity, stop_final_step
+from .sedimentation import (Sedimentation, compute_net_gravity, l_int_box, l_int_z,
+                            stop_final_step)
 from .solver import Context
 
 
@@ -110,6 +116,14 @@ class ParticleBank:
         self.vel_half = np.zeros_like(self.vel)
         self.omega_half = np.zeros(self.P)
         self.live = np.ones(self.P, dtype=bool)
+        # explicit Euler (sedimentation_multi.py:127-202): the fluid's angular momentum inside
+        # each disc, read from its member's lattice after each step
+        self.euler = _uniform(members, "time_integrator") == "euler_explicit"
+        self.scheme = _uniform(members, "sedimentation_euler_update_scheme")
+        self.l_int_prev = np.array([float(p.l_int_prev) for p in parts])
+        self.r_domain = np.array([float(p.r_domain) for p in parts])
+        self.inst = [getattr(members[mi], "k", mi) for mi in owner]  # the member's instance
+        self.grid = [(members[mi].st.nx, members[mi].st.ny) for mi in owner]
         # marker layout
         lbs = np.array([p.Lb for p in parts])
         starts = np.concatenate([[0], np.cumsum(lbs)[:-1]])
@@ -138,6 +152,13 @@ class ParticleBank:
         self.Lx = np.empty(self.n_markers)
         self.Ly = np.empty(self.n_markers)
         self.ud = np.empty((self.n_markers, 2))
+        if self.euler:  # the first step runs on the initial rings at rest
+            for g in self.groups:
+                for i, r in enumerate(g["rows"]):
+                    dst = g["dst"][i]
+                    self.Lx[dst] = parts[r].Lx
+                    self.Ly[dst] = parts[r].Ly
+                    self.ud[dst] = parts[r].desired
 
     def _keep_frozen(self, new, old):
         if self.live.all():
@@ -162,6 +183,15 @@ class ParticleBank:
             self.angle = self._keep_frozen(self.angle + self.dt * self.omega_half, self.angle)
         else:
             self.omega_half = np.zeros(self.P)
+        om = None
+        if self.semi_implicit and self.rotation:
+            om = np.where(self.multi, self.omega, self.omega_half)
+        return self._place_markers(self.vel_half, om)
+
+    def _place_markers(self, vel, om):
+        """Every ring at its particle's position and angle, with desired velocity vel (P, 2)
+        plus, where om (P,) is given and non-zero, the rotation om x r
+        (``update_markers`` / ``desired_velocity``, ``markers.py:23-57``, array form)."""
         for g in self.groups:
             r = g["rows"]
             rx = g["Lx_c"] - g["cx0"][:, None]
@@ -175,14 +205,14 @@ class ParticleBank:
             Lx = c * rx - s * ry + px
             Ly = s * rx + c * ry + py
             g["Lx"], g["Ly"] = Lx, Ly
-            d0 = np.broadcast_to(self.vel_half[r, 0][:, None], Lx.shape)
-            d1 = np.broadcast_to(self.vel_half[r, 1][:, None], Lx.shape)
-            if self.semi_implicit and self.rotation:  # markers.py:41-57
-                om = np.where(self.multi[r], self.omega[r], self.omega_half[r])[:, None]
+            d0 = np.broadcast_to(vel[r, 0][:, None], Lx.shape)
+            d1 = np.broadcast_to(vel[r, 1][:, None], Lx.shape)
+            if om is not None:  # markers.py:41-57
+                omr = om[r][:, None]
                 dxp = self.dx[r][:, None]
-                on = om != 0.0
-                d0 = np.where(on, d0 + (-om * (Ly - py)) / dxp, d0)
-                d1 = np.where(on, d1 + (om * (Lx - px)) / dxp, d1)
+                on = omr != 0.0
+                d0 = np.where(on, d0 + (-omr * (Ly - py)) / dxp, d0)
+                d1 = np.where(on, d1 + (omr * (Lx - px)) / dxp, d1)
             if g["span"] is not None:
                 a, b = g["span"]
                 self.Lx[a:b] = Lx.ravel()
@@ -250,6 +280,50 @@ class ParticleBank:
         self.vel = self._keep_frozen(vel_new, self.vel)
         self.force = self._keep_frozen(F, self.force)
 
+    def euler_step(self, F, T, state_box):
+        """Explicit Euler step of every particle (``euler_explicit``, ``euler_rotation`` and
+        ``post_correction``, array form, each element in the scalar form's order), then every
+        ring and desired velocity for the next step (the new velocity, and the new rate of
+        rotation whatever the coupling).  state_box(k, x0, y0, w, h) reads rho and U of
+        particle k's member on a box after the step."""
+        F_total = F + self.grav
+        vel_new = self.vel + (self.dt[:, None] * F_total) / self.mass[:, None]
+        if self.hcoef is not None:
+            vel_new = vel_new + self.hcoef[:, None] * (self.vel - self.vel_prev)
+        if self.scheme == "new_velocity":
+            pos = self.pos + (self.dt[:, None] * vel_new) * self.dx[:, None]
+        else:
+            pos = self.pos + ((0.5 * self.dt)[:, None] * (self.vel + vel_new)) * self.dx[:, None]
+        om = None
+        if self.rotation:
+            dl = np.zeros(self.P)
+            for k in np.nonzero(self.live)[0]:  # l_int_z about the position before the step
+                nx, ny = self.grid[k]
+                cx, cy = float(self.pos[k, 0]), float(self.pos[k, 1])
+                i_lo, i_hi, j_lo, j_hi = l_int_box(nx, ny, float(self.dx[k]), cx, cy,
+                                                   float(self.r_domain[k]))
+                ro, U = state_box(k, i_lo, j_lo, i_hi - i_lo, j_hi - j_lo)
+                l_now = l_int_z(ro, U, i_lo, j_lo, float(self.dx[k]), cx, cy,
+                                float(self.r_domain[k]))
+                dl[k] = l_now - self.l_int_prev[k]
+                self.l_int_prev[k] = l_now
+            omega_new = self.omega + (dl + self.dt * T) / self.I
+            if self.scheme == "new_velocity":
+                inc = self.dt * omega_new
+            else:
+                inc = (0.5 * self.dt) * (self.omega + omega_new)
+            self.angle = self._keep_frozen(self.angle + inc, self.angle)
+            self.torque = self._keep_frozen(T, self.torque)
+            self.omega = self._keep_frozen(omega_new, self.omega)
+            om = self.omega
+        if self.rinv is not None:  # post_correction
+            vel_new = vel_new + self.rinv[:, None] * (self.vel - self.vel_prev)
+        self.vel_prev = self._keep_frozen(self.vel.copy(), self.vel_prev)
+        self.vel = self._keep_frozen(vel_new, self.vel)
+        self.pos = self._keep_frozen(pos, self.pos)
+        self.force = self._keep_frozen(F, self.force)
+        return self._place_markers(self.vel, om)
+
     def to_particles(self):
         """Write the arrays back into the members' ``Particle`` objects."""
         for k, p in enumerate(self.parts):
@@ -283,9 +357,6 @@ class SedimentationEnsemble:
         index) at every check interval; ``finalize_outputs`` completes them."""
         if not cfgs:
             raise ValueError("an ensemble needs at least one member")
-        if any(c.time_integrator != "verlet" for c in cfgs):
-            raise ValueError("an ensemble steps its particles with Velocity Verlet; run "
-                             "explicit-Euler members one at a time (sedimentation.run)")
         self.ctx = Context(device)
         cache = {}
         self.members = [Sedimentation(c, device=device, precision=precision, ctx=self.ctx,
@@ -333,7 +404,10 @@ class SedimentationEnsemble:
         """First half of a time step: particle half step, batched marker upload and the step
         launch, which runs on while the host returns."""
         self.ttt += 1
-        Lx, Ly, ud = self.bank.half_step()
+        if self.bank.euler:  # the rings the previous step's update placed (explicit Euler)
+            Lx, Ly, ud = self.bank.Lx, self.bank.Ly, self.bank.ud
+        else:
+            Lx, Ly, ud = self.bank.half_step()
         center = self.bank.pos[self.first]
         self.ctx.set_markers_batch(self.ids, self.lb, Lx, Ly, ud, self.larea, center)
         self.ctx.step(1, diag_every=1 if self.ttt % self.check_interval == 0 else 0)
@@ -342,7 +416,12 @@ class SedimentationEnsemble:
         """Second half: the forces (waits for this context's step), the full step, the
         checks at the check interval."""
         F, T = self._forces()
-        self.bank.full_step(F, T)
+        if self.bank.euler:
+            bank = self.bank
+            self.bank.euler_step(F, T, lambda k, x0, y0, w, h: self.ctx.download_state_box(
+                bank.inst[k], x0, y0, w, h))
+        else:
+            self.bank.full_step(F, T)
         if self.ttt % self.check_interval == 0:
             self._check()
 

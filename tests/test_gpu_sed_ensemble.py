@@ -206,3 +206,35 @@ def test_ensemble_mdf_member_writes_the_single_runs_files(tmp_path):
         hist = json.load(fh)
     recs = hist if isinstance(hist, list) else hist.get("history", hist)
     assert any("mdf_iter" in r for r in recs)
+
+
+def _euler_variants(name, shifts):
+    base = golden_cfg(golden(name)[1])
+    out = [base]
+    for dx, dy in shifts:
+        if base.particles_config is not None:
+            pc = [dict(p) for p in base.particles_config]
+            c = pc[1]["center"]
+            pc[1]["center"] = [c[0] + dx, c[1] + dy]
+            out.append(dataclasses.replace(base, particles_config=pc))
+        else:
+            c = base.cylinder_center
+            out.append(dataclasses.replace(base, cylinder_center=(c[0] + dx, c[1] + dy)))
+    return out
+
+
+@pytest.mark.parametrize("name", ["sed_multi_df_inc_euler", "sed_single_df_euler"])
+def test_euler_ensemble_matches_single_runs(name):
+    """Explicit Euler members (the reference's two-disc default: incompressible LBGK,
+    semi-implicit coupling; and one disc) stepped as an ensemble: each member's trajectory and
+    populations equal its own single run bit for bit, the first member the reference fixture."""
+    n = 150
+    cfgs = _euler_variants(name, [(0.02, -0.05), (-0.03, 0.04)])
+    print(f"explicit-Euler ensemble {name}: 3 members, {n} steps, fp64", flush=True)
+    rows, fs, batched = _ensemble(cfgs, n)
+    assert batched == (name == "sed_multi_df_inc_euler")
+    for k, (cfg, r, f) in enumerate(zip(cfgs, rows, fs)):
+        r1, f1 = _single(cfg, n)
+        assert np.array_equal(r, r1), k
+        assert np.array_equal(f, f1), k
+    _check_fixture(name, rows[0], n)

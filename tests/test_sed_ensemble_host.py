@@ -74,6 +74,62 @@ def test_bank_matches_per_particle_code(variant):
                                   _bits([float(getattr(p, name)) for p in parts])), (step, name)
 
 
+@pytest.mark.parametrize("scheme", ["new_velocity", "trapezoidal"])
+def test_bank_euler_matches_per_particle_code(scheme):
+    """The vectorised explicit Euler update against ``euler_step_particles``, with random
+    forces and torques and a random rho / U field standing in for the lattice (each box read
+    from it): positions, velocities, rotation, the internal angular momentum and the next
+    step's rings and desired velocities, bit for bit."""
+    from iblbm_amd.sedimentation import euler_step_particles
+
+    def members():
+        out = []
+        for name in FIXTURES:
+            cfg = dataclasses.replace(golden_cfg(golden(name)[1]),
+                                      time_integrator="euler_explicit",
+                                      sedimentation_euler_update_scheme=scheme)
+            out.append(_member(cfg))
+        return out
+
+    scalar, vec = members(), members()
+    bank = ParticleBank(vec)
+    assert bank.euler
+    rng = np.random.default_rng(12)
+    fields = {}
+
+    def field(mi, m):  # a fixed rho / U field per member and step
+        if mi not in fields:
+            nx, ny = m.st.nx, m.st.ny
+            fields[mi] = (1.0 + 0.01 * rng.standard_normal((ny, nx)),
+                          0.01 * rng.standard_normal((ny, nx, 2)))
+        return fields[mi]
+
+    owner = [mi for mi, m in enumerate(scalar) for _ in m.particles]
+    for step in range(30):
+        fields.clear()
+        F = rng.normal(scale=1e-3, size=(bank.P, 2))
+        T = rng.normal(scale=1e-4, size=bank.P)
+        Lx, Ly, ud = bank.euler_step(F, T, lambda k, x0, y0, w, h: tuple(
+            a[y0:y0 + h, x0:x0 + w] for a in field(owner[k], scalar[owner[k]])))
+        k = 0
+        for mi, m in enumerate(scalar):
+            n = len(m.particles)
+            euler_step_particles(m.particles, [(F[k + j], T[k + j]) for j in range(n)], m.cfg,
+                                 m.st, m.g, lambda x0, y0, w, h, mi=mi, m=m: tuple(
+                                     a[y0:y0 + h, x0:x0 + w] for a in field(mi, m)))
+            k += n
+        parts = [p for m in scalar for p in m.particles]
+        for name in ["pos", "vel", "vel_prev", "force"]:
+            assert np.array_equal(_bits(getattr(bank, name)),
+                                  _bits([getattr(p, name) for p in parts])), (step, name)
+        for name in ["omega", "angle", "torque", "l_int_prev"]:
+            assert np.array_equal(_bits(getattr(bank, name)),
+                                  _bits([float(getattr(p, name)) for p in parts])), (step, name)
+        assert np.array_equal(_bits(Lx), _bits(np.concatenate([p.Lx for p in parts])))
+        assert np.array_equal(_bits(Ly), _bits(np.concatenate([p.Ly for p in parts])))
+        assert np.array_equal(_bits(ud), _bits(np.concatenate([p.desired for p in parts])))
+
+
 def test_frozen_member_keeps_its_particles():
     members = _members("indirect")
     bank = ParticleBank(members)
