@@ -255,9 +255,10 @@ int iblbm_set_schedule(void *ctx, const iblbm_schedule *in);
  * the context qualifies for it. */
 int iblbm_row_layout_in_use(void *ctx, int *layout);
 /* Times every candidate schedule for `steps` steps (after 2 warm-up steps) on the context's
- * current state, restores the populations and the step counter after each trial, and keeps
- * the fastest.  best_ms (may be NULL) receives its time per step; n_tried the number of
- * candidates.  Refused while a learned-predictor mode is on (trial steps would record
+ * current state, then the four fastest again for 4 x `steps` steps, twice in turn; restores
+ * the populations and the step counter before each trial, and keeps the fastest of the
+ * re-timed ones.  best_ms (may be NULL) receives its mean time per step over those two
+ * trials; n_tried the number of candidates.  Refused while a learned-predictor mode is on (trial steps would record
  * samples).  The last step's diagnostics and MDF statistics afterwards are a trial's. */
 int iblbm_autotune(void *ctx, int steps, double *best_ms, int *n_tried);
 
