@@ -312,6 +312,12 @@ def make_case(cfg, st, precision=_abi.FP64) -> _abi.Case:
         raise KeyError(f"unknown ibm_method={cfg.ibm_method!r} (registered: ['DF', 'DFC', 'MDF'])")
     if cfg.delta_type not in _abi.DELTA:
         raise KeyError(cfg.delta_type)
+    if cfg.stream_first and cfg.ibm_method == "DFC":
+        # DF and MDF already collide after the IBM with the current step's force (the
+        # force-level scheduler, runtime/loop.py:60-87 and runtime/scheduler.py:20-27), which
+        # is the stream-first order; DFC collides before it
+        raise ValueError("stream_first=True with DFC is outside the GPU hot path (DF and MDF "
+                         "run it: their steps already take that order)")
     c = _abi.Case()
     c.nx, c.ny = st.nx, st.ny
     c.collision = _abi.COLL[cfg.collision_model]
@@ -538,8 +544,6 @@ def run(cfg, verbose: bool = True, callback=None, save_dir=None, perturb=None,
     if cfg.motion_type == "sedimentation":
         raise ValueError(f"motion_type={cfg.motion_type!r}: sedimentation runs through "
                          "iblbm_amd.sedimentation.run")
-    if cfg.stream_first:
-        raise ValueError("stream_first=True is outside the GPU hot path")
     sim = FixedCylinder(cfg, perturb=perturb, device=device, precision=precision)
     mon = None
     if save_dir:  # status.json in the reference's format

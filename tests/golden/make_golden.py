@@ -312,6 +312,18 @@ def case_run_mdf_trt_p4():
         perturb=(1234, 1e-3), record_mdf=True)
 
 
+def case_run_sf_mdf_trt_p4():
+    # run_mdf_trt_p4 with stream_first=True (Stream -> Macro -> IBM -> Collision)
+    _run_case("run_sf_mdf_trt_p4", dict(
+        Re=10.0, NN=25, xmax=2.0, ymax=1.0, cylinder_center=(0.5, 0.5), cylinder_D_ratio=0.25,
+        lattice_u=0.04, inflow_u=0.04, bc_type="dirichlet_neumann", delta_type="peskin4pt",
+        ibm_method="MDF", collision_model="TRT", trt_magic_param=0.25,
+        mdf_iterations=20, mdf_min_iterations=5, mdf_tolerance=1e-4,
+        marker_spacing_factor=1.0, max_steps=120, check_interval=1, use_convergence=False,
+        stream_first=True),
+        perturb=(1234, 1e-3), record_mdf=True)
+
+
 def case_run_dfc_cm_hat():
     _run_case("run_dfc_cm_hat", dict(
         Re=8.0, NN=21, xmax=2.0, ymax=1.0, cylinder_center=(0.45, 0.5), cylinder_D_ratio=0.25,
