@@ -411,8 +411,9 @@ class Sedimentation:
                              "'euler_explicit'")
         if cfg.rotation_coupling == "iterative" and cfg.enable_rotation:
             raise ValueError("rotation_coupling='iterative' is outside the GPU hot path")
-        if cfg.prescribed_velocity is not None:
-            raise ValueError("prescribed_velocity is outside the GPU hot path")
+        if cfg.prescribed_velocity is not None and cfg.particles_config is not None:
+            # the reference's particle-list runtime has no prescribed-velocity branch
+            raise ValueError("prescribed_velocity applies to the single-particle run")
         if cfg.gravity <= 0.0:
             raise ValueError(f"sedimentation needs gravity > 0 (got {cfg.gravity})")
         if cfg.rho_ratio <= 1.0:
@@ -545,8 +546,28 @@ class Sedimentation:
     def _full_step(self, forces):
         full_step_particles(self.particles, forces, self.cfg, self.st, self.g)
 
+    def _prescribed_step(self, ci):
+        """A disc moved at a prescribed velocity (``sedimentation_single.py:92-98``,
+        ``:176-181``): it moves by dt u dx before the IBM, its ring follows without rotation,
+        the desired velocity is u, and after the step its velocity is u and its force the
+        hydrodynamic force (no torque, no inertia model)."""
+        st, p = self.st, self.particles[0]
+        pv = np.array(self.cfg.prescribed_velocity, dtype=float)
+        p.pos = p.pos + st.dt * pv * st.dx
+        p.Lx, p.Ly = update_markers(p.Lx_c, p.Ly_c, p.cx_init, p.cy_init, p.pos)
+        p.desired = desired_velocity(pv, p.Lb)
+        self._push_markers()
+        self.ctx.step(1, diag_every=1 if self.ttt % ci == 0 else 0)
+        F, _ = self._forces()[0]
+        p.vel = np.array(self.cfg.prescribed_velocity, dtype=float)
+        p.force = F
+
     def step(self):
         self.ttt += 1
+        if self.cfg.prescribed_velocity is not None:
+            self._prescribed_step(int(self.cfg.check_interval))
+            self._mdf_read = None
+            return
         euler = self.cfg.time_integrator == "euler_explicit"
         if not euler:  # Verlet: half step before the IBM (explicit Euler moves after it)
             self._half_step()

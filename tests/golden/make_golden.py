@@ -602,6 +602,17 @@ def case_sed_single_df_euler():
         settling_inertia_model="explicit_history", time_integrator="euler_explicit"))
 
 
+def case_sed_single_df_prescribed():
+    # a disc moved at a prescribed velocity (towed down and slightly sideways)
+    _run_sed("sed_single_df_prescribed", dict(
+        Re=2.0, NN=41, xmax=1.0, ymax=3.0, cylinder_center=(0.48, 2.2), cylinder_D_ratio=0.25,
+        lattice_u=0.02, inflow_u=0.0, bc_type="settling_channel", delta_type="peskin4pt",
+        ibm_method="DF", collision_model="BGK", marker_spacing_factor=1.0, max_steps=120,
+        check_interval=1, use_convergence=False, motion_type="sedimentation", rho_ratio=1.5,
+        gravity=2e-5, enable_rotation=True, rotation_coupling="indirect",
+        settling_inertia_model="explicit_history", prescribed_velocity=(0.001, -0.004)))
+
+
 def _multi_df_inc(time_integrator):
     return dict(
         Re=2.0, NN=81, xmax=1.0, ymax=4.0, cylinder_center=(0.5, 3.0), cylinder_D_ratio=0.25,

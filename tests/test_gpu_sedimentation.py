@@ -2,7 +2,7 @@
 one particle (DF, MDF) and two (DFC spread as one ring; DF and coupled MDF spread per particle,
 with stencils that share nodes); Velocity Verlet and the explicit Euler integrator (whose
 rotation takes the change of the fluid's angular momentum inside the disc), compressible and
-incompressible LBGK.
+incompressible LBGK, and a disc moved at a prescribed velocity.
 
 Trajectories (position, velocity, angular velocity, angle, force, torque per step), the MDF
 statistics logged with them (iteration count, last residual, and for coupled MDF the guard trips
@@ -19,7 +19,8 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("case", ["sed_single_df_bgk_hat", "sed_single_mdf_cm_p4",
                                   "sed_multi_dfc_cm_hat", "sed_multi_df_bgk_hat",
                                   "sed_multi_mdf_cm_p4", "sed_single_df_euler",
-                                  "sed_multi_df_inc_verlet", "sed_multi_df_inc_euler"])
+                                  "sed_multi_df_inc_verlet", "sed_multi_df_inc_euler",
+                                  "sed_single_df_prescribed"])
 def test_sedimentation_matches_reference(case):
     g, sc = golden(case)
     cfg = golden_cfg(sc)
