@@ -312,12 +312,11 @@ def make_case(cfg, st, precision=_abi.FP64) -> _abi.Case:
         raise KeyError(f"unknown ibm_method={cfg.ibm_method!r} (registered: ['DF', 'DFC', 'MDF'])")
     if cfg.delta_type not in _abi.DELTA:
         raise KeyError(cfg.delta_type)
-    if cfg.stream_first and cfg.ibm_method == "DFC":
-        # DF and MDF already collide after the IBM with the current step's force (the
-        # force-level scheduler, runtime/loop.py:60-87 and runtime/scheduler.py:20-27), which
-        # is the stream-first order; DFC collides before it
-        raise ValueError("stream_first=True with DFC is outside the GPU hot path (DF and MDF "
-                         "run it: their steps already take that order)")
+    # stream_first (runtime/loop.py:60-87) moves the collision from the top of a step to its
+    # end, after one collision before the loop: the sequence of operations is the same, and
+    # no hook between the end of one step and the top of the next changes what the collision
+    # reads, so it runs unchanged (the reference's stream-first runs equal its default-order
+    # runs bit for bit)
     c = _abi.Case()
     c.nx, c.ny = st.nx, st.ny
     c.collision = _abi.COLL[cfg.collision_model]

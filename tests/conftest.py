@@ -43,7 +43,7 @@ def perturb_of(sc):
 
 
 RUN_CASES = ["run_df_bgk_hat", "run_mdf_trt_p4", "run_dfc_cm_hat", "run_df_cm_p4",
-             "run_dfc_trt_p4", "run_mdf_bgk_hat", "run_sf_mdf_trt_p4"]
+             "run_dfc_trt_p4", "run_mdf_bgk_hat", "run_sf_mdf_trt_p4", "run_sf_dfc_cm_hat"]
 # runs with the velocity-inlet, open and Taylor-Green boundaries (oscillating cylinder and
 # Taylor-Green vortex scenarios)
 BC_RUN_CASES = ["run_vi_df_bgk_hat", "run_vi_mdf_cm_p4", "run_osc_df_bgk_hat",

@@ -332,6 +332,15 @@ def case_run_dfc_cm_hat():
         max_steps=150, check_interval=10, use_convergence=False))
 
 
+def case_run_sf_dfc_cm_hat():
+    # run_dfc_cm_hat with stream_first=True
+    _run_case("run_sf_dfc_cm_hat", dict(
+        Re=8.0, NN=21, xmax=2.0, ymax=1.0, cylinder_center=(0.45, 0.5), cylinder_D_ratio=0.25,
+        lattice_u=0.05, inflow_u=0.05, bc_type="dirichlet_neumann", delta_type="hat",
+        ibm_method="DFC", collision_model="CM_MRT", marker_spacing_factor=1.0,
+        max_steps=150, check_interval=10, use_convergence=False, stream_first=True))
+
+
 def case_run_df_cm_p4():
     _run_case("run_df_cm_p4", dict(
         Re=12.0, NN=21, xmax=2.0, ymax=1.0, cylinder_center=(0.55, 0.48), cylinder_D_ratio=0.3,

@@ -135,25 +135,20 @@ def test_stop_rules_several_particles():
     assert stop_reason(c, [_p(0.5, 0.04), b], dx, True) == "contact"
 
 
-def test_stream_first_is_the_force_level_order_for_df_and_mdf():
-    """The reference's MDF run with stream_first=True (Stream -> Macro -> IBM -> Collision) is
-    its default-order run bit for bit (both fixtures written from the reference): DF and MDF
-    already collide after the IBM with the current step's force, so the device runs
-    stream_first for them unchanged; DFC, whose default order differs, is refused."""
-    import dataclasses
-
-    import pytest
+def test_stream_first_runs_equal_the_default_order():
+    """The reference's runs with stream_first=True (a collision before the loop, then Stream
+    -> Macro -> IBM -> Collision each step) are its default-order runs bit for bit, for MDF
+    and for DFC (fixtures written from the reference): the same operations in the same
+    sequence, the loop boundary moved.  The device runs stream_first unchanged."""
     from conftest import golden, golden_cfg
     from iblbm_amd.init import initialize
     from iblbm_amd.solver import make_case
-    a, sa = golden("run_mdf_trt_p4")
-    b, sb = golden("run_sf_mdf_trt_p4")
-    assert sb["cfg"]["stream_first"] is True and not sa["cfg"].get("stream_first", False)
-    for k in a.files:
-        assert np.array_equal(a[k], b[k]), k
-    cfg = golden_cfg(sb)
-    st = initialize(cfg, fields=False)
-    make_case(cfg, st)  # accepted
-    dcfg = dataclasses.replace(golden_cfg(golden("run_dfc_cm_hat")[1]), stream_first=True)
-    with pytest.raises(ValueError):
-        make_case(dcfg, initialize(dcfg, fields=False))
+    for base, sf in (("run_mdf_trt_p4", "run_sf_mdf_trt_p4"),
+                     ("run_dfc_cm_hat", "run_sf_dfc_cm_hat")):
+        a, sa = golden(base)
+        b, sb = golden(sf)
+        assert sb["cfg"]["stream_first"] is True and not sa["cfg"].get("stream_first", False)
+        for k in a.files:
+            assert np.array_equal(a[k], b[k]), (sf, k)
+        cfg = golden_cfg(sb)
+        make_case(cfg, initialize(cfg, fields=False))  # accepted
