@@ -107,3 +107,25 @@ def test_state_box_matches_full_state(case):
     with pytest.raises(ValueError):
         sim.ctx.download_state_box(sim.k, nx - 2, 0, 3, 1)
     sim.ctx.close()
+
+
+@pytest.mark.parametrize("case", ["sed_single_df_bgk_hat", "sed_single_df_euler",
+                                  "sed_single_df_prescribed"])
+def test_settling_channel_under_the_lds_look_ahead(case):
+    """The fused launch's row queue with the LDS look-ahead (edge rows copied from their
+    clamped addresses, then the settling channel's wall closures) on settling runs whose
+    window takes the fused path: every logged column and the final populations equal the
+    reference's run bit for bit."""
+    g, sc = golden(case)
+    sim = Sedimentation(golden_cfg(sc))
+    sim.ctx.set_schedule(mode="fused", step_rows=3, step_order="queue")
+    assert sim.ctx.step_mode() == "fused"
+    rows = []
+    for _ in range(sc["final_step"]):
+        sim.step()
+        p = sim.particles[0]
+        rows.append([sim.ttt, p.pos[0], p.pos[1], p.vel[0], p.vel[1], p.omega, p.angle,
+                     p.force[0], p.force[1], p.torque])
+    np.testing.assert_array_equal(np.array(rows), g["particles"])
+    np.testing.assert_array_equal(sim.ctx.download_f(sim.k), g["f_post"])
+    sim.ctx.close()
